@@ -1,0 +1,127 @@
+"""GPU single-porosity solver: the host-side mirror of the reference's swSp.
+
+``SpSolver(case)`` plays the part of ``swParent::create("sp")`` +
+``swSp::init`` (src/sw2dSolver/models/sp/swSp.cpp:720-969); ``execute()`` is
+``swSp::execute`` (swSp.cpp:482-670); ``step(n)`` runs n iterations of its
+time loop.  All arithmetic runs in the HIP kernels of libsw2dg.so; there is
+no CPU path here.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .cases import Case, MemCase
+
+
+class SpSolver:
+    def __init__(self, case, device=-1, params=None):
+        if not isinstance(case, (Case, MemCase)):
+            case = Case(case)
+        self.case = case
+        self.lib = N.gpu_lib()
+        a = case.a
+        self._keep = {}
+
+        def keep(name, arr, dtype):
+            arr = np.ascontiguousarray(arr, dtype=dtype)
+            self._keep[name] = arr
+            return arr
+
+        m = N.Mesh()
+        m.n_cell, m.n_bnd, m.n_int = case.n_cell, case.n_bnd, case.n_int
+        for f in ("cell_area", "cell_x", "cell_y", "cell_zb", "bnd_len", "bnd_nx", "bnd_ny",
+                  "int_len", "int_nx", "int_ny", "int_xm", "int_ym"):
+            setattr(m, f, N.dptr(keep(f, a[f], np.float64)))
+        m.bnd_cell = N.i64ptr(keep("bnd_cell", a["bnd_cell"], np.int64))
+        m.int_L = N.i64ptr(keep("int_L", a["int_L"], np.int64))
+        m.int_R = N.i64ptr(keep("int_R", a["int_R"], np.int64))
+        m.bnd_series = N.i32ptr(keep("bnd_series", a["bnd_series"], np.int32))
+        self.params = params if params is not None else case.params
+        ctx = C.c_void_p()
+        rc = self.lib.sw2dg_ctx_create(C.byref(m), C.byref(self.params), device, C.byref(ctx))
+        if rc != 0:
+            raise N.SW2DError(rc, "sw2dg_ctx_create failed")
+        self.ctx = ctx
+        fric = self.params.friction != N.FRIC_NONE
+        hl = self.params.head_loss != 0
+        arr = lambda k, on=True: N.dptr(keep(k, a[k], np.float64)) if on else None
+        self._check(self.lib.sw2dg_set_cell_params(
+            ctx, arr("phi"), arr("beta"), arr("fc1", fric), arr("fc2", fric), arr("falpha", fric),
+            arr("hsing", hl), arr("hk1", hl), arr("hk2", hl), arr("halpha", hl)))
+        st = keep("series_type", a["series_type"], np.int32)
+        sv = keep("series_v", a["series_v"], np.float64)
+        stt = keep("series_t", a["series_t"], np.float64)
+        self._check(self.lib.sw2dg_set_bc_series(ctx, case.n_series, case.n_times, N.i32ptr(st),
+                                                 N.dptr(stt), N.dptr(sv)))
+        self.set_state(a["h0"], a["qx0"], a["qy0"], self.params.t0)
+
+    # -- plumbing -----------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise N.SW2DError(rc, self.lib.sw2dg_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            self.lib.sw2dg_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------
+    def set_state(self, h, qx, qy, t):
+        f = lambda x: np.ascontiguousarray(np.broadcast_to(x, (self.case.n_cell,)), dtype=np.float64)
+        h, qx, qy = f(h), f(qx), f(qy)
+        self._check(self.lib.sw2dg_set_state(self.ctx, N.dptr(h), N.dptr(qx), N.dptr(qy), float(t)))
+
+    def state(self):
+        n = self.case.n_cell
+        h, qx, qy = np.empty(n), np.empty(n), np.empty(n)
+        self._check(self.lib.sw2dg_get_state(self.ctx, N.dptr(h), N.dptr(qx), N.dptr(qy)))
+        return h, qx, qy
+
+    def clock(self):
+        t, dt = C.c_double(), C.c_double()
+        steps = C.c_int64()
+        flags = C.c_int32()
+        self._check(self.lib.sw2dg_get_clock(self.ctx, C.byref(t), C.byref(dt), C.byref(steps),
+                                             C.byref(flags)))
+        return {"t": t.value, "dt": dt.value, "steps": steps.value, "store_flags": flags.value}
+
+    @property
+    def t(self):
+        return self.clock()["t"]
+
+    def bnd_f1(self):
+        out = np.empty(self.case.n_bnd)
+        self._check(self.lib.sw2dg_get_bnd_f1(self.ctx, N.dptr(out)))
+        return out
+
+    # -- time loop -------------------------------------------------------------
+    def run(self, t_stop=np.inf, max_steps=2 ** 62, pause_on_store=False):
+        n = C.c_int64()
+        self._check(self.lib.sw2dg_run(self.ctx, float(min(t_stop, 1e300)), int(max_steps),
+                                       1 if pause_on_store else 0, C.byref(n)))
+        return n.value
+
+    def step(self, n=1):
+        return self.run(max_steps=n)
+
+    def execute(self):
+        """swSp::execute: run to t_max."""
+        return self.run(t_stop=self.params.t_max)
+
+    def profile(self, n_steps):
+        avg = (C.c_double * 8)()
+        names = (C.c_char_p * 8)()
+        nk = C.c_int32()
+        self._check(self.lib.sw2dg_profile_kernels(self.ctx, int(n_steps), 8, avg, names,
+                                                   C.byref(nk)))
+        return {names[i].decode(): avg[i] for i in range(nk.value)}
+
+    def device_bytes(self):
+        return self.lib.sw2dg_device_bytes(self.ctx)

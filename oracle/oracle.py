@@ -1,0 +1,156 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle is the checker for the HIP path.  Only
+tests, the smoke check and the benchmark's CPU-baseline leg use it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+REF_SOLVER = os.path.join(HERE, "_ref", "sw2dSolver")
+
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_i32 = C.POINTER(C.c_int32)
+
+
+class OrcCase(C.Structure):
+    _fields_ = [
+        ("n_cell", C.c_int64), ("n_bnd", C.c_int64), ("n_int", C.c_int64),
+        ("cell_area", _d), ("cell_x", _d), ("cell_y", _d), ("cell_zb", _d),
+        ("phi", _d), ("beta", _d),
+        ("fric_c1", _d), ("fric_c2", _d), ("fric_alpha_deg", _d),
+        ("hl_hsing", _d), ("hl_c1", _d), ("hl_c2", _d), ("hl_alpha_deg", _d),
+        ("bnd_cell", _i64), ("bnd_len", _d), ("bnd_nx", _d), ("bnd_ny", _d), ("bnd_series", _i32),
+        ("int_L", _i64), ("int_R", _i64), ("int_len", _d), ("int_nx", _d), ("int_ny", _d),
+        ("int_xm", _d), ("int_ym", _d),
+        ("n_series", C.c_int32), ("n_times", C.c_int32),
+        ("series_type", _i32), ("series_t", _d), ("series_v", _d),
+        ("scheme", C.c_int32), ("topo", C.c_int32), ("friction", C.c_int32),
+        ("head_loss", C.c_int32), ("neg_depth_detection", C.c_int32),
+        ("h_min", C.c_double), ("phi_min", C.c_double), ("cfl_max", C.c_double),
+        ("dt_max", C.c_double), ("dt_map", C.c_double), ("t0", C.c_double), ("t_max", C.c_double),
+        ("v_max", C.c_double), ("fr_max", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-C", HERE], stdout=subprocess.DEVNULL)
+        L = C.CDLL(LIB)
+        L.orc_create.argtypes = [C.POINTER(OrcCase), _d, _d, _d]
+        L.orc_create.restype = C.c_void_p
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p]
+        L.orc_step.restype = C.c_int
+        L.orc_run.argtypes = [C.c_void_p, C.c_double, C.c_int64, _i64]
+        L.orc_run.restype = C.c_int
+        L.orc_get_state.argtypes = [C.c_void_p, _d, _d, _d]
+        L.orc_set_state.argtypes = [C.c_void_p, _d, _d, _d]
+        L.orc_time.argtypes = [C.c_void_p]
+        L.orc_time.restype = C.c_double
+        L.orc_last_dt.argtypes = [C.c_void_p]
+        L.orc_last_dt.restype = C.c_double
+        L.orc_get_bnd_f1.argtypes = [C.c_void_p, _d]
+        L.orc_hllc.argtypes = [C.c_int, C.c_double, C.c_double] + [C.c_double] * 16 + [_d, C.POINTER(C.c_int)]
+        L.orc_bnd_flux.argtypes = [C.c_double] * 8 + [C.c_int, C.c_double, _d, C.POINTER(C.c_int)]
+        L.orc_friction_factor.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_friction_factor.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None or len(a) == 0 else a.ctypes.data_as(_d)
+
+
+class OracleSolver:
+    """CPU restatement of swSp's time loop on one case (arrays in reference order)."""
+
+    def __init__(self, case, params=None):
+        a = case.a
+        p = params if params is not None else case.params
+        self.n_cell = case.n_cell
+        self._keep = []
+
+        def k(arr, dt=np.float64):
+            arr = np.ascontiguousarray(arr, dtype=dt)
+            self._keep.append(arr)
+            return arr
+
+        c = OrcCase()
+        c.n_cell, c.n_bnd, c.n_int = case.n_cell, case.n_bnd, case.n_int
+        for f in ("cell_area", "cell_x", "cell_y", "cell_zb", "phi", "beta", "bnd_len", "bnd_nx",
+                  "bnd_ny", "int_len", "int_nx", "int_ny", "int_xm", "int_ym"):
+            setattr(c, f, _dp(k(a[f])))
+        for f, src in (("fric_c1", "fc1"), ("fric_c2", "fc2"), ("fric_alpha_deg", "falpha"),
+                       ("hl_hsing", "hsing"), ("hl_c1", "hk1"), ("hl_c2", "hk2"),
+                       ("hl_alpha_deg", "halpha")):
+            setattr(c, f, _dp(k(a[src])))
+        c.bnd_cell = k(a["bnd_cell"], np.int64).ctypes.data_as(_i64)
+        c.bnd_series = k(a["bnd_series"], np.int32).ctypes.data_as(_i32)
+        c.int_L = k(a["int_L"], np.int64).ctypes.data_as(_i64)
+        c.int_R = k(a["int_R"], np.int64).ctypes.data_as(_i64)
+        c.n_series, c.n_times = case.n_series, case.n_times
+        c.series_type = k(a["series_type"], np.int32).ctypes.data_as(_i32)
+        c.series_t = _dp(k(a["series_t"]))
+        c.series_v = _dp(k(a["series_v"]))
+        for f in ("scheme", "topo", "friction", "head_loss", "neg_depth_detection", "h_min",
+                  "phi_min", "cfl_max", "dt_max", "dt_map", "t0", "t_max", "v_max", "fr_max"):
+            setattr(c, f, getattr(p, f))
+        h0, qx0, qy0 = k(a["h0"]), k(a["qx0"]), k(a["qy0"])
+        self._s = lib().orc_create(C.byref(c), _dp(h0), _dp(qx0), _dp(qy0))
+        if not self._s:
+            raise RuntimeError("orc_create failed")
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().orc_destroy(self._s)
+            self._s = None
+
+    def step(self, n=1):
+        done = C.c_int64()
+        st = lib().orc_run(self._s, 1e300, int(n), C.byref(done))
+        if st:
+            raise RuntimeError(f"oracle error {st}")
+        return done.value
+
+    def run(self, t_end=1e300, max_steps=2 ** 62):
+        done = C.c_int64()
+        st = lib().orc_run(self._s, float(t_end), int(max_steps), C.byref(done))
+        if st:
+            raise RuntimeError(f"oracle error {st}")
+        return done.value
+
+    def state(self):
+        n = self.n_cell
+        h, qx, qy = np.empty(n), np.empty(n), np.empty(n)
+        lib().orc_get_state(self._s, _dp(h), _dp(qx), _dp(qy))
+        return h, qx, qy
+
+    @property
+    def t(self):
+        return lib().orc_time(self._s)
+
+    @property
+    def dt(self):
+        return lib().orc_last_dt(self._s)
+
+
+def run_reference(folder, timeout=600):
+    """Run the compiled reference sw2dSolver (oracle/_ref) on a case folder."""
+    if not os.path.exists(REF_SOLVER):
+        raise FileNotFoundError(REF_SOLVER)
+    r = subprocess.run([REF_SOLVER, str(folder)], capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference solver failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    return r

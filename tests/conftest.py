@@ -1,0 +1,75 @@
+import glob
+import hashlib
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+PKG_DIR = os.path.join(REPO, "inria-internship-project-sw2d-enzyme_amd")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, GOLDEN)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+def load_pkg():
+    if "sw2d_amd" in sys.modules:
+        return sys.modules["sw2d_amd"]
+    spec = importlib.util.spec_from_file_location("sw2d_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["sw2d_amd"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+@pytest.fixture(scope="session")
+def sw():
+    return load_pkg()
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as O
+    O.lib()
+    return O
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+
+
+def regen_case(sw, name, folder):
+    """Rebuild the inputs of a golden case and check they are byte-identical."""
+    from make_golden import golden_cases
+    gen, kw = golden_cases()[name]
+    from sw2d_amd import cases
+    getattr(cases, gen)(folder, **kw)
+    g = load_golden(name)
+    for fname, dig in zip(g["digest_names"], g["digest_values"]):
+        with open(os.path.join(folder, "input", str(fname)), "rb") as f:
+            assert hashlib.sha256(f.read()).hexdigest() == str(dig), f"{name}: {fname} differs"
+    return g
+
+
+def time_of(result_file):
+    """Time (s) encoded in a ResultHydro_pDDDDdHHhMMmSSsMMM.txt name (swIO.cpp:1194-1226)."""
+    s = result_file.split("_p")[1]
+    d = int(s[0:4]); h = int(s[5:7]); m = int(s[8:10]); sec = int(s[11:13]); ms = int(s[14:17])
+    return d * 86400 + h * 3600 + m * 60 + sec + ms / 1000.0
+
+
+def rel_err(a, b):
+    """max |a-b| / max |b| (field-normalised), 0 when both are 0."""
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / scale

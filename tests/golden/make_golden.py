@@ -1,0 +1,121 @@
+"""Generate the golden fixtures of the SP forward step from the compiled reference.
+
+Run in the development container (needs oracle/_ref/sw2dSolver, built from
+/root/reference by `make -C oracle ref`):
+
+    python tests/golden/make_golden.py
+
+For every case below the script writes the case folder with the package's
+case tooling (deterministic: fixed seeds), runs the reference's own
+`sw2dSolver <folder>` on it, and stores the state maps it writes
+(ResultHydro_p*.txt, 17 decimals = exact FP64 round trip) in
+tests/golden/<name>.npz together with SHA-256 digests of the input files, so
+a test can regenerate the same inputs and check it did.  Cases where the
+reference stops on a fatal error record that error instead.
+"""
+import glob
+import hashlib
+import importlib.util
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+PKG = os.path.join(REPO, "inria-internship-project-sw2d-enzyme_amd")
+REF = os.path.join(REPO, "oracle", "_ref", "sw2dSolver")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_pkg():
+    if "sw2d_amd" in sys.modules:
+        return sys.modules["sw2d_amd"]
+    spec = importlib.util.spec_from_file_location("sw2d_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["sw2d_amd"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+# name -> (generator, kwargs).  Shared with the tests through golden_cases().
+def golden_cases():
+    return {
+        "dam_small_godunov": ("dambreak_case", dict(nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0,
+                                                     scheme="godunov", friction="none", t_max=3.0,
+                                                     dt_map=1.0)),
+        "dam_small_muscl_manning": ("dambreak_case", dict(nx=20, ny=10, Lx=40.0, Ly=20.0,
+                                                          x_dam=20.0, scheme="muscl_evr",
+                                                          friction="manning", t_max=3.0,
+                                                          dt_map=1.0)),
+        "dambreak_20k_muscl_manning": ("dambreak_case", dict(scheme="muscl_evr",
+                                                             friction="manning", t_max=10.0)),
+        **{f"mixed_{s}_{t}_{f}": ("mixed_case", dict(scheme=s, topo=t, friction=f, t_max=20.0,
+                                                     dt_map=5.0,
+                                                     closed_fraction=0.0 if s == "muscl_evr" else 0.04))
+           for s in ("godunov", "muscl_evr") for t in ("discontinuous", "blending")
+           for f in ("strickler", "manning")},
+        # the reference stops on an impossible HLLC regime (swSpHyp.cpp:729-739)
+        "mixed_muscl_closed_cells_error": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
+                                                             t_max=20.0, closed_fraction=0.04)),
+    }
+
+
+def make_case(sw, folder, gen, kw):
+    from sw2d_amd import cases
+    getattr(cases, gen)(folder, **kw)
+
+
+def input_digests(folder):
+    d = {}
+    for p in sorted(glob.glob(os.path.join(folder, "input", "*"))):
+        with open(p, "rb") as f:
+            d[os.path.basename(p)] = hashlib.sha256(f.read()).hexdigest()
+    return d
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit(f"{REF} missing: run `make -C {os.path.join(REPO, 'oracle')} ref` first")
+    sw = load_pkg()
+    for name, (gen, kw) in golden_cases().items():
+        tmp = tempfile.mkdtemp(prefix="sw2d_golden_")
+        try:
+            make_case(sw, tmp, gen, kw)
+            r = subprocess.run([REF, tmp], capture_output=True, text=True, timeout=1200)
+            dig = input_digests(tmp)
+            meta = dict(name=name, generator=gen, kwargs=repr(kw),
+                        digest_names=np.array(list(dig.keys())),
+                        digest_values=np.array(list(dig.values())))
+            if r.returncode != 0:
+                log = open(os.path.join(tmp, "log", "console_Solver.log")).read()
+                fatal = [l for l in log.splitlines() if "[Fat.]" in l]
+                np.savez_compressed(os.path.join(OUT, name + ".npz"), error=fatal[-1] if fatal else "fatal",
+                                    **meta)
+                print(name, "reference error:", fatal[-1] if fatal else r.returncode)
+                continue
+            outs = sorted(glob.glob(os.path.join(tmp, "output", "ResultHydro_p*.txt")))
+            times, H, QX, QY = [], [], [], []
+            for p in outs:
+                m = sw.read_result_map(p)
+                base = os.path.basename(p)
+                times.append(base)
+                H.append(m["water_depth"])
+                QX.append(m["unit_discharge_x"])
+                QY.append(m["unit_discharge_y"])
+            keep = list(range(len(outs)))
+            if name.startswith("dambreak_20k"):
+                keep = [len(outs) - 1]  # final map only for the full-size case
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), files=np.array([times[k] for k in keep]),
+                                h=np.array([H[k] for k in keep]), qx=np.array([QX[k] for k in keep]),
+                                qy=np.array([QY[k] for k in keep]), **meta)
+            print(name, "maps:", [times[k] for k in keep])
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()
