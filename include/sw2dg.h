@@ -103,6 +103,59 @@ int sw2dg_get_bnd_f1(sw2dg_ctx *ctx, double *f1);
  * HIP events on the ctx stream.  names may be NULL.  k_max entries max. */
 int sw2dg_profile_kernels(sw2dg_ctx *ctx, int64_t n_steps, int32_t k_max, double *avg_ms,
                           const char **names, int32_t *n_kernels);
+/* ---- 4D-Var layer: the reference's differentiable ("_E") forward and its
+ * adjoint (assim/swSpE_checkpoint.cpp).  Godunov reconstruction. ---- */
+
+/* Observations.  Gauges: cost sum_g w_g (h[cell_g] - gauge_h[k*n_gauges+g])^2
+ * (compute_cost_gauges_ptr, assim/swSpE_gauges.cpp:166-179).  Frames: cost
+ * sum_i w_h (h - frame_h)^2 + w_q ((qx - frame_qx)^2 + (qy - frame_qy)^2)
+ * (compute_cost_frame_ptr, assim/swSpE_ad.cpp:273-308); frame arrays are
+ * [n_obs][n_cell] in reference order, w_h / w_q may be NULL (1).  Both
+ * divided by n_obs when normalize != 0 (cost_normalize). */
+typedef struct sw2dg_obs {
+    int32_t n_obs;
+    const double *t;            /* [n_obs] observation times, ascending */
+    int32_t n_gauges;
+    const int64_t *gauge_cell;  /* [n_gauges] cells (reference order)     */
+    const double *gauge_w;      /* [n_gauges] weights or NULL (1)         */
+    const double *gauge_h;      /* [n_obs * n_gauges] observed depths     */
+    const double *frame_h, *frame_qx, *frame_qy, *w_h, *w_q;
+    int32_t normalize;
+} sw2dg_obs;
+
+/* Allocates the adjoint buffers: up to max_steps steps, a state checkpoint
+ * every `stride` steps (0 = ceil(sqrt(max_steps))), the window
+ * [t_window_start, t_max] (segments starting before it are not reversed,
+ * -1e300 = whole run) split into n_tiles tiles whose adjoint is reset at the
+ * tile top (get_spatial_gradient_checkpointed, swSpE_checkpoint.cpp:785-953). */
+int sw2dg_assim_setup(sw2dg_ctx *ctx, const sw2dg_obs *obs, int32_t stride, int32_t n_tiles,
+                      double t_window_start, int64_t max_steps);
+/* Dry run of the _E forward from the state given to sw2dg_set_state
+ * (build_checkpoint_records, swSpE_checkpoint.cpp:97-455): fixes dt, the
+ * per-series boundary values and observation triggers of every step and
+ * keeps the checkpoints.  Controls in reference order; NULL = the ctx's
+ * porosity / friction maps. */
+int sw2dg_assim_record(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
+                       int64_t *n_steps);
+/* The recorded schedule: t_seq [n+1], dt_seq [n], bc_seq [n*n_series], obs_trig [n] (-1 = none). */
+int sw2dg_assim_get_record(sw2dg_ctx *ctx, double *t_seq, double *dt_seq, double *bc_seq,
+                           int32_t *obs_trig);
+/* Replace the recorded boundary values (hydrograph controls) [n*n_series]. */
+int sw2dg_assim_set_bc_seq(sw2dg_ctx *ctx, const double *bc_seq);
+/* Cost on the frozen schedule (replay_trajectory_cost, swSpE_checkpoint.cpp:1311-1433). */
+int sw2dg_assim_cost(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
+                     double *J);
+/* Dry run at the given controls, then the adjoint (replaces the Enzyme reverse
+ * pass of get_spatial_gradient_checkpointed, swSpE_checkpoint.cpp:722-1281):
+ * J, dJ/dphi, dJ/dK1, dJ/dK2 per cell (reference order), dJ/d(boundary value)
+ * per step and series [n*n_series] (sized for max_steps), the number of
+ * recorded steps.  Any output may be NULL. */
+int sw2dg_assim_gradient(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
+                         double *J, double *dJ_dphi, double *dJ_dK1, double *dJ_dK2,
+                         double *dJ_dbc, int64_t *n_steps);
+/* State at the end of the recorded run (reference order). */
+int sw2dg_assim_final_state(sw2dg_ctx *ctx, double *h, double *qx, double *qy);
+
 /* Bytes of device memory held by the ctx. */
 int64_t sw2dg_device_bytes(sw2dg_ctx *ctx);
 const char *sw2dg_last_error(sw2dg_ctx *ctx);

@@ -1,0 +1,206 @@
+"""4D-Var layer: host mirror of the reference's swSpE assimilation entry points.
+
+``Assim`` wraps the C-ABI of include/sw2dg.h (sw2dg_assim_*) on an SpSolver
+context:
+  record()   ~ swSpE::build_checkpoint_records  (assim/swSpE_checkpoint.cpp:97-455)
+  cost()     ~ swSpE::replay_trajectory_cost    (:1311-1433)
+  gradient() ~ swSpE::get_spatial_gradient_checkpointed (:722-1281)
+All of them run the _E forward and the hand-written adjoint on the GPU.
+
+``Controls`` maps the 68 controls of the calibration workload (Manning
+roughness of 8 zones and a 60-knot inflow hydrograph) to the per-cell /
+per-step inputs of the kernels, and the per-cell / per-step gradients back.
+The friction of the _E forward is the semi-implicit Strickler law
+(swSpE_friction.cpp:19-76), so a Manning n enters as K = 1/n.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class Obs(C.Structure):
+    """sw2dg_obs (include/sw2dg.h)."""
+
+    _fields_ = [
+        ("n_obs", C.c_int32), ("t", N._d), ("n_gauges", C.c_int32), ("gauge_cell", N._i64),
+        ("gauge_w", N._d), ("gauge_h", N._d), ("frame_h", N._d), ("frame_qx", N._d),
+        ("frame_qy", N._d), ("w_h", N._d), ("w_q", N._d), ("normalize", C.c_int32),
+    ]
+
+
+_DECL = {
+    "sw2dg_assim_setup": ([C.c_void_p, C.POINTER(Obs), C.c_int32, C.c_int32, C.c_double, C.c_int64], C.c_int),
+    "sw2dg_assim_record": ([C.c_void_p, N._d, N._d, N._d, N._i64], C.c_int),
+    "sw2dg_assim_get_record": ([C.c_void_p, N._d, N._d, N._d, N._i32], C.c_int),
+    "sw2dg_assim_set_bc_seq": ([C.c_void_p, N._d], C.c_int),
+    "sw2dg_assim_cost": ([C.c_void_p, N._d, N._d, N._d, N._d], C.c_int),
+    "sw2dg_assim_gradient": ([C.c_void_p, N._d, N._d, N._d, N._d, N._d, N._d, N._d, N._d, N._i64],
+                             C.c_int),
+    "sw2dg_assim_final_state": ([C.c_void_p, N._d, N._d, N._d], C.c_int),
+}
+ASSIM_SYMBOLS = tuple(_DECL)
+
+
+def _declare(lib):
+    for k, (a, r) in _DECL.items():
+        f = getattr(lib, k)
+        f.argtypes = a
+        f.restype = r
+
+
+class Assim:
+    """Frozen-trajectory cost and adjoint on one SpSolver context."""
+
+    def __init__(self, solver, obs_t, gauge_cells=None, gauge_h=None, gauge_w=None, frames=None,
+                 frame_w=None, normalize=False, stride=0, n_tiles=1, t_window=-1e300,
+                 max_steps=100000):
+        self.s = solver
+        self.lib = solver.lib
+        _declare(self.lib)
+        self._keep = []
+        self.max_steps = int(max_steps)
+        o = Obs()
+        obs_t = self._hold(np.atleast_1d(np.asarray(obs_t, dtype=np.float64)))
+        o.n_obs = len(obs_t)
+        o.t = N.dptr(obs_t)
+        if gauge_cells is not None:
+            gc = self._hold(np.asarray(gauge_cells, dtype=np.int64))
+            o.n_gauges = len(gc)
+            o.gauge_cell = N.i64ptr(gc)
+            o.gauge_h = N.dptr(self._hold(np.asarray(gauge_h, dtype=np.float64).reshape(-1)))
+            if gauge_w is not None:
+                o.gauge_w = N.dptr(self._hold(np.asarray(gauge_w, dtype=np.float64)))
+        if frames is not None:
+            fh, fqx, fqy = (self._hold(np.asarray(f, dtype=np.float64).reshape(-1)) for f in frames)
+            o.frame_h, o.frame_qx, o.frame_qy = N.dptr(fh), N.dptr(fqx), N.dptr(fqy)
+            if frame_w is not None:
+                o.w_h = N.dptr(self._hold(np.asarray(frame_w[0], dtype=np.float64)))
+                o.w_q = N.dptr(self._hold(np.asarray(frame_w[1], dtype=np.float64)))
+        o.normalize = 1 if normalize else 0
+        self.obs = o
+        self.n_obs = o.n_obs
+        self.s._check(self.lib.sw2dg_assim_setup(solver.ctx, C.byref(o), int(stride), int(n_tiles),
+                                                 float(t_window), self.max_steps))
+        self.n_steps = 0
+
+    def _hold(self, a):
+        a = np.ascontiguousarray(a)
+        self._keep.append(a)
+        return a
+
+    def _ctl(self, x):
+        if x is None:
+            return None
+        return N.dptr(self._hold(np.broadcast_to(np.asarray(x, dtype=np.float64),
+                                                 (self.s.case.n_cell,)).copy()))
+
+    def record(self, phi=None, K1=None, K2=None):
+        n = C.c_int64()
+        self.s._check(self.lib.sw2dg_assim_record(self.s.ctx, self._ctl(phi), self._ctl(K1),
+                                                  self._ctl(K2), C.byref(n)))
+        self.n_steps = n.value
+        return self.n_steps
+
+    def get_record(self):
+        n = self.n_steps
+        ns = max(self.s.case.n_series, 1)
+        t = np.empty(n + 1)
+        dt = np.empty(max(n, 1))
+        bc = np.empty(max(n, 1) * ns)
+        trig = np.empty(max(n, 1), dtype=np.int32)
+        self.s._check(self.lib.sw2dg_assim_get_record(self.s.ctx, N.dptr(t), N.dptr(dt),
+                                                      N.dptr(bc), N.i32ptr(trig)))
+        return {"t": t, "dt": dt[:n], "bc": bc[: n * ns].reshape(n, ns), "trig": trig[:n]}
+
+    def set_bc_seq(self, bc):
+        bc = self._hold(np.asarray(bc, dtype=np.float64).reshape(-1))
+        self.s._check(self.lib.sw2dg_assim_set_bc_seq(self.s.ctx, N.dptr(bc)))
+
+    def cost(self, phi=None, K1=None, K2=None):
+        J = C.c_double()
+        self.s._check(self.lib.sw2dg_assim_cost(self.s.ctx, self._ctl(phi), self._ctl(K1),
+                                                self._ctl(K2), C.byref(J)))
+        return J.value
+
+    def gradient(self, phi=None, K1=None, K2=None):
+        """Dry run at the controls, then the adjoint.  Returns J and dJ/d(phi, K1, K2, bc)."""
+        nc = self.s.case.n_cell
+        ns = max(self.s.case.n_series, 1)
+        J = C.c_double()
+        n = C.c_int64()
+        gphi, gK1, gK2 = np.empty(nc), np.empty(nc), np.empty(nc)
+        gbc = np.empty(self.max_steps * ns)
+        self.s._check(self.lib.sw2dg_assim_gradient(self.s.ctx, self._ctl(phi), self._ctl(K1),
+                                                    self._ctl(K2), C.byref(J), N.dptr(gphi),
+                                                    N.dptr(gK1), N.dptr(gK2), N.dptr(gbc),
+                                                    C.byref(n)))
+        self.n_steps = n.value
+        return {"J": J.value, "phi": gphi, "K1": gK1, "K2": gK2,
+                "bc": gbc[: n.value * ns].reshape(n.value, ns), "n_steps": n.value}
+
+    def final_state(self):
+        nc = self.s.case.n_cell
+        h, qx, qy = np.empty(nc), np.empty(nc), np.empty(nc)
+        self.s._check(self.lib.sw2dg_assim_final_state(self.s.ctx, N.dptr(h), N.dptr(qx),
+                                                       N.dptr(qy)))
+        return h, qx, qy
+
+
+def interp_weights(tk, t):
+    """Weights W[n, k] of the _E time interpolation (assim/swSpE.h:138-153):
+    value(t_n) = sum_k W[n, k] v_k.  Used to map boundary-value gradients per
+    step onto hydrograph knots."""
+    tk = np.asarray(tk, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    W = np.zeros((len(t), len(tk)))
+    nk = len(tk)
+    for n, tn in enumerate(t):
+        if nk == 1 or tn <= tk[0] - 1e-12:
+            W[n, 0] = 1.0
+            continue
+        if tn >= tk[-1]:
+            W[n, -1] = 1.0
+            continue
+        it = 0
+        while it < nk - 1 and tn > tk[it + 1]:
+            it += 1
+        den = tk[it + 1] - tk[it]
+        if den < 1e-12:
+            W[n, it] = 1.0
+            continue
+        a = (tn - tk[it]) / den
+        W[n, it] += 1.0 - a
+        W[n, it + 1] += a
+    return W
+
+
+class Controls:
+    """Manning zones + inflow hydrograph <-> per-cell K and per-step BC values.
+
+    zone[c] in [0, n_zones); hydrograph knots (t_k, Q_k) in m3/s on the
+    boundary series `series` whose edges have total length `width` (unit
+    discharge = Q / width)."""
+
+    def __init__(self, zone, n_zones, t_knots, series, width):
+        self.zone = np.asarray(zone, dtype=np.int64)
+        self.n_zones = int(n_zones)
+        self.t_knots = np.asarray(t_knots, dtype=np.float64)
+        self.series = int(series)
+        self.width = float(width)
+
+    def K(self, n_zone):
+        return 1.0 / np.asarray(n_zone, dtype=np.float64)[self.zone]
+
+    def grad_zones(self, n_zone, dK1, dK2):
+        """dJ/dn_z from dJ/dK per cell (K1 = K2 = 1/n)."""
+        n_zone = np.asarray(n_zone, dtype=np.float64)
+        g = np.zeros(self.n_zones)
+        np.add.at(g, self.zone, dK1 + dK2)
+        return g * (-1.0 / n_zone ** 2)
+
+    def grad_knots(self, t_steps, dbc):
+        """dJ/dQ_k from dJ/d(boundary value) per step (frozen step times)."""
+        W = interp_weights(self.t_knots, t_steps)
+        return (W.T @ dbc[:, self.series]) / self.width

@@ -1,0 +1,625 @@
+// sp_adjoint.h -- the reference's differentiable ("_E") single-porosity
+// physics and hand-written reverse-mode derivatives (vector-Jacobian
+// products) of each local function.  Shared by the HIP adjoint kernels and a
+// host build used by the per-function derivative tests.
+//
+// Forward semantics (the function the reference differentiates):
+//   hypGodunovComputeLRStates_E  assim/swSpE_hyp.cpp:507-573
+//   hypBoundaryFlux_E            assim/swSpE_hyp.cpp:143-239
+//   hypHllc_E                    assim/swSpE_hyp.cpp:725-832
+//   hypComputeBalance_E          assim/swSpE_hyp.cpp:336-394
+//   sourceFrictionStrickler_E    assim/swSpE_friction.cpp:19-76
+//   sourceSingularHeadLoss_E     assim/swSpE_friction.cpp:79-126
+// Derivative conventions: branch decisions (wet/dry regimes, min/max
+// selections, clamps) are taken from the primal values and held fixed; the
+// derivative of sqrt at 0 is taken as 0.
+#pragma once
+
+#include "sp_physics.h"
+
+namespace spe {
+
+using sp::kG;
+using sp::kSmall;
+
+constexpr double kPhiEps = 1e-3;  // smoothed min of the porosities (swSpE_hyp.cpp:794)
+
+SPD double dsqrt_arg(double s, double ds) { return (s > 0.0) ? ds * 0.5 / s : 0.0; }
+
+// ---------------------------------------------------------------------------
+// Godunov Riemann state on one side (swSpE_hyp.cpp:532-565)
+// ---------------------------------------------------------------------------
+struct Side {
+    double z, q, r, lm, lp;
+};
+
+SPD Side lr_fwd(double h, double qx, double qy, double zb, double b, double nx, double ny,
+                double h_min) {
+    Side s;
+    s.z = h + zb;
+    s.q = qx * nx + qy * ny;
+    s.r = -qx * ny + qy * nx;
+    const double u = (h > h_min) ? s.q / h : 0.0;
+    const double c = sqrt(kG * h);
+    s.lm = sp::lambda_minus(b, c, u);
+    s.lp = sp::lambda_plus(b, c, u);
+    return s;
+}
+
+// adjoint of lr_fwd: (dz, dq, dr, dlm, dlp) -> accumulate into (dh, dqx, dqy)
+SPD void lr_vjp(double h, double qx, double qy, double b, double nx, double ny, double h_min,
+                double dz, double dq, double dr, double dlm, double dlp, double &dh, double &dqx,
+                double &dqy) {
+    const double q = qx * nx + qy * ny;
+    const double u = (h > h_min) ? q / h : 0.0;
+    const double c = sqrt(kG * h);
+    const double A = (b - 1.0) * b * u * u + c * c;
+    const double s = sqrt(A);
+    double du = b * (dlm + dlp);
+    const double ds = dlp - dlm;
+    const double dA = dsqrt_arg(s, ds);
+    du += dA * (b - 1.0) * b * 2.0 * u;
+    const double dc = dA * 2.0 * c;
+    double dhh = dz + dsqrt_arg(c, dc) * kG;
+    double dqq = dq;
+    if (h > h_min) {
+        dqq += du / h;
+        dhh += -du * q / (h * h);
+    }
+    dh += dhh;
+    dqx += dqq * nx - dr * ny;
+    dqy += dqq * ny + dr * nx;
+}
+
+// ---------------------------------------------------------------------------
+// HLLC with the smoothed porosity min (swSpE_hyp.cpp:725-832).
+// in: zL qL rL zR qR rR lmL lmR lpL lpR phiL phiR ; out: F1 F2 F3 sL sR
+// ---------------------------------------------------------------------------
+enum Regime { R_NOWHERE = 0, R_BOTH = 1, R_LHS = 2, R_RHS = 3 };
+
+SPD int hllc_regime(double h_min, double phi_min, double phiL, double phiR, double zbL, double zbR,
+                    double zL, double zR, double hL, double hR) {
+    bool both = false, lhs = false, rhs = false, nowhere = false;
+    if (((hL <= h_min) && (hR <= h_min)) || ((phiL <= phi_min) && (phiR <= phi_min))) {
+        nowhere = true;
+    } else {
+        both = true;
+        if ((phiR <= phi_min) || ((phiR > phi_min) && (hR <= h_min) && (zL <= zbR + h_min))) {
+            lhs = true;
+            both = false;
+        }
+        if ((phiL <= phi_min) || ((phiL > phi_min) && (hL <= h_min) && (zR <= zbL + h_min))) {
+            rhs = true;
+            both = false;
+        }
+    }
+    // the reference's if-else chain: both, then rhs, then lhs, else nothing
+    if (both) return R_BOTH;
+    if (rhs) return R_RHS;
+    if (lhs) return R_LHS;
+    return R_NOWHERE;
+}
+
+SPD void hllc_fwd(double h_min, double phi_min, double bL, double bR, double phiL, double phiR,
+                  double zbL, double zbR, double zL, double qL, double rL, double zR, double qR,
+                  double rR, double lmL, double lmR, double lpL, double lpR, double *f) {
+    const double hL = (zL - zbL > 0.0) ? (zL - zbL) : 0.0;
+    const double hR = (zR - zbR > 0.0) ? (zR - zbR) : 0.0;
+    const int reg = hllc_regime(h_min, phi_min, phiL, phiR, zbL, zbR, zL, zR, hL, hR);
+    double f1 = 0.0, f2 = 0.0, f3 = 0.0, sl = 0.0, sr = 0.0;
+    if (reg == R_BOTH) {
+        const double hLs = (hL > kSmall) ? hL : kSmall;
+        const double hRs = (hR > kSmall) ? hR : kSmall;
+        const double uL = qL / hLs, uR = qR / hRs;
+        double L1 = (lmL < lmR) ? lmL : lmR;
+        if (L1 > 0.0) L1 = 0.0;
+        double L2 = (lpL > lpR) ? lpL : lpR;
+        if (L2 < 0.0) L2 = 0.0;
+        double den = L2 - L1;
+        if (den < kSmall) den = kSmall;
+        const double d = phiL - phiR;
+        const double phiLR = 0.5 * (phiL + phiR - sqrt(d * d + kPhiEps * kPhiEps));
+        f1 = phiLR * (L2 * qL - L1 * qR + L1 * L2 * (zR - zL)) / den;
+        const double fact = 0.5 * phiLR * (hL + hR);
+        double s2 = (phiR * hR * hR - phiL * hL * hL) * 0.5 + fact * (zL - zR);
+        s2 = s2 * kG / den;
+        sl = -L1 * s2;
+        sr = L2 * s2;
+        const double FL = sp::specific_force(bL, phiL, hL, uL);
+        const double FR = sp::specific_force(bR, phiR, hR, uR);
+        f2 = (L2 * (FL + sl) - L1 * (FR - sr) + L1 * L2 * (phiR * qR - phiL * qL)) / den;
+        const double vL = rL / hLs, vR = rR / hRs;
+        f3 = (f1 >= 0.0) ? (bL * f1 * vL) : (bR * f1 * vR);
+    } else if (reg == R_RHS) {
+        sr = phiR * 0.5 * kG * hR * hR;
+    } else if (reg == R_LHS) {
+        sl = -phiL * 0.5 * kG * hL * hL;
+    }
+    f[0] = f1;
+    f[1] = f2;
+    f[2] = f3;
+    f[3] = sl;
+    f[4] = sr;
+}
+
+// d[0..11] accumulates the adjoints of zL qL rL zR qR rR lmL lmR lpL lpR phiL phiR
+SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phiL, double phiR,
+                  double zbL, double zbR, double zL, double qL, double rL, double zR, double qR,
+                  double rR, double lmL, double lmR, double lpL, double lpR, const double *df,
+                  double *d) {
+    const double hL = (zL - zbL > 0.0) ? (zL - zbL) : 0.0;
+    const double hR = (zR - zbR > 0.0) ? (zR - zbR) : 0.0;
+    const int reg = hllc_regime(h_min, phi_min, phiL, phiR, zbL, zbR, zL, zR, hL, hR);
+    double dhL = 0.0, dhR = 0.0;
+    if (reg == R_BOTH) {
+        const double hLs = (hL > kSmall) ? hL : kSmall;
+        const double hRs = (hR > kSmall) ? hR : kSmall;
+        const double uL = qL / hLs, uR = qR / hRs;
+        int s1 = (lmL < lmR) ? 1 : 2;
+        double L1 = (s1 == 1) ? lmL : lmR;
+        if (L1 > 0.0) {
+            L1 = 0.0;
+            s1 = 0;
+        }
+        int s2s = (lpL > lpR) ? 1 : 2;
+        double L2 = (s2s == 1) ? lpL : lpR;
+        if (L2 < 0.0) {
+            L2 = 0.0;
+            s2s = 0;
+        }
+        const double den0 = L2 - L1;
+        const bool den_free = !(den0 < kSmall);
+        const double den = den_free ? den0 : kSmall;
+        const double dphi = phiL - phiR;
+        const double sq = sqrt(dphi * dphi + kPhiEps * kPhiEps);
+        const double phiLR = 0.5 * (phiL + phiR - sq);
+        const double A1 = L2 * qL - L1 * qR + L1 * L2 * (zR - zL);
+        const double N1 = phiLR * A1;
+        const double f1 = N1 / den;
+        const double fact = 0.5 * phiLR * (hL + hR);
+        const double s2a = (phiR * hR * hR - phiL * hL * hL) * 0.5 + fact * (zL - zR);
+        const double s2b = s2a * kG;
+        const double s2 = s2b / den;
+        const double sl = -L1 * s2, sr = L2 * s2;
+        const double FL = sp::specific_force(bL, phiL, hL, uL);
+        const double FR = sp::specific_force(bR, phiR, hR, uR);
+        const double P = phiR * qR - phiL * qL;
+        const double A2 = L2 * (FL + sl) - L1 * (FR - sr) + L1 * L2 * P;
+        const double vL = rL / hLs, vR = rR / hRs;
+        double df1 = df[0];
+        const double df2 = df[1], df3 = df[2];
+        double dsl = df[3], dsr = df[4];
+        double dhLs = 0.0, dhRs = 0.0, dL1 = 0.0, dL2 = 0.0, dden = 0.0, dphiLR = 0.0;
+        double duL = 0.0, duR = 0.0;
+        // f3
+        if (f1 >= 0.0) {
+            df1 += df3 * bL * vL;
+            const double dvL = df3 * bL * f1;
+            d[2] += dvL / hLs;
+            dhLs += -dvL * rL / (hLs * hLs);
+        } else {
+            df1 += df3 * bR * vR;
+            const double dvR = df3 * bR * f1;
+            d[5] += dvR / hRs;
+            dhRs += -dvR * rR / (hRs * hRs);
+        }
+        // f2 = A2 / den
+        const double dA2 = df2 / den;
+        dden += -df2 * A2 / (den * den);
+        dL2 += dA2 * (FL + sl) + dA2 * L1 * P;
+        const double dFL = dA2 * L2;
+        dsl += dA2 * L2;
+        dL1 += -dA2 * (FR - sr) + dA2 * L2 * P;
+        const double dFR = -dA2 * L1;
+        dsr += dA2 * L1;
+        const double dP = dA2 * L1 * L2;
+        d[11] += dP * qR;
+        d[4] += dP * phiR;
+        d[10] += -dP * qL;
+        d[1] += -dP * phiL;
+        // specific forces
+        d[10] += dFL * (bL * hL * uL * uL + 0.5 * kG * hL * hL);
+        dhL += dFL * phiL * (bL * uL * uL + kG * hL);
+        duL += dFL * phiL * bL * hL * 2.0 * uL;
+        d[11] += dFR * (bR * hR * uR * uR + 0.5 * kG * hR * hR);
+        dhR += dFR * phiR * (bR * uR * uR + kG * hR);
+        duR += dFR * phiR * bR * hR * 2.0 * uR;
+        // sl = -L1 s2, sr = L2 s2
+        dL1 += -dsl * s2;
+        dL2 += dsr * s2;
+        const double ds2 = -dsl * L1 + dsr * L2;
+        const double ds2a = ds2 * kG / den;
+        dden += -ds2 * s2b / (den * den);
+        d[11] += ds2a * 0.5 * hR * hR;
+        dhR += ds2a * phiR * hR;
+        d[10] += -ds2a * 0.5 * hL * hL;
+        dhL += -ds2a * phiL * hL;
+        const double dfact = ds2a * (zL - zR);
+        d[0] += ds2a * fact;
+        d[3] += -ds2a * fact;
+        dphiLR += dfact * 0.5 * (hL + hR);
+        dhL += dfact * 0.5 * phiLR;
+        dhR += dfact * 0.5 * phiLR;
+        // f1 = phiLR A1 / den
+        const double dN1 = df1 / den;
+        dden += -df1 * N1 / (den * den);
+        dphiLR += dN1 * A1;
+        const double dA1 = dN1 * phiLR;
+        dL2 += dA1 * (qL + L1 * (zR - zL));
+        d[1] += dA1 * L2;
+        dL1 += dA1 * (-qR + L2 * (zR - zL));
+        d[4] += -dA1 * L1;
+        d[3] += dA1 * L1 * L2;
+        d[0] += -dA1 * L1 * L2;
+        // smoothed min
+        const double dsq = -0.5 * dphiLR;
+        d[10] += 0.5 * dphiLR;
+        d[11] += 0.5 * dphiLR;
+        const double dd = (sq > 0.0) ? dsq * dphi / sq : 0.0;
+        d[10] += dd;
+        d[11] -= dd;
+        // den
+        if (den_free) {
+            dL2 += dden;
+            dL1 -= dden;
+        }
+        if (s1 == 1) d[6] += dL1;
+        if (s1 == 2) d[7] += dL1;
+        if (s2s == 1) d[8] += dL2;
+        if (s2s == 2) d[9] += dL2;
+        // velocities
+        d[1] += duL / hLs;
+        dhLs += -duL * qL / (hLs * hLs);
+        d[4] += duR / hRs;
+        dhRs += -duR * qR / (hRs * hRs);
+        if (hL > kSmall) dhL += dhLs;
+        if (hR > kSmall) dhR += dhRs;
+    } else if (reg == R_RHS) {
+        d[11] += df[4] * 0.5 * kG * hR * hR;
+        dhR += df[4] * phiR * kG * hR;
+    } else if (reg == R_LHS) {
+        d[10] += -df[3] * 0.5 * kG * hL * hL;
+        dhL += -df[3] * phiL * kG * hL;
+    }
+    if (zL - zbL > 0.0) d[0] += dhL;
+    if (zR - zbR > 0.0) d[3] += dhR;
+}
+
+// ---------------------------------------------------------------------------
+// boundary flux (swSpE_hyp.cpp:143-239); in: zR qR rR lpR phiR bc ; out F1 F2 F3
+// ---------------------------------------------------------------------------
+SPD void bnd_fwd(double h_min, double phi_min, double bR, double phiR, double zbR, double zR,
+                 double qR, double rR, double lpR, int type, double bc, double *f) {
+    const double hR = zR - zbR;
+    const bool dry = (hR < h_min);
+    const double q1 = dry ? 0.0 : qR, r1 = dry ? 0.0 : rR;
+    const double hRe = dry ? 0.0 : hR;
+    const double hden = (hRe > kSmall) ? hRe : kSmall;
+    const double cR = sqrt(kG * hden);
+    const double uR = q1 / hden;
+    const double L1 = (lpR > 0.0) ? lpR : 0.0;
+    const double f2R = sp::specific_force(bR, phiR, hRe, uR);
+    double f1 = 0.0, f2 = f2R, f3;
+    if (type == sp::BC_Q) {
+        f1 = bc;
+        if (bc > kSmall) {
+            const double ps = (phiR > phi_min) ? phiR : phi_min;
+            const double hc = exp((1.0 / 3.0) * log((bc / ps) / kG));
+            if (hc >= hRe) {
+                const double hcs = (hc > kSmall) ? hc : kSmall;
+                f1 = phiR * 1.5 * sqrt(kG * hcs) * hc;
+            }
+        }
+        f2 = f2R + (f1 - q1 * phiR) * L1;
+    } else if (type == sp::BC_FR) {
+        const double fr = (bc < 0.0) ? 0.0 : ((bc > 1.0) ? 1.0 : bc);
+        double c = (uR - 2.0 * cR) / (-fr - 2.0);
+        c = (c > 0.0) ? c : 0.0;
+        const double u = -fr * c;
+        const double h = c * c / kG;
+        f1 = phiR * u * h;
+        f2 = f2R + (f1 - q1 * phiR) * L1;
+    } else if (type == sp::BC_H || type == sp::BC_Z) {
+        const double hb = (type == sp::BC_Z) ? ((bc - zbR > 0.0) ? bc - zbR : 0.0)
+                                             : ((bc > 0.0) ? bc : 0.0);
+        if (hRe > h_min) {
+            f1 = phiR * (q1 + L1 * (hb - hRe));
+            f2 = f2R + (f1 - q1 * phiR) * L1;
+        } else {
+            const double c = sqrt(kG * ((hb > kSmall) ? hb : kSmall));
+            f1 = phiR * hb * c;
+            f2 = sp::specific_force(bR, phiR, hb, c);
+        }
+    }
+    if (f1 > 0.0) {
+        f3 = 0.0;
+    } else {
+        const double vR = r1 / ((hRe > kSmall) ? hRe : kSmall);
+        f3 = f1 * vR;
+    }
+    f[0] = f1;
+    f[1] = f2;
+    f[2] = f3;
+}
+
+// d[0..5] accumulates the adjoints of zR qR rR lpR phiR bc
+SPD void bnd_vjp(double h_min, double phi_min, double bR, double phiR, double zbR, double zR,
+                 double qR, double rR, double lpR, int type, double bc, const double *df, double *d) {
+    const double hR = zR - zbR;
+    const bool dry = (hR < h_min);
+    const double q1 = dry ? 0.0 : qR, r1 = dry ? 0.0 : rR;
+    const double hRe = dry ? 0.0 : hR;
+    const double hden = (hRe > kSmall) ? hRe : kSmall;
+    const double cR = sqrt(kG * hden);
+    const double uR = q1 / hden;
+    const double L1 = (lpR > 0.0) ? lpR : 0.0;
+    double f[3];
+    bnd_fwd(h_min, phi_min, bR, phiR, zbR, zR, qR, rR, lpR, type, bc, f);
+    const double f1 = f[0];
+    double df1 = df[0];
+    const double df2 = df[1], df3 = df[2];
+    double dq1 = 0.0, dr1 = 0.0, dhRe = 0.0, duR = 0.0, dcR = 0.0, dL1 = 0.0, dphi = 0.0,
+           dbc = 0.0, df2R = 0.0;
+    // f3
+    if (!(f1 > 0.0)) {
+        const double hd = (hRe > kSmall) ? hRe : kSmall;
+        df1 += df3 * r1 / hd;
+        dr1 += df3 * f1 / hd;
+        if (hRe > kSmall) dhRe += -df3 * f1 * r1 / (hd * hd);
+    }
+    if (type == sp::BC_Q || type == sp::BC_FR ||
+        ((type == sp::BC_H || type == sp::BC_Z) && hRe > h_min)) {
+        // f2 = f2R + (f1 - q1 phiR) L1
+        df2R += df2;
+        df1 += df2 * L1;
+        dq1 += -df2 * L1 * phiR;
+        dphi += -df2 * L1 * q1;
+        dL1 += df2 * (f1 - q1 * phiR);
+    }
+    if (type == sp::BC_Q) {
+        if (bc > kSmall) {
+            const double ps = (phiR > phi_min) ? phiR : phi_min;
+            const double x = (bc / ps) / kG;
+            const double hc = exp((1.0 / 3.0) * log(x));
+            if (hc >= hRe) {
+                const double hcs = (hc > kSmall) ? hc : kSmall;
+                const double sg = sqrt(kG * hcs);
+                dphi += df1 * 1.5 * sg * hc;
+                double dhc = df1 * phiR * 1.5 * sg;
+                if (hc > kSmall) dhc += dsqrt_arg(sg, df1 * phiR * 1.5 * hc) * kG;
+                const double dx = dhc * hc * (1.0 / 3.0) / x;
+                dbc += dx / (ps * kG);
+                if (phiR > phi_min) dphi += -dx * bc / (ps * ps * kG);
+            } else {
+                dbc += df1;
+            }
+        } else {
+            dbc += df1;
+        }
+    } else if (type == sp::BC_FR) {
+        const double fr = (bc < 0.0) ? 0.0 : ((bc > 1.0) ? 1.0 : bc);
+        const double B = -fr - 2.0;
+        const double A = uR - 2.0 * cR;
+        const double c0 = A / B;
+        const double c = (c0 > 0.0) ? c0 : 0.0;
+        const double u = -fr * c;
+        const double h = c * c / kG;
+        dphi += df1 * u * h;
+        const double du = df1 * phiR * h;
+        const double dh = df1 * phiR * u;
+        double dc = dh * 2.0 * c / kG - du * fr;
+        double dfr = -du * c;
+        if (c0 > 0.0) {
+            const double dA = dc / B;
+            const double dB = -dc * A / (B * B);
+            duR += dA;
+            dcR += -2.0 * dA;
+            dfr += -dB;
+        }
+        if (!(bc < 0.0) && !(bc > 1.0)) dbc += dfr;
+    } else if (type == sp::BC_H || type == sp::BC_Z) {
+        const double hb = (type == sp::BC_Z) ? ((bc - zbR > 0.0) ? bc - zbR : 0.0)
+                                             : ((bc > 0.0) ? bc : 0.0);
+        double dhb = 0.0;
+        if (hRe > h_min) {
+            dphi += df1 * (q1 + L1 * (hb - hRe));
+            dq1 += df1 * phiR;
+            dL1 += df1 * phiR * (hb - hRe);
+            dhb += df1 * phiR * L1;
+            dhRe += -df1 * phiR * L1;
+        } else {
+            const double c = sqrt(kG * ((hb > kSmall) ? hb : kSmall));
+            dphi += df1 * hb * c + df2 * (bR * hb * c * c + 0.5 * kG * hb * hb);
+            dhb += df1 * phiR * c + df2 * phiR * (bR * c * c + kG * hb);
+            const double dc = df1 * phiR * hb + df2 * phiR * bR * hb * 2.0 * c;
+            if (hb > kSmall) dhb += dsqrt_arg(c, dc) * kG;
+        }
+        if (type == sp::BC_Z ? (bc - zbR > 0.0) : (bc > 0.0)) dbc += dhb;
+    }
+    // f2R = phiR (bR hRe uR^2 + g hRe^2 / 2)
+    dphi += df2R * (bR * hRe * uR * uR + 0.5 * kG * hRe * hRe);
+    dhRe += df2R * phiR * (bR * uR * uR + kG * hRe);
+    duR += df2R * phiR * bR * hRe * 2.0 * uR;
+    // L1 = max(lpR, 0)
+    if (lpR > 0.0) d[3] += dL1;
+    // uR = q1 / hden ; cR = sqrt(g hden)
+    dq1 += duR / hden;
+    double dhden = -duR * q1 / (hden * hden) + dsqrt_arg(cR, dcR) * kG;
+    if (hRe > kSmall) dhRe += dhden;
+    if (!dry) {
+        d[1] += dq1;
+        d[2] += dr1;
+        d[0] += dhRe;
+    }
+    d[4] += dphi;
+    d[5] += dbc;
+}
+
+// ---------------------------------------------------------------------------
+// semi-implicit Strickler friction (swSpE_friction.cpp:19-76): (h,qx,qy,K1,K2) -> (qx,qy)
+// ---------------------------------------------------------------------------
+SPD void fric_fwd(double h_min, double h, double &qx, double &qy, double ca, double sa, double K1in,
+                  double K2in, double dt) {
+    const double hthr = (h_min > kSmall) ? h_min : kSmall;
+    if (h > hthr) {
+        const double hs = h + kSmall;
+        const double u = qx / hs, v = qy / hs;
+        double ul = u * ca + v * sa;
+        double vl = -u * sa + v * ca;
+        const double un = sqrt(u * u + v * v + 1e-16);
+        const double K1 = (K1in > 1e-10) ? K1in : 20.0;
+        const double K2 = (K2in > 1e-10) ? K2in : 20.0;
+        const double h43 = h * cbrt(h + 1e-16);
+        const double d1 = 1.0 + (dt * kG * un) / (K1 * K1 * h43 + 1e-10);
+        const double d2 = 1.0 + (dt * kG * un) / (K2 * K2 * h43 + 1e-10);
+        ul /= d1;
+        vl /= d2;
+        qx = (ul * ca - vl * sa) * h;
+        qy = (ul * sa + vl * ca) * h;
+    } else {
+        qx = 0.0;
+        qy = 0.0;
+    }
+}
+
+// adjoint: dqx, dqy (of the outputs) in/out become the adjoints of the inputs;
+// dh accumulates; dK1, dK2 accumulate.
+SPD void fric_vjp(double h_min, double h, double qx, double qy, double ca, double sa, double K1in,
+                  double K2in, double dt, double &dh, double &dqx, double &dqy, double &dK1,
+                  double &dK2) {
+    const double hthr = (h_min > kSmall) ? h_min : kSmall;
+    if (!(h > hthr)) {
+        dqx = 0.0;
+        dqy = 0.0;
+        return;
+    }
+    const double hs = h + kSmall;
+    const double u = qx / hs, v = qy / hs;
+    const double ul = u * ca + v * sa;
+    const double vl = -u * sa + v * ca;
+    const double un = sqrt(u * u + v * v + 1e-16);
+    const double K1 = (K1in > 1e-10) ? K1in : 20.0;
+    const double K2 = (K2in > 1e-10) ? K2in : 20.0;
+    const double cb = cbrt(h + 1e-16);
+    const double h43 = h * cb;
+    const double D1 = K1 * K1 * h43 + 1e-10, D2 = K2 * K2 * h43 + 1e-10;
+    const double d1 = 1.0 + (dt * kG * un) / D1;
+    const double d2 = 1.0 + (dt * kG * un) / D2;
+    const double ul2 = ul / d1, vl2 = vl / d2;
+    const double T1 = ul2 * ca - vl2 * sa, T2 = ul2 * sa + vl2 * ca;
+    dh += dqx * T1 + dqy * T2;
+    const double dT1 = dqx * h, dT2 = dqy * h;
+    const double dul2 = dT1 * ca + dT2 * sa;
+    const double dvl2 = -dT1 * sa + dT2 * ca;
+    const double dul = dul2 / d1, dvl = dvl2 / d2;
+    const double dd1 = -dul2 * ul / (d1 * d1), dd2 = -dvl2 * vl / (d2 * d2);
+    const double dun = dd1 * dt * kG / D1 + dd2 * dt * kG / D2;
+    const double dD1 = -dd1 * dt * kG * un / (D1 * D1), dD2 = -dd2 * dt * kG * un / (D2 * D2);
+    if (K1in > 1e-10) dK1 += dD1 * 2.0 * K1 * h43;
+    if (K2in > 1e-10) dK2 += dD2 * 2.0 * K2 * h43;
+    const double dh43 = dD1 * K1 * K1 + dD2 * K2 * K2;
+    dh += dh43 * (cb + h * cb / (3.0 * (h + 1e-16)));
+    double du = dun * u / un + dul * ca - dvl * sa;
+    double dv = dun * v / un + dul * sa + dvl * ca;
+    dqx = du / hs;
+    dqy = dv / hs;
+    dh += -du * qx / (hs * hs) - dv * qy / (hs * hs);
+}
+
+// ---------------------------------------------------------------------------
+// singular head loss (swSpE_friction.cpp:79-126): (h,qx,qy) -> (qx,qy)
+// ---------------------------------------------------------------------------
+SPD void hl_fwd(double h_min, double h, double &qx, double &qy, double ca, double sa, double hsing,
+                double k1, double k2, double dt) {
+    const double hthr = (h_min > kSmall) ? h_min : kSmall;
+    double ax = qx * ca + qy * sa;
+    double ay = -qx * sa + qy * ca;
+    if (h > hthr) {
+        double Ax = 0.0, Ay = 0.0;
+        if (h > hsing) {
+            const double hs = h + kSmall;
+            const double qn = sqrt(ax * ax + ay * ay + 1e-16);
+            Ax = qn * k1 / hs;
+            Ay = qn * k2 / hs;
+        }
+        ax = ax * exp(-Ax * dt);
+        ay = ay * exp(-Ay * dt);
+    } else {
+        ax = 0.0;
+        ay = 0.0;
+    }
+    qx = ax * ca - ay * sa;
+    qy = ax * sa + ay * ca;
+}
+
+SPD void hl_vjp(double h_min, double h, double qx, double qy, double ca, double sa, double hsing,
+                double k1, double k2, double dt, double &dh, double &dqx, double &dqy) {
+    const double hthr = (h_min > kSmall) ? h_min : kSmall;
+    const double dbx = dqx * ca + dqy * sa;
+    const double dby = -dqx * sa + dqy * ca;
+    double dax = 0.0, day = 0.0;
+    if (h > hthr) {
+        const double ax = qx * ca + qy * sa;
+        const double ay = -qx * sa + qy * ca;
+        double Ax = 0.0, Ay = 0.0, qn = 0.0, hs = 0.0;
+        if (h > hsing) {
+            hs = h + kSmall;
+            qn = sqrt(ax * ax + ay * ay + 1e-16);
+            Ax = qn * k1 / hs;
+            Ay = qn * k2 / hs;
+        }
+        const double ex = exp(-Ax * dt), ey = exp(-Ay * dt);
+        dax = dbx * ex;
+        day = dby * ey;
+        const double dAx = -dbx * ax * ex * dt, dAy = -dby * ay * ey * dt;
+        if (h > hsing) {
+            const double dqn = dAx * k1 / hs + dAy * k2 / hs;
+            dh += -(dAx * qn * k1 + dAy * qn * k2) / (hs * hs);
+            dax += dqn * ax / qn;
+            day += dqn * ay / qn;
+        }
+    }
+    dqx = dax * ca - day * sa;
+    dqy = dax * sa + day * ca;
+}
+
+// Balance contributions of one internal-edge side (swSpE_hyp.cpp:356-383) and
+// their adjoint.  side 0 = left cell (factors -c,-c nx,+c ny,+c nx,...).
+SPD void side_contrib(int side, double co, double nx, double ny, const double *F, double dt,
+                      double &dh, double &dqx, double &dqy) {
+    const double s = F[3 + side];
+    if (side == 0) {
+        dh = (-co) * (F[0] * dt);
+        dqx = ((-co * nx) * F[1] + (co * nx) * s + (co * ny) * F[2]) * dt;
+        dqy = ((-co * ny) * F[1] + (co * ny) * s + (-co * nx) * F[2]) * dt;
+    } else {
+        dh = (co) * (F[0] * dt);
+        dqx = ((co * nx) * F[1] + (co * nx) * s + (-co * ny) * F[2]) * dt;
+        dqy = ((co * ny) * F[1] + (co * ny) * s + (co * nx) * F[2]) * dt;
+    }
+}
+
+// adjoint of side_contrib w.r.t. F (accumulated into dF[5]) and the factor co
+SPD double side_contrib_vjp(int side, double co, double nx, double ny, const double *F, double dt,
+                            double ah, double ax, double ay, double *dF) {
+    const double sg = side == 0 ? -1.0 : 1.0;
+    dF[0] += ah * sg * co * dt;
+    dF[1] += dt * (ax * sg * co * nx + ay * sg * co * ny);
+    dF[2] += dt * (ax * (-sg) * co * ny + ay * sg * co * nx);
+    dF[3 + side] += dt * (ax * co * nx + ay * co * ny);
+    const double s = F[3 + side];
+    return ah * sg * F[0] * dt +
+           dt * (ax * (sg * nx * F[1] + nx * s - sg * ny * F[2]) +
+                 ay * (sg * ny * F[1] + ny * s + sg * nx * F[2]));
+}
+
+// coeff = w / area / phi (floored as computeEdgeFactors_E) and d coeff / d phi
+SPD double edge_coeff(double w, double area, double phi, double phi_min) {
+    const double pc = (phi > phi_min) ? phi : phi_min;
+    return (phi > phi_min) ? (w / area / pc) : 0.0;
+}
+SPD double edge_coeff_dphi(double w, double area, double phi, double phi_min) {
+    return (phi > phi_min) ? -(w / area / phi) / phi : 0.0;
+}
+
+}  // namespace spe

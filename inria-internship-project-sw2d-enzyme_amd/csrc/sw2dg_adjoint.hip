@@ -1,0 +1,1024 @@
+// sw2dg_adjoint.hip -- 4D-Var layer on the MI355X: the reference's
+// differentiable ("_E") forward and its hand-written adjoint.
+//
+// Reference path (assim/swSpE_checkpoint.cpp):
+//   build_checkpoint_records (97-455)  -> assim_record: dry run that fixes dt,
+//       the per-series BC values and the observation triggers of every step,
+//       landing on observation times, and keeps a state checkpoint every
+//       `stride` steps (sqrt(N) checkpointing)
+//   ckpt_segment_E (584-715)           -> k_e_edges + k_e_update replayed with
+//       the frozen schedule; cost at observation steps
+//   get_spatial_gradient_checkpointed (722-1281) -> assim_gradient: reverse
+//       loop over segments (K-1 .. window start), each segment recomputed
+//       forward from its checkpoint, then reversed step by step by
+//       hand-written adjoint kernels (Enzyme's role in the reference); time
+//       tiling resets the adjoint at tile tops (:932-970)
+//   replay_trajectory_cost (1311-1433) -> assim_cost
+// Controls: porosity per cell (the reference's control), friction
+// coefficients per cell (Strickler K; Manning zones map to them on the host)
+// and boundary values per step and series (hydrograph controls).
+// All gathers run in fixed reference order: no atomics, deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sw2dg.h"
+#include "sp_adjoint.h"
+#include "sp_physics.h"
+#include "sw2dg_internal.h"
+
+using namespace sw2dg_detail;
+using sp::kG;
+using sp::kSmall;
+
+namespace sw2dg_detail {
+
+struct EClock {
+    int oi;                  // next observation index (dry run)
+    unsigned int counter;    // last-block detection
+    int n_obs;
+    int n_series;
+    double t_max, dt_max, cfl;
+};
+
+struct Controls {
+    const double *phi, *K1, *K2;
+};
+
+struct AssimState {
+    // observations (device order)
+    int n_obs = 0, n_gauges = 0, normalize = 0;
+    bool frames = false;
+    std::vector<double> obs_t;
+    double *d_obs_t = nullptr;
+    int *d_gcell = nullptr;
+    double *d_gw = nullptr, *d_gobs = nullptr;
+    double *d_fh = nullptr, *d_fqx = nullptr, *d_fqy = nullptr, *d_wh = nullptr, *d_wq = nullptr;
+    // schedule
+    int stride = 0, n_tiles = 1;
+    double t_window = -1e300;
+    int64_t max_steps = 0, n_steps = 0;
+    double *d_T = nullptr, *d_dt = nullptr, *d_bc = nullptr;
+    int *d_trig = nullptr;
+    EClock *d_ck = nullptr;
+    bool recorded = false;
+    // states: checkpoints and one segment of trajectory, layout [h | qx | qy] per state
+    double *d_ckpt = nullptr;  // (n_ckpt) x 3nc
+    int64_t n_ckpt = 0;
+    double *d_seg = nullptr;   // (stride + 1) x 3nc
+    // work
+    double *d_F = nullptr;     // 5 n_int
+    double *d_sw = nullptr;    // 2 n_int
+    double *d_partial = nullptr;
+    double *d_J = nullptr;     // [2]: cost accumulator, scratch
+    double *d_cpart = nullptr; // frame-cost block partials
+    // adjoint
+    double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
+    double *d_gphi = nullptr, *d_gK1 = nullptr, *d_gK2 = nullptr, *d_gbce = nullptr,
+           *d_gbc = nullptr;
+    // controls (device order)
+    double *d_phi = nullptr, *d_K1 = nullptr, *d_K2 = nullptr;
+    // series -> boundary edges (renumbered) in reference order
+    int *d_ser_ptr = nullptr, *d_ser_edges = nullptr;
+    std::vector<void *> allocs;
+    template <class T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        if (hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        allocs.push_back(p);
+        return (T *)p;
+    }
+    ~AssimState() {
+        for (void *p : allocs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+void assim_destroy(AssimState *a) { delete a; }
+
+}  // namespace sw2dg_detail
+
+namespace {
+
+__device__ __forceinline__ double smax_(double a, double b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// _E forward kernels
+// ---------------------------------------------------------------------------
+
+// per internal edge: Godunov _E states, HLLC_E -> F[5e..5e+4]; optional CFL swept terms
+__global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls cs,
+                                                    const double *__restrict__ U,
+                                                    double *__restrict__ F, double *__restrict__ sw,
+                                                    const double *T, int64_t n, double t_max) {
+    if (T && !(T[n] < t_max)) return;
+    const int e = xcd_block() * kBlock + (int)threadIdx.x;
+    if (e >= m.n_int) return;
+    const int nc = m.n_cell;
+    const int L = m.eL[e], R = m.eR[e];
+    const double nx = m.enx[e], ny = m.eny[e];
+    const double hL = U[L], hR = U[R];
+    const spe::Side sL = spe::lr_fwd(hL, U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], nx, ny, o.h_min);
+    const spe::Side sR = spe::lr_fwd(hR, U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], nx, ny, o.h_min);
+    double f[5];
+    spe::hllc_fwd(o.h_min, o.phi_min, m.beta[L], m.beta[R], cs.phi[L], cs.phi[R], m.zb[L], m.zb[R],
+                  sL.z, sL.q, sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+    double *Fe = F + 5 * (size_t)e;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Fe[k] = f[k];
+    if (sw) {
+        double m1 = fabs(sL.lm), v = fabs(sL.lp);
+        if (m1 < v) m1 = v;
+        if (m1 < (v = fabs(sR.lm))) m1 = v;
+        if (m1 < (v = fabs(sR.lp))) m1 = v;
+        const double len = m.elen[e];
+        sw[2 * e] = (hL > o.h_min) ? m1 * len : 0.0;
+        sw[2 * e + 1] = (hR > o.h_min) ? m1 * len : 0.0;
+    }
+}
+
+// interpolateTimeSeries_E (assim/swSpE.h:138-153)
+__device__ double interp_e(const double *tt, const double *vv, int n, double t) {
+    if (n == 0) return 0.0;
+    if (n == 1 || t <= tt[0] - 1.0e-12) return vv[0];
+    if (t >= tt[n - 1]) return vv[n - 1];
+    int it = 0;
+    while (it < n - 1 && t > tt[it + 1]) it++;
+    const double t1 = tt[it], t2 = tt[it + 1], v1 = vv[it], v2 = vv[it + 1];
+    const double den = t2 - t1;
+    if (den < 1e-12) return v1;
+    return v1 + (v2 - v1) * (t - t1) / den;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o2 = __shfl_xor(v, off, 64);
+        v = (o2 < v) ? o2 : v;
+    }
+    return v;
+}
+
+// dry-run CFL (hypComputeTimeStep_E, swSpE_hyp.cpp:578-635) and the step
+// schedule of build_checkpoint_records (swSpE_checkpoint.cpp:224-366)
+__global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const double *__restrict__ U,
+                                                  const double *__restrict__ sw, BcDev bc,
+                                                  EClock *ck, double *partial, double *T, double *dtv,
+                                                  double *bcv, int *trig, const double *obs_t,
+                                                  int64_t n) {
+    if (!(T[n] < ck->t_max)) return;
+    __shared__ double red[kBlock / 64];
+    __shared__ int is_last;
+    const int lb = xcd_block();
+    const int i = lb * kBlock + (int)threadIdx.x;
+    const double dt_max = ck->dt_max;
+    double cand = dt_max;
+    const int nc = m.n_cell;
+    if (i < nc) {
+        const double h = U[i];
+        double swept = 0.0;
+        if (h > o.h_min) {
+            for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
+                const int j = m.bnd_inc[k];
+                const spe::Side s = spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i],
+                                                m.bnx[j], m.bny[j], o.h_min);
+                const double lm = (fabs(s.lp) < fabs(s.lm)) ? fabs(s.lm) : fabs(s.lp);
+                swept += lm * m.blen[j];
+            }
+        }
+        for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+            const int inc = m.int_inc[k];
+            swept += sw[2 * (size_t)(inc >> 1) + (inc & 1)];
+        }
+        const double den = smax_(swept, kSmall);
+        const double x = ck->cfl * (m.area[i] / den);
+        if (x < cand) cand = x;
+    }
+    cand = wave_min(cand);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = cand;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < kBlock / 64; ++w) v = (red[w] < v) ? red[w] : v;
+        partial[lb] = v;
+        __threadfence();
+        is_last = (atomicAdd(&ck->counter, 1u) == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double v = dt_max;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) {
+        const double x = partial[k];
+        v = (x < v) ? x : v;
+    }
+    v = wave_min(v);
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double dtc = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) dtc = (red[w] < dtc) ? red[w] : dtc;
+    ck->counter = 0u;
+    const double t = T[n];
+    const double t_max = ck->t_max;
+    if ((t + dtc) > t_max) dtc = t_max - t;
+    if (dtc < 1e-6) dtc = 1e-6;
+    double target = dt_max;
+    int oi = ck->oi;
+    if (oi < ck->n_obs) {
+        const double to = obs_t[oi] - t;
+        if (to - 10e-6 > 0.0 && to < target) target = to;
+    }
+    double dt = (dtc < target) ? dtc : target;
+    if ((t + dt) > t_max) dt = t_max - t;
+    if (dt < 1e-6) dt = 1e-6;
+    for (int k = 0; k < ck->n_series; ++k)
+        bcv[n * ck->n_series + k] = interp_e(bc.t, bc.v + (size_t)k * bc.n_times, bc.n_times, t);
+    const double tn = t + dt;
+    while (oi < ck->n_obs && tn > obs_t[oi] + 1e-6) oi++;
+    int tr = -1;
+    if (oi < ck->n_obs && fabs(tn - obs_t[oi]) < 1e-6) tr = oi++;
+    ck->oi = oi;
+    trig[n] = tr;
+    dtv[n] = dt;
+    T[n + 1] = tn;
+}
+
+struct CellSt {
+    double h, qx, qy;
+};
+
+// balance of one cell (hypComputeBalance_E, swSpE_hyp.cpp:336-394) before the
+// round-off clean-up; boundary fluxes computed here from the cell state.
+__device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, const Controls &cs,
+                                            const BcDev &bc, const double *U, const double *F,
+                                            const double *bcv, int i, double dt) {
+    const int nc = m.n_cell;
+    const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i];
+    CellSt s{h0, qx0, qy0};
+    const double phi = cs.phi[i], area = m.area[i];
+    for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
+        const int j = m.bnd_inc[k];
+        const double nx = m.bnx[j], ny = m.bny[j];
+        const spe::Side sd = spe::lr_fwd(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny, o.h_min);
+        const int ser = m.bser[j];
+        double f[3];
+        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp,
+                     bc.type[ser], bcv[ser], f);
+        const double co = spe::edge_coeff(m.blen[j], area, phi, o.phi_min);
+        s.h += (+co) * f[0] * dt;
+        s.qx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
+        s.qy += ((+co * ny) * f[1] + (+co * ny) * 0.0 + (+co * nx) * f[2]) * dt;
+    }
+    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+        const int inc = m.int_inc[k];
+        const int e = inc >> 1, side = inc & 1;
+        const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
+        double a, b, c;
+        spe::side_contrib(side, co, m.enx[e], m.eny[e], F + 5 * (size_t)e, dt, a, b, c);
+        s.h += a;
+        s.qx += b;
+        s.qy += c;
+    }
+    return s;
+}
+
+// per cell: balance, clean-up, clamp, friction _E, head loss _E
+__global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls cs, BcDev bc,
+                                                     const double *__restrict__ Uin,
+                                                     double *__restrict__ Uout,
+                                                     const double *__restrict__ F,
+                                                     const double *dtv, const double *bcv_all,
+                                                     int n_series, double *T, int64_t n,
+                                                     double t_max) {
+    const int nc = m.n_cell;
+    if (T && !(T[n] < t_max)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) T[n + 1] = T[n];
+        const int i = xcd_block() * kBlock + (int)threadIdx.x;
+        if (i < nc) {
+            Uout[i] = Uin[i];
+            Uout[nc + i] = Uin[nc + i];
+            Uout[2 * nc + i] = Uin[2 * nc + i];
+        }
+        return;
+    }
+    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+    if (i >= nc) return;
+    const double dt = dtv[n];
+    CellSt s = e_balance(m, o, cs, bc, Uin, F, bcv_all + n * n_series, i, dt);
+    if (fabs(s.h) < kSmall) s.h = 0.0;
+    if (fabs(s.qx) < kSmall) s.qx = 0.0;
+    if (fabs(s.qy) < kSmall) s.qy = 0.0;
+    if (s.h < 0.) {
+        s.h = 0.;
+        s.qx = 0.;
+        s.qy = 0.;
+    }
+    if (o.friction != sp::FRIC_NONE)
+        spe::fric_fwd(o.h_min, s.h, s.qx, s.qy, m.fcos[i], m.fsin[i], cs.K1[i], cs.K2[i], dt);
+    if (o.head_loss)
+        spe::hl_fwd(o.h_min, s.h, s.qx, s.qy, m.hcos[i], m.hsin[i], m.hsing[i], m.hk1[i], m.hk2[i],
+                    dt);
+    Uout[i] = s.h;
+    Uout[nc + i] = s.qx;
+    Uout[2 * nc + i] = s.qy;
+}
+
+// gauge cost (compute_cost_gauges_ptr, swSpE_gauges.cpp:166-179): one thread,
+// fixed order.  J[0] += sum_g w_g (h - obs)^2 / norm
+__global__ void k_cost_gauges(const double *__restrict__ U, const int *gcell, const double *gw,
+                              const double *gobs, int n_g, double norm, double *J) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double Jo = 0.0;
+    for (int g = 0; g < n_g; ++g) {
+        const double d = U[gcell[g]] - gobs[g];
+        Jo += gw[g] * d * d;
+    }
+    J[0] += Jo / norm;
+}
+
+// frame cost (compute_cost_frame_ptr, swSpE_ad.cpp:273-308): per-block partial
+// sums, then one block sums them in block order (deterministic).
+__global__ __launch_bounds__(kBlock) void k_cost_frame_part(const double *__restrict__ U, int nc,
+                                                             const double *fh, const double *fqx,
+                                                             const double *fqy, const double *wh,
+                                                             const double *wq, double *part) {
+    __shared__ double red[kBlock];
+    const int i = blockIdx.x * kBlock + (int)threadIdx.x;
+    double v = 0.0;
+    if (i < nc) {
+        const double dh = U[i] - fh[i], dqx = U[nc + i] - fqx[i], dqy = U[2 * nc + i] - fqy[i];
+        v = wh[i] * (dh * dh) + wq[i] * ((dqx * dqx) + (dqy * dqy));
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_cost_frame_sum(const double *part, int nparts, double norm, double *J) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < nparts; ++k) s += part[k];
+    J[0] += s / norm;
+}
+
+// ---------------------------------------------------------------------------
+// reverse kernels
+// ---------------------------------------------------------------------------
+
+// cost seeds: lam += d J_obs / d U
+__global__ void k_seed_gauges(const double *__restrict__ U, const int *gcell, const double *gw,
+                              const double *gobs, int n_g, double norm, double *lam) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int g = 0; g < n_g; ++g) {
+        const int c = gcell[g];
+        lam[c] += 2.0 * gw[g] * (U[c] - gobs[g]) / norm;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_seed_frame(const double *__restrict__ U, int nc,
+                                                        const double *fh, const double *fqx,
+                                                        const double *fqy, const double *wh,
+                                                        const double *wq, double norm, double *lam) {
+    const int i = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (i >= nc) return;
+    lam[i] += 2.0 * wh[i] * (U[i] - fh[i]) / norm;
+    lam[nc + i] += 2.0 * wq[i] * (U[nc + i] - fqx[i]) / norm;
+    lam[2 * nc + i] += 2.0 * wq[i] * (U[2 * nc + i] - fqy[i]) / norm;
+}
+
+// per cell: reverse of sources, clamp and clean-up; boundary fluxes; factor
+// derivatives.  lam (in) = adjoint of U^{n+1}; lams (out) = adjoint of the
+// post-balance state; part (out) = cell-local part of the adjoint of U^n.
+__global__ __launch_bounds__(kBlock) void k_r_cell(MeshDev m, Opts o, Controls cs, BcDev bc,
+                                                   const double *__restrict__ U,
+                                                   const double *__restrict__ F, double dt,
+                                                   const double *__restrict__ bcv,
+                                                   const double *__restrict__ lam,
+                                                   double *__restrict__ lams,
+                                                   double *__restrict__ part, double *gphi,
+                                                   double *gK1, double *gK2, double *gbce) {
+    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+    const int nc = m.n_cell;
+    if (i >= nc) return;
+    // forward recompute of the cell
+    const CellSt pre = e_balance(m, o, cs, bc, U, F, bcv, i, dt);
+    CellSt s = pre;
+    const bool kh = !(fabs(s.h) < kSmall), kx = !(fabs(s.qx) < kSmall), ky = !(fabs(s.qy) < kSmall);
+    if (!kh) s.h = 0.0;
+    if (!kx) s.qx = 0.0;
+    if (!ky) s.qy = 0.0;
+    const bool clamped = s.h < 0.;
+    if (clamped) {
+        s.h = 0.;
+        s.qx = 0.;
+        s.qy = 0.;
+    }
+    const CellSt after_clamp = s;
+    if (o.friction != sp::FRIC_NONE)
+        spe::fric_fwd(o.h_min, s.h, s.qx, s.qy, m.fcos[i], m.fsin[i], cs.K1[i], cs.K2[i], dt);
+    // reverse
+    double ah = lam[i], ax = lam[nc + i], ay = lam[2 * nc + i];
+    if (o.head_loss)
+        spe::hl_vjp(o.h_min, s.h, s.qx, s.qy, m.hcos[i], m.hsin[i], m.hsing[i], m.hk1[i], m.hk2[i],
+                    dt, ah, ax, ay);
+    if (o.friction != sp::FRIC_NONE) {
+        double dk1 = 0.0, dk2 = 0.0;
+        spe::fric_vjp(o.h_min, after_clamp.h, after_clamp.qx, after_clamp.qy, m.fcos[i], m.fsin[i],
+                      cs.K1[i], cs.K2[i], dt, ah, ax, ay, dk1, dk2);
+        gK1[i] += dk1;
+        gK2[i] += dk2;
+    }
+    if (clamped) {
+        ah = 0.0;
+        ax = 0.0;
+        ay = 0.0;
+    }
+    if (!kh) ah = 0.0;
+    if (!kx) ax = 0.0;
+    if (!ky) ay = 0.0;
+    lams[i] = ah;
+    lams[nc + i] = ax;
+    lams[2 * nc + i] = ay;
+    // balance: identity part + boundary fluxes + factor (porosity) derivatives
+    double ph = ah, px = ax, py = ay, gp = 0.0;
+    const double phi = cs.phi[i], area = m.area[i];
+    const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i];
+    for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
+        const int j = m.bnd_inc[k];
+        const double nx = m.bnx[j], ny = m.bny[j];
+        const spe::Side sd = spe::lr_fwd(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny, o.h_min);
+        const int ser = m.bser[j];
+        double f[3];
+        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp,
+                     bc.type[ser], bcv[ser], f);
+        const double co = spe::edge_coeff(m.blen[j], area, phi, o.phi_min);
+        double df[3];
+        df[0] = ah * co * dt;
+        df[1] = dt * (ax * co * nx + ay * co * ny);
+        df[2] = dt * (-ax * co * ny + ay * co * nx);
+        const double dco = ah * f[0] * dt + dt * (ax * (nx * f[1] - ny * f[2]) + ay * (ny * f[1] + nx * f[2]));
+        gp += dco * spe::edge_coeff_dphi(m.blen[j], area, phi, o.phi_min);
+        double d[6] = {0, 0, 0, 0, 0, 0};
+        spe::bnd_vjp(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp,
+                     bc.type[ser], bcv[ser], df, d);
+        spe::lr_vjp(h0, qx0, qy0, m.beta[i], nx, ny, o.h_min, d[0], d[1], d[2], 0.0, d[3], ph, px, py);
+        gp += d[4];
+        gbce[j] = d[5];
+    }
+    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+        const int inc = m.int_inc[k];
+        const int e = inc >> 1, side = inc & 1;
+        const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
+        double dF[5] = {0, 0, 0, 0, 0};
+        const double dco = spe::side_contrib_vjp(side, co, m.enx[e], m.eny[e], F + 5 * (size_t)e,
+                                                 dt, ah, ax, ay, dF);
+        gp += dco * spe::edge_coeff_dphi(m.elen[e], area, phi, o.phi_min);
+    }
+    part[i] = ph;
+    part[nc + i] = px;
+    part[2 * nc + i] = py;
+    gphi[i] += gp;
+}
+
+// per internal edge: flux adjoints from both sides, HLLC_E and Godunov _E reverse
+__global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls cs,
+                                                   const double *__restrict__ U, double dt,
+                                                   const double *__restrict__ lams,
+                                                   double *__restrict__ side_out) {
+    const int e = xcd_block() * kBlock + (int)threadIdx.x;
+    if (e >= m.n_int) return;
+    const int nc = m.n_cell;
+    const int L = m.eL[e], R = m.eR[e];
+    const double nx = m.enx[e], ny = m.eny[e], len = m.elen[e];
+    const double phiL = cs.phi[L], phiR = cs.phi[R];
+    const double coL = spe::edge_coeff(len, m.area[L], phiL, o.phi_min);
+    const double coR = spe::edge_coeff(len, m.area[R], phiR, o.phi_min);
+    const double Fz[5] = {0, 0, 0, 0, 0};
+    double dF[5] = {0, 0, 0, 0, 0};
+    spe::side_contrib_vjp(0, coL, nx, ny, Fz, dt, lams[L], lams[nc + L], lams[2 * nc + L], dF);
+    spe::side_contrib_vjp(1, coR, nx, ny, Fz, dt, lams[R], lams[nc + R], lams[2 * nc + R], dF);
+    const double hL = U[L], qxL = U[nc + L], qyL = U[2 * nc + L];
+    const double hR = U[R], qxR = U[nc + R], qyR = U[2 * nc + R];
+    const double bL = m.beta[L], bR = m.beta[R];
+    const spe::Side sL = spe::lr_fwd(hL, qxL, qyL, m.zb[L], bL, nx, ny, o.h_min);
+    const spe::Side sR = spe::lr_fwd(hR, qxR, qyR, m.zb[R], bR, nx, ny, o.h_min);
+    double d[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    spe::hllc_vjp(o.h_min, o.phi_min, bL, bR, phiL, phiR, m.zb[L], m.zb[R], sL.z, sL.q, sL.r, sR.z,
+                  sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, dF, d);
+    double dhL = 0, dqxL = 0, dqyL = 0, dhR = 0, dqxR = 0, dqyR = 0;
+    spe::lr_vjp(hL, qxL, qyL, bL, nx, ny, o.h_min, d[0], d[1], d[2], d[6], d[8], dhL, dqxL, dqyL);
+    spe::lr_vjp(hR, qxR, qyR, bR, nx, ny, o.h_min, d[3], d[4], d[5], d[7], d[9], dhR, dqxR, dqyR);
+    double *so = side_out + 8 * (size_t)e;
+    so[0] = dhL;
+    so[1] = dqxL;
+    so[2] = dqyL;
+    so[3] = d[10];
+    so[4] = dhR;
+    so[5] = dqxR;
+    so[6] = dqyR;
+    so[7] = d[11];
+}
+
+// per cell: adjoint of U^n = cell part + edge-side contributions, fixed order
+__global__ __launch_bounds__(kBlock) void k_r_gather(MeshDev m, const double *__restrict__ part,
+                                                     const double *__restrict__ side,
+                                                     double *__restrict__ lam, double *gphi) {
+    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+    const int nc = m.n_cell;
+    if (i >= nc) return;
+    double a = part[i], b = part[nc + i], c = part[2 * nc + i], g = 0.0;
+    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+        const int inc = m.int_inc[k];
+        const double *s = side + 8 * (size_t)(inc >> 1) + 4 * (inc & 1);
+        a += s[0];
+        b += s[1];
+        c += s[2];
+        g += s[3];
+    }
+    lam[i] = a;
+    lam[nc + i] = b;
+    lam[2 * nc + i] = c;
+    gphi[i] += g;
+}
+
+// per series: sum of the boundary-value adjoints of its edges, reference order
+__global__ void k_r_bc_reduce(const int *ser_ptr, const int *ser_edges, const double *gbce,
+                              int n_series, double *gbc_n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_series) return;
+    double s = 0.0;
+    for (int j = ser_ptr[k]; j < ser_ptr[k + 1]; ++j) s += gbce[ser_edges[j]];
+    gbc_n[k] = s;
+}
+
+__global__ void k_zero(double *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+int zero(sw2dg_ctx *c, double *p, size_t n) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, p, n);
+    return SW2DG_OK;
+}
+
+int upload_ctl(sw2dg_ctx *c, AssimState *a, const double *phi, const double *K1, const double *K2) {
+    const int64_t nc = c->nc;
+    auto put = [&](const double *src, double *dst, const double *dflt) -> int {
+        if (src) {
+            std::vector<double> v = c->to_dev_cells(src);
+            HIPCHK(c, hipMemcpyAsync(dst, v.data(), nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, hipMemcpyAsync(dst, dflt, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        return SW2DG_OK;
+    };
+    int st;
+    if ((st = put(phi, a->d_phi, c->m.phi))) return st;
+    if ((st = put(K1, a->d_K1, c->m.fc1))) return st;
+    if ((st = put(K2, a->d_K2, c->m.fc2))) return st;
+    return SW2DG_OK;
+}
+
+Controls ctl(AssimState *a) { return Controls{a->d_phi, a->d_K1, a->d_K2}; }
+
+double *seg_state(AssimState *a, sw2dg_ctx *c, int j) { return a->d_seg + (size_t)j * 3 * c->nc; }
+double *ckpt_state(AssimState *a, sw2dg_ctx *c, int64_t k) {
+    return a->d_ckpt + (size_t)k * 3 * c->nc;
+}
+
+void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout, int64_t n,
+                   bool dry) {
+    const dim3 blk(kBlock);
+    const Controls cs = ctl(a);
+    hipLaunchKernelGGL(k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Uin,
+                       a->d_F, dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max);
+    if (dry)
+        hipLaunchKernelGGL(k_e_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
+                           c->bc, a->d_ck, a->d_partial, a->d_T, a->d_dt, a->d_bc, a->d_trig,
+                           a->d_obs_t, n);
+    hipLaunchKernelGGL(k_e_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin,
+                       Uout, a->d_F, a->d_dt, a->d_bc, std::max(c->n_series, 1),
+                       dry ? a->d_T : nullptr, n, c->prm.t_max);
+}
+
+void launch_cost(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
+    const double norm = (a->normalize && a->n_obs > 0) ? (double)a->n_obs : 1.0;
+    if (a->n_gauges > 0)
+        hipLaunchKernelGGL(k_cost_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
+                           a->d_gobs + (size_t)oi * a->n_gauges, a->n_gauges, norm, a->d_J);
+    if (a->frames) {
+        const size_t o = (size_t)oi * c->nc;
+        const int nb = nblocks(c->nc);
+        hipLaunchKernelGGL(k_cost_frame_part, dim3(nb), dim3(kBlock), 0, c->stream, U, (int)c->nc,
+                           a->d_fh + o, a->d_fqx + o, a->d_fqy + o, a->d_wh, a->d_wq, a->d_cpart);
+        hipLaunchKernelGGL(k_cost_frame_sum, dim3(1), dim3(64), 0, c->stream, a->d_cpart, nb, norm,
+                           a->d_J);
+    }
+}
+
+void launch_seed(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
+    const double norm = (a->normalize && a->n_obs > 0) ? (double)a->n_obs : 1.0;
+    if (a->n_gauges > 0)
+        hipLaunchKernelGGL(k_seed_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
+                           a->d_gobs + (size_t)oi * a->n_gauges, a->n_gauges, norm, a->d_lam);
+    if (a->frames) {
+        const size_t o = (size_t)oi * c->nc;
+        hipLaunchKernelGGL(k_seed_frame, dim3(nblocks(c->nc)), dim3(kBlock), 0, c->stream, U,
+                           (int)c->nc, a->d_fh + o, a->d_fqx + o, a->d_fqy + o, a->d_wh, a->d_wq,
+                           norm, a->d_lam);
+    }
+}
+
+// segments, window and tiles (get_spatial_gradient_checkpointed, :785-953)
+struct Plan {
+    int64_t K = 0, first = 0;
+    std::vector<char> tile_top;
+    std::vector<int64_t> tile_start;
+};
+
+Plan make_plan(const AssimState *a, const std::vector<double> &T) {
+    Plan p;
+    const int64_t S = a->stride;
+    p.K = (a->n_steps + S - 1) / S;
+    p.first = 0;
+    if (a->t_window > -1e299) {
+        for (int64_t k = 0; k < p.K; ++k) {
+            if (T[k * S] <= a->t_window + 1e-9)
+                p.first = k;
+            else
+                break;
+        }
+    }
+    p.tile_top.assign(p.K, 0);
+    const int64_t M = p.K - p.first;
+    int64_t n = std::max<int64_t>(1, std::min<int64_t>(a->n_tiles, M));
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t s_t = p.first + (t * M) / n;
+        const int64_t e_t = p.first + ((t + 1) * M) / n - 1;
+        p.tile_start.push_back(s_t);
+        p.tile_top[e_t] = 1;
+    }
+    return p;
+}
+
+int fetch_record(sw2dg_ctx *c, AssimState *a, std::vector<double> &T, std::vector<int> &trig,
+                 std::vector<double> &dt) {
+    T.resize(a->n_steps + 1);
+    trig.resize(a->n_steps);
+    dt.resize(a->n_steps);
+    HIPCHK(c, hipMemcpy(T.data(), a->d_T, T.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (a->n_steps) {
+        HIPCHK(c, hipMemcpy(trig.data(), a->d_trig, trig.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(dt.data(), a->d_dt, dt.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return SW2DG_OK;
+}
+
+// replay segment k from its checkpoint into the segment buffer; cost of its
+// observation steps added to J when with_cost
+int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int> &trig,
+                   bool with_cost, const double *start) {
+    const int64_t S = a->stride;
+    const int64_t n0 = k * S, n1 = std::min(a->n_steps, n0 + S);
+    HIPCHK(c, hipMemcpyAsync(seg_state(a, c, 0), start ? start : ckpt_state(a, c, k),
+                             3 * c->nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    for (int64_t n = n0; n < n1; ++n) {
+        const int j = (int)(n - n0);
+        launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), n, false);
+        if (with_cost && trig[n] >= 0) launch_cost(c, a, seg_state(a, c, j + 1), trig[n]);
+    }
+    return SW2DG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_t n_tiles,
+                      double t_window_start, int64_t max_steps) {
+    if (!c || !obs || max_steps <= 0 || n_tiles < 1 || stride < 0) return SW2DG_ERR_ARG;
+    if (!c->state_set) return fail(c, SW2DG_ERR_ARG, "sw2dg_set_state must be called first");
+    if (c->o.scheme != SW2DG_GODUNOV)
+        return fail(c, SW2DG_ERR_ARG, "the adjoint covers the Godunov reconstruction");
+    if (obs->n_obs < 0 || obs->n_gauges < 0 || (obs->n_obs > 0 && !obs->t)) return SW2DG_ERR_ARG;
+    if (obs->n_gauges > 0 && (!obs->gauge_cell || !obs->gauge_h)) return SW2DG_ERR_ARG;
+    for (int g = 0; g < obs->n_gauges; ++g)
+        if (obs->gauge_cell[g] < 0 || obs->gauge_cell[g] >= c->nc) return SW2DG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    assim_destroy(c->assim);
+    c->assim = nullptr;
+    auto *a = new AssimState();
+    c->assim = a;
+    const int64_t nc = c->nc, ni = c->ni, nb = c->nb;
+    const int ns = std::max(c->n_series, 1);
+    a->n_obs = obs->n_obs;
+    a->n_gauges = obs->n_gauges;
+    a->normalize = obs->normalize;
+    a->obs_t.assign(obs->t, obs->t + obs->n_obs);
+    a->max_steps = max_steps;
+    a->stride = stride > 0 ? stride : (int)std::max<double>(1.0, std::ceil(std::sqrt((double)max_steps)));
+    a->n_tiles = n_tiles;
+    a->t_window = t_window_start;
+    a->d_obs_t = a->alloc<double>(std::max(1, obs->n_obs));
+    if (obs->n_obs)
+        HIPCHK(c, hipMemcpy(a->d_obs_t, obs->t, obs->n_obs * sizeof(double), hipMemcpyHostToDevice));
+    if (obs->n_gauges) {
+        std::vector<int> gc(obs->n_gauges);
+        for (int g = 0; g < obs->n_gauges; ++g) gc[g] = c->inv[obs->gauge_cell[g]];
+        std::vector<double> gw(obs->n_gauges, 1.0);
+        if (obs->gauge_w) gw.assign(obs->gauge_w, obs->gauge_w + obs->n_gauges);
+        a->d_gcell = a->alloc<int>(obs->n_gauges);
+        a->d_gw = a->alloc<double>(obs->n_gauges);
+        a->d_gobs = a->alloc<double>((size_t)obs->n_gauges * std::max(1, obs->n_obs));
+        HIPCHK(c, hipMemcpy(a->d_gcell, gc.data(), gc.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(a->d_gw, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(a->d_gobs, obs->gauge_h,
+                            (size_t)obs->n_gauges * obs->n_obs * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (obs->frame_h) {
+        if (!obs->frame_qx || !obs->frame_qy) return fail(c, SW2DG_ERR_ARG, "frame observations incomplete");
+        a->frames = true;
+        const size_t tot = (size_t)nc * std::max(1, obs->n_obs);
+        a->d_fh = a->alloc<double>(tot);
+        a->d_fqx = a->alloc<double>(tot);
+        a->d_fqy = a->alloc<double>(tot);
+        std::vector<double> v(nc);
+        const double *src[3] = {obs->frame_h, obs->frame_qx, obs->frame_qy};
+        double *dst[3] = {a->d_fh, a->d_fqx, a->d_fqy};
+        for (int f = 0; f < 3; ++f)
+            for (int o = 0; o < obs->n_obs; ++o) {
+                v = c->to_dev_cells(src[f] + (size_t)o * nc);
+                HIPCHK(c, hipMemcpy(dst[f] + (size_t)o * nc, v.data(), nc * sizeof(double),
+                                    hipMemcpyHostToDevice));
+            }
+        std::vector<double> one(nc, 1.0);
+        a->d_wh = a->alloc<double>(nc);
+        a->d_wq = a->alloc<double>(nc);
+        std::vector<double> wh = obs->w_h ? c->to_dev_cells(obs->w_h) : one;
+        std::vector<double> wq = obs->w_q ? c->to_dev_cells(obs->w_q) : one;
+        HIPCHK(c, hipMemcpy(a->d_wh, wh.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(a->d_wq, wq.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    a->d_T = a->alloc<double>(max_steps + 1);
+    a->d_dt = a->alloc<double>(max_steps);
+    a->d_bc = a->alloc<double>((size_t)max_steps * ns);
+    a->d_trig = a->alloc<int>(max_steps);
+    a->d_ck = a->alloc<EClock>(1);
+    a->n_ckpt = (max_steps + a->stride - 1) / a->stride + 1;
+    a->d_ckpt = a->alloc<double>((size_t)a->n_ckpt * 3 * nc);
+    a->d_seg = a->alloc<double>((size_t)(a->stride + 1) * 3 * nc);
+    a->d_F = a->alloc<double>(5 * (size_t)std::max<int64_t>(ni, 1));
+    a->d_sw = a->alloc<double>(2 * (size_t)std::max<int64_t>(ni, 1));
+    a->d_partial = a->alloc<double>(nblocks(nc));
+    a->d_cpart = a->alloc<double>(nblocks(nc));
+    a->d_J = a->alloc<double>(2);
+    a->d_lam = a->alloc<double>(3 * nc);
+    a->d_lams = a->alloc<double>(3 * nc);
+    a->d_part = a->alloc<double>(3 * nc);
+    a->d_side = a->alloc<double>(8 * (size_t)std::max<int64_t>(ni, 1));
+    a->d_gphi = a->alloc<double>(nc);
+    a->d_gK1 = a->alloc<double>(nc);
+    a->d_gK2 = a->alloc<double>(nc);
+    a->d_gbce = a->alloc<double>(std::max<int64_t>(nb, 1));
+    a->d_gbc = a->alloc<double>((size_t)max_steps * ns);
+    a->d_phi = a->alloc<double>(nc);
+    a->d_K1 = a->alloc<double>(nc);
+    a->d_K2 = a->alloc<double>(nc);
+    // boundary edges of each series in reference order (renumbered indices)
+    std::vector<int> sp(ns + 1, 0), se;
+    for (int k = 0; k < c->n_series; ++k) {
+        for (int64_t e = 0; e < nb; ++e)
+            if (c->bser_host[e] == k) se.push_back(c->binv[e]);
+        sp[k + 1] = (int)se.size();
+    }
+    if (se.empty()) se.push_back(0);
+    a->d_ser_ptr = a->alloc<int>(sp.size());
+    a->d_ser_edges = a->alloc<int>(se.size());
+    HIPCHK(c, hipMemcpy(a->d_ser_ptr, sp.data(), sp.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(a->d_ser_edges, se.data(), se.size() * sizeof(int), hipMemcpyHostToDevice));
+    for (void *p : a->allocs)
+        if (!p) return fail(c, SW2DG_ERR_HIP, "out of device memory for the adjoint buffers");
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
+                       int64_t *n_steps) {
+    if (n_steps) *n_steps = 0;
+    if (!c || !c->assim) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    int st = upload_ctl(c, a, phi, K1, K2);
+    if (st) return st;
+    const int64_t nc = c->nc;
+    // initial state -> checkpoint 0 and the dry-run buffers
+    std::vector<double> U0(3 * nc);
+    std::copy(c->h0.begin(), c->h0.end(), U0.begin());
+    std::copy(c->qx0.begin(), c->qx0.end(), U0.begin() + nc);
+    std::copy(c->qy0.begin(), c->qy0.end(), U0.begin() + 2 * nc);
+    HIPCHK(c, hipMemcpy(ckpt_state(a, c, 0), U0.data(), 3 * nc * sizeof(double), hipMemcpyHostToDevice));
+    EClock ek{};
+    ek.oi = 0;
+    ek.counter = 0;
+    ek.n_obs = a->n_obs;
+    ek.n_series = c->n_series;
+    ek.t_max = c->prm.t_max;
+    ek.dt_max = c->prm.dt_max;
+    ek.cfl = c->prm.cfl_max;
+    HIPCHK(c, hipMemcpy(a->d_ck, &ek, sizeof(EClock), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(a->d_T, &c->t0, sizeof(double), hipMemcpyHostToDevice));
+    const int64_t S = a->stride;
+    int64_t n = 0;
+    int64_t k = 0;
+    double tk = c->t0;
+    while (n < a->max_steps && tk < c->prm.t_max) {
+        // one segment: ping-pong in the segment buffer, then checkpoint its end
+        HIPCHK(c, hipMemcpyAsync(seg_state(a, c, 0), ckpt_state(a, c, k), 3 * nc * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+        const int64_t n1 = std::min(a->max_steps, n + S);
+        for (int64_t m2 = n; m2 < n1; ++m2) {
+            const int j = (int)(m2 - n);
+            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true);
+        }
+        HIPCHK(c, hipMemcpyAsync(ckpt_state(a, c, k + 1), seg_state(a, c, (int)(n1 - n)),
+                                 3 * nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&tk, a->d_T + n1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        n = n1;
+        ++k;
+    }
+    // number of active steps: first n with T[n] >= t_max
+    std::vector<double> T(n + 1);
+    HIPCHK(c, hipMemcpy(T.data(), a->d_T, (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    int64_t cnt = 0;
+    while (cnt < n && T[cnt] < c->prm.t_max) ++cnt;
+    a->n_steps = cnt;
+    a->recorded = true;
+    if (n_steps) *n_steps = cnt;
+    if (cnt >= a->max_steps && T[cnt] < c->prm.t_max)
+        return fail(c, SW2DG_ERR_ARG, "max_steps reached before t_max");
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_get_record(sw2dg_ctx *c, double *t_seq, double *dt_seq, double *bc_seq,
+                           int32_t *obs_trig) {
+    if (!c || !c->assim || !c->assim->recorded) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = a->n_steps;
+    const int ns = std::max(c->n_series, 1);
+    if (t_seq) HIPCHK(c, hipMemcpy(t_seq, a->d_T, (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (dt_seq && n) HIPCHK(c, hipMemcpy(dt_seq, a->d_dt, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (bc_seq && n && c->n_series)
+        HIPCHK(c, hipMemcpy(bc_seq, a->d_bc, (size_t)n * ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (obs_trig && n)
+        HIPCHK(c, hipMemcpy(obs_trig, a->d_trig, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_set_bc_seq(sw2dg_ctx *c, const double *bc_seq) {
+    if (!c || !c->assim || !c->assim->recorded || !bc_seq) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int ns = std::max(c->n_series, 1);
+    if (a->n_steps && c->n_series)
+        HIPCHK(c, hipMemcpy(a->d_bc, bc_seq, (size_t)a->n_steps * ns * sizeof(double),
+                            hipMemcpyHostToDevice));
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_cost(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
+                     double *J) {
+    if (!c || !c->assim || !c->assim->recorded || !J) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    int st = upload_ctl(c, a, phi, K1, K2);
+    if (st) return st;
+    std::vector<double> T, dt;
+    std::vector<int> trig;
+    if ((st = fetch_record(c, a, T, trig, dt))) return st;
+    const Plan p = make_plan(a, T);
+    zero(c, a->d_J, 2);
+    // tiles restart from their frozen checkpoints; segments inside a tile chain
+    for (size_t ti = 0; ti < p.tile_start.size(); ++ti) {
+        const int64_t s_t = p.tile_start[ti];
+        const int64_t e_t = (ti + 1 < p.tile_start.size()) ? p.tile_start[ti + 1] - 1 : p.K - 1;
+        for (int64_t k = s_t; k <= e_t; ++k) {
+            const double *start = nullptr;
+            if (k > s_t) {
+                // continue from the end of the previous (full) segment; copy it out
+                // first because the segment buffer is reused
+                HIPCHK(c, hipMemcpyAsync(a->d_part, seg_state(a, c, a->stride), 3 * c->nc * sizeof(double),
+                                         hipMemcpyDeviceToDevice, c->stream));
+                start = a->d_part;
+            }
+            if ((st = replay_segment(c, a, k, trig, true, start))) return st;
+        }
+    }
+    HIPCHK(c, hipMemcpyAsync(J, a->d_J, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
+                         double *J, double *dJ_dphi, double *dJ_dK1, double *dJ_dK2,
+                         double *dJ_dbc, int64_t *n_steps) {
+    if (n_steps) *n_steps = 0;
+    if (!c || !c->assim) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t nst = 0;
+    int st = sw2dg_assim_record(c, phi, K1, K2, &nst);
+    if (st) return st;
+    std::vector<double> T, dt;
+    std::vector<int> trig;
+    if ((st = fetch_record(c, a, T, trig, dt))) return st;
+    const Plan p = make_plan(a, T);
+    const int64_t nc = c->nc;
+    const int ns = std::max(c->n_series, 1);
+    zero(c, a->d_J, 2);
+    zero(c, a->d_lam, 3 * nc);
+    zero(c, a->d_gphi, nc);
+    zero(c, a->d_gK1, nc);
+    zero(c, a->d_gK2, nc);
+    zero(c, a->d_gbc, (size_t)std::max<int64_t>(a->n_steps, 1) * ns);
+    const Controls cs = ctl(a);
+    const dim3 blk(kBlock);
+    for (int64_t k = p.K - 1; k >= p.first; --k) {
+        if (p.tile_top[k]) zero(c, a->d_lam, 3 * nc);
+        // forward recompute of segment k from its checkpoint (with its costs)
+        if ((st = replay_segment(c, a, k, trig, true, nullptr))) return st;
+        const int64_t n0 = k * a->stride, n1 = std::min(a->n_steps, n0 + a->stride);
+        for (int64_t n = n1 - 1; n >= n0; --n) {
+            const int j = (int)(n - n0);
+            const double *Un = seg_state(a, c, j);
+            if (trig[n] >= 0) launch_seed(c, a, seg_state(a, c, j + 1), trig[n]);
+            hipLaunchKernelGGL(k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
+                               a->d_F, nullptr, nullptr, n, 0.0);
+            hipLaunchKernelGGL(k_r_cell, dim3(nblocks(nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
+                               Un, a->d_F, dt[n], a->d_bc + (size_t)n * ns, a->d_lam, a->d_lams,
+                               a->d_part, a->d_gphi, a->d_gK1, a->d_gK2, a->d_gbce);
+            hipLaunchKernelGGL(k_r_edge, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
+                               dt[n], a->d_lams, a->d_side);
+            hipLaunchKernelGGL(k_r_gather, dim3(nblocks(nc)), blk, 0, c->stream, c->m, a->d_part,
+                               a->d_side, a->d_lam, a->d_gphi);
+            if (c->n_series > 0)
+                hipLaunchKernelGGL(k_r_bc_reduce, dim3(1), dim3(64), 0, c->stream, a->d_ser_ptr,
+                                   a->d_ser_edges, a->d_gbce, c->n_series, a->d_gbc + (size_t)n * ns);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> v(nc);
+    auto pull = [&](const double *src, double *dst) -> int {
+        if (!dst) return SW2DG_OK;
+        HIPCHK(c, hipMemcpy(v.data(), src, nc * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < nc; ++q) {
+            const double g = v[q];
+            // non-finite gradient entries are zeroed (swSpE_checkpoint.cpp:1197-1220)
+            dst[c->perm[q]] = std::isfinite(g) ? g : 0.0;
+        }
+        return SW2DG_OK;
+    };
+    if ((st = pull(a->d_gphi, dJ_dphi))) return st;
+    if ((st = pull(a->d_gK1, dJ_dK1))) return st;
+    if ((st = pull(a->d_gK2, dJ_dK2))) return st;
+    if (dJ_dbc && a->n_steps && c->n_series)
+        HIPCHK(c, hipMemcpy(dJ_dbc, a->d_gbc, (size_t)a->n_steps * ns * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    if (J) HIPCHK(c, hipMemcpy(J, a->d_J, sizeof(double), hipMemcpyDeviceToHost));
+    if (n_steps) *n_steps = a->n_steps;
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_final_state(sw2dg_ctx *c, double *h, double *qx, double *qy) {
+    if (!c || !c->assim || !c->assim->recorded) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nc = c->nc;
+    const int64_t K = (a->n_steps + a->stride - 1) / a->stride;
+    std::vector<double> U(3 * nc);
+    HIPCHK(c, hipMemcpy(U.data(), ckpt_state(a, c, K), 3 * nc * sizeof(double), hipMemcpyDeviceToHost));
+    double *dst[3] = {h, qx, qy};
+    for (int f = 0; f < 3; ++f)
+        if (dst[f])
+            for (int64_t q = 0; q < nc; ++q) dst[f][c->perm[q]] = U[f * nc + q];
+    return SW2DG_OK;
+}
+
+}  // extern "C"

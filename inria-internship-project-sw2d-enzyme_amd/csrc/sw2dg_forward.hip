@@ -32,81 +32,18 @@
 
 #include "../../include/sw2dg.h"
 #include "sp_physics.h"
+#include "sw2dg_internal.h"
 
 using sp::kG;
 using sp::kSmall;
 using sp::smax;
 using sp::smin;
 
+using namespace sw2dg_detail;
+
 namespace {
 
-constexpr int kBlock = 256;
 constexpr int kStepsPerGraph = 32;  // even: the clock double buffer returns to slot 0
-
-// ---------------------------------------------------------------------------
-// device-resident clock (one per ctx).  Slot p holds the values valid at the
-// start of a step launched with parity p; the step writes slot 1-p.
-// ---------------------------------------------------------------------------
-struct Clock {
-    double T[2];         // time at the start of the step
-    double DT[2];        // dt of the previous step (MUSCL reconstruction)
-    long long STEP[2];   // steps taken since set_state
-    int ACT[2];          // step runs (1) or is a no-op (0)
-    int err;             // first error code raised on the device
-    int store_flags;     // storage instants hit by the last step
-    int paused;          // a storage instant stopped the run
-    int pause_on_store;
-    double t_stop, t_max, dt_max, dt_map, dt_ts, cfl;
-    long long step_stop;
-    double dt_last;      // dt of the last active step
-    unsigned int k_counter;  // last-block detection in k_cfl
-    int n_series;
-};
-
-struct BcDev {
-    const double *t;     // [n_times]
-    const double *v;     // [n_series*n_times]
-    const int *type;     // [n_series]
-    double *vinf, *vsup, *tinf, *tsup, *val;
-    int *rank, *atend;
-    int n_times;
-};
-
-struct MeshDev {
-    int n_cell, n_bnd, n_int;
-    // cells (renumbered)
-    double *h, *qx, *qy;
-    const double *zb, *area, *phi, *beta, *cx, *cy;
-    const double *fc1, *fc2, *fcos, *fsin;               // friction
-    const double *hsing, *hk1, *hk2, *hcos, *hsin;       // singular head loss
-    double *dzdx, *dzdy;                                 // MUSCL slopes
-    // incidence lists in reference order
-    const int *int_ptr, *int_inc;  // entries (edge << 1) | side
-    const int *bnd_ptr, *bnd_inc;  // entries: renumbered boundary edge
-    // internal edges (renumbered)
-    const int *eL, *eR;
-    const double *elen, *enx, *eny, *exm, *eym;
-    double *rec;                   // [8*n_int]: side 0 {dh, dqx, dqy, swept}, side 1 same
-    // boundary edges (renumbered)
-    const double *blen, *bnx, *bny;
-    const int *bser;
-    double *bF1;
-    double *partial;               // [n_blocks of k_cfl]
-};
-
-struct Opts {
-    int scheme, topo, friction, head_loss, neg_det;
-    double h_min, phi_min, v_max, fr_max;
-};
-
-__device__ __forceinline__ int xcd_block() {
-    // Workgroup b is dispatched to XCD b % 8.  Give each XCD a contiguous
-    // range of logical blocks (bijection on [0, gridDim.x)).
-    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
-    const int q = nb >> 3, r = nb & 7;
-    const int x = b & 7, k = b >> 3;
-    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
 
 __device__ __forceinline__ bool step_active(const Clock *ck, int p) {
     return ck->ACT[p] != 0 && ck->err == 0;
@@ -580,61 +517,14 @@ uint32_t hilbert_d(uint32_t x, uint32_t y) {
 
 }  // namespace
 
-struct sw2dg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    sw2dg_params prm{};
-    int64_t nc = 0, nb = 0, ni = 0;
-    std::vector<int> perm, inv;     // new cell -> old, old -> new
-    std::vector<int> eperm, binv;   // new edge -> old edge ; old bnd -> new bnd
-    std::vector<int> bperm;
-    std::vector<int> bser_host;     // reference order
-    MeshDev m{};
-    Opts o{};
-    BcDev bc{};
-    Clock *ck = nullptr;
-    Clock hck{};
-    std::vector<void *> allocs;
-    int64_t bytes = 0;
-    int n_series = 0, n_times = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    bool state_set = false;
-
-    template <class T>
-    T *dalloc(size_t n) {
-        void *p = nullptr;
-        const size_t b = std::max<size_t>(n, 1) * sizeof(T);
-        if (hipMalloc(&p, b) != hipSuccess) return nullptr;
-        hipMemset(p, 0, b);
-        allocs.push_back(p);
-        bytes += (int64_t)b;
-        return (T *)p;
-    }
-    template <class T>
-    T *upload(const std::vector<T> &v) {
-        T *d = dalloc<T>(v.size());
-        if (d && !v.empty()) hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-        return d;
-    }
-};
-
-namespace {
-
+namespace sw2dg_detail {
 int fail(sw2dg_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
 }
+}  // namespace sw2dg_detail
 
-#define HIPCHK(c, x)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = (x);                                                              \
-        if (e_ != hipSuccess)                                                             \
-            return fail((c), SW2DG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+namespace {
 
 void launch_step(sw2dg_ctx *c, int p) {
     const dim3 blk(kBlock);
@@ -956,6 +846,9 @@ int sw2dg_set_bc_series(sw2dg_ctx *c, int32_t n_series, int32_t n_times, const i
     if (tv.empty()) tv.push_back(0.0);
     if (vv.empty()) vv.push_back(0.0);
     if (ty.empty()) ty.push_back(3);
+    c->series_t_host = tv;
+    c->series_v_host = vv;
+    c->series_type_host = ty;
     c->bc.t = c->upload(tv);
     c->bc.v = c->upload(vv);
     c->bc.type = c->upload(ty);
@@ -985,10 +878,13 @@ int sw2dg_set_state(sw2dg_ctx *c, const double *h, const double *qx, const doubl
     std::vector<double> v(nc);
     double *dst[3] = {c->m.h, c->m.qx, c->m.qy};
     const double *src[3] = {h, qx, qy};
+    std::vector<double> *keep[3] = {&c->h0, &c->qx0, &c->qy0};
     for (int a = 0; a < 3; ++a) {
         for (int64_t k = 0; k < nc; ++k) v[k] = src[a][c->perm[k]];
         HIPCHK(c, hipMemcpy(dst[a], v.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+        *keep[a] = v;
     }
+    c->t0 = t;
     // fluxes of dry-dry edges start at zero (reference vectors are value-initialised)
     HIPCHK(c, hipMemset(c->m.rec, 0, 8 * (size_t)std::max<int64_t>(c->ni, 1) * sizeof(double)));
     // BC cursors rewound (swBoundaryConditionTimeSeries::initialise)
@@ -1191,6 +1087,8 @@ void sw2dg_ctx_destroy(sw2dg_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
+    sw2dg_detail::assim_destroy(c->assim);
+    c->assim = nullptr;
     for (void *p : c->allocs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);

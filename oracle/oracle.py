@@ -73,6 +73,107 @@ def _dp(a):
     return None if a is None or len(a) == 0 else a.ctypes.data_as(_d)
 
 
+def build_case(case, params=None, keep=None):
+    """OrcCase view of a loaded case (arrays kept alive in `keep`)."""
+    a = case.a
+    p = params if params is not None else case.params
+    keep = keep if keep is not None else []
+
+    def k(arr, dt=np.float64):
+        arr = np.ascontiguousarray(arr, dtype=dt)
+        keep.append(arr)
+        return arr
+
+    c = OrcCase()
+    c.n_cell, c.n_bnd, c.n_int = case.n_cell, case.n_bnd, case.n_int
+    for f in ("cell_area", "cell_x", "cell_y", "cell_zb", "phi", "beta", "bnd_len", "bnd_nx",
+              "bnd_ny", "int_len", "int_nx", "int_ny", "int_xm", "int_ym"):
+        setattr(c, f, _dp(k(a[f])))
+    for f, src in (("fric_c1", "fc1"), ("fric_c2", "fc2"), ("fric_alpha_deg", "falpha"),
+                   ("hl_hsing", "hsing"), ("hl_c1", "hk1"), ("hl_c2", "hk2"),
+                   ("hl_alpha_deg", "halpha")):
+        setattr(c, f, _dp(k(a[src])))
+    c.bnd_cell = k(a["bnd_cell"], np.int64).ctypes.data_as(_i64)
+    c.bnd_series = k(a["bnd_series"], np.int32).ctypes.data_as(_i32)
+    c.int_L = k(a["int_L"], np.int64).ctypes.data_as(_i64)
+    c.int_R = k(a["int_R"], np.int64).ctypes.data_as(_i64)
+    c.n_series, c.n_times = case.n_series, case.n_times
+    c.series_type = k(a["series_type"], np.int32).ctypes.data_as(_i32)
+    c.series_t = _dp(k(a["series_t"]))
+    c.series_v = _dp(k(a["series_v"]))
+    for f in ("scheme", "topo", "friction", "head_loss", "neg_depth_detection", "h_min",
+              "phi_min", "cfl_max", "dt_max", "dt_map", "t0", "t_max", "v_max", "fr_max"):
+        setattr(c, f, getattr(p, f))
+    return c, keep
+
+
+def _e_decl():
+    L = lib()
+    if getattr(L, "_e_ready", False):
+        return L
+    L.orc_e_record.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int32, _d, C.c_int64,
+                               _d, _d, _i32]
+    L.orc_e_record.restype = C.c_int64
+    L.orc_e_replay.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d, _d, _i32,
+                               C.c_int32, C.c_int32, _i64, _d, _d, C.c_int32, _d, _d, _d]
+    L.orc_e_replay.restype = C.c_double
+    L._e_ready = True
+    return L
+
+
+class OracleE:
+    """CPU restatement of the reference's _E forward: dry run and frozen replay."""
+
+    def __init__(self, case):
+        self.case = case
+        self.c, self._keep = build_case(case)
+        self.nc = case.n_cell
+        self.ns = max(case.n_series, 1)
+
+    def _ctl(self, x, default):
+        v = default if x is None else np.broadcast_to(np.asarray(x, dtype=np.float64), (self.nc,))
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._keep.append(v)
+        return v
+
+    def record(self, obs_t, phi=None, K1=None, K2=None, max_steps=100000):
+        a = self.case.a
+        phi = self._ctl(phi, a["phi"])
+        K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
+        K2 = self._ctl(K2, a["fc2"] if len(a["fc2"]) else np.zeros(self.nc))
+        obs_t = np.ascontiguousarray(obs_t, dtype=np.float64)
+        dt = np.empty(max_steps)
+        bc = np.empty(max_steps * self.ns)
+        trig = np.empty(max_steps, dtype=np.int32)
+        h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
+        n = _e_decl().orc_e_record(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+                                   _dp(qy0), len(obs_t), _dp(obs_t), max_steps, _dp(dt), _dp(bc),
+                                   trig.ctypes.data_as(_i32))
+        return {"dt": dt[:n].copy(), "bc": bc[: n * self.ns].reshape(n, self.ns).copy(),
+                "trig": trig[:n].copy()}
+
+    def replay(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=None, normalize=False,
+               n_obs=None, bc=None):
+        a = self.case.a
+        phi = self._ctl(phi, a["phi"])
+        K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
+        K2 = self._ctl(K2, a["fc2"] if len(a["fc2"]) else np.zeros(self.nc))
+        dt = np.ascontiguousarray(rec["dt"], dtype=np.float64)
+        bcs = np.ascontiguousarray(rec["bc"] if bc is None else bc, dtype=np.float64).reshape(-1)
+        trig = np.ascontiguousarray(rec["trig"], dtype=np.int32)
+        h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
+        gc = np.ascontiguousarray(gauges if gauges is not None else np.zeros(0), dtype=np.int64)
+        gw = np.ones(len(gc))
+        ob = np.ascontiguousarray(obs if obs is not None else np.zeros(1), dtype=np.float64).reshape(-1)
+        nob = int(n_obs if n_obs is not None else (trig.max() + 1 if len(trig) else 0))
+        H, QX, QY = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
+        J = _e_decl().orc_e_replay(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+                                   _dp(qy0), len(dt), _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32),
+                                   nob, len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
+                                   1 if normalize else 0, _dp(H), _dp(QX), _dp(QY))
+        return J, (H, QX, QY)
+
+
 class OracleSolver:
     """CPU restatement of swSp's time loop on one case (arrays in reference order)."""
 

@@ -42,7 +42,13 @@ def oracle():
 
 
 def golden_names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    """Fixtures of the forward solver (the e_* ones belong to the 4D-Var layer)."""
+    return sorted(n for n in (os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+                  if not n.startswith("e_"))
+
+
+def e_golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "e_*.npz")))
 
 
 def load_golden(name):
@@ -51,8 +57,8 @@ def load_golden(name):
 
 def regen_case(sw, name, folder):
     """Rebuild the inputs of a golden case and check they are byte-identical."""
-    from make_golden import golden_cases
-    gen, kw = golden_cases()[name]
+    from make_golden import e_golden_cases, golden_cases
+    gen, kw = (golden_cases()[name] if name in golden_cases() else e_golden_cases()[name][:2])
     from sw2d_amd import cases
     getattr(cases, gen)(folder, **kw)
     g = load_golden(name)

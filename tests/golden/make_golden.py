@@ -12,6 +12,10 @@ case tooling (deterministic: fixed seeds), runs the reference's own
 tests/golden/<name>.npz together with SHA-256 digests of the input files, so
 a test can regenerate the same inputs and check it did.  Cases where the
 reference stops on a fatal error record that error instead.
+
+The e_* fixtures hold the differentiable forward of the 4D-Var layer
+(dt / observation-trigger schedule of the dry run and the final state),
+produced by oracle/_ref/ref_e_driver from the reference's assim/swSpE_*.cpp.
 """
 import glob
 import hashlib
@@ -27,6 +31,7 @@ import numpy as np
 REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
 PKG = os.path.join(REPO, "inria-internship-project-sw2d-enzyme_amd")
 REF = os.path.join(REPO, "oracle", "_ref", "sw2dSolver")
+REF_E = os.path.join(REPO, "oracle", "_ref", "ref_e_driver")
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -62,6 +67,58 @@ def golden_cases():
         "mixed_muscl_closed_cells_error": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
                                                              t_max=20.0, closed_fraction=0.04)),
     }
+
+
+# Differentiable (_E) forward: name -> (generator, kwargs, observation times).
+# The reference's _E physics (assim/swSpE_*.cpp) runs through
+# oracle/ref_e_driver.cpp, which prints dt and the observation trigger of
+# every dry-run step and the final state.
+def e_golden_cases():
+    return {
+        "e_scale_godunov_strickler": ("scale_case", dict(px=30, py=30, L=300.0, scheme="godunov",
+                                                         t_max=120.0, friction="strickler",
+                                                         manning=30.0), [30.0, 60.0, 90.0, 120.0]),
+        "e_mixed_godunov_strickler": ("mixed_case", dict(scheme="godunov", friction="strickler",
+                                                         t_max=20.0), [5.0, 10.0, 15.0, 20.0]),
+        "e_mixed_godunov_manning_nohl": ("mixed_case", dict(scheme="godunov", friction="manning",
+                                                            t_max=20.0, head_loss=False),
+                                         [5.0, 10.0, 15.0, 20.0]),
+    }
+
+
+def run_ref_e(folder, obs):
+    r = subprocess.run([REF_E, folder, str(len(obs))] + [repr(float(x)) for x in obs],
+                       capture_output=True, text=True, timeout=1200)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_e_driver failed ({r.returncode}): {r.stderr[-400:]}")
+    dt, trig, S = [], [], []
+    for line in r.stdout.splitlines():
+        p = line.split()
+        if p and p[0] == "step":
+            dt.append(float(p[2]))
+            trig.append(int(p[3]))
+        elif p and p[0] == "cell":
+            S.append([float(p[2]), float(p[3]), float(p[4])])
+    S = np.array(S)
+    return np.array(dt), np.array(trig, dtype=np.int32), S[:, 0], S[:, 1], S[:, 2]
+
+
+def main_e(sw):
+    if not os.path.exists(REF_E):
+        sys.exit(f"{REF_E} missing: run `make -C {os.path.join(REPO, 'oracle')} ref` first")
+    for name, (gen, kw, obs) in e_golden_cases().items():
+        tmp = tempfile.mkdtemp(prefix="sw2d_golden_e_")
+        try:
+            make_case(sw, tmp, gen, kw)
+            dig = input_digests(tmp)
+            dt, trig, h, qx, qy = run_ref_e(tmp, obs)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), obs_t=np.array(obs), dt=dt,
+                                trig=trig, h=h, qx=qx, qy=qy, name=name, generator=gen,
+                                kwargs=repr(kw), digest_names=np.array(list(dig.keys())),
+                                digest_values=np.array(list(dig.values())))
+            print(name, "steps:", len(dt))
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
 
 
 def make_case(sw, folder, gen, kw):
@@ -115,6 +172,7 @@ def main():
             print(name, "maps:", [times[k] for k in keep])
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
+    main_e(sw)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,143 @@
+"""Hand-written reverse-mode derivatives of the local _E functions against
+central finite differences (dot-product form: <J^T w, v> = d/de <w, f(x+e v)>).
+
+Inputs are drawn in smooth regions (away from the regime switches) so the
+finite differences see one branch.  A planted error (a derivative with a
+dropped term) must fail the same check, so the check has teeth."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import PKG_DIR
+
+LIB = C.CDLL(os.path.join(PKG_DIR, "lib", "libsw2d_adjtest.so"))
+dp = C.POINTER(C.c_double)
+HMIN, PHIMIN = 1e-5, 1e-16
+
+
+def arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def fwd(name, p, nout, *extra):
+    out = np.zeros(nout)
+    getattr(LIB, name)(arr(p).ctypes.data_as(dp), *extra, out.ctypes.data_as(dp))
+    return out
+
+
+def vjp(name, p, w, nin, *extra):
+    d = np.zeros(nin)
+    getattr(LIB, name)(arr(p).ctypes.data_as(dp), *extra, arr(w).ctypes.data_as(dp),
+                       d.ctypes.data_as(dp))
+    return d
+
+
+def check(f, g, x, active, nout, rng, eps=1e-6, tol=1e-6, trials=5):
+    """f(x) -> outputs; g(x, w) -> adjoints of the active inputs."""
+    worst = 0.0
+    for _ in range(trials):
+        w = rng.standard_normal(nout)
+        v = np.zeros_like(x)
+        v[active] = rng.standard_normal(len(active))
+        scale = np.maximum(np.abs(x), 1.0)
+        vx = v * scale
+        fd = (w @ f(x + eps * vx) - w @ f(x - eps * vx)) / (2 * eps)
+        ad = g(x, w) @ vx[active]
+        err = abs(fd - ad) / max(abs(fd), abs(ad), 1e-12)
+        worst = max(worst, err)
+    return worst
+
+
+def test_lr_state(rng=np.random.default_rng(0)):
+    for _ in range(20):
+        h = rng.uniform(0.05, 3.0)
+        x = np.array([h, rng.normal(0, 1), rng.normal(0, 1), rng.uniform(-1, 1), 1.0, 0.6, 0.8])
+        e = check(lambda p: fwd("sw2dt_lr_fwd", p, 5, C.c_double(HMIN)),
+                  lambda p, w: vjp("sw2dt_lr_vjp", p, w, 3, C.c_double(HMIN)),
+                  x, [0, 1, 2], 5, rng)
+        assert e < 1e-6, e
+
+
+def _hllc_point(rng, regime="both"):
+    bL = bR = 1.0
+    zbL, zbR = rng.uniform(0, 0.5, 2)
+    hL, hR = rng.uniform(0.3, 2.0, 2)
+    if regime == "rhs":
+        hL = 0.0
+        zbL = zbR + hR + 0.5
+    phiL, phiR = rng.uniform(0.4, 1.0, 2)
+    qL, rL, qR, rR = rng.normal(0, 0.5, 4)
+    lam = lambda h, q: (q / h - np.sqrt(9.81 * h), q / h + np.sqrt(9.81 * h)) if h > 0 else (0.0, 0.0)
+    lmL, lpL = lam(hL, qL)
+    lmR, lpR = lam(hR, qR)
+    return np.array([bL, bR, phiL, phiR, zbL, zbR, zbL + hL, qL, rL, zbR + hR, qR, rR, lmL, lmR, lpL, lpR])
+
+
+@pytest.mark.parametrize("regime", ["both", "rhs"])
+def test_hllc(regime):
+    rng = np.random.default_rng(1)
+    active = [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 2, 3]  # matches the adjoint's d[0..11] order
+    for _ in range(30):
+        x = _hllc_point(rng, regime)
+        f = lambda p: fwd("sw2dt_hllc_fwd", p, 5, C.c_double(HMIN), C.c_double(PHIMIN))
+        g = lambda p, w: vjp("sw2dt_hllc_vjp", p, w, 12, C.c_double(HMIN), C.c_double(PHIMIN))
+        assert check(f, g, x, active, 5, rng) < 2e-6
+
+
+def test_hllc_planted_error_is_detected():
+    """Drop the porosity derivative of the smoothed min: the check must fail."""
+    rng = np.random.default_rng(2)
+    x = _hllc_point(rng)
+    active = [6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 2, 3]
+    f = lambda p: fwd("sw2dt_hllc_fwd", p, 5, C.c_double(HMIN), C.c_double(PHIMIN))
+
+    def g_bad(p, w):
+        d = vjp("sw2dt_hllc_vjp", p, w, 12, C.c_double(HMIN), C.c_double(PHIMIN))
+        d[10] = 0.0
+        return d
+
+    assert check(f, g_bad, x, active, 5, rng) > 1e-3
+
+
+@pytest.mark.parametrize("btype,bc,h", [(3, 0.8, 0.02), (3, 0.8, 1.5), (0, 0.65, 1.2), (1, 1.7, 1.2),
+                                        (2, 2.4, 1.0), (1, 1.1, 0.0)])
+def test_boundary_flux(btype, bc, h):
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        zb = rng.uniform(0, 0.3)
+        q, r = rng.normal(0, 0.3, 2)
+        u = q / h if h > HMIN else 0.0
+        lp = u + np.sqrt(9.81 * h)
+        x = np.array([1.0, rng.uniform(0.5, 1.0), zb, zb + h, q, r, lp, bc])
+        f = lambda p: fwd("sw2dt_bnd_fwd", p, 3, C.c_int(btype), C.c_double(HMIN), C.c_double(PHIMIN))
+        g = lambda p, w: vjp("sw2dt_bnd_vjp", p, w, 6, C.c_int(btype), C.c_double(HMIN), C.c_double(PHIMIN))
+        active = [3, 4, 5, 6, 1, 7] if h > 0 else [4, 5, 6, 1, 7]
+        if h == 0:
+            g2 = lambda p, w: g(p, w)[[1, 2, 3, 4, 5]]
+            assert check(f, g2, x, active, 3, rng, eps=1e-7) < 2e-6
+        else:
+            assert check(f, g, x, active, 3, rng) < 2e-6
+
+
+def test_friction():
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        a = rng.uniform(-0.5, 0.5)
+        x = np.array([rng.uniform(0.1, 3), rng.normal(0, 1), rng.normal(0, 1), np.cos(a), np.sin(a),
+                      rng.uniform(15, 60), rng.uniform(15, 60), rng.uniform(0.01, 1.0)])
+        f = lambda p: fwd("sw2dt_fric_fwd", p, 2, C.c_double(HMIN))
+        g = lambda p, w: vjp("sw2dt_fric_vjp", p, w, 5, C.c_double(HMIN))
+        assert check(f, g, x, [0, 1, 2, 5, 6], 2, rng) < 2e-6
+
+
+def test_head_loss():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        a = rng.uniform(-0.5, 0.5)
+        x = np.array([rng.uniform(0.2, 3), rng.normal(0, 1), rng.normal(0, 1), np.cos(a), np.sin(a),
+                      0.05, rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0.01, 0.5)])
+        f = lambda p: fwd("sw2dt_hl_fwd", p, 2, C.c_double(HMIN))
+        g = lambda p, w: vjp("sw2dt_hl_vjp", p, w, 3, C.c_double(HMIN))
+        assert check(f, g, x, [0, 1, 2], 2, rng) < 2e-6
