@@ -1,17 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark of the SP hot path on one MI355X.
+"""Benchmark of the SP hot path on one MI355X: forward + adjoint.
 
-Workload: SP forward at scale -- 5 x 5 km jittered-lattice Delaunay mesh
-(707 x 707 lattice -> 999,698 triangles, 1,498,133 internal edges), bed slope
-1e-3 + 20 seeded Gaussian bumps, porosity in [0.3, 1], partially dry start
-(~40 % wet), triangular west inflow, Manning 0.03, MUSCL-EVR, FP64.  One
-"step" is one time step of the whole mesh (swSp::execute loop body).  The
-inputs are built in memory, uploaded once, and are resident in HBM when the
-timed region starts; the timed region runs K steps through the HIP-graph
-path and ends with a stream synchronisation.
+Workload (the calibration gradient on the 1M mesh): 5 x 5 km jittered-lattice
+Delaunay mesh (707 x 707 lattice -> 999,698 triangles), bed slope 1e-3 + 20
+seeded Gaussian bumps, porosity in [0.3, 1], partially dry start (~40 %
+wet), Godunov + HLLC, Strickler-law friction of the differentiable forward
+with 8 Manning zones (k-means of the centroids), a 60-knot west inflow
+hydrograph and 20 depth gauges observed every 30 s over a 90 s window
+(~500 time steps).  The observations come from a twin run at seeded "true"
+controls; the gradient is taken at the default controls (n = 0.03, the
+unperturbed hydrograph).
 
-Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
-runs on one GPU.
+One "step" of this benchmark is one gradient evaluation: the dry run that
+fixes the time-step schedule (with sqrt(N) checkpoints), the forward
+recomputation of each segment and the reverse sweep -- i.e. one forward and
+one adjoint pass over every time step of the window.  `value` counts
+fwd+adj cell-steps: n_cell x time steps x gradient evaluations / wall time.
+All inputs are resident in HBM when the timed region starts; each call
+copies the 3 x n_cell control arrays in and the gradients out (<0.1 % of
+the time).
+
+The native (non-differentiable) MUSCL-EVR forward on the same mesh is timed
+as well and reported under "forward".
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
 import importlib.util
@@ -20,9 +32,12 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(REPO, "inria-internship-project-sw2d-enzyme_amd")
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E vendor peak
+METRIC = "FP64 cell-steps/s fwd+adj (%HBM roofline); 4D-Var time-to-misfit learned vs plain"
 
 
 def load_pkg():
@@ -34,112 +49,240 @@ def load_pkg():
     return m
 
 
-def algorithmic_bytes(n_cell, n_int, n_bnd, scheme):
-    """Compulsory HBM bytes of one forward step (inputs read once, outputs
-    written once, no intermediates):
-      Godunov   80 Nc + 32 Ei + 32 Eb
-      MUSCL-EVR 96 Nc + 48 Ei + 32 Eb
-    per cell: state in 24 + state out 24 + {zb, A, phi, n} 32 (+ centroid 16);
-    per internal edge: 2 int32 cells + {nx, ny, L} 24 (+ midpoint 16);
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def pass_bytes(nc, ni, nb, scheme):
+    """Compulsory HBM bytes of one time step (inputs read once, outputs
+    written once, no intermediates).
+      forward  Godunov   80 Nc + 32 Ei + 32 Eb,  MUSCL-EVR 96 Nc + 48 Ei + 32 Eb
+      reverse  Godunov  120 Nc + 32 Ei + 32 Eb,  MUSCL-EVR 120 Nc + 48 Ei + 32 Eb
+    per cell: state 24 in + 24 out + {zb, A, phi, n} 32 (+ centroid 16, MUSCL);
+    reverse per cell: primal state 24 + statics 48 + adjoint 24 in + 24 out;
+    per internal edge: 2 int32 cells + {nx, ny, L} 24 (+ midpoint 16, MUSCL);
     per boundary edge: int32 cell + int32 BC + {nx, ny, L} 24."""
     if scheme == "muscl_evr":
-        return 96 * n_cell + 48 * n_int + 32 * n_bnd
-    return 80 * n_cell + 32 * n_int + 32 * n_bnd
+        return 96 * nc + 48 * ni + 32 * nb, 120 * nc + 48 * ni + 32 * nb
+    return 80 * nc + 32 * ni + 32 * nb, 120 * nc + 32 * ni + 32 * nb
+
+
+def kernel_bytes(name, nc, ni, nb, n_gauges):
+    """Compulsory bytes of one launch of each kernel of the gradient (every
+    array element the launch touches, read once / written once; int32
+    indices, FP64 values; friction on, head loss off)."""
+    table = {
+        # cells U, zb, beta, phi; edges L, R, nx, ny; F out (+ length, swept out)
+        "k_e_edges": 48 * nc + 64 * ni,
+        # cells U, zb, beta, area, CSR ptrs; incidences + swept; boundary edges
+        "k_e_cfl": 56 * nc + 24 * ni + 28 * nb,
+        "k_e_dt": 8 * ((nc + 255) // 256),
+        # cells U in/out, phi, area, zb, beta, K1, K2, cos, sin, ptrs; incidences,
+        # F, edge geometry; boundary edges
+        "k_e_update": 120 * nc + 72 * ni + 32 * nb,
+        # + lam in, lams/part out, gphi/gK1/gK2 read-modify-write, gbce out
+        "k_r_cell": 216 * nc + 72 * ni + 40 * nb,
+        # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, L; sides out
+        "k_r_edge": 80 * nc + 96 * ni,
+        # cells part, ptrs, lam out, gphi rmw; incidences + sides
+        "k_r_gather": 72 * nc + 72 * ni,
+        "k_gauge_sample": 16 * n_gauges,
+        "k_cost_gauges": 32 * n_gauges,
+        "k_seed_gauges": 48 * n_gauges,
+        "k_r_bc_reduce": 12 * nb,
+        "k_cost_total": 64,
+    }
+    return table.get(name)
+
+
+def fwd_adj_bench(sw, args):
+    from sw2d_amd import assim, calibration
+    t0 = time.perf_counter()
+    cal = calibration.Calibration(px=args.px, window=args.window)
+    case = cal.case
+    nc, ni, nb = case.n_cell, case.n_int, case.n_bnd
+    log(f"calibration mesh {nc} cells, {ni} internal edges, {nb} boundary edges "
+        f"(built in {time.perf_counter() - t0:.1f} s)")
+    s = sw.SpSolver(case, device=0)
+    # twin run at the true controls -> gauge observations
+    K_true = cal.K(cal.n_true)
+    cal.apply_hydrograph(s, cal.Q_true)
+    ng, no = len(cal.gauge_cells), len(cal.obs_t)
+    T = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=np.zeros((no, ng)),
+                    max_steps=args.max_steps)
+    T.record(K1=K_true, K2=K_true)
+    obs = T.gauge_series()
+    # gradient at the default controls
+    cal.apply_hydrograph(s, cal.Q_default)
+    K0 = cal.K(cal.n_default)
+    A = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=obs,
+                    max_steps=args.max_steps)
+    for _ in range(args.warmup):
+        G = A.gradient(K1=K0, K2=K0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        G = A.gradient(K1=K0, K2=K0)  # returns after a stream synchronisation
+    elapsed = time.perf_counter() - t0
+    n_steps = G["n_steps"]
+    rec = A.get_record()
+    gn, gQ = cal.grad_controls(G, cal.n_default, rec["t"][:-1])
+    log(f"{args.steps} gradients x {n_steps} time steps in {elapsed:.3f} s; J = {G['J']:.6e}, "
+        f"|dJ/dn| = {np.linalg.norm(gn):.3e}, |dJ/dQ| = {np.linalg.norm(gQ):.3e}")
+    prof = A.profile(K1=K0, K2=K0)
+    out = dict(cal=cal, case=case, solver=s, assim=A, n_steps=n_steps, elapsed=elapsed,
+               J=G["J"], prof=prof, rec=rec, obs=obs, K0=K0, gn=gn, gQ=gQ,
+               stride=int(np.ceil(np.sqrt(args.max_steps))))
+    return out
+
+
+def roofline(res, args):
+    case = res["case"]
+    nc, ni, nb = case.n_cell, case.n_int, case.n_bnd
+    n_steps = res["n_steps"]
+    ng = len(res["cal"].gauge_cells)
+    kernels = {}
+    total_ms = 0.0
+    for name, (avg, cnt) in res["prof"].items():
+        total_ms += avg * cnt
+        b = kernel_bytes(name, nc, ni, nb, ng)
+        kernels[name] = {"avg_ms": avg, "launches": cnt, "bytes_per_launch": b,
+                         "gbs": (b / (avg * 1e-3) / 1e9) if (b and avg > 0) else None}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    d = kernels[dom]
+    achieved = d["gbs"]
+    Bf, Ba = pass_bytes(nc, ni, nb, "godunov")
+    step_ms = total_ms / n_steps  # device time per time step (fwd + recompute + adj)
+    step_gbs = (Bf + Ba) / (step_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic_fwd_adj_1m.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "bytes_per_launch": d["bytes_per_launch"],
+        "avg_launch_ms": d["avg_ms"],
+        "step": {
+            "algorithmic_bytes_fwd_adj": Bf + Ba,
+            "device_ms_per_time_step": step_ms,
+            "achieved": step_gbs,
+            "frac": step_gbs / HBM_PEAK_GBS,
+            "note": "algorithmic fwd+adj bytes of one time step over the device time of all "
+                    "kernels of one time step (dry run + recomputation + reverse)",
+        },
+    }, kernels
+
+
+def forward_bench(sw, args):
+    case, info = sw.scale_case(None, px=args.px, py=args.px, L=5000.0, seed=1234, t_max=3600.0,
+                               scheme="muscl_evr")
+    s = sw.SpSolver(case, device=0)
+    s.run(max_steps=64)
+    t0 = time.perf_counter()
+    n = s.run(max_steps=args.fwd_steps)
+    el = time.perf_counter() - t0
+    prof = s.profile(64)
+    Bf, _ = pass_bytes(case.n_cell, case.n_int, case.n_bnd, "muscl_evr")
+    step_ms = sum(prof.values())
+    s.close()
+    log(f"native MUSCL forward: {n} steps in {el:.3f} s")
+    return {"workload": "SP forward at scale, MUSCL-EVR + Manning 0.03, native semantics",
+            "value": case.n_cell * n / el, "unit": "cell-steps/s", "steps": n,
+            "ms_per_step": el * 1e3 / n, "kernels_ms": prof,
+            "roofline_step": {"algorithmic_bytes": Bf, "device_ms": step_ms,
+                              "achieved": Bf / (step_ms * 1e-3) / 1e9,
+                              "frac": Bf / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+
+def cpu_baseline(res, args):
+    """The oracle's _E dry run + frozen replay + reverse sweep on the first
+    cpu_steps time steps of the same workload, one host thread."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    case = res["case"]
+    cal = res["cal"]
+    E = O.OracleE(case)
+    m = args.cpu_steps
+    t_m = float(res["rec"]["t"][m])
+    K0 = res["K0"]
+    obs = res["obs"][:1]
+    tc = time.perf_counter()
+    rec = E.record([t_m], K1=K0, K2=K0, max_steps=m)
+    g = E.gradient(rec, K1=K0, K2=K0, gauges=cal.gauge_cells, obs=obs, n_obs=1)
+    tc = time.perf_counter() - tc
+    n = len(rec["dt"])
+    return {
+        "value": case.n_cell * n / tc,
+        "unit": "fwd+adj cell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {n} time steps of the same {case.n_cell}-cell gradient (C oracle: _E dry run "
+                  f"+ replay + reverse sweep, all states kept, no recomputation), 1 thread, "
+                  f"{tc:.1f} s; J = {g['J']:.6e}",
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=3, help="timed gradient evaluations")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--px", type=int, default=707, help="lattice size (707 -> ~1e6 cells)")
-    ap.add_argument("--profile-steps", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=60, help="oracle sample (steps of the same mesh)")
+    ap.add_argument("--window", type=float, default=90.0, help="assimilation window (s)")
+    ap.add_argument("--max-steps", type=int, default=2000)
+    ap.add_argument("--fwd-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forward", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-
     sw = load_pkg()
-    t_gen = time.perf_counter()
-    case, info = sw.scale_case(None, px=args.px, py=args.px, L=5000.0, seed=1234, t_max=3600.0,
-                               scheme="muscl_evr")
-    t_gen = time.perf_counter() - t_gen
-    print(f"[bench] mesh {info['n_cell']} cells, {info['n_int']} internal edges "
-          f"(built in {t_gen:.1f} s)", file=sys.stderr, flush=True)
-    s = sw.SpSolver(case, device=0)
-    s.run(max_steps=args.warmup)
-    c0 = s.clock()
-    t0 = time.perf_counter()
-    n = s.run(max_steps=args.steps)  # ends with hipStreamSynchronize
-    elapsed = time.perf_counter() - t0
-    c1 = s.clock()
-    print(f"[bench] {n} steps in {elapsed:.3f} s, t {c0['t']:.2f} -> {c1['t']:.2f}",
-          file=sys.stderr, flush=True)
-    nc, ni, nb = case.n_cell, case.n_int, case.n_bnd
-    value = nc * n / elapsed
-    # per-kernel device durations, HIP events on the solver's stream
-    prof = s.profile(args.profile_steps)
-    step_ms_events = sum(prof.values())
-    B = algorithmic_bytes(nc, ni, nb, "muscl_evr")
-    achieved = B / (step_ms_events * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_sp_step_1m.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_step")
-    dominant = max(prof, key=prof.get)
+    res = fwd_adj_bench(sw, args)
+    roof, kernels = roofline(res, args)
+    case = res["case"]
+    nc = case.n_cell
+    n_steps = res["n_steps"]
+    el = res["elapsed"]
     line = {
-        "metric": "FP64 cell-steps/s fwd+adj (%HBM roofline); 4D-Var time-to-misfit learned vs plain",
-        "value": value,
-        "unit": "cell-steps/s (forward)",
+        "metric": METRIC,
+        "value": nc * n_steps * args.steps / el,
+        "unit": "fwd+adj cell-steps/s",
         "n_gpus": 1,
-        "steps": n,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / max(n, 1),
+        "ms_per_step": el * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: seeded 1M-cell jittered-lattice Delaunay mesh, built in memory",
+        "data": "synthetic: seeded 1M-cell jittered-lattice Delaunay mesh built in memory; "
+                "gauge observations from a twin run at seeded true controls",
         "config": {
-            "workload": "SP forward at scale (5x5 km, 999,698-cell Delaunay, MUSCL-EVR, Manning 0.03, "
-                        "wet/dry start, west inflow); forward step only",
-            "n_cell": nc, "n_int": ni, "n_bnd": nb, "scheme": "muscl_evr",
-            "topo": "discontinuous", "sim_time_s": [c0["t"], c1["t"]],
+            "workload": "adjoint gradient on the 1M mesh: 8 Manning zones + 60-knot inflow "
+                        "hydrograph (68 controls), 20 gauges every 30 s, 90 s window, "
+                        "sqrt(N) checkpointing, Godunov/HLLC _E forward",
+            "n_cell": nc, "n_int": case.n_int, "n_bnd": case.n_bnd,
+            "time_steps_per_gradient": n_steps,
+            "checkpoint_stride": res["stride"],
+            "cost_J": res["J"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "SP step = " + " + ".join(prof),
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_step": B,
-            "step_ms_events": step_ms_events,
-        },
-        "kernels_ms": prof,
-        "dominant_kernel": dominant,
+        "roofline": roof,
+        "kernels": kernels,
     }
+    res["solver"].close()
+    if not args.no_forward:
+        line["forward"] = forward_bench(sw, args)
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(REPO, "oracle"))
-        import oracle as O
-        o = O.OracleSolver(case)
-        tc = time.perf_counter()
-        done = o.step(args.cpu_steps)
-        tc = time.perf_counter() - tc
-        line["cpu_baseline"] = {
-            "value": nc * done / tc,
-            "unit": "cell-steps/s (forward)",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"first {done} steps of the same {nc}-cell workload, C oracle "
-                      f"(bit-identical to the compiled reference on the golden cases), 1 thread, {tc:.1f} s",
-        }
-    s.close()
+        line["cpu_baseline"] = cpu_baseline(res, args)
     print(json.dumps(line))
 
 

@@ -134,7 +134,8 @@ int sw2dg_assim_setup(sw2dg_ctx *ctx, const sw2dg_obs *obs, int32_t stride, int3
  * (build_checkpoint_records, swSpE_checkpoint.cpp:97-455): fixes dt, the
  * per-series boundary values and observation triggers of every step and
  * keeps the checkpoints.  Controls in reference order; NULL = the ctx's
- * porosity / friction maps. */
+ * porosity / friction maps.  The run stops at t_max or after max_steps
+ * steps (the window then ends at t_seq[n_steps] < t_max). */
 int sw2dg_assim_record(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
                        int64_t *n_steps);
 /* The recorded schedule: t_seq [n+1], dt_seq [n], bc_seq [n*n_series], obs_trig [n] (-1 = none). */
@@ -153,6 +154,18 @@ int sw2dg_assim_cost(sw2dg_ctx *ctx, const double *phi, const double *K1, const 
 int sw2dg_assim_gradient(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
                          double *J, double *dJ_dphi, double *dJ_dK1, double *dJ_dK2,
                          double *dJ_dbc, int64_t *n_steps);
+/* Depths at the gauge cells sampled by the last dry run at each observation
+ * step, [n_obs * n_gauges] (0 for observations the run did not reach): the
+ * series a twin experiment observes.  No reference counterpart (the
+ * reference reads observations from files, swSpE_ad.cpp:47-154). */
+int sw2dg_assim_gauge_series(sw2dg_ctx *ctx, double *out);
+/* One sw2dg_assim_gradient at the given controls with an event pair around
+ * every kernel launch on the ctx stream: average device duration (ms) and
+ * launch count per kernel (names are static strings), up to k_max entries.
+ * Counterpart of the reference's per-phase timers (swSp.cpp:652-660). */
+int sw2dg_assim_profile(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
+                        int32_t k_max, double *avg_ms, int64_t *launches, const char **names,
+                        int32_t *n_kernels);
 /* State at the end of the recorded run (reference order). */
 int sw2dg_assim_final_state(sw2dg_ctx *ctx, double *h, double *qx, double *qy);
 

@@ -39,6 +39,9 @@ _DECL = {
     "sw2dg_assim_gradient": ([C.c_void_p, N._d, N._d, N._d, N._d, N._d, N._d, N._d, N._d, N._i64],
                              C.c_int),
     "sw2dg_assim_final_state": ([C.c_void_p, N._d, N._d, N._d], C.c_int),
+    "sw2dg_assim_gauge_series": ([C.c_void_p, N._d], C.c_int),
+    "sw2dg_assim_profile": ([C.c_void_p, N._d, N._d, N._d, C.c_int32, N._d, N._i64,
+                             C.POINTER(C.c_char_p), C.POINTER(C.c_int32)], C.c_int),
 }
 ASSIM_SYMBOLS = tuple(_DECL)
 
@@ -139,6 +142,24 @@ class Assim:
         self.n_steps = n.value
         return {"J": J.value, "phi": gphi, "K1": gK1, "K2": gK2,
                 "bc": gbc[: n.value * ns].reshape(n.value, ns), "n_steps": n.value}
+
+    def profile(self, phi=None, K1=None, K2=None, k_max=32):
+        """One gradient with per-launch HIP events: {kernel: (avg_ms, launches)}."""
+        avg = np.zeros(k_max)
+        cnt = np.zeros(k_max, dtype=np.int64)
+        names = (C.c_char_p * k_max)()
+        nk = C.c_int32()
+        self.s._check(self.lib.sw2dg_assim_profile(self.s.ctx, self._ctl(phi), self._ctl(K1),
+                                                   self._ctl(K2), k_max, N.dptr(avg),
+                                                   N.i64ptr(cnt), names, C.byref(nk)))
+        return {names[k].decode(): (float(avg[k]), int(cnt[k])) for k in range(min(nk.value, k_max))}
+
+    def gauge_series(self):
+        """[n_obs, n_gauges] depths at the gauge cells sampled by the last dry run."""
+        ng = self.obs.n_gauges
+        out = np.zeros(max(self.n_obs * ng, 1))
+        self.s._check(self.lib.sw2dg_assim_gauge_series(self.s.ctx, N.dptr(out)))
+        return out[: self.n_obs * ng].reshape(self.n_obs, ng)
 
     def final_state(self):
         nc = self.s.case.n_cell

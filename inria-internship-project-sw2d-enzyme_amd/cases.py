@@ -200,7 +200,7 @@ def scale_case(folder, px=500, py=500, L=5000.0, seed=1234, t_max=3600.0, scheme
     nc, nb, ni, nn = g.sizes()
     info = {"n_cell": nc, "n_bnd": nb, "n_int": ni, "n_node": nn,
             "flips": getattr(g, "n_flips", 0), "west_len": west_len, "z0": z0,
-            "wet_fraction": float(np.mean(h0 > 0))}
+            "wet_fraction": float(np.mean(h0 > 0)), "geometry": g}
     bc_v = np.stack([np.zeros_like(tq), qunit])
     if folder is None:
         # same inputs, kept in memory (a written case reads back bit-identical: %.17e)

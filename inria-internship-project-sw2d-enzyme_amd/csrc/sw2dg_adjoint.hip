@@ -39,7 +39,6 @@ namespace sw2dg_detail {
 
 struct EClock {
     int oi;                  // next observation index (dry run)
-    unsigned int counter;    // last-block detection
     int n_obs;
     int n_series;
     double t_max, dt_max, cfl;
@@ -57,6 +56,7 @@ struct AssimState {
     double *d_obs_t = nullptr;
     int *d_gcell = nullptr;
     double *d_gw = nullptr, *d_gobs = nullptr;
+    double *d_gser = nullptr;  // [n_obs][n_gauges] depths sampled by the last dry run
     double *d_fh = nullptr, *d_fqx = nullptr, *d_fqy = nullptr, *d_wh = nullptr, *d_wq = nullptr;
     // schedule
     int stride = 0, n_tiles = 1;
@@ -74,7 +74,7 @@ struct AssimState {
     double *d_F = nullptr;     // 5 n_int
     double *d_sw = nullptr;    // 2 n_int
     double *d_partial = nullptr;
-    double *d_J = nullptr;     // [2]: cost accumulator, scratch
+    double *d_J = nullptr;     // [1 + n_obs]: total, then the cost of each observation
     double *d_cpart = nullptr; // frame-cost block partials
     // adjoint
     double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
@@ -84,6 +84,12 @@ struct AssimState {
     double *d_phi = nullptr, *d_K1 = nullptr, *d_K2 = nullptr;
     // series -> boundary edges (renumbered) in reference order
     int *d_ser_ptr = nullptr, *d_ser_edges = nullptr;
+    // optional per-launch timing (sw2dg_assim_profile): an event pair per launch
+    hipStream_t stream = nullptr;
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    std::vector<std::pair<const char *, size_t>> marks;
     std::vector<void *> allocs;
     template <class T>
     T *alloc(size_t n) {
@@ -96,10 +102,36 @@ struct AssimState {
     ~AssimState() {
         for (void *p : allocs)
             if (p) (void)hipFree(p);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
 };
 
 void assim_destroy(AssimState *a) { delete a; }
+
+// event pair around one launch when profiling; -1 otherwise
+inline long prof_begin(AssimState *a) {
+    if (!a->prof) return -1;
+    while (a->ev.size() < a->ev_used + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        a->ev.push_back(e);
+    }
+    const size_t i = a->ev_used;
+    a->ev_used += 2;
+    (void)hipEventRecord(a->ev[i], a->stream);
+    return (long)i;
+}
+inline void prof_end(AssimState *a, long i, const char *name) {
+    if (i < 0) return;
+    (void)hipEventRecord(a->ev[i + 1], a->stream);
+    a->marks.emplace_back(name, (size_t)i);
+}
+#define KLAUNCH(A, KERN, ...)                       \
+    do {                                            \
+        const long pk_ = prof_begin(A);             \
+        hipLaunchKernelGGL(KERN, __VA_ARGS__);      \
+        prof_end(A, pk_, #KERN);                    \
+    } while (0)
 
 }  // namespace sw2dg_detail
 
@@ -167,13 +199,10 @@ __device__ __forceinline__ double wave_min(double v) {
 // dry-run CFL (hypComputeTimeStep_E, swSpE_hyp.cpp:578-635) and the step
 // schedule of build_checkpoint_records (swSpE_checkpoint.cpp:224-366)
 __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const double *__restrict__ U,
-                                                  const double *__restrict__ sw, BcDev bc,
-                                                  EClock *ck, double *partial, double *T, double *dtv,
-                                                  double *bcv, int *trig, const double *obs_t,
-                                                  int64_t n) {
+                                                  const double *__restrict__ sw, const EClock *ck,
+                                                  double *partial, const double *T, int64_t n) {
     if (!(T[n] < ck->t_max)) return;
     __shared__ double red[kBlock / 64];
-    __shared__ int is_last;
     const int lb = xcd_block();
     const int i = lb * kBlock + (int)threadIdx.x;
     const double dt_max = ck->dt_max;
@@ -207,24 +236,31 @@ __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const doubl
         double v = red[0];
         for (int w = 1; w < kBlock / 64; ++w) v = (red[w] < v) ? red[w] : v;
         partial[lb] = v;
-        __threadfence();
-        is_last = (atomicAdd(&ck->counter, 1u) == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// one block: min of the per-block candidates, then the step schedule
+__global__ __launch_bounds__(kBlock) void k_e_dt(BcDev bc, EClock *ck, const double *partial,
+                                                 int n_part, double *T, double *dtv, double *bcv,
+                                                 int *trig, const double *obs_t, int64_t n) {
+    if (!(T[n] < ck->t_max)) {  // past t_max: no step, no observation
+        if (threadIdx.x == 0) trig[n] = -1;
+        return;
+    }
+    __shared__ double red[kBlock / 64];
+    const double dt_max = ck->dt_max;
     double v = dt_max;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) {
+    for (int k = threadIdx.x; k < n_part; k += kBlock) {
         const double x = partial[k];
         v = (x < v) ? x : v;
     }
     v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     if (threadIdx.x != 0) return;
     double dtc = red[0];
     for (int w = 1; w < kBlock / 64; ++w) dtc = (red[w] < dtc) ? red[w] : dtc;
-    ck->counter = 0u;
     const double t = T[n];
     const double t_max = ck->t_max;
     if ((t + dtc) > t_max) dtc = t_max - t;
@@ -331,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls
 }
 
 // gauge cost (compute_cost_gauges_ptr, swSpE_gauges.cpp:166-179): one thread,
-// fixed order.  J[0] += sum_g w_g (h - obs)^2 / norm
+// fixed order.  Jo[0] += sum_g w_g (h - obs)^2 / norm (Jo = slot of the observation)
 __global__ void k_cost_gauges(const double *__restrict__ U, const int *gcell, const double *gw,
                               const double *gobs, int n_g, double norm, double *J) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -341,6 +377,15 @@ __global__ void k_cost_gauges(const double *__restrict__ U, const int *gcell, co
         Jo += gw[g] * d * d;
     }
     J[0] += Jo / norm;
+}
+
+// total cost: observation costs summed in observation (= time) order, so the
+// result does not depend on the order segments were replayed in
+__global__ void k_cost_total(double *J, int n_obs) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < n_obs; ++k) s += J[1 + k];
+    J[0] = s;
 }
 
 // frame cost (compute_cost_frame_ptr, swSpE_ad.cpp:273-308): per-block partial
@@ -370,6 +415,15 @@ __global__ void k_cost_frame_sum(const double *part, int nparts, double norm, do
     double s = 0.0;
     for (int k = 0; k < nparts; ++k) s += part[k];
     J[0] += s / norm;
+}
+
+// gauge sampling of the dry run: depth at the gauge cells after a step that
+// reaches an observation time (the series a twin experiment observes)
+__global__ void k_gauge_sample(const double *__restrict__ U, const int *gcell, int n_g,
+                               const int *trig, int64_t n, double *gser) {
+    const int oi = trig[n];
+    if (oi < 0) return;
+    for (int g = threadIdx.x; g < n_g; g += blockDim.x) gser[(size_t)oi * n_g + g] = U[gcell[g]];
 }
 
 // ---------------------------------------------------------------------------
@@ -606,40 +660,45 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
                    bool dry) {
     const dim3 blk(kBlock);
     const Controls cs = ctl(a);
-    hipLaunchKernelGGL(k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Uin,
+    KLAUNCH(a, k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Uin,
                        a->d_F, dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max);
-    if (dry)
-        hipLaunchKernelGGL(k_e_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
-                           c->bc, a->d_ck, a->d_partial, a->d_T, a->d_dt, a->d_bc, a->d_trig,
-                           a->d_obs_t, n);
-    hipLaunchKernelGGL(k_e_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin,
+    if (dry) {
+        KLAUNCH(a, k_e_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
+                a->d_ck, a->d_partial, a->d_T, n);
+        KLAUNCH(a, k_e_dt, dim3(1), blk, 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
+                a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
+    }
+    KLAUNCH(a, k_e_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin,
                        Uout, a->d_F, a->d_dt, a->d_bc, std::max(c->n_series, 1),
                        dry ? a->d_T : nullptr, n, c->prm.t_max);
+    if (dry && a->n_gauges > 0)
+        KLAUNCH(a, k_gauge_sample, dim3(1), dim3(64), 0, c->stream, Uout, a->d_gcell,
+                           a->n_gauges, a->d_trig, n, a->d_gser);
 }
 
 void launch_cost(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
     const double norm = (a->normalize && a->n_obs > 0) ? (double)a->n_obs : 1.0;
     if (a->n_gauges > 0)
-        hipLaunchKernelGGL(k_cost_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
-                           a->d_gobs + (size_t)oi * a->n_gauges, a->n_gauges, norm, a->d_J);
+        KLAUNCH(a, k_cost_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
+                           a->d_gobs + (size_t)oi * a->n_gauges, a->n_gauges, norm, a->d_J + 1 + oi);
     if (a->frames) {
         const size_t o = (size_t)oi * c->nc;
         const int nb = nblocks(c->nc);
-        hipLaunchKernelGGL(k_cost_frame_part, dim3(nb), dim3(kBlock), 0, c->stream, U, (int)c->nc,
+        KLAUNCH(a, k_cost_frame_part, dim3(nb), dim3(kBlock), 0, c->stream, U, (int)c->nc,
                            a->d_fh + o, a->d_fqx + o, a->d_fqy + o, a->d_wh, a->d_wq, a->d_cpart);
-        hipLaunchKernelGGL(k_cost_frame_sum, dim3(1), dim3(64), 0, c->stream, a->d_cpart, nb, norm,
-                           a->d_J);
+        KLAUNCH(a, k_cost_frame_sum, dim3(1), dim3(64), 0, c->stream, a->d_cpart, nb, norm,
+                           a->d_J + 1 + oi);
     }
 }
 
 void launch_seed(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
     const double norm = (a->normalize && a->n_obs > 0) ? (double)a->n_obs : 1.0;
     if (a->n_gauges > 0)
-        hipLaunchKernelGGL(k_seed_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
+        KLAUNCH(a, k_seed_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
                            a->d_gobs + (size_t)oi * a->n_gauges, a->n_gauges, norm, a->d_lam);
     if (a->frames) {
         const size_t o = (size_t)oi * c->nc;
-        hipLaunchKernelGGL(k_seed_frame, dim3(nblocks(c->nc)), dim3(kBlock), 0, c->stream, U,
+        KLAUNCH(a, k_seed_frame, dim3(nblocks(c->nc)), dim3(kBlock), 0, c->stream, U,
                            (int)c->nc, a->d_fh + o, a->d_fqx + o, a->d_fqy + o, a->d_wh, a->d_wq,
                            norm, a->d_lam);
     }
@@ -725,6 +784,7 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     c->assim = nullptr;
     auto *a = new AssimState();
     c->assim = a;
+    a->stream = c->stream;
     const int64_t nc = c->nc, ni = c->ni, nb = c->nb;
     const int ns = std::max(c->n_series, 1);
     a->n_obs = obs->n_obs;
@@ -746,6 +806,7 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
         a->d_gcell = a->alloc<int>(obs->n_gauges);
         a->d_gw = a->alloc<double>(obs->n_gauges);
         a->d_gobs = a->alloc<double>((size_t)obs->n_gauges * std::max(1, obs->n_obs));
+        a->d_gser = a->alloc<double>((size_t)obs->n_gauges * std::max(1, obs->n_obs));
         HIPCHK(c, hipMemcpy(a->d_gcell, gc.data(), gc.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(a->d_gw, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(a->d_gobs, obs->gauge_h,
@@ -787,7 +848,7 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_sw = a->alloc<double>(2 * (size_t)std::max<int64_t>(ni, 1));
     a->d_partial = a->alloc<double>(nblocks(nc));
     a->d_cpart = a->alloc<double>(nblocks(nc));
-    a->d_J = a->alloc<double>(2);
+    a->d_J = a->alloc<double>(1 + (size_t)std::max(1, obs->n_obs));
     a->d_lam = a->alloc<double>(3 * nc);
     a->d_lams = a->alloc<double>(3 * nc);
     a->d_part = a->alloc<double>(3 * nc);
@@ -834,7 +895,6 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
     HIPCHK(c, hipMemcpy(ckpt_state(a, c, 0), U0.data(), 3 * nc * sizeof(double), hipMemcpyHostToDevice));
     EClock ek{};
     ek.oi = 0;
-    ek.counter = 0;
     ek.n_obs = a->n_obs;
     ek.n_series = c->n_series;
     ek.t_max = c->prm.t_max;
@@ -842,6 +902,7 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
     ek.cfl = c->prm.cfl_max;
     HIPCHK(c, hipMemcpy(a->d_ck, &ek, sizeof(EClock), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(a->d_T, &c->t0, sizeof(double), hipMemcpyHostToDevice));
+    if (a->n_gauges > 0) zero(c, a->d_gser, (size_t)a->n_gauges * std::max(1, a->n_obs));
     const int64_t S = a->stride;
     int64_t n = 0;
     int64_t k = 0;
@@ -869,9 +930,7 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
     while (cnt < n && T[cnt] < c->prm.t_max) ++cnt;
     a->n_steps = cnt;
     a->recorded = true;
-    if (n_steps) *n_steps = cnt;
-    if (cnt >= a->max_steps && T[cnt] < c->prm.t_max)
-        return fail(c, SW2DG_ERR_ARG, "max_steps reached before t_max");
+    if (n_steps) *n_steps = cnt;  // < t_max reached when max_steps cut the window
     return SW2DG_OK;
 }
 
@@ -913,7 +972,7 @@ int sw2dg_assim_cost(sw2dg_ctx *c, const double *phi, const double *K1, const do
     std::vector<int> trig;
     if ((st = fetch_record(c, a, T, trig, dt))) return st;
     const Plan p = make_plan(a, T);
-    zero(c, a->d_J, 2);
+    zero(c, a->d_J, 1 + (size_t)std::max(1, a->n_obs));
     // tiles restart from their frozen checkpoints; segments inside a tile chain
     for (size_t ti = 0; ti < p.tile_start.size(); ++ti) {
         const int64_t s_t = p.tile_start[ti];
@@ -930,6 +989,7 @@ int sw2dg_assim_cost(sw2dg_ctx *c, const double *phi, const double *K1, const do
             if ((st = replay_segment(c, a, k, trig, true, start))) return st;
         }
     }
+    KLAUNCH(a, k_cost_total, dim3(1), dim3(64), 0, c->stream, a->d_J, a->n_obs);
     HIPCHK(c, hipMemcpyAsync(J, a->d_J, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SW2DG_OK;
@@ -951,7 +1011,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     const Plan p = make_plan(a, T);
     const int64_t nc = c->nc;
     const int ns = std::max(c->n_series, 1);
-    zero(c, a->d_J, 2);
+    zero(c, a->d_J, 1 + (size_t)std::max(1, a->n_obs));
     zero(c, a->d_lam, 3 * nc);
     zero(c, a->d_gphi, nc);
     zero(c, a->d_gK1, nc);
@@ -968,20 +1028,21 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             const int j = (int)(n - n0);
             const double *Un = seg_state(a, c, j);
             if (trig[n] >= 0) launch_seed(c, a, seg_state(a, c, j + 1), trig[n]);
-            hipLaunchKernelGGL(k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
+            KLAUNCH(a, k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
                                a->d_F, nullptr, nullptr, n, 0.0);
-            hipLaunchKernelGGL(k_r_cell, dim3(nblocks(nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
+            KLAUNCH(a, k_r_cell, dim3(nblocks(nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
                                Un, a->d_F, dt[n], a->d_bc + (size_t)n * ns, a->d_lam, a->d_lams,
                                a->d_part, a->d_gphi, a->d_gK1, a->d_gK2, a->d_gbce);
-            hipLaunchKernelGGL(k_r_edge, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
+            KLAUNCH(a, k_r_edge, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
                                dt[n], a->d_lams, a->d_side);
-            hipLaunchKernelGGL(k_r_gather, dim3(nblocks(nc)), blk, 0, c->stream, c->m, a->d_part,
+            KLAUNCH(a, k_r_gather, dim3(nblocks(nc)), blk, 0, c->stream, c->m, a->d_part,
                                a->d_side, a->d_lam, a->d_gphi);
             if (c->n_series > 0)
-                hipLaunchKernelGGL(k_r_bc_reduce, dim3(1), dim3(64), 0, c->stream, a->d_ser_ptr,
+                KLAUNCH(a, k_r_bc_reduce, dim3(1), dim3(64), 0, c->stream, a->d_ser_ptr,
                                    a->d_ser_edges, a->d_gbce, c->n_series, a->d_gbc + (size_t)n * ns);
         }
     }
+    KLAUNCH(a, k_cost_total, dim3(1), dim3(64), 0, c->stream, a->d_J, a->n_obs);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> v(nc);
@@ -1003,6 +1064,56 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
                             hipMemcpyDeviceToHost));
     if (J) HIPCHK(c, hipMemcpy(J, a->d_J, sizeof(double), hipMemcpyDeviceToHost));
     if (n_steps) *n_steps = a->n_steps;
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_profile(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
+                        int32_t k_max, double *avg_ms, int64_t *launches, const char **names,
+                        int32_t *n_kernels) {
+    if (!c || !c->assim || k_max < 0 || !n_kernels) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    a->prof = true;
+    a->ev_used = 0;
+    a->marks.clear();
+    double J = 0.0;
+    int64_t n = 0;
+    const int st = sw2dg_assim_gradient(c, phi, K1, K2, &J, nullptr, nullptr, nullptr, nullptr, &n);
+    a->prof = false;
+    if (st) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<const char *> nm;
+    std::vector<double> sum;
+    std::vector<int64_t> cnt;
+    for (const auto &mk : a->marks) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, a->ev[mk.second], a->ev[mk.second + 1]));
+        size_t k = 0;
+        while (k < nm.size() && std::strcmp(nm[k], mk.first) != 0) ++k;
+        if (k == nm.size()) {
+            nm.push_back(mk.first);
+            sum.push_back(0.0);
+            cnt.push_back(0);
+        }
+        sum[k] += ms;
+        cnt[k] += 1;
+    }
+    a->marks.clear();
+    *n_kernels = (int32_t)nm.size();
+    for (size_t k = 0; k < nm.size() && (int32_t)k < k_max; ++k) {
+        if (avg_ms) avg_ms[k] = sum[k] / (double)cnt[k];
+        if (launches) launches[k] = cnt[k];
+        if (names) names[k] = nm[k];
+    }
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_gauge_series(sw2dg_ctx *c, double *out) {
+    if (!c || !c->assim || !c->assim->recorded || !out) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    if (a->n_gauges == 0 || a->n_obs == 0) return SW2DG_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(out, a->d_gser, (size_t)a->n_obs * a->n_gauges * sizeof(double),
+                        hipMemcpyDeviceToHost));
     return SW2DG_OK;
 }
 

@@ -10,8 +10,9 @@
 //                           previous contribution), contributions factor*flux
 //                           for both sides, CFL swept terms
 //   k_cfl                   per cell: swept = fixed-order sum, dt candidate,
-//                           wave/block/grid min; the last block finalises dt,
-//                           the clock and the boundary values
+//                           wave/block min -> per-block partial
+//   k_dt                    one block: min of the partials, dt landing, the
+//                           clock and the boundary values
 //   k_update                per cell: fixed-order gather of boundary fluxes and
 //                           edge contributions (no atomics: deterministic and
 //                           in the reference's summation order), round-off
@@ -286,20 +287,21 @@ __device__ __forceinline__ double wave_min(double v) {
 
 // ---------------------------------------------------------------------------
 // k_cfl: per-cell swept sum in reference order (boundary edges first, then
-// internal edges, swSpHyp.cpp:440-477), dt candidate (:481-493) and a
-// deterministic min (min is exact, so any reduction order gives the same dt).
-// The last block to finish finalises the step's dt (map landing and t_max
-// clamp, :495-508), the boundary values at the step's start time, the clock
-// and the storage flags (swParent::checkStore, swParent.cpp:46-88).
+// internal edges, swSpHyp.cpp:440-477), dt candidate (:481-493), wavefront and
+// block min -> one partial per block (min is exact, so the reduction order
+// does not matter).  k_dt, one block, then reduces the partials and finalises
+// the step's dt (map landing and t_max clamp, :495-508), the boundary values
+// at the step's start time, the clock and the storage flags
+// (swParent::checkStore, swParent.cpp:46-88).  The kernel boundary orders the
+// two: no grid-wide fence or atomic (a device-scope release writes back the
+// L2 of an XCD, which per block costs more than the whole reduction).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, BcDev bc, Clock *ck, int p) {
+__global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, Clock *ck, int p) {
     if (!step_active(ck, p)) return;
     __shared__ double red[kBlock / 64];
-    __shared__ int is_last;
     const int lb = xcd_block();
     const int i = lb * kBlock + (int)threadIdx.x;
-    const double dt_max = ck->dt_max;
-    double cand = dt_max;
+    double cand = ck->dt_max;
     if (i < m.n_cell) {
         const double h = m.h[i];
         double swept = 0.;
@@ -328,25 +330,25 @@ __global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, BcDev bc, Clo
         double v = red[0];
         for (int w = 1; w < kBlock / 64; ++w) v = (red[w] < v) ? red[w] : v;
         m.partial[lb] = v;
-        __threadfence();
-        const unsigned int done = atomicAdd(&ck->k_counter, 1u);
-        is_last = (done == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+__global__ __launch_bounds__(kBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck, int p, int n_part) {
+    if (!step_active(ck, p)) return;
+    __shared__ double red[kBlock / 64];
+    const double dt_max = ck->dt_max;
     double v = dt_max;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) {
+    for (int k = threadIdx.x; k < n_part; k += kBlock) {
         const double x = m.partial[k];
         v = (x < v) ? x : v;
     }
     v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     if (threadIdx.x != 0) return;
     double dt = red[0];
     for (int w = 1; w < kBlock / 64; ++w) dt = (red[w] < dt) ? red[w] : dt;
-    ck->k_counter = 0u;
     const double t = ck->T[p];
     if (ck->dt_map > kSmall) {
         const int now = (int)(t / ck->dt_map);
@@ -534,7 +536,8 @@ void launch_step(sw2dg_ctx *c, int p) {
     } else {
         hipLaunchKernelGGL(k_edges<0>, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, c->ck, p);
     }
-    hipLaunchKernelGGL(k_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p);
+    hipLaunchKernelGGL(k_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->ck, p);
+    hipLaunchKernelGGL(k_dt, dim3(1), blk, 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
     hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p);
 }
 
@@ -838,6 +841,17 @@ int sw2dg_set_bc_series(sw2dg_ctx *c, int32_t n_series, int32_t n_times, const i
         if (c->bser_host[e] >= n_series)
             return fail(c, SW2DG_ERR_ARG, "a boundary edge refers to a missing BC series");
     HIPCHK(c, hipSetDevice(c->device));
+    if (c->bc.t && n_series == c->n_series && n_times == c->n_times && n_series > 0) {
+        // same shape (a new hydrograph in a calibration loop): overwrite in place
+        HIPCHK(c, hipMemcpy((void *)c->bc.t, t, n_times * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->bc.v, v, (size_t)n_series * n_times * sizeof(double),
+                            hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->bc.type, type, n_series * sizeof(int), hipMemcpyHostToDevice));
+        c->series_t_host.assign(t, t + n_times);
+        c->series_v_host.assign(v, v + (size_t)n_series * n_times);
+        c->series_type_host.assign(type, type + n_series);
+        return SW2DG_OK;
+    }
     c->n_series = n_series;
     c->n_times = n_times;
     const int ns = std::max(n_series, 1), nt = std::max(n_times, 1);
@@ -923,7 +937,6 @@ int sw2dg_set_state(sw2dg_ctx *c, const double *h, const double *qx, const doubl
     c->hck.err = 0;
     c->hck.paused = 0;
     c->hck.store_flags = 0;
-    c->hck.k_counter = 0;
     c->hck.dt_last = 0.0;
     c->state_set = true;
     return push_clock(c);
@@ -959,7 +972,6 @@ int sw2dg_run(sw2dg_ctx *c, double t_stop, int64_t max_steps, int32_t pause_on_s
     c->hck.step_stop = step0 + max_steps;
     c->hck.pause_on_store = pause_on_store;
     c->hck.paused = 0;
-    c->hck.k_counter = 0;
     const double t = c->hck.T[0];
     c->hck.ACT[0] = (t < c->hck.t_max) && (t < t_stop) && (max_steps > 0);
     c->hck.T[1] = t;
@@ -1009,9 +1021,9 @@ int sw2dg_profile_kernels(sw2dg_ctx *c, int64_t n_steps, int32_t k_max, double *
     if (st) return st;
     // run each step's kernels one by one, bracketed by events on the ctx stream
     const bool muscl = c->o.scheme == 1;
-    const int nk = muscl ? 4 : 3;
-    static const char *kn_m[4] = {"k_slopes", "k_edges<1>", "k_cfl", "k_update"};
-    static const char *kn_g[3] = {"k_edges<0>", "k_cfl", "k_update"};
+    const int nk = muscl ? 5 : 4;
+    static const char *kn_m[5] = {"k_slopes", "k_edges<1>", "k_cfl", "k_dt", "k_update"};
+    static const char *kn_g[4] = {"k_edges<0>", "k_cfl", "k_dt", "k_update"};
     std::vector<hipEvent_t> ev(2 * nk);
     for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
     std::vector<double> acc(nk, 0.0);
@@ -1042,7 +1054,11 @@ int sw2dg_profile_kernels(sw2dg_ctx *c, int64_t n_steps, int32_t k_max, double *
             ++k;
         }
         hipEventRecord(ev[2 * k], c->stream);
-        hipLaunchKernelGGL(k_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p);
+        hipLaunchKernelGGL(k_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->ck, p);
+        hipEventRecord(ev[2 * k + 1], c->stream);
+        ++k;
+        hipEventRecord(ev[2 * k], c->stream);
+        hipLaunchKernelGGL(k_dt, dim3(1), blk, 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
         hipEventRecord(ev[2 * k + 1], c->stream);
         ++k;
         hipEventRecord(ev[2 * k], c->stream);

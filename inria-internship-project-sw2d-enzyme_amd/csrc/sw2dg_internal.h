@@ -29,7 +29,6 @@ struct Clock {
     double t_stop, t_max, dt_max, dt_map, dt_ts, cfl;
     long long step_stop;
     double dt_last;      // dt of the last active step
-    unsigned int k_counter;  // last-block detection in k_cfl
     int n_series;
 };
 
@@ -61,7 +60,7 @@ struct MeshDev {
     const double *blen, *bnx, *bny;
     const int *bser;
     double *bF1;
-    double *partial;               // [n_blocks of k_cfl]
+    double *partial;               // [n_blocks of k_cfl]: per-block dt candidates
 };
 
 struct Opts {

@@ -72,6 +72,26 @@ class SpSolver:
         except Exception:
             pass
 
+    def set_bc_series(self, t, v, types=None):
+        """Replace the boundary time series: t [n_times], v [n_series, n_times]
+        (swParent::initBoundaryConditions for time series).  Call set_state
+        afterwards before running the native time loop."""
+        t = self._hold("bc_t", t)
+        v = np.atleast_2d(np.asarray(v, dtype=np.float64))
+        ns = v.shape[0]
+        if types is None:
+            types = self.case.a["series_type"][:ns]
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        self._keep["bc_type"] = ty
+        vv = self._hold("bc_v", v.reshape(-1))
+        self._check(self.lib.sw2dg_set_bc_series(self.ctx, ns, len(t), N.i32ptr(ty), N.dptr(t),
+                                                 N.dptr(vv)))
+
+    def _hold(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self._keep[name] = arr
+        return arr
+
     # -- state ---------------------------------------------------------------
     def set_state(self, h, qx, qy, t):
         f = lambda x: np.ascontiguousarray(np.broadcast_to(x, (self.case.n_cell,)), dtype=np.float64)

@@ -117,6 +117,10 @@ def _e_decl():
     L.orc_e_replay.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d, _d, _i32,
                                C.c_int32, C.c_int32, _i64, _d, _d, C.c_int32, _d, _d, _d]
     L.orc_e_replay.restype = C.c_double
+    L.orc_e_gradient.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d, _d,
+                                 _i32, C.c_int32, C.c_int32, _i64, _d, _d, C.c_int32, _d, _d, _d,
+                                 _d]
+    L.orc_e_gradient.restype = C.c_double
     L._e_ready = True
     return L
 
@@ -172,6 +176,35 @@ class OracleE:
                                    nob, len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
                                    1 if normalize else 0, _dp(H), _dp(QX), _dp(QY))
         return J, (H, QX, QY)
+
+
+def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=None,
+                       normalize=False, n_obs=None, bc=None, gauge_w=None):
+    """Gauge cost of the frozen replay and its gradient (reverse sweep of the
+    oracle's own replay): {"J", "phi", "K1", "K2", "bc" [n_steps, n_series]}."""
+    a = self.case.a
+    phi = self._ctl(phi, a["phi"])
+    K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
+    K2 = self._ctl(K2, a["fc2"] if len(a["fc2"]) else np.zeros(self.nc))
+    dt = np.ascontiguousarray(rec["dt"], dtype=np.float64)
+    n = len(dt)
+    bcs = np.ascontiguousarray(rec["bc"] if bc is None else bc, dtype=np.float64).reshape(-1)
+    trig = np.ascontiguousarray(rec["trig"], dtype=np.int32)
+    h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
+    gc = np.ascontiguousarray(gauges if gauges is not None else np.zeros(0), dtype=np.int64)
+    gw = np.ascontiguousarray(np.ones(len(gc)) if gauge_w is None else gauge_w, dtype=np.float64)
+    ob = np.ascontiguousarray(obs if obs is not None else np.zeros(1), dtype=np.float64).reshape(-1)
+    nob = int(n_obs if n_obs is not None else (trig.max() + 1 if len(trig) else 0))
+    gphi, gK1, gK2 = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
+    gbc = np.empty(max(n, 1) * self.ns)
+    J = _e_decl().orc_e_gradient(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+                                 _dp(qy0), n, _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32), nob,
+                                 len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
+                                 1 if normalize else 0, _dp(gphi), _dp(gK1), _dp(gK2), _dp(gbc))
+    return {"J": J, "phi": gphi, "K1": gK1, "K2": gK2, "bc": gbc[: n * self.ns].reshape(n, self.ns)}
+
+
+OracleE.gradient = _oracle_e_gradient
 
 
 class OracleSolver:

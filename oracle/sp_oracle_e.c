@@ -491,3 +491,605 @@ double orc_e_replay(const orc_case *c, const double *phi, const double *K1, cons
     e_free(&s);
     return J;
 }
+
+/* ------------------------------------------------------------------------
+ * Reverse sweep of the frozen replay: the derivative that the reference
+ * obtains from Enzyme (get_spatial_gradient_checkpointed,
+ * swSpE_checkpoint.cpp:722-1281, reversing ckpt_segment_E :584-715), written
+ * out by hand for the functions above, in the same loop order.  All states
+ * of the trajectory are kept (no checkpointing: test sizes).
+ * Conventions at non-differentiable points: the derivative of sqrt at 0 is
+ * taken as 0; min/max/clamps pass the derivative to the branch the forward
+ * took; the round-off clean-up and the negative-depth clamp zero it.
+ * ------------------------------------------------------------------------ */
+
+static double dsq(double x) { return (x > 0.0) ? 0.5 / sqrt(x) : 0.0; } /* d sqrt(x)/dx */
+
+/* lam_m / lam_p adjoint: returns d lam / du and d lam / dc */
+static void lam_grad(double b, double c, double u, int plus, double *du, double *dc) {
+    double a = (b - 1.0) * b * u * u + c * c;
+    double ds = dsq(a);
+    double sg = plus ? 1.0 : -1.0;
+    *du = b + sg * ds * 2.0 * (b - 1.0) * b * u;
+    *dc = sg * ds * 2.0 * c;
+}
+
+/* adjoint of one side of hypGodunovComputeLRStates_E: (z, q, r, lm, lp) of a
+ * cell state rotated to the edge normal -> accumulates into (ah, aqx, aqy) */
+static void lr_adj(double h, double qx, double qy, double b, double nx, double ny, double hmin,
+                   double az, double aq, double ar, double alm, double alp, double *ah,
+                   double *aqx, double *aqy) {
+    double q = qx * nx + qy * ny;
+    double u = (h > hmin) ? q / h : 0.0;
+    double c = sqrt(G * h);
+    double dum, dcm, dup, dcp;
+    lam_grad(b, c, u, 0, &dum, &dcm);
+    lam_grad(b, c, u, 1, &dup, &dcp);
+    double au = alm * dum + alp * dup;
+    double ac = alm * dcm + alp * dcp;
+    double ahh = az + ac * G * dsq(G * h);
+    if (h > hmin) {
+        aq += au / h;
+        ahh += -au * q / (h * h);
+    }
+    *ah += ahh;
+    *aqx += aq * nx - ar * ny;
+    *aqy += aq * ny + ar * nx;
+}
+
+/* adjoint of hypHllc_E: df[5] -> d[12] = d(zL qL rL zR qR rR lmL lmR lpL lpR phiL phiR) */
+static void hllc_adj(double h_min, double phi_min, double bL, double bR, double phiL, double phiR,
+                     double zbL, double zbR, double zL, double qL, double rL, double zR, double qR,
+                     double rR, double lmL, double lmR, double lpL, double lpR, const double *df,
+                     double *d) {
+    double hL = (zL - zbL > 0.0) ? (zL - zbL) : 0.0;
+    double hR = (zR - zbR > 0.0) ? (zR - zbR) : 0.0;
+    int both = 0, lhs = 0, rhs = 0;
+    if (!(((hL <= h_min) && (hR <= h_min)) || ((phiL <= phi_min) && (phiR <= phi_min)))) {
+        both = 1;
+        if ((phiR <= phi_min) || ((phiR > phi_min) && (hR <= h_min) && (zL <= zbR + h_min))) {
+            lhs = 1;
+            both = 0;
+        }
+        if ((phiL <= phi_min) || ((phiL > phi_min) && (hL <= h_min) && (zR <= zbL + h_min))) {
+            rhs = 1;
+            both = 0;
+        }
+    }
+    double ahL = 0, ahR = 0;
+    for (int k = 0; k < 12; k++) d[k] = 0.0;
+    if (both) {
+        double hLs = (hL > SMALL) ? hL : SMALL, hRs = (hR > SMALL) ? hR : SMALL;
+        double uL = qL / hLs, uR = qR / hRs;
+        double L1 = (lmL < lmR) ? lmL : lmR;
+        int L1z = (L1 > 0.0);
+        if (L1z) L1 = 0.0;
+        double L2 = (lpL > lpR) ? lpL : lpR;
+        int L2z = (L2 < 0.0);
+        if (L2z) L2 = 0.0;
+        double den = L2 - L1;
+        int denf = (den < SMALL);
+        if (denf) den = SMALL;
+        double dd = phiL - phiR;
+        double sq = sqrt(dd * dd + 1e-3 * 1e-3);
+        double phiLR = 0.5 * (phiL + phiR - sq);
+        double N1 = L2 * qL - L1 * qR + L1 * L2 * (zR - zL);
+        double f1 = phiLR * N1 / den;
+        double fact = 0.5 * phiLR * (hL + hR);
+        double s2a = (phiR * hR * hR - phiL * hL * hL) * 0.5 + fact * (zL - zR);
+        double s2 = s2a * G / den;
+        double sl = -L1 * s2, sr = L2 * s2;
+        double FLk = bL * hL * uL * uL + 0.5 * G * hL * hL, FRk = bR * hR * uR * uR + 0.5 * G * hR * hR;
+        double FL = phiL * FLk, FR = phiR * FRk;
+        double P = phiR * qR - phiL * qL;
+        double num2 = L2 * (FL + sl) - L1 * (FR - sr) + L1 * L2 * P;
+        double f2 = num2 / den;
+        double vL = rL / hLs, vR = rR / hRs;
+        double af1 = df[0], af2 = df[1], af3 = df[2], asl = df[3], asr = df[4];
+        double ahLs = 0, ahRs = 0, auL = 0, auR = 0, aL1 = 0, aL2 = 0, aden = 0, aphiLR = 0;
+        /* f3 */
+        if (f1 >= 0.0) {
+            af1 += af3 * bL * vL;
+            double avL = af3 * bL * f1;
+            d[2] += avL / hLs;
+            ahLs += -avL * rL / (hLs * hLs);
+        } else {
+            af1 += af3 * bR * vR;
+            double avR = af3 * bR * f1;
+            d[5] += avR / hRs;
+            ahRs += -avR * rR / (hRs * hRs);
+        }
+        /* f2 */
+        double an2 = af2 / den;
+        aden += -af2 * f2 / den;
+        aL2 += an2 * (FL + sl) + an2 * L1 * P;
+        double aFL = an2 * L2;
+        asl += an2 * L2;
+        aL1 += -an2 * (FR - sr) + an2 * L2 * P;
+        double aFR = -an2 * L1;
+        asr += an2 * L1;
+        double aP = an2 * L1 * L2;
+        d[11] += aP * qR;
+        d[4] += aP * phiR;
+        d[10] += -aP * qL;
+        d[1] += -aP * phiL;
+        /* FL, FR */
+        d[10] += aFL * FLk;
+        ahL += aFL * phiL * (bL * uL * uL + G * hL);
+        auL += aFL * phiL * 2.0 * bL * hL * uL;
+        d[11] += aFR * FRk;
+        ahR += aFR * phiR * (bR * uR * uR + G * hR);
+        auR += aFR * phiR * 2.0 * bR * hR * uR;
+        /* sl, sr */
+        aL1 += -asl * s2;
+        double as2 = -asl * L1;
+        aL2 += asr * s2;
+        as2 += asr * L2;
+        /* s2 */
+        double as2a = as2 * G / den;
+        aden += -as2 * s2 / den;
+        d[11] += as2a * 0.5 * hR * hR;
+        ahR += as2a * phiR * hR;
+        d[10] += -as2a * 0.5 * hL * hL;
+        ahL += -as2a * phiL * hL;
+        double afact = as2a * (zL - zR);
+        d[0] += as2a * fact;
+        d[3] += -as2a * fact;
+        aphiLR += afact * 0.5 * (hL + hR);
+        ahL += afact * 0.5 * phiLR;
+        ahR += afact * 0.5 * phiLR;
+        /* f1 */
+        aphiLR += af1 * N1 / den;
+        double aN1 = af1 * phiLR / den;
+        aden += -af1 * f1 / den;
+        aL2 += aN1 * (qL + L1 * (zR - zL));
+        d[1] += aN1 * L2;
+        aL1 += aN1 * (-qR + L2 * (zR - zL));
+        d[4] += -aN1 * L1;
+        d[3] += aN1 * L1 * L2;
+        d[0] += -aN1 * L1 * L2;
+        /* den, phiLR, u, h */
+        if (!denf) {
+            aL2 += aden;
+            aL1 += -aden;
+        }
+        d[10] += aphiLR * 0.5 * (1.0 - dd / sq);
+        d[11] += aphiLR * 0.5 * (1.0 + dd / sq);
+        d[1] += auL / hLs;
+        ahLs += -auL * qL / (hLs * hLs);
+        d[4] += auR / hRs;
+        ahRs += -auR * qR / (hRs * hRs);
+        if (hL > SMALL) ahL += ahLs;
+        if (hR > SMALL) ahR += ahRs;
+        if (!L2z) {
+            if (lpL > lpR) d[8] += aL2;
+            else d[9] += aL2;
+        }
+        if (!L1z) {
+            if (lmL < lmR) d[6] += aL1;
+            else d[7] += aL1;
+        }
+    } else if (rhs) {
+        d[11] += df[4] * 0.5 * G * hR * hR;
+        ahR += df[4] * phiR * G * hR;
+    } else if (lhs) {
+        d[10] += -df[3] * 0.5 * G * hL * hL;
+        ahL += -df[3] * phiL * G * hL;
+    }
+    if (zL - zbL > 0.0) d[0] += ahL;
+    if (zR - zbR > 0.0) d[3] += ahR;
+}
+
+/* adjoint of hypBoundaryFlux_E: df[3] -> d[6] = d(zR qR rR lpR phiR bc) */
+static void bnd_adj(double h_min, double phi_min, double bR, double phiR, double zbR, double zR,
+                    double qR, double rR, double lpR, int type, double bc, const double *df,
+                    double *d) {
+    for (int k = 0; k < 6; k++) d[k] = 0.0;
+    double hR = zR - zbR;
+    int dry = (hR < h_min);
+    double q1 = dry ? 0.0 : qR, r1 = dry ? 0.0 : rR;
+    double hRe = dry ? 0.0 : hR;
+    int hden_f = !(hRe > SMALL);
+    double hden = hden_f ? SMALL : hRe;
+    double cR = sqrt(G * hden);
+    double uR = q1 / hden;
+    double L1 = (lpR > 0.0) ? lpR : 0.0;
+    double f2Rk = bR * hRe * uR * uR + 0.5 * G * hRe * hRe;
+    /* forward values needed for the reverse */
+    double f1 = 0.0;
+    double aq1 = 0, ar1 = 0, ahRe = 0, auR = 0, acR = 0, aL1 = 0, aphi = 0, abc = 0;
+    double af1 = df[0], af2 = df[1], af3 = df[2];
+    /* recompute f1 by type, keeping what the reverse needs */
+    int q_crit = 0;
+    double ps = 0, hc = 0, hcs = 0;
+    double fr = 0, cc = 0, cc0 = 0, uu = 0, hh = 0;
+    double hb = 0, ccw = 0;
+    int hz_wet = 0, fr_in = 0, cc_pos = 0, hb_pos = 0, hbs_f = 0;
+    if (type == ORC_BC_Q) {
+        f1 = bc;
+        if (bc > SMALL) {
+            ps = (phiR > phi_min) ? phiR : phi_min;
+            hc = exp((1.0 / 3.0) * log((bc / ps) / G));
+            if (hc >= hRe) {
+                q_crit = 1;
+                hcs = (hc > SMALL) ? hc : SMALL;
+                f1 = phiR * 1.5 * sqrt(G * hcs) * hc;
+            }
+        }
+    } else if (type == ORC_BC_FR) {
+        fr_in = (bc >= 0.0 && bc <= 1.0);
+        fr = (bc < 0.0) ? 0.0 : ((bc > 1.0) ? 1.0 : bc);
+        cc0 = (uR - 2.0 * cR) / (-fr - 2.0);
+        cc_pos = (cc0 > 0.0);
+        cc = cc_pos ? cc0 : 0.0;
+        uu = -fr * cc;
+        hh = cc * cc / G;
+        f1 = phiR * uu * hh;
+    } else if (type == ORC_BC_H || type == ORC_BC_Z) {
+        hb_pos = (type == ORC_BC_Z) ? (bc - zbR > 0.0) : (bc > 0.0);
+        hb = hb_pos ? ((type == ORC_BC_Z) ? bc - zbR : bc) : 0.0;
+        hz_wet = (hRe > h_min);
+        if (hz_wet) {
+            f1 = phiR * (q1 + L1 * (hb - hRe));
+        } else {
+            hbs_f = !(hb > SMALL);
+            ccw = sqrt(G * (hbs_f ? SMALL : hb));
+            f1 = phiR * hb * ccw;
+        }
+    }
+    /* f3 */
+    if (!(f1 > 0.0)) {
+        double hv = (hRe > SMALL) ? hRe : SMALL;
+        double vR = r1 / hv;
+        af1 += af3 * vR;
+        double avR = af3 * f1;
+        ar1 += avR / hv;
+        if (hRe > SMALL) ahRe += -avR * r1 / (hv * hv);
+    }
+    /* f2 and f1 by type */
+    int f2_std = 1;
+    if (type == ORC_BC_H || type == ORC_BC_Z) f2_std = hz_wet;
+    if (type != ORC_BC_Q && type != ORC_BC_FR && type != ORC_BC_H && type != ORC_BC_Z) f2_std = 2;
+    double af2R = 0;
+    if (f2_std == 1) {
+        /* f2 = f2R + (f1 - q1 phiR) L1 */
+        af2R += af2;
+        af1 += af2 * L1;
+        aq1 += -af2 * phiR * L1;
+        aphi += -af2 * q1 * L1;
+        aL1 += af2 * (f1 - q1 * phiR);
+    } else if (f2_std == 0) {
+        /* f2 = sforce(bR, phiR, hb, ccw) */
+        aphi += af2 * (bR * hb * ccw * ccw + 0.5 * G * hb * hb);
+        double ahb = af2 * phiR * (bR * ccw * ccw + G * hb);
+        double accw = af2 * phiR * 2.0 * bR * hb * ccw;
+        /* f1 = phiR hb ccw */
+        aphi += af1 * hb * ccw;
+        ahb += af1 * phiR * ccw;
+        accw += af1 * phiR * hb;
+        af1 = 0.0;
+        if (!hbs_f) ahb += accw * G * dsq(G * hb);
+        if (hb_pos) abc += ahb;
+    } else {
+        af2R += af2;
+    }
+    if (type == ORC_BC_Q) {
+        if (q_crit) {
+            /* f1 = phiR 1.5 sqrt(G hcs) hc */
+            double sg = sqrt(G * hcs);
+            aphi += af1 * 1.5 * sg * hc;
+            double ahc = af1 * phiR * 1.5 * sg;
+            if (hc > SMALL) ahc += af1 * phiR * 1.5 * hc * G * dsq(G * hcs);
+            /* hc = ((bc/ps)/G)^(1/3): d hc / d bc = hc / (3 bc), d hc / d ps = -hc / (3 ps) */
+            abc += ahc * hc / (3.0 * bc);
+            if (phiR > phi_min) aphi += -ahc * hc / (3.0 * ps);
+        } else {
+            abc += af1;
+        }
+    } else if (type == ORC_BC_FR) {
+        /* f1 = phiR uu hh, uu = -fr cc, hh = cc^2/G */
+        aphi += af1 * uu * hh;
+        double auu = af1 * phiR * hh, ahh = af1 * phiR * uu;
+        double acc = -auu * fr + ahh * 2.0 * cc / G;
+        double afr = -auu * cc;
+        if (cc_pos) {
+            /* cc0 = (uR - 2 cR) / (-fr - 2) */
+            double den = -fr - 2.0;
+            auR += acc / den;
+            acR += -2.0 * acc / den;
+            afr += acc * cc0 / den; /* d cc0 / d fr = cc0 / den (den' = -1) */
+        }
+        if (fr_in) abc += afr;
+    } else if (type == ORC_BC_H || type == ORC_BC_Z) {
+        if (hz_wet) {
+            /* f1 = phiR (q1 + L1 (hb - hRe)) */
+            aphi += af1 * (q1 + L1 * (hb - hRe));
+            aq1 += af1 * phiR;
+            aL1 += af1 * phiR * (hb - hRe);
+            double ahb = af1 * phiR * L1;
+            ahRe += -af1 * phiR * L1;
+            if (hb_pos) abc += ahb;
+        }
+    }
+    /* f2R = phiR (bR hRe uR^2 + 0.5 G hRe^2) */
+    aphi += af2R * f2Rk;
+    ahRe += af2R * phiR * (bR * uR * uR + G * hRe);
+    auR += af2R * phiR * 2.0 * bR * hRe * uR;
+    /* uR = q1 / hden, cR = sqrt(G hden) */
+    aq1 += auR / hden;
+    double ahden = -auR * q1 / (hden * hden) + acR * G * dsq(G * hden);
+    if (!hden_f) ahRe += ahden;
+    /* L1 = max(lpR, 0) */
+    if (lpR > 0.0) d[3] += aL1;
+    if (!dry) {
+        d[0] += ahRe; /* hR = zR - zbR */
+        d[1] += aq1;
+        d[2] += ar1;
+    }
+    d[4] += aphi;
+    d[5] += abc;
+}
+
+/* adjoint of the semi-implicit Strickler friction (sourceFrictionStrickler_E) */
+static void fric_adj(double hthr, double h, double qx, double qy, double ca, double sa,
+                     double K1in, double K2in, double dt, double *ah, double *aqx, double *aqy,
+                     double *aK1, double *aK2) {
+    if (!(h > hthr)) {
+        *aqx = 0.0;
+        *aqy = 0.0;
+        return;
+    }
+    double hs = h + SMALL;
+    double u = qx / hs, v = qy / hs;
+    double ul = u * ca + v * sa, vl = -u * sa + v * ca;
+    double un = sqrt(u * u + v * v + 1e-16);
+    int k1f = !(K1in > 1e-10), k2f = !(K2in > 1e-10);
+    double K1 = k1f ? 20.0 : K1in, K2 = k2f ? 20.0 : K2in;
+    double cb = cbrt(h + 1e-16);
+    double h43 = h * cb;
+    double D1 = K1 * K1 * h43 + 1e-10, D2 = K2 * K2 * h43 + 1e-10;
+    double d1 = 1.0 + (dt * G * un) / D1, d2 = 1.0 + (dt * G * un) / D2;
+    double ul2 = ul / d1, vl2 = vl / d2;
+    double bx = *aqx, by = *aqy;
+    double aul2 = (bx * ca + by * sa) * h;
+    double avl2 = (-bx * sa + by * ca) * h;
+    double ahh = bx * (ul2 * ca - vl2 * sa) + by * (ul2 * sa + vl2 * ca);
+    double aul = aul2 / d1, ad1 = -aul2 * ul2 / d1;
+    double avl = avl2 / d2, ad2 = -avl2 * vl2 / d2;
+    double aun = ad1 * dt * G / D1 + ad2 * dt * G / D2;
+    double aD1 = -ad1 * dt * G * un / (D1 * D1), aD2 = -ad2 * dt * G * un / (D2 * D2);
+    if (!k1f) *aK1 += aD1 * 2.0 * K1 * h43;
+    if (!k2f) *aK2 += aD2 * 2.0 * K2 * h43;
+    double ah43 = aD1 * K1 * K1 + aD2 * K2 * K2;
+    ahh += ah43 * (cb + h * cb / (3.0 * (h + 1e-16)));
+    double au = aun * u / un + aul * ca - avl * sa;
+    double av = aun * v / un + aul * sa + avl * ca;
+    *aqx = au / hs;
+    *aqy = av / hs;
+    ahh += -(au * u + av * v) / hs;
+    *ah += ahh;
+}
+
+/* adjoint of sourceSingularHeadLoss_E */
+static void hl_adj(double hthr, double h, double qx, double qy, double ca, double sa, double hsing,
+                   double c1, double c2, double dt, double *ah, double *aqx, double *aqy) {
+    if (!(h > hthr)) {
+        *aqx = 0.0;
+        *aqy = 0.0;
+        return;
+    }
+    double xr = qx * ca + qy * sa, yr = -qx * sa + qy * ca;
+    double ax = 0, ay = 0, hs = h + SMALL, qn = 1.0;
+    int act = (h > hsing);
+    if (act) {
+        qn = sqrt(xr * xr + yr * yr + 1e-16);
+        ax = qn * c1 / hs;
+        ay = qn * c2 / hs;
+    }
+    double ex = exp(-ax * dt), ey = exp(-ay * dt);
+    double bx = *aqx, by = *aqy;
+    double axr2 = bx * ca + by * sa, ayr2 = -bx * sa + by * ca;
+    double axr = axr2 * ex, ayr = ayr2 * ey;
+    if (act) {
+        double aax = -axr2 * xr * dt * ex, aay = -ayr2 * yr * dt * ey;
+        double aqn = aax * c1 / hs + aay * c2 / hs;
+        *ah += -(aax * ax + aay * ay) / hs;
+        axr += aqn * xr / qn;
+        ayr += aqn * yr / qn;
+    }
+    *aqx = axr * ca - ayr * sa;
+    *aqy = axr * sa + ayr * ca;
+}
+
+static void fac_dphi(const orc_case *c, const double *phi, int64_t cell, double w, double *out) {
+    *out = (phi[cell] > c->phi_min) ? -(w / c->cell_area[cell] / phi[cell]) / phi[cell] : 0.0;
+}
+
+/* Gradient of the frozen-replay gauge cost (orc_e_replay) with respect to
+ * porosity, the two Strickler coefficients per cell and the boundary value of
+ * every step and series.  Returns J. */
+double orc_e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
+                      const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
+                      const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
+                      int32_t n_obs, int32_t n_gauges, const int64_t *gauge_cell,
+                      const double *gauge_w, const double *obs, int32_t normalize, double *g_phi,
+                      double *g_K1, double *g_K2, double *g_bc) {
+    const int64_t nc = c->n_cell;
+    const int ns = c->n_series;
+    const double hthr = (c->h_min > SMALL) ? c->h_min : SMALL;
+    const double norm = (normalize && n_obs > 0) ? (double)n_obs : 1.0;
+    esolver s;
+    e_init(&s, c, phi, K1, K2, h0, qx0, qy0);
+    /* forward: keep every state */
+    double *U = (double *)malloc(sizeof(double) * 3 * (size_t)nc * (size_t)(n_steps + 1));
+    memcpy(U, h0, sizeof(double) * nc);
+    memcpy(U + nc, qx0, sizeof(double) * nc);
+    memcpy(U + 2 * nc, qy0, sizeof(double) * nc);
+    double J = 0.0;
+    for (int64_t n = 0; n < n_steps; n++) {
+        e_step(&s, dt_seq[n], bc_seq + n * ns);
+        double *Un = U + 3 * (size_t)nc * (size_t)(n + 1);
+        memcpy(Un, s.h, sizeof(double) * nc);
+        memcpy(Un + nc, s.qx, sizeof(double) * nc);
+        memcpy(Un + 2 * nc, s.qy, sizeof(double) * nc);
+        int32_t oi = obs_trig[n];
+        if (oi >= 0 && oi < n_obs && n_gauges > 0) {
+            double Jo = 0.0;
+            for (int g = 0; g < n_gauges; g++) {
+                double d = s.h[gauge_cell[g]] - obs[(int64_t)oi * n_gauges + g];
+                Jo += gauge_w[g] * d * d;
+            }
+            if (normalize && n_obs > 0) Jo /= (double)n_obs;
+            J += Jo;
+        }
+    }
+    /* reverse */
+    double *lam = (double *)calloc(3 * (size_t)nc, sizeof(double));  /* adjoint of U^{n+1} */
+    double *lams = (double *)calloc(3 * (size_t)nc, sizeof(double)); /* adjoint after balance */
+    double *bh = (double *)malloc(sizeof(double) * 3 * (size_t)nc);
+    for (int64_t i = 0; i < nc; i++) g_phi[i] = g_K1[i] = g_K2[i] = 0.0;
+    for (int64_t k = 0; k < n_steps * (ns > 0 ? ns : 1); k++) g_bc[k] = 0.0;
+    for (int64_t n = n_steps - 1; n >= 0; n--) {
+        const double dt = dt_seq[n];
+        const double *bcn = bc_seq + n * ns;
+        const double *Un = U + 3 * (size_t)nc * (size_t)n;
+        const double *Un1 = U + 3 * (size_t)nc * (size_t)(n + 1);
+        int32_t oi = obs_trig[n];
+        if (oi >= 0 && oi < n_obs && n_gauges > 0)
+            for (int g = 0; g < n_gauges; g++) {
+                int64_t gc = gauge_cell[g];
+                lam[gc] += 2.0 * gauge_w[g] * (Un1[gc] - obs[(int64_t)oi * n_gauges + g]) / norm;
+            }
+        /* recompute the step's intermediates */
+        memcpy(s.h, Un, sizeof(double) * nc);
+        memcpy(s.qx, Un + nc, sizeof(double) * nc);
+        memcpy(s.qy, Un + 2 * nc, sizeof(double) * nc);
+        lr(&s);
+        fluxes(&s, bcn);
+        memcpy(bh, Un, sizeof(double) * 3 * nc);
+        {
+            double *h = bh, *qx = bh + nc, *qy = bh + 2 * nc;
+            for (int64_t e = 0; e < c->n_bnd; e++) {
+                int64_t id = c->bnd_cell[e];
+                double co = s.fac_b[e], nx = c->bnd_nx[e], ny = c->bnd_ny[e];
+                h[id] += (+co) * s.F1b[e] * dt;
+                qx[id] += ((+co * nx) * s.F2b[e] + (+co * nx) * 0.0 + (-co * ny) * s.F3b[e]) * dt;
+                qy[id] += ((+co * ny) * s.F2b[e] + (+co * ny) * 0.0 + (+co * nx) * s.F3b[e]) * dt;
+            }
+            for (int64_t e = 0; e < c->n_int; e++) {
+                int64_t L = c->int_L[e], R = c->int_R[e];
+                double cl = s.fac_L[e], cr = s.fac_R[e], nx = c->int_nx[e], ny = c->int_ny[e];
+                double dh = s.F1i[e] * dt;
+                h[L] += (-cl) * dh;
+                h[R] += (+cr) * dh;
+                qx[L] += ((-cl * nx) * s.F2i[e] + (+cl * nx) * s.sLi[e] + (+cl * ny) * s.F3i[e]) * dt;
+                qx[R] += ((+cr * nx) * s.F2i[e] + (+cr * nx) * s.sRi[e] + (-cr * ny) * s.F3i[e]) * dt;
+                qy[L] += ((-cl * ny) * s.F2i[e] + (+cl * ny) * s.sLi[e] + (-cl * nx) * s.F3i[e]) * dt;
+                qy[R] += ((+cr * ny) * s.F2i[e] + (+cr * ny) * s.sRi[e] + (+cr * nx) * s.F3i[e]) * dt;
+            }
+        }
+        /* per cell: sources, clamp and clean-up in reverse */
+        for (int64_t i = 0; i < nc; i++) {
+            double h = bh[i], qx = bh[nc + i], qy = bh[2 * nc + i];
+            int kh = !(fabs(h) < SMALL), kx = !(fabs(qx) < SMALL), ky = !(fabs(qy) < SMALL);
+            if (!kh) h = 0.0;
+            if (!kx) qx = 0.0;
+            if (!ky) qy = 0.0;
+            int clamped = (h < 0.);
+            if (clamped) h = qx = qy = 0.0;
+            double fx = qx, fy = qy; /* state after friction */
+            if (c->friction != ORC_FRIC_NONE) {
+                if (h > hthr) {
+                    double hs = h + SMALL;
+                    double u = qx / hs, v = qy / hs;
+                    double ca = s.fcos[i], sa = s.fsin[i];
+                    double ul = u * ca + v * sa, vl = -u * sa + v * ca;
+                    double un = sqrt(u * u + v * v + 1e-16);
+                    double k1 = (K1[i] > 1e-10) ? K1[i] : 20.0, k2 = (K2[i] > 1e-10) ? K2[i] : 20.0;
+                    double h43 = h * cbrt(h + 1e-16);
+                    double d1 = 1.0 + (dt * G * un) / (k1 * k1 * h43 + 1e-10);
+                    double d2 = 1.0 + (dt * G * un) / (k2 * k2 * h43 + 1e-10);
+                    ul /= d1;
+                    vl /= d2;
+                    fx = (ul * ca - vl * sa) * h;
+                    fy = (ul * sa + vl * ca) * h;
+                } else {
+                    fx = fy = 0.0;
+                }
+            }
+            double ah = lam[i], ax = lam[nc + i], ay = lam[2 * nc + i];
+            if (c->head_loss)
+                hl_adj(hthr, h, fx, fy, s.hcos[i], s.hsin[i], c->hl_hsing[i], c->hl_c1[i],
+                       c->hl_c2[i], dt, &ah, &ax, &ay);
+            if (c->friction != ORC_FRIC_NONE)
+                fric_adj(hthr, h, qx, qy, s.fcos[i], s.fsin[i], K1[i], K2[i], dt, &ah, &ax, &ay,
+                         &g_K1[i], &g_K2[i]);
+            if (clamped) ah = ax = ay = 0.0;
+            if (!kh) ah = 0.0;
+            if (!kx) ax = 0.0;
+            if (!ky) ay = 0.0;
+            lams[i] = ah;
+            lams[nc + i] = ax;
+            lams[2 * nc + i] = ay;
+        }
+        /* balance: identity part, then every edge in reverse */
+        memcpy(lam, lams, sizeof(double) * 3 * nc);
+        for (int64_t e = 0; e < c->n_bnd; e++) {
+            int64_t id = c->bnd_cell[e];
+            double co = s.fac_b[e], nx = c->bnd_nx[e], ny = c->bnd_ny[e];
+            double lh = lams[id], lx = lams[nc + id], ly = lams[2 * nc + id];
+            double df[3] = {lh * co * dt, (lx * co * nx + ly * co * ny) * dt,
+                            (-lx * co * ny + ly * co * nx) * dt};
+            double dco = lh * s.F1b[e] * dt +
+                         (lx * (nx * s.F2b[e] - ny * s.F3b[e]) + ly * (ny * s.F2b[e] + nx * s.F3b[e])) * dt;
+            double dfp;
+            fac_dphi(c, phi, id, c->bnd_len[e], &dfp);
+            g_phi[id] += dco * dfp;
+            int k = c->bnd_series[e];
+            double d[6];
+            bnd_adj(c->h_min, c->phi_min, c->beta[id], phi[id], c->cell_zb[id], s.zrb[e], s.qrb[e],
+                    s.rrb[e], s.lpb[e], c->series_type[k], bcn[k], df, d);
+            g_phi[id] += d[4];
+            g_bc[n * ns + k] += d[5];
+            lr_adj(Un[id], Un[nc + id], Un[2 * nc + id], c->beta[id], nx, ny, c->h_min, d[0], d[1],
+                   d[2], 0.0, d[3], &lam[id], &lam[nc + id], &lam[2 * nc + id]);
+        }
+        for (int64_t e = 0; e < c->n_int; e++) {
+            int64_t L = c->int_L[e], R = c->int_R[e];
+            double cl = s.fac_L[e], cr = s.fac_R[e], nx = c->int_nx[e], ny = c->int_ny[e];
+            double lhL = lams[L], lxL = lams[nc + L], lyL = lams[2 * nc + L];
+            double lhR = lams[R], lxR = lams[nc + R], lyR = lams[2 * nc + R];
+            double F1 = s.F1i[e], F2 = s.F2i[e], F3 = s.F3i[e], sL = s.sLi[e], sR = s.sRi[e];
+            double df[5];
+            df[0] = (-lhL * cl + lhR * cr) * dt;
+            df[1] = (-lxL * cl * nx + lxR * cr * nx - lyL * cl * ny + lyR * cr * ny) * dt;
+            df[2] = (lxL * cl * ny - lxR * cr * ny - lyL * cl * nx + lyR * cr * nx) * dt;
+            df[3] = (lxL * cl * nx + lyL * cl * ny) * dt;
+            df[4] = (lxR * cr * nx + lyR * cr * ny) * dt;
+            double dcl = (-lhL * F1 + lxL * (-nx * F2 + nx * sL + ny * F3) + lyL * (-ny * F2 + ny * sL - nx * F3)) * dt;
+            double dcr = (lhR * F1 + lxR * (nx * F2 + nx * sR - ny * F3) + lyR * (ny * F2 + ny * sR + nx * F3)) * dt;
+            double dpl, dpr;
+            fac_dphi(c, phi, L, c->int_len[e], &dpl);
+            fac_dphi(c, phi, R, c->int_len[e], &dpr);
+            g_phi[L] += dcl * dpl;
+            g_phi[R] += dcr * dpr;
+            double d[12];
+            hllc_adj(c->h_min, c->phi_min, c->beta[L], c->beta[R], phi[L], phi[R], c->cell_zb[L],
+                     c->cell_zb[R], s.zri[2 * e], s.qri[2 * e], s.rri[2 * e], s.zri[2 * e + 1],
+                     s.qri[2 * e + 1], s.rri[2 * e + 1], s.lmi[2 * e], s.lmi[2 * e + 1], s.lpi[2 * e],
+                     s.lpi[2 * e + 1], df, d);
+            g_phi[L] += d[10];
+            g_phi[R] += d[11];
+            lr_adj(Un[L], Un[nc + L], Un[2 * nc + L], c->beta[L], nx, ny, c->h_min, d[0], d[1], d[2],
+                   d[6], d[8], &lam[L], &lam[nc + L], &lam[2 * nc + L]);
+            lr_adj(Un[R], Un[nc + R], Un[2 * nc + R], c->beta[R], nx, ny, c->h_min, d[3], d[4], d[5],
+                   d[7], d[9], &lam[R], &lam[nc + R], &lam[2 * nc + R]);
+        }
+    }
+    free(U);
+    free(lam);
+    free(lams);
+    free(bh);
+    e_free(&s);
+    return J;
+}
