@@ -90,13 +90,25 @@ class Calibration:
         wig = (wig - wig.mean()) / (np.abs(wig - wig.mean()).max() + 1e-12)
         self.Q_true = base * (1.0 + 0.3 * wig)
         self.west_len = info["west_len"]
-        # gauges at seeded wet points
+        # initial state: a free surface sloping down to the east at half the
+        # bed slope, at rest -- water moves everywhere from the start, so the
+        # roughness of every zone shapes the gauge series; dry where the
+        # bumps emerge and near the west edge, where the inflow wets the bed
+        zs = info["z0"] + 0.5e-3 * (5000.0 - a["cell_x"])
+        a["h0"] = np.maximum(zs - a["cell_zb"], 0.0)
+        a["qx0"] = np.zeros(self.case.n_cell)
+        a["qy0"] = np.zeros(self.case.n_cell)
+        # gauges: a third at seeded points near the inflow (x < 300 m), the
+        # rest at seeded wet points anywhere
         h0 = a["h0"]
         cells = []
+        n_west = n_gauges * 2 // 5
         while len(cells) < n_gauges:
-            x, y = rng.uniform(0.0, 5000.0, 2)
+            west = len(cells) < n_west
+            x = rng.uniform(10.0, 300.0) if west else rng.uniform(0.0, 5000.0)
+            y = rng.uniform(0.0, 5000.0)
             c = int(locate_cells(g, [x], [y])[0])
-            if c >= 0 and h0[c] > 0.2 and c not in cells:
+            if c >= 0 and (west or h0[c] > 0.2) and c not in cells:
                 cells.append(c)
         self.gauge_cells = np.array(cells, dtype=np.int64)
         self.obs_t = np.arange(obs_dt, self.window + 1e-9, obs_dt)

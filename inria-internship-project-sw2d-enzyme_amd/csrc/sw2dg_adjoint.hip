@@ -240,17 +240,17 @@ __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const doubl
 }
 
 // one block: min of the per-block candidates, then the step schedule
-__global__ __launch_bounds__(kBlock) void k_e_dt(BcDev bc, EClock *ck, const double *partial,
+__global__ __launch_bounds__(kDtBlock) void k_e_dt(BcDev bc, EClock *ck, const double *partial,
                                                  int n_part, double *T, double *dtv, double *bcv,
                                                  int *trig, const double *obs_t, int64_t n) {
     if (!(T[n] < ck->t_max)) {  // past t_max: no step, no observation
         if (threadIdx.x == 0) trig[n] = -1;
         return;
     }
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[kDtBlock / 64];
     const double dt_max = ck->dt_max;
     double v = dt_max;
-    for (int k = threadIdx.x; k < n_part; k += kBlock) {
+    for (int k = threadIdx.x; k < n_part; k += kDtBlock) {
         const double x = partial[k];
         v = (x < v) ? x : v;
     }
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_e_dt(BcDev bc, EClock *ck, const dou
     __syncthreads();
     if (threadIdx.x != 0) return;
     double dtc = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) dtc = (red[w] < dtc) ? red[w] : dtc;
+    for (int w = 1; w < kDtBlock / 64; ++w) dtc = (red[w] < dtc) ? red[w] : dtc;
     const double t = T[n];
     const double t_max = ck->t_max;
     if ((t + dtc) > t_max) dtc = t_max - t;
@@ -607,13 +607,22 @@ __global__ __launch_bounds__(kBlock) void k_r_gather(MeshDev m, const double *__
 }
 
 // per series: sum of the boundary-value adjoints of its edges, reference order
-__global__ void k_r_bc_reduce(const int *ser_ptr, const int *ser_edges, const double *gbce,
-                              int n_series, double *gbc_n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_series) return;
+// One block per series: each thread sums a strided slice of the series'
+// edges in order, then a fixed tree over the block -- the same order on
+// every run (deterministic), and no long serial dependency chain.
+__global__ __launch_bounds__(kBlock) void k_r_bc_reduce(const int *ser_ptr, const int *ser_edges,
+                                                        const double *gbce, double *gbc_n) {
+    __shared__ double red[kBlock];
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int j = ser_ptr[k]; j < ser_ptr[k + 1]; ++j) s += gbce[ser_edges[j]];
-    gbc_n[k] = s;
+    for (int j = ser_ptr[k] + (int)threadIdx.x; j < ser_ptr[k + 1]; j += kBlock) s += gbce[ser_edges[j]];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gbc_n[k] = red[0];
 }
 
 __global__ void k_zero(double *p, size_t n) {
@@ -665,7 +674,7 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
     if (dry) {
         KLAUNCH(a, k_e_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
                 a->d_ck, a->d_partial, a->d_T, n);
-        KLAUNCH(a, k_e_dt, dim3(1), blk, 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
+        KLAUNCH(a, k_e_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
                 a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
     }
     KLAUNCH(a, k_e_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin,
@@ -1038,8 +1047,8 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             KLAUNCH(a, k_r_gather, dim3(nblocks(nc)), blk, 0, c->stream, c->m, a->d_part,
                                a->d_side, a->d_lam, a->d_gphi);
             if (c->n_series > 0)
-                KLAUNCH(a, k_r_bc_reduce, dim3(1), dim3(64), 0, c->stream, a->d_ser_ptr,
-                                   a->d_ser_edges, a->d_gbce, c->n_series, a->d_gbc + (size_t)n * ns);
+                KLAUNCH(a, k_r_bc_reduce, dim3(c->n_series), blk, 0, c->stream, a->d_ser_ptr,
+                        a->d_ser_edges, a->d_gbce, a->d_gbc + (size_t)n * ns);
         }
     }
     KLAUNCH(a, k_cost_total, dim3(1), dim3(64), 0, c->stream, a->d_J, a->n_obs);

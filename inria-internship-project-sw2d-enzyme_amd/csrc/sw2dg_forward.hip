@@ -333,12 +333,12 @@ __global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, Clock *ck, in
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck, int p, int n_part) {
+__global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck, int p, int n_part) {
     if (!step_active(ck, p)) return;
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[kDtBlock / 64];
     const double dt_max = ck->dt_max;
     double v = dt_max;
-    for (int k = threadIdx.x; k < n_part; k += kBlock) {
+    for (int k = threadIdx.x; k < n_part; k += kDtBlock) {
         const double x = m.partial[k];
         v = (x < v) ? x : v;
     }
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck, i
     __syncthreads();
     if (threadIdx.x != 0) return;
     double dt = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) dt = (red[w] < dt) ? red[w] : dt;
+    for (int w = 1; w < kDtBlock / 64; ++w) dt = (red[w] < dt) ? red[w] : dt;
     const double t = ck->T[p];
     if (ck->dt_map > kSmall) {
         const int now = (int)(t / ck->dt_map);
@@ -537,7 +537,7 @@ void launch_step(sw2dg_ctx *c, int p) {
         hipLaunchKernelGGL(k_edges<0>, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, c->ck, p);
     }
     hipLaunchKernelGGL(k_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->ck, p);
-    hipLaunchKernelGGL(k_dt, dim3(1), blk, 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
     hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p);
 }
 
@@ -1058,7 +1058,7 @@ int sw2dg_profile_kernels(sw2dg_ctx *c, int64_t n_steps, int32_t k_max, double *
         hipEventRecord(ev[2 * k + 1], c->stream);
         ++k;
         hipEventRecord(ev[2 * k], c->stream);
-        hipLaunchKernelGGL(k_dt, dim3(1), blk, 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
+        hipLaunchKernelGGL(k_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
         hipEventRecord(ev[2 * k + 1], c->stream);
         ++k;
         hipEventRecord(ev[2 * k], c->stream);

@@ -13,6 +13,7 @@
 namespace sw2dg_detail {
 
 constexpr int kBlock = 256;
+constexpr int kDtBlock = 1024;  // the one-block dt reductions
 
 // device-resident clock of the native forward (one per ctx).  Slot p holds the
 // values valid at the start of a step launched with parity p; the step
