@@ -72,24 +72,27 @@ def kernel_bytes(name, nc, ni, nb, n_gauges):
     array element the launch touches, read once / written once; int32
     indices, FP64 values; friction on, head loss off)."""
     table = {
-        # cells U, zb, beta, phi; edges L, R, nx, ny; F out (+ length, swept out)
-        "k_e_edges": 48 * nc + 64 * ni,
+        # cells U, zb, beta, phi; internal edges L, R, nx, ny, F out (+ length and
+        # swept out in the dry run); boundary edges cell, nx, ny, series, F out
+        "k_e_edges": 48 * nc + 64 * ni + 48 * nb,
         # cells U, zb, beta, area, CSR ptrs; incidences + swept; boundary edges
         "k_e_cfl": 56 * nc + 24 * ni + 28 * nb,
         "k_e_dt": 8 * ((nc + 255) // 256),
-        # cells U in/out, phi, area, zb, beta, K1, K2, cos, sin, ptrs; incidences,
-        # F, edge geometry; boundary edges
-        "k_e_update": 120 * nc + 72 * ni + 32 * nb,
-        # + lam in, lams/part out, gphi/gK1/gK2 read-modify-write, gbce out
-        "k_r_cell": 216 * nc + 72 * ni + 40 * nb,
-        # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, L; sides out
+        # cells U in/out, phi, area, K1, K2, cos, sin, ptrs; incidences, F, edge
+        # geometry; boundary incidences, geometry and F
+        "k_e_update<false>": 104 * nc + 72 * ni + 52 * nb,
+        # + balanced state and internal increments out
+        "k_e_update<true>": 152 * nc + 72 * ni + 52 * nb,
+        # cells part in/out, ptrs, P in, phi, K1, K2, cos, sin, lams out,
+        # gphi/gK1/gK2 read-modify-write; incidences + edge sides
+        "k_r_gc<true>": 216 * nc + 72 * ni,
+        "k_r_gc<false>": 80 * nc + 72 * ni,
+        # boundary cells: state, statics, lams, part, gphi; edges geometry, F, gbce
+        "k_r_bnd": 160 * nb,
+        # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, length; sides out
         "k_r_edge": 80 * nc + 96 * ni,
-        # cells part, ptrs, lam out, gphi rmw; incidences + sides
-        "k_r_gather": 72 * nc + 72 * ni,
         "k_gauge_sample": 16 * n_gauges,
         "k_cost_gauges": 32 * n_gauges,
-        "k_seed_gauges": 48 * n_gauges,
-        "k_r_bc_reduce": 12 * nb,
         "k_cost_total": 64,
     }
     return table.get(name)

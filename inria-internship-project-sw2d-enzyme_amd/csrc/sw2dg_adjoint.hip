@@ -57,6 +57,7 @@ struct AssimState {
     int *d_gcell = nullptr;
     double *d_gw = nullptr, *d_gobs = nullptr;
     double *d_gser = nullptr;  // [n_obs][n_gauges] depths sampled by the last dry run
+    int *d_gfirst = nullptr, *d_gnext = nullptr;  // per-cell gauge chains (gauge order)
     double *d_fh = nullptr, *d_fqx = nullptr, *d_fqy = nullptr, *d_wh = nullptr, *d_wq = nullptr;
     // schedule
     int stride = 0, n_tiles = 1;
@@ -71,7 +72,9 @@ struct AssimState {
     int64_t n_ckpt = 0;
     double *d_seg = nullptr;   // (stride + 1) x 3nc
     // work
-    double *d_F = nullptr;     // 5 n_int
+    double *d_F = nullptr;     // 5 n_int + 3 n_bnd: edge fluxes of the dry-run step
+    double *d_Fseg = nullptr;  // stride x the same: fluxes of every step of the replayed segment
+    double *d_Pseg = nullptr;  // stride x 6 nc: balanced state + internal increments per step
     double *d_sw = nullptr;    // 2 n_int
     double *d_partial = nullptr;
     double *d_J = nullptr;     // [1 + n_obs]: total, then the cost of each observation
@@ -143,15 +146,51 @@ __device__ __forceinline__ double smax_(double a, double b) { return (a < b) ? b
 // _E forward kernels
 // ---------------------------------------------------------------------------
 
-// per internal edge: Godunov _E states, HLLC_E -> F[5e..5e+4]; optional CFL swept terms
-__global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls cs,
+// interpolateTimeSeries_E (assim/swSpE.h:138-153)
+__device__ double interp_e(const double *tt, const double *vv, int n, double t) {
+    if (n == 0) return 0.0;
+    if (n == 1 || t <= tt[0] - 1.0e-12) return vv[0];
+    if (t >= tt[n - 1]) return vv[n - 1];
+    int it = 0;
+    while (it < n - 1 && t > tt[it + 1]) it++;
+    const double t1 = tt[it], t2 = tt[it + 1], v1 = vv[it], v2 = vv[it + 1];
+    const double den = t2 - t1;
+    if (den < 1e-12) return v1;
+    return v1 + (v2 - v1) * (t - t1) / den;
+}
+
+// Edge fluxes of one _E step.  Threads [0, n_int): internal edges, Godunov _E
+// states + HLLC_E -> F[5e..5e+4] (+ CFL swept terms in the dry run).
+// Threads [n_int, n_int + n_bnd): boundary edges, hypBoundaryFlux_E ->
+// F[5 n_int + 3j ..] with the step's boundary value (bcv, or interpolated at
+// T[n] in the dry run, as build_checkpoint_records does).
+__global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls cs, BcDev bc,
                                                     const double *__restrict__ U,
                                                     double *__restrict__ F, double *__restrict__ sw,
-                                                    const double *T, int64_t n, double t_max) {
+                                                    const double *T, int64_t n, double t_max,
+                                                    const double *bcv) {
     if (T && !(T[n] < t_max)) return;
     const int e = xcd_block() * kBlock + (int)threadIdx.x;
-    if (e >= m.n_int) return;
     const int nc = m.n_cell;
+    if (e >= m.n_int) {
+        const int j = e - m.n_int;
+        if (j >= m.n_bnd) return;
+        const int i = m.bcell[j];
+        const double h = U[i];
+        const spe::Side sd = spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j],
+                                         m.bny[j], o.h_min);
+        const int ser = m.bser[j];
+        const double v = bcv ? bcv[ser]
+                             : interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n]);
+        double f[3];
+        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], cs.phi[i], m.zb[i], sd.z, sd.q, sd.r, sd.lp,
+                     bc.type[ser], v, f);
+        double *Fb = F + 5 * (size_t)m.n_int + 3 * (size_t)j;
+        Fb[0] = f[0];
+        Fb[1] = f[1];
+        Fb[2] = f[2];
+        return;
+    }
     const int L = m.eL[e], R = m.eR[e];
     const double nx = m.enx[e], ny = m.eny[e];
     const double hL = U[L], hR = U[R];
@@ -172,19 +211,6 @@ __global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls 
         sw[2 * e] = (hL > o.h_min) ? m1 * len : 0.0;
         sw[2 * e + 1] = (hR > o.h_min) ? m1 * len : 0.0;
     }
-}
-
-// interpolateTimeSeries_E (assim/swSpE.h:138-153)
-__device__ double interp_e(const double *tt, const double *vv, int n, double t) {
-    if (n == 0) return 0.0;
-    if (n == 1 || t <= tt[0] - 1.0e-12) return vv[0];
-    if (t >= tt[n - 1]) return vv[n - 1];
-    int it = 0;
-    while (it < n - 1 && t > tt[it + 1]) it++;
-    const double t1 = tt[it], t2 = tt[it + 1], v1 = vv[it], v2 = vv[it + 1];
-    const double den = t2 - t1;
-    if (den < 1e-12) return v1;
-    return v1 + (v2 - v1) * (t - t1) / den;
 }
 
 __device__ __forceinline__ double wave_min(double v) {
@@ -291,22 +317,20 @@ struct CellSt {
 };
 
 // balance of one cell (hypComputeBalance_E, swSpE_hyp.cpp:336-394) before the
-// round-off clean-up; boundary fluxes computed here from the cell state.
+// round-off clean-up, from the step's stored edge fluxes: boundary edges
+// first, then internal edges, each in reference order.
+// dint (optional): sum of the internal-edge contributions alone, in the same order
 __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, const Controls &cs,
-                                            const BcDev &bc, const double *U, const double *F,
-                                            const double *bcv, int i, double dt) {
+                                            const double *U, const double *F, int i, double dt,
+                                            CellSt *dint = nullptr) {
     const int nc = m.n_cell;
-    const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i];
-    CellSt s{h0, qx0, qy0};
+    CellSt s{U[i], U[nc + i], U[2 * nc + i]};
     const double phi = cs.phi[i], area = m.area[i];
+    const double *Fb = F + 5 * (size_t)m.n_int;
     for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
         const int j = m.bnd_inc[k];
         const double nx = m.bnx[j], ny = m.bny[j];
-        const spe::Side sd = spe::lr_fwd(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny, o.h_min);
-        const int ser = m.bser[j];
-        double f[3];
-        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp,
-                     bc.type[ser], bcv[ser], f);
+        const double *f = Fb + 3 * (size_t)j;
         const double co = spe::edge_coeff(m.blen[j], area, phi, o.phi_min);
         s.h += (+co) * f[0] * dt;
         s.qx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
@@ -321,18 +345,25 @@ __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, con
         s.h += a;
         s.qx += b;
         s.qy += c;
+        if (dint) {
+            dint->h += a;
+            dint->qx += b;
+            dint->qy += c;
+        }
     }
     return s;
 }
 
-// per cell: balance, clean-up, clamp, friction _E, head loss _E
-__global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls cs, BcDev bc,
+// per cell: balance, clean-up, clamp, friction _E, head loss _E.  STORE (the
+// replay before a reverse sweep): also keep the balanced state before the
+// clean-up and the sum of its internal-edge increments, P = [pre | dint]
+template <bool STORE>
+__global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls cs,
                                                      const double *__restrict__ Uin,
                                                      double *__restrict__ Uout,
                                                      const double *__restrict__ F,
-                                                     const double *dtv, const double *bcv_all,
-                                                     int n_series, double *T, int64_t n,
-                                                     double t_max) {
+                                                     const double *dtv, double *T, int64_t n,
+                                                     double t_max, double *__restrict__ P) {
     const int nc = m.n_cell;
     if (T && !(T[n] < t_max)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) T[n + 1] = T[n];
@@ -347,7 +378,16 @@ __global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= nc) return;
     const double dt = dtv[n];
-    CellSt s = e_balance(m, o, cs, bc, Uin, F, bcv_all + n * n_series, i, dt);
+    CellSt dint{0.0, 0.0, 0.0};
+    CellSt s = e_balance(m, o, cs, Uin, F, i, dt, STORE ? &dint : nullptr);
+    if (STORE) {
+        P[i] = s.h;
+        P[nc + i] = s.qx;
+        P[2 * nc + i] = s.qy;
+        P[3 * nc + i] = dint.h;
+        P[4 * nc + i] = dint.qx;
+        P[5 * nc + i] = dint.qy;
+    }
     if (fabs(s.h) < kSmall) s.h = 0.0;
     if (fabs(s.qx) < kSmall) s.qx = 0.0;
     if (fabs(s.qy) < kSmall) s.qy = 0.0;
@@ -430,44 +470,43 @@ __global__ void k_gauge_sample(const double *__restrict__ U, const int *gcell, i
 // reverse kernels
 // ---------------------------------------------------------------------------
 
-// cost seeds: lam += d J_obs / d U
-__global__ void k_seed_gauges(const double *__restrict__ U, const int *gcell, const double *gw,
-                              const double *gobs, int n_g, double norm, double *lam) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int g = 0; g < n_g; ++g) {
-        const int c = gcell[g];
-        lam[c] += 2.0 * gw[g] * (U[c] - gobs[g]) / norm;
+// cost seed of one observation, applied where the adjoint of the observed
+// state is formed: dJ_obs/dU of the gauge misfit (cells chained through
+// gfirst/gnext in gauge order) and of the frame misfit
+struct SeedArgs {
+    int oi;                               // observation index, -1 = none
+    const double *U;                      // the observed state
+    const int *gfirst, *gnext;            // per-cell gauge chain (-1 ends)
+    const double *gw, *gobs;              // weights, this observation's depths
+    const double *fh, *fqx, *fqy, *wh, *wq;  // this observation's frame (fh = nullptr: none)
+    double norm;
+};
+
+__device__ __forceinline__ void seed_cell(const SeedArgs &sd, int nc, int i, double &ah, double &ax,
+                                          double &ay) {
+    if (sd.oi < 0) return;
+    if (sd.gfirst)
+        for (int g = sd.gfirst[i]; g >= 0; g = sd.gnext[g]) ah += 2.0 * sd.gw[g] * (sd.U[i] - sd.gobs[g]) / sd.norm;
+    if (sd.fh) {
+        ah += 2.0 * sd.wh[i] * (sd.U[i] - sd.fh[i]) / sd.norm;
+        ax += 2.0 * sd.wq[i] * (sd.U[nc + i] - sd.fqx[i]) / sd.norm;
+        ay += 2.0 * sd.wq[i] * (sd.U[2 * nc + i] - sd.fqy[i]) / sd.norm;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_seed_frame(const double *__restrict__ U, int nc,
-                                                        const double *fh, const double *fqx,
-                                                        const double *fqy, const double *wh,
-                                                        const double *wq, double norm, double *lam) {
-    const int i = blockIdx.x * kBlock + (int)threadIdx.x;
-    if (i >= nc) return;
-    lam[i] += 2.0 * wh[i] * (U[i] - fh[i]) / norm;
-    lam[nc + i] += 2.0 * wq[i] * (U[nc + i] - fqx[i]) / norm;
-    lam[2 * nc + i] += 2.0 * wq[i] * (U[2 * nc + i] - fqy[i]) / norm;
-}
-
-// per cell: reverse of sources, clamp and clean-up; boundary fluxes; factor
-// derivatives.  lam (in) = adjoint of U^{n+1}; lams (out) = adjoint of the
-// post-balance state; part (out) = cell-local part of the adjoint of U^n.
-__global__ __launch_bounds__(kBlock) void k_r_cell(MeshDev m, Opts o, Controls cs, BcDev bc,
-                                                   const double *__restrict__ U,
-                                                   const double *__restrict__ F, double dt,
-                                                   const double *__restrict__ bcv,
-                                                   const double *__restrict__ lam,
-                                                   double *__restrict__ lams,
-                                                   double *__restrict__ part, double *gphi,
-                                                   double *gK1, double *gK2, double *gbce) {
-    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+// Reverse of the cell-local part of one step (sources, clamp, clean-up) and
+// the porosity derivative of its internal-edge factors.  (ah, ax, ay) in =
+// adjoint of U^{n+1}; out: lams = adjoint of the balanced state, part = the
+// cell-local part of the adjoint of U^n (boundary cells: completed by
+// k_r_bnd).  P = [balanced state | internal-edge increments] from the replay.
+__device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, const Controls &cs,
+                                             const double *__restrict__ P, double dt, int i,
+                                             double ah, double ax, double ay,
+                                             double *__restrict__ lams, double *__restrict__ part,
+                                             double &gp, double *gK1, double *gK2) {
     const int nc = m.n_cell;
-    if (i >= nc) return;
-    // forward recompute of the cell
-    const CellSt pre = e_balance(m, o, cs, bc, U, F, bcv, i, dt);
-    CellSt s = pre;
+    const CellSt dint{P[3 * nc + i], P[4 * nc + i], P[5 * nc + i]};
+    CellSt s{P[i], P[nc + i], P[2 * nc + i]};
     const bool kh = !(fabs(s.h) < kSmall), kx = !(fabs(s.qx) < kSmall), ky = !(fabs(s.qy) < kSmall);
     if (!kh) s.h = 0.0;
     if (!kx) s.qx = 0.0;
@@ -479,13 +518,12 @@ __global__ __launch_bounds__(kBlock) void k_r_cell(MeshDev m, Opts o, Controls c
         s.qy = 0.;
     }
     const CellSt after_clamp = s;
-    if (o.friction != sp::FRIC_NONE)
-        spe::fric_fwd(o.h_min, s.h, s.qx, s.qy, m.fcos[i], m.fsin[i], cs.K1[i], cs.K2[i], dt);
-    // reverse
-    double ah = lam[i], ax = lam[nc + i], ay = lam[2 * nc + i];
-    if (o.head_loss)
+    if (o.head_loss) {
+        if (o.friction != sp::FRIC_NONE)
+            spe::fric_fwd(o.h_min, s.h, s.qx, s.qy, m.fcos[i], m.fsin[i], cs.K1[i], cs.K2[i], dt);
         spe::hl_vjp(o.h_min, s.h, s.qx, s.qy, m.hcos[i], m.hsin[i], m.hsing[i], m.hk1[i], m.hk2[i],
                     dt, ah, ax, ay);
+    }
     if (o.friction != sp::FRIC_NONE) {
         double dk1 = 0.0, dk2 = 0.0;
         spe::fric_vjp(o.h_min, after_clamp.h, after_clamp.qx, after_clamp.qy, m.fcos[i], m.fsin[i],
@@ -504,18 +542,105 @@ __global__ __launch_bounds__(kBlock) void k_r_cell(MeshDev m, Opts o, Controls c
     lams[i] = ah;
     lams[nc + i] = ax;
     lams[2 * nc + i] = ay;
-    // balance: identity part + boundary fluxes + factor (porosity) derivatives
-    double ph = ah, px = ax, py = ay, gp = 0.0;
+    // balance: identity part; internal-edge factors co_e = w_e / area / phi
+    // enter every contribution linearly, so
+    // sum_e (dJ/dco_e) (dco_e/dphi) = -(lam . dint) / phi
+    part[i] = ah;
+    part[nc + i] = ax;
+    part[2 * nc + i] = ay;
+    const double phi = cs.phi[i];
+    if (phi > o.phi_min) gp += -((ah * dint.h + ax * dint.qx) + ay * dint.qy) / phi;
+}
+
+// One fused reverse kernel per step boundary n:
+//   gather  the adjoint of U^n = part + the edge-side contributions of step
+//           n in the cell's reference incidence order (reset: the tile top,
+//           the adjoint restarts from 0; the porosity terms still count)
+//   seed    + dJ_obs/dU^n when U^n is observed
+//   CELL    reverse of the cell part of step n-1 (else: write lam = adj U^n)
+// Blocks past n_cell_blocks sum the boundary-value adjoints of step n per
+// series (one block per series, fixed order).
+template <bool CELL>
+__global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
+                                                 double *__restrict__ part,
+                                                 const double *__restrict__ side, int gather,
+                                                 SeedArgs sd, const double *__restrict__ P,
+                                                 double dt, double *__restrict__ lams,
+                                                 double *__restrict__ lam, double *gphi,
+                                                 double *gK1, double *gK2, int n_cell_blocks,
+                                                 const int *ser_ptr, const int *ser_edges,
+                                                 const double *gbce, double *gbc_n) {
+    if ((int)blockIdx.x >= n_cell_blocks) {
+        __shared__ double red[kBlock];
+        const int k = (int)blockIdx.x - n_cell_blocks;
+        double v = 0.0;
+        for (int q = ser_ptr[k] + (int)threadIdx.x; q < ser_ptr[k + 1]; q += kBlock) v += gbce[ser_edges[q]];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) gbc_n[k] = red[0];
+        return;
+    }
+    const int i = xcd_remap((int)blockIdx.x, n_cell_blocks) * kBlock + (int)threadIdx.x;
+    const int nc = m.n_cell;
+    if (i >= nc) return;
+    double ah = 0.0, ax = 0.0, ay = 0.0, gp = 0.0;
+    if (gather >= 0) {
+        const double a0 = part[i], b0 = part[nc + i], c0 = part[2 * nc + i];
+        double a = a0, b = b0, c = c0;
+        for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+            const int inc = m.int_inc[k];
+            const double *sv = side + 8 * (size_t)(inc >> 1) + 4 * (inc & 1);
+            a += sv[0];
+            b += sv[1];
+            c += sv[2];
+            gp += sv[3];
+        }
+        if (gather > 0) {  // not a tile top
+            ah = a;
+            ax = b;
+            ay = c;
+        }
+    }
+    seed_cell(sd, nc, i, ah, ax, ay);
+    if (CELL) {
+        cell_reverse(m, o, cs, P, dt, i, ah, ax, ay, lams, part, gp, gK1, gK2);
+    } else {
+        lam[i] = ah;
+        lam[nc + i] = ax;
+        lam[2 * nc + i] = ay;
+    }
+    gphi[i] += gp;
+}
+
+// per boundary cell: reverse of its boundary fluxes (hypBoundaryFlux_E and
+// the Godunov _E states) and of their factors, in reference order, added to
+// the cell part of the adjoint; boundary-value adjoint per edge
+__global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs, BcDev bc,
+                                                  const double *__restrict__ U,
+                                                  const double *__restrict__ F, double dt,
+                                                  const double *__restrict__ bcv,
+                                                  const double *__restrict__ lams,
+                                                  double *__restrict__ part, double *gphi,
+                                                  double *gbce) {
+    const int t = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (t >= m.n_bnd_cells) return;
+    const int i = m.bnd_cells[t];
+    const int nc = m.n_cell;
+    const double ah = lams[i], ax = lams[nc + i], ay = lams[2 * nc + i];
+    double ph = part[i], px = part[nc + i], py = part[2 * nc + i], gp = 0.0;
     const double phi = cs.phi[i], area = m.area[i];
     const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i];
+    const double *Fb = F + 5 * (size_t)m.n_int;
     for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
         const int j = m.bnd_inc[k];
         const double nx = m.bnx[j], ny = m.bny[j];
+        const double *f = Fb + 3 * (size_t)j;
         const spe::Side sd = spe::lr_fwd(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny, o.h_min);
         const int ser = m.bser[j];
-        double f[3];
-        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp,
-                     bc.type[ser], bcv[ser], f);
         const double co = spe::edge_coeff(m.blen[j], area, phi, o.phi_min);
         double df[3];
         df[0] = ah * co * dt;
@@ -529,15 +654,6 @@ __global__ __launch_bounds__(kBlock) void k_r_cell(MeshDev m, Opts o, Controls c
         spe::lr_vjp(h0, qx0, qy0, m.beta[i], nx, ny, o.h_min, d[0], d[1], d[2], 0.0, d[3], ph, px, py);
         gp += d[4];
         gbce[j] = d[5];
-    }
-    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
-        const int inc = m.int_inc[k];
-        const int e = inc >> 1, side = inc & 1;
-        const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
-        double dF[5] = {0, 0, 0, 0, 0};
-        const double dco = spe::side_contrib_vjp(side, co, m.enx[e], m.eny[e], F + 5 * (size_t)e,
-                                                 dt, ah, ax, ay, dF);
-        gp += dco * spe::edge_coeff_dphi(m.elen[e], area, phi, o.phi_min);
     }
     part[i] = ph;
     part[nc + i] = px;
@@ -584,47 +700,6 @@ __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls c
     so[7] = d[11];
 }
 
-// per cell: adjoint of U^n = cell part + edge-side contributions, fixed order
-__global__ __launch_bounds__(kBlock) void k_r_gather(MeshDev m, const double *__restrict__ part,
-                                                     const double *__restrict__ side,
-                                                     double *__restrict__ lam, double *gphi) {
-    const int i = xcd_block() * kBlock + (int)threadIdx.x;
-    const int nc = m.n_cell;
-    if (i >= nc) return;
-    double a = part[i], b = part[nc + i], c = part[2 * nc + i], g = 0.0;
-    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
-        const int inc = m.int_inc[k];
-        const double *s = side + 8 * (size_t)(inc >> 1) + 4 * (inc & 1);
-        a += s[0];
-        b += s[1];
-        c += s[2];
-        g += s[3];
-    }
-    lam[i] = a;
-    lam[nc + i] = b;
-    lam[2 * nc + i] = c;
-    gphi[i] += g;
-}
-
-// per series: sum of the boundary-value adjoints of its edges, reference order
-// One block per series: each thread sums a strided slice of the series'
-// edges in order, then a fixed tree over the block -- the same order on
-// every run (deterministic), and no long serial dependency chain.
-__global__ __launch_bounds__(kBlock) void k_r_bc_reduce(const int *ser_ptr, const int *ser_edges,
-                                                        const double *gbce, double *gbc_n) {
-    __shared__ double red[kBlock];
-    const int k = blockIdx.x;
-    double s = 0.0;
-    for (int j = ser_ptr[k] + (int)threadIdx.x; j < ser_ptr[k + 1]; j += kBlock) s += gbce[ser_edges[j]];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gbc_n[k] = red[0];
-}
-
 __global__ void k_zero(double *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -665,21 +740,36 @@ double *ckpt_state(AssimState *a, sw2dg_ctx *c, int64_t k) {
     return a->d_ckpt + (size_t)k * 3 * c->nc;
 }
 
+// fluxes of one step: 5 per internal edge, then 3 per boundary edge
+size_t flux_len(const sw2dg_ctx *c) {
+    return 5 * (size_t)std::max<int64_t>(c->ni, 1) + 3 * (size_t)std::max<int64_t>(c->nb, 1);
+}
+
+double *seg_flux(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Fseg + (size_t)j * flux_len(c); }
+double *seg_pre(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Pseg + (size_t)j * 6 * c->nc; }
+
+// one _E step U_in -> U_out; the edge fluxes go to F (kept for the reverse
+// sweep when F is a slot of the segment flux buffer)
 void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout, int64_t n,
-                   bool dry) {
+                   bool dry, double *F, double *P = nullptr) {
     const dim3 blk(kBlock);
     const Controls cs = ctl(a);
-    KLAUNCH(a, k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Uin,
-                       a->d_F, dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max);
+    const int ns = std::max(c->n_series, 1);
+    KLAUNCH(a, k_e_edges, dim3(nblocks(c->ni + c->nb)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
+            Uin, F, dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max,
+            dry ? nullptr : a->d_bc + (size_t)n * ns);
     if (dry) {
         KLAUNCH(a, k_e_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
                 a->d_ck, a->d_partial, a->d_T, n);
         KLAUNCH(a, k_e_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
                 a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
     }
-    KLAUNCH(a, k_e_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin,
-                       Uout, a->d_F, a->d_dt, a->d_bc, std::max(c->n_series, 1),
-                       dry ? a->d_T : nullptr, n, c->prm.t_max);
+    if (P)
+        KLAUNCH(a, k_e_update<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, Uin,
+                Uout, F, a->d_dt, dry ? a->d_T : nullptr, n, c->prm.t_max, P);
+    else
+        KLAUNCH(a, k_e_update<false>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, Uin,
+                Uout, F, a->d_dt, dry ? a->d_T : nullptr, n, c->prm.t_max, P);
     if (dry && a->n_gauges > 0)
         KLAUNCH(a, k_gauge_sample, dim3(1), dim3(64), 0, c->stream, Uout, a->d_gcell,
                            a->n_gauges, a->d_trig, n, a->d_gser);
@@ -700,17 +790,28 @@ void launch_cost(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
     }
 }
 
-void launch_seed(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
-    const double norm = (a->normalize && a->n_obs > 0) ? (double)a->n_obs : 1.0;
-    if (a->n_gauges > 0)
-        KLAUNCH(a, k_seed_gauges, dim3(1), dim3(64), 0, c->stream, U, a->d_gcell, a->d_gw,
-                           a->d_gobs + (size_t)oi * a->n_gauges, a->n_gauges, norm, a->d_lam);
+SeedArgs seed_args(sw2dg_ctx *c, AssimState *a, int oi, const double *U) {
+    SeedArgs sd{};
+    sd.oi = -1;
+    if (oi < 0) return sd;
+    sd.oi = oi;
+    sd.U = U;
+    sd.norm = (a->normalize && a->n_obs > 0) ? (double)a->n_obs : 1.0;
+    if (a->n_gauges > 0) {
+        sd.gfirst = a->d_gfirst;
+        sd.gnext = a->d_gnext;
+        sd.gw = a->d_gw;
+        sd.gobs = a->d_gobs + (size_t)oi * a->n_gauges;
+    }
     if (a->frames) {
         const size_t o = (size_t)oi * c->nc;
-        KLAUNCH(a, k_seed_frame, dim3(nblocks(c->nc)), dim3(kBlock), 0, c->stream, U,
-                           (int)c->nc, a->d_fh + o, a->d_fqx + o, a->d_fqy + o, a->d_wh, a->d_wq,
-                           norm, a->d_lam);
+        sd.fh = a->d_fh + o;
+        sd.fqx = a->d_fqx + o;
+        sd.fqy = a->d_fqy + o;
+        sd.wh = a->d_wh;
+        sd.wq = a->d_wq;
     }
+    return sd;
 }
 
 // segments, window and tiles (get_spatial_gradient_checkpointed, :785-953)
@@ -760,15 +861,18 @@ int fetch_record(sw2dg_ctx *c, AssimState *a, std::vector<double> &T, std::vecto
 
 // replay segment k from its checkpoint into the segment buffer; cost of its
 // observation steps added to J when with_cost
+// replay segment k from its checkpoint (or `start`) into the segment
+// buffers; store = keep what the reverse sweep reads (balanced states)
 int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int> &trig,
-                   bool with_cost, const double *start) {
+                   bool with_cost, const double *start, bool store = false) {
     const int64_t S = a->stride;
     const int64_t n0 = k * S, n1 = std::min(a->n_steps, n0 + S);
     HIPCHK(c, hipMemcpyAsync(seg_state(a, c, 0), start ? start : ckpt_state(a, c, k),
                              3 * c->nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     for (int64_t n = n0; n < n1; ++n) {
         const int j = (int)(n - n0);
-        launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), n, false);
+        launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), n, false, seg_flux(a, c, j),
+                      store ? seg_pre(a, c, j) : nullptr);
         if (with_cost && trig[n] >= 0) launch_cost(c, a, seg_state(a, c, j + 1), trig[n]);
     }
     return SW2DG_OK;
@@ -816,6 +920,17 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
         a->d_gw = a->alloc<double>(obs->n_gauges);
         a->d_gobs = a->alloc<double>((size_t)obs->n_gauges * std::max(1, obs->n_obs));
         a->d_gser = a->alloc<double>((size_t)obs->n_gauges * std::max(1, obs->n_obs));
+        std::vector<int> gfirst(nc, -1), gnext(obs->n_gauges, -1), glast(nc, -1);
+        for (int g = 0; g < obs->n_gauges; ++g) {
+            const int cell = gc[g];
+            if (glast[cell] < 0) gfirst[cell] = g;
+            else gnext[glast[cell]] = g;
+            glast[cell] = g;
+        }
+        a->d_gfirst = a->alloc<int>(nc);
+        a->d_gnext = a->alloc<int>(obs->n_gauges);
+        HIPCHK(c, hipMemcpy(a->d_gfirst, gfirst.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(a->d_gnext, gnext.data(), gnext.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(a->d_gcell, gc.data(), gc.size() * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(a->d_gw, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(a->d_gobs, obs->gauge_h,
@@ -853,7 +968,9 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->n_ckpt = (max_steps + a->stride - 1) / a->stride + 1;
     a->d_ckpt = a->alloc<double>((size_t)a->n_ckpt * 3 * nc);
     a->d_seg = a->alloc<double>((size_t)(a->stride + 1) * 3 * nc);
-    a->d_F = a->alloc<double>(5 * (size_t)std::max<int64_t>(ni, 1));
+    a->d_F = a->alloc<double>(flux_len(c));
+    a->d_Fseg = a->alloc<double>((size_t)a->stride * flux_len(c));
+    a->d_Pseg = a->alloc<double>((size_t)a->stride * 6 * (size_t)nc);
     a->d_sw = a->alloc<double>(2 * (size_t)std::max<int64_t>(ni, 1));
     a->d_partial = a->alloc<double>(nblocks(nc));
     a->d_cpart = a->alloc<double>(nblocks(nc));
@@ -923,7 +1040,7 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
         const int64_t n1 = std::min(a->max_steps, n + S);
         for (int64_t m2 = n; m2 < n1; ++m2) {
             const int j = (int)(m2 - n);
-            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true);
+            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true, a->d_F);
         }
         HIPCHK(c, hipMemcpyAsync(ckpt_state(a, c, k + 1), seg_state(a, c, (int)(n1 - n)),
                                  3 * nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -1028,29 +1145,57 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     zero(c, a->d_gbc, (size_t)std::max<int64_t>(a->n_steps, 1) * ns);
     const Controls cs = ctl(a);
     const dim3 blk(kBlock);
+    const int ncb = nblocks(nc);
+    // fused gather (step n) + cell reverse (step n-1) [+ boundary-value sums of step n]
+    auto gc = [&](bool cell, int gather, int64_t bc_step, const SeedArgs &sd, const double *P,
+                  double dtv) {
+        const int extra = (bc_step >= 0) ? c->n_series : 0;
+        double *gbcn = (bc_step >= 0) ? a->d_gbc + (size_t)bc_step * ns : nullptr;
+        if (cell)
+            KLAUNCH(a, k_r_gc<true>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
+                    a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn);
+        else
+            KLAUNCH(a, k_r_gc<false>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
+                    a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn);
+    };
+    auto bnd = [&](int64_t n, int j) {
+        if (c->m.n_bnd_cells > 0)
+            KLAUNCH(a, k_r_bnd, dim3(nblocks(c->m.n_bnd_cells)), blk, 0, c->stream, c->m, c->o, cs,
+                    c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
+                    a->d_lams, a->d_part, a->d_gphi, a->d_gbce);
+    };
+    bool started = false;
+    int64_t n_last = -1;  // first step of the last reversed segment
     for (int64_t k = p.K - 1; k >= p.first; --k) {
-        if (p.tile_top[k]) zero(c, a->d_lam, 3 * nc);
-        // forward recompute of segment k from its checkpoint (with its costs)
-        if ((st = replay_segment(c, a, k, trig, true, nullptr))) return st;
+        // forward recompute of segment k from its checkpoint (with its costs);
+        // the boundary n1 between segments k and k+1 is crossed after it
+        if ((st = replay_segment(c, a, k, trig, true, nullptr, true))) return st;
         const int64_t n0 = k * a->stride, n1 = std::min(a->n_steps, n0 + a->stride);
+        {
+            const int64_t nt = n1 - 1;
+            const SeedArgs sd = seed_args(c, a, trig[nt], seg_state(a, c, (int)(n1 - n0)));
+            if (!started)
+                gc(true, -1, -1, sd, seg_pre(a, c, (int)(nt - n0)), dt[nt]);
+            else  // tile top: the adjoint restarts from 0
+                gc(true, p.tile_top[k] ? 0 : 1, n1, sd, seg_pre(a, c, (int)(nt - n0)), dt[nt]);
+            started = true;
+            bnd(nt, (int)(nt - n0));
+        }
         for (int64_t n = n1 - 1; n >= n0; --n) {
             const int j = (int)(n - n0);
-            const double *Un = seg_state(a, c, j);
-            if (trig[n] >= 0) launch_seed(c, a, seg_state(a, c, j + 1), trig[n]);
-            KLAUNCH(a, k_e_edges, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
-                               a->d_F, nullptr, nullptr, n, 0.0);
-            KLAUNCH(a, k_r_cell, dim3(nblocks(nc)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
-                               Un, a->d_F, dt[n], a->d_bc + (size_t)n * ns, a->d_lam, a->d_lams,
-                               a->d_part, a->d_gphi, a->d_gK1, a->d_gK2, a->d_gbce);
-            KLAUNCH(a, k_r_edge, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs, Un,
-                               dt[n], a->d_lams, a->d_side);
-            KLAUNCH(a, k_r_gather, dim3(nblocks(nc)), blk, 0, c->stream, c->m, a->d_part,
-                               a->d_side, a->d_lam, a->d_gphi);
-            if (c->n_series > 0)
-                KLAUNCH(a, k_r_bc_reduce, dim3(c->n_series), blk, 0, c->stream, a->d_ser_ptr,
-                        a->d_ser_edges, a->d_gbce, a->d_gbc + (size_t)n * ns);
+            KLAUNCH(a, k_r_edge, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs,
+                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side);
+            if (n > n0) {
+                const SeedArgs sd = seed_args(c, a, trig[n - 1], seg_state(a, c, j));
+                gc(true, 1, n, sd, seg_pre(a, c, j - 1), dt[n - 1]);
+                bnd(n - 1, j - 1);
+            }
         }
+        n_last = n0;
     }
+    if (started) gc(false, 1, n_last, seed_args(c, a, -1, nullptr), nullptr, 0.0);
     KLAUNCH(a, k_cost_total, dim3(1), dim3(64), 0, c->stream, a->d_J, a->n_obs);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));

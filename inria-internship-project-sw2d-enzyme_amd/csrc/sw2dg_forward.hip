@@ -706,14 +706,19 @@ int sw2dg_ctx_create(const sw2dg_mesh *mesh, const sw2dg_params *prm, int32_t de
         eym[k] = mesh->int_ym[e];
     }
     std::vector<double> blen(nb), bnx(nb), bny(nb);
-    std::vector<int> bser(nb);
+    std::vector<int> bser(nb), bcell(std::max<int64_t>(nb, 1), 0), bcells;
     for (int64_t k = 0; k < nb; ++k) {
         const int e = c->bperm[k];
         blen[k] = mesh->bnd_len[e];
         bnx[k] = mesh->bnd_nx[e];
         bny[k] = mesh->bnd_ny[e];
         bser[k] = mesh->bnd_series[e];
+        bcell[k] = c->inv[mesh->bnd_cell[e]];
     }
+    for (int64_t i = 0; i < nc; ++i)
+        if (bptr[i + 1] > bptr[i]) bcells.push_back((int)i);
+    const int n_bcells = (int)bcells.size();
+    if (bcells.empty()) bcells.push_back(0);
     c->bser_host.assign(mesh->bnd_series, mesh->bnd_series + nb);
     MeshDev &m = c->m;
     m.n_cell = (int)nc;
@@ -756,6 +761,9 @@ int sw2dg_ctx_create(const sw2dg_mesh *mesh, const sw2dg_params *prm, int32_t de
     m.bnx = c->upload(bnx);
     m.bny = c->upload(bny);
     m.bser = c->upload(bser);
+    m.bcell = c->upload(bcell);
+    m.bnd_cells = c->upload(bcells);
+    m.n_bnd_cells = n_bcells;
     m.bF1 = c->dalloc<double>(nb);
     m.partial = c->dalloc<double>(nblocks(nc));
     c->ck = c->dalloc<Clock>(1);

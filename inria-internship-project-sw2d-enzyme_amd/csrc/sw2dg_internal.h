@@ -60,6 +60,9 @@ struct MeshDev {
     // boundary edges (renumbered)
     const double *blen, *bnx, *bny;
     const int *bser;
+    const int *bcell;              // cell of each boundary edge
+    const int *bnd_cells;          // cells with at least one boundary edge, ascending
+    int n_bnd_cells;
     double *bF1;
     double *partial;               // [n_blocks of k_cfl]: per-block dt candidates
 };
@@ -69,14 +72,14 @@ struct Opts {
     double h_min, phi_min, v_max, fr_max;
 };
 
-__device__ __forceinline__ int xcd_block() {
-    // Workgroup b is dispatched to XCD b % 8.  Give each XCD a contiguous
-    // range of logical blocks (bijection on [0, gridDim.x)).
-    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+// Workgroup b is dispatched to XCD b % 8.  Give each XCD a contiguous range
+// of logical blocks (bijection on [0, nb)).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
     const int q = nb >> 3, r = nb & 7;
     const int x = b & 7, k = b >> 3;
     return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_remap((int)blockIdx.x, (int)gridDim.x); }
 
 inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
