@@ -121,6 +121,12 @@ def _e_decl():
                                  _i32, C.c_int32, C.c_int32, _i64, _d, _d, C.c_int32, _d, _d, _d,
                                  _d]
     L.orc_e_gradient.restype = C.c_double
+    L.orc_e_replay_frames.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d, _d,
+                                      _i32, C.c_int32, _d, _d, _d, C.c_int32]
+    L.orc_e_replay_frames.restype = C.c_double
+    L.orc_e_gradient_frames.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d,
+                                        _d, _i32, C.c_int32, _d, _d, _d, C.c_int32, _d, _d, _d, _d]
+    L.orc_e_gradient_frames.restype = C.c_double
     L._e_ready = True
     return L
 
@@ -205,6 +211,66 @@ def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=N
 
 
 OracleE.gradient = _oracle_e_gradient
+
+
+def _oracle_e_frame_cost(self, rec, frames, phi=None, K1=None, K2=None, w_h=None, w_q=None,
+                         normalize=False, n_obs=None):
+    """Frame-misfit cost of the frozen replay; frames [n_obs, 3, n_cell] or
+    [n_obs, n_cell, 3] (h, qx, qy)."""
+    a = self.case.a
+    phi = self._ctl(phi, a["phi"])
+    K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
+    K2 = self._ctl(K2, a["fc2"] if len(a["fc2"]) else np.zeros(self.nc))
+    fr = np.asarray(frames, dtype=np.float64)
+    if fr.shape[-1] == 3 and fr.shape[1] == self.nc:
+        fr = fr.transpose(0, 2, 1)
+    fr = np.ascontiguousarray(fr)
+    self._keep.append(fr)
+    dt = np.ascontiguousarray(rec["dt"], dtype=np.float64)
+    bcs = np.ascontiguousarray(rec["bc"], dtype=np.float64).reshape(-1)
+    trig = np.ascontiguousarray(rec["trig"], dtype=np.int32)
+    h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
+    wh = None if w_h is None else _dp(self._ctl(w_h, None))
+    wq = None if w_q is None else _dp(self._ctl(w_q, None))
+    nob = int(n_obs if n_obs is not None else fr.shape[0])
+    return _e_decl().orc_e_replay_frames(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
+                                         _dp(qx0), _dp(qy0), len(dt), _dp(dt), _dp(bcs),
+                                         trig.ctypes.data_as(_i32), nob, _dp(fr), wh, wq,
+                                         1 if normalize else 0)
+
+
+OracleE.frame_cost = _oracle_e_frame_cost
+
+
+def _oracle_e_frame_gradient(self, rec, frames, phi=None, K1=None, K2=None, normalize=False,
+                             n_obs=None):
+    """Frame-misfit cost and its gradient (uniform weights)."""
+    a = self.case.a
+    phi = self._ctl(phi, a["phi"])
+    K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
+    K2 = self._ctl(K2, a["fc2"] if len(a["fc2"]) else np.zeros(self.nc))
+    fr = np.asarray(frames, dtype=np.float64)
+    if fr.shape[-1] == 3 and fr.shape[1] == self.nc:
+        fr = fr.transpose(0, 2, 1)
+    fr = np.ascontiguousarray(fr)
+    self._keep.append(fr)
+    dt = np.ascontiguousarray(rec["dt"], dtype=np.float64)
+    n = len(dt)
+    bcs = np.ascontiguousarray(rec["bc"], dtype=np.float64).reshape(-1)
+    trig = np.ascontiguousarray(rec["trig"], dtype=np.int32)
+    h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
+    gphi, gK1, gK2 = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
+    gbc = np.empty(max(n, 1) * self.ns)
+    nob = int(n_obs if n_obs is not None else fr.shape[0])
+    J = _e_decl().orc_e_gradient_frames(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
+                                        _dp(qx0), _dp(qy0), n, _dp(dt), _dp(bcs),
+                                        trig.ctypes.data_as(_i32), nob, _dp(fr), None, None,
+                                        1 if normalize else 0, _dp(gphi), _dp(gK1), _dp(gK2),
+                                        _dp(gbc))
+    return {"J": J, "phi": gphi, "K1": gK1, "K2": gK2, "bc": gbc[: n * self.ns].reshape(n, self.ns)}
+
+
+OracleE.frame_gradient = _oracle_e_frame_gradient
 
 
 class OracleSolver:

@@ -908,12 +908,13 @@ static void fac_dphi(const orc_case *c, const double *phi, int64_t cell, double 
 /* Gradient of the frozen-replay gauge cost (orc_e_replay) with respect to
  * porosity, the two Strickler coefficients per cell and the boundary value of
  * every step and series.  Returns J. */
-double orc_e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
-                      const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
-                      const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
-                      int32_t n_obs, int32_t n_gauges, const int64_t *gauge_cell,
-                      const double *gauge_w, const double *obs, int32_t normalize, double *g_phi,
-                      double *g_K1, double *g_K2, double *g_bc) {
+static double e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
+                         const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
+                         const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
+                         int32_t n_obs, int32_t n_gauges, const int64_t *gauge_cell,
+                         const double *gauge_w, const double *obs, const double *frames,
+                         const double *w_h, const double *w_q, int32_t normalize, double *g_phi,
+                         double *g_K1, double *g_K2, double *g_bc) {
     const int64_t nc = c->n_cell;
     const int ns = c->n_series;
     const double hthr = (c->h_min > SMALL) ? c->h_min : SMALL;
@@ -942,6 +943,17 @@ double orc_e_gradient(const orc_case *c, const double *phi, const double *K1, co
             if (normalize && n_obs > 0) Jo /= (double)n_obs;
             J += Jo;
         }
+        if (oi >= 0 && oi < n_obs && frames) {
+            const double *fh = frames + (int64_t)oi * 3 * nc, *fqx = fh + nc, *fqy = fh + 2 * nc;
+            double Jo = 0.0;
+            for (int64_t i = 0; i < nc; i++) {
+                double dh = s.h[i] - fh[i], dqx = s.qx[i] - fqx[i], dqy = s.qy[i] - fqy[i];
+                double wh = w_h ? w_h[i] : 1.0, wq = w_q ? w_q[i] : 1.0;
+                Jo += wh * (dh * dh) + wq * ((dqx * dqx) + (dqy * dqy));
+            }
+            if (normalize && n_obs > 0) Jo /= (double)n_obs;
+            J += Jo;
+        }
     }
     /* reverse */
     double *lam = (double *)calloc(3 * (size_t)nc, sizeof(double));  /* adjoint of U^{n+1} */
@@ -960,6 +972,15 @@ double orc_e_gradient(const orc_case *c, const double *phi, const double *K1, co
                 int64_t gc = gauge_cell[g];
                 lam[gc] += 2.0 * gauge_w[g] * (Un1[gc] - obs[(int64_t)oi * n_gauges + g]) / norm;
             }
+        if (oi >= 0 && oi < n_obs && frames) {
+            const double *fh = frames + (int64_t)oi * 3 * nc, *fqx = fh + nc, *fqy = fh + 2 * nc;
+            for (int64_t i = 0; i < nc; i++) {
+                double wh = w_h ? w_h[i] : 1.0, wq = w_q ? w_q[i] : 1.0;
+                lam[i] += 2.0 * wh * (Un1[i] - fh[i]) / norm;
+                lam[nc + i] += 2.0 * wq * (Un1[nc + i] - fqx[i]) / norm;
+                lam[2 * nc + i] += 2.0 * wq * (Un1[2 * nc + i] - fqy[i]) / norm;
+            }
+        }
         /* recompute the step's intermediates */
         memcpy(s.h, Un, sizeof(double) * nc);
         memcpy(s.qx, Un + nc, sizeof(double) * nc);
@@ -1092,4 +1113,59 @@ double orc_e_gradient(const orc_case *c, const double *phi, const double *K1, co
     free(bh);
     e_free(&s);
     return J;
+}
+
+/* Frozen replay with full-frame observations (compute_cost_frame_ptr,
+ * assim/swSpE_ad.cpp:273-308): frames [n_obs][h | qx | qy] (3 n_cell each,
+ * reference cell order), per-cell weights w_h / w_q (NULL = 1); each frame's
+ * misfit divided by n_obs when normalize != 0, summed in time order. */
+double orc_e_replay_frames(const orc_case *c, const double *phi, const double *K1, const double *K2,
+                           const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
+                           const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
+                           int32_t n_obs, const double *frames, const double *w_h,
+                           const double *w_q, int32_t normalize) {
+    esolver s;
+    e_init(&s, c, phi, K1, K2, h0, qx0, qy0);
+    const int64_t nc = c->n_cell;
+    double J = 0.0;
+    for (int64_t n = 0; n < n_steps; n++) {
+        e_step(&s, dt_seq[n], bc_seq + n * c->n_series);
+        int32_t oi = obs_trig[n];
+        if (oi >= 0 && oi < n_obs) {
+            const double *fh = frames + (int64_t)oi * 3 * nc, *fqx = fh + nc, *fqy = fh + 2 * nc;
+            double Jo = 0.0;
+            for (int64_t i = 0; i < nc; i++) {
+                double dh = s.h[i] - fh[i], dqx = s.qx[i] - fqx[i], dqy = s.qy[i] - fqy[i];
+                double wh = w_h ? w_h[i] : 1.0, wq = w_q ? w_q[i] : 1.0;
+                Jo += wh * (dh * dh) + wq * ((dqx * dqx) + (dqy * dqy));
+            }
+            if (normalize && n_obs > 0) Jo /= (double)n_obs;
+            J += Jo;
+        }
+    }
+    e_free(&s);
+    return J;
+}
+
+double orc_e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
+                      const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
+                      const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
+                      int32_t n_obs, int32_t n_gauges, const int64_t *gauge_cell,
+                      const double *gauge_w, const double *obs, int32_t normalize, double *g_phi,
+                      double *g_K1, double *g_K2, double *g_bc) {
+    return e_gradient(c, phi, K1, K2, h0, qx0, qy0, n_steps, dt_seq, bc_seq, obs_trig, n_obs,
+                      n_gauges, gauge_cell, gauge_w, obs, NULL, NULL, NULL, normalize, g_phi, g_K1,
+                      g_K2, g_bc);
+}
+
+/* The same with full-frame observations (frames [n_obs][h | qx | qy]). */
+double orc_e_gradient_frames(const orc_case *c, const double *phi, const double *K1,
+                             const double *K2, const double *h0, const double *qx0,
+                             const double *qy0, int64_t n_steps, const double *dt_seq,
+                             const double *bc_seq, const int32_t *obs_trig, int32_t n_obs,
+                             const double *frames, const double *w_h, const double *w_q,
+                             int32_t normalize, double *g_phi, double *g_K1, double *g_K2,
+                             double *g_bc) {
+    return e_gradient(c, phi, K1, K2, h0, qx0, qy0, n_steps, dt_seq, bc_seq, obs_trig, n_obs, 0,
+                      NULL, NULL, NULL, frames, w_h, w_q, normalize, g_phi, g_K1, g_K2, g_bc);
 }

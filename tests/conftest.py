@@ -42,9 +42,12 @@ def oracle():
 
 
 def golden_names():
-    """Fixtures of the forward solver (the e_* ones belong to the 4D-Var layer)."""
+    """Fixtures of the forward solver made by make_golden.py from the compiled
+    reference (the e_* ones belong to the 4D-Var layer, sp_toce to its own tests)."""
+    from make_golden import golden_cases
+    names = set(golden_cases())
     return sorted(n for n in (os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
-                  if not n.startswith("e_"))
+                  if n in names)
 
 
 def e_golden_names():
@@ -79,3 +82,19 @@ def rel_err(a, b):
     """max |a-b| / max |b| (field-normalised), 0 when both are 0."""
     scale = max(float(np.max(np.abs(b))), 1e-300)
     return float(np.max(np.abs(a - b))) / scale
+
+
+def load_sp_toce(sw):
+    """The sp_toce fixture (tests/golden/make_sp_toce.py): an in-memory case in
+    reference order plus its observation frames, assimilation settings and the
+    reference's first descent-log row."""
+    from sw2d_amd import _native as N
+    from sw2d_amd.cases import MemCase
+    g = dict(np.load(os.path.join(GOLDEN, "sp_toce.npz"), allow_pickle=False))
+    arrays = {k[2:]: g[k] for k in g if k.startswith("a_")}
+    p = N.Params()
+    for k in g:
+        if k.startswith("p_"):
+            setattr(p, k[2:], g[k].item())
+    case = MemCase(arrays, p, int(g["n_series"]), int(g["n_times"]))
+    return case, g
