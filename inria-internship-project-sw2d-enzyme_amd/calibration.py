@@ -1,27 +1,37 @@
 """Calibration workload of the 4D-Var layer: Manning zones, an inflow
-hydrograph and gauges on the SP mesh at scale.
+hydrograph and gauges on the SP mesh at scale, and the 4D-Var that
+calibrates them.
 
-The reference calibrates porosity per cell (assim/swSpE_checkpoint.cpp) and
-keeps friction and inflow constant; this workload instead calibrates 68
-controls -- the Manning roughness n of 8 zones (k-means clusters of the cell
-centroids) and 60 knots of the west inflow hydrograph -- from 20 depth gauges
-sampled every 30 s, through the same differentiable forward and adjoint.
-Gauges sit at seeded points located in their cell by a point-in-triangle
-test (the role of getCellFromCoordinates, src/common/geometry/swMesh.cpp:695-759).
+The reference calibrates porosity per cell (assim/swSpE_checkpoint.cpp,
+minimised by L-BFGS-B in assim/swSpE_minimize.cpp:267-1073) and keeps
+friction and inflow constant; this workload calibrates 68 controls -- the
+Manning roughness n of 8 zones and 60 knots of the west inflow hydrograph --
+from 20 depth gauges sampled every 30 s, through the same differentiable
+forward and the GPU adjoint.  Gauges sit at seeded points located in their
+cell by a point-in-triangle test (the role of getCellFromCoordinates,
+src/common/geometry/swMesh.cpp:695-759).
+
+Every quantity that defines the physical problem (bed, porosity field, zone
+centres, gauge points, start level, hydrographs) is a function of position
+and seed only, so meshes of different resolution of the same domain carry
+the same problem: the batched ensemble runs coarse meshes of it, the 4D-Var
+the fine one.
 """
+import time
+
 import numpy as np
 
 from . import _native as N
 from .cases import scale_case
 
+L_DOMAIN = 5000.0
 
-def kmeans_zones(x, y, k, seed, iters=25, fit_max=50000):
-    """Lloyd's k-means of the cell centroids (fitted on a seeded subsample of
-    at most fit_max cells, then every cell assigned to its nearest centre);
-    zones renumbered by centre x."""
+
+def zone_centres(k, seed, L=L_DOMAIN, iters=25, n_fit=50000):
+    """Centres of k Manning zones: Lloyd's k-means of seeded uniform points of
+    the L x L domain, ordered by x."""
     rng = np.random.default_rng(seed)
-    P = np.stack([x, y], axis=1)
-    S = P[rng.choice(len(P), min(len(P), fit_max), replace=False)]
+    S = rng.uniform(0.0, L, (n_fit, 2))
     C = S[rng.choice(len(S), k, replace=False)].copy()
     for _ in range(iters):
         lab = np.argmin(((S[:, None, :] - C[None, :, :]) ** 2).sum(axis=2), axis=1)
@@ -29,16 +39,19 @@ def kmeans_zones(x, y, k, seed, iters=25, fit_max=50000):
             sel = lab == j
             if np.any(sel):
                 C[j] = S[sel].mean(axis=0)
-    lab = np.argmin(((P[:, None, :] - C[None, :, :]) ** 2).sum(axis=2), axis=1)
-    order = np.argsort(C[:, 0])
-    rank = np.empty(k, dtype=np.int64)
-    rank[order] = np.arange(k)
-    return rank[lab]
+    return C[np.argsort(C[:, 0])]
+
+
+def kmeans_zones(x, y, k, seed, L=L_DOMAIN):
+    """Zone of every cell: the nearest of the k zone centres."""
+    C = zone_centres(k, seed, L)
+    P = np.stack([x, y], axis=1)
+    return np.argmin(((P[:, None, :] - C[None, :, :]) ** 2).sum(axis=2), axis=1).astype(np.int64)
 
 
 def locate_cells(geom, px, py):
     """Cell containing each point (-1 if outside): sign test against the
-    three edges of every triangle, nearest centroid first."""
+    three edges of the nearest triangles."""
     nx_, ny_ = geom.array("node_x"), geom.array("node_y")
     tri = geom.array("cell_tri").reshape(-1, 3)
     cx, cy = nx_[tri].mean(axis=1), ny_[tri].mean(axis=1)
@@ -58,61 +71,78 @@ def locate_cells(geom, px, py):
     return np.array(out, dtype=np.int64)
 
 
+def bed(bumps, x, y, L=L_DOMAIN):
+    """Bed elevation of the scale case at points (x, y): slope 1e-3 down to
+    the east plus the Gaussian bumps."""
+    z = 1e-3 * (L - np.asarray(x, dtype=np.float64))
+    for cx, cy, amp, sig in bumps:
+        z = z + amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2.0 * sig * sig))
+    return z
+
+
 class Calibration:
-    """The workload: mesh + truth / default controls + gauges.
+    """The workload on one mesh: mesh + truth / default controls + gauges.
 
     Controls: n_zone[8] (Manning n per zone; the _E forward applies the
     Strickler law, so K = 1/n per cell) and Q[60] (m3/s at the knots
     t_knots, turned into unit discharge on the west edges)."""
 
     def __init__(self, px=707, seed=1234, n_zones=8, n_knots=60, n_gauges=20, obs_dt=30.0,
-                 window=90.0, q_peak=500.0, t_event=3600.0):
+                 window=90.0, q_peak=500.0, t_event=3600.0, truth_seed=None, level_quantile=0.4):
         self.window = float(window)
         self.t_knots = np.linspace(0.0, self.window, n_knots)
         # default hydrograph: the rising limb of the event's triangular
         # hydrograph (0 -> q_peak over t_event / 2) on the knots
         base = q_peak * np.minimum(self.t_knots / (0.5 * t_event), 1.0) + 5.0
-        self.case, info = scale_case(None, px=px, py=px, L=5000.0, seed=seed, t_max=self.window,
+        self.case, info = scale_case(None, px=px, py=px, L=L_DOMAIN, seed=seed, t_max=self.window,
                                      scheme="godunov", friction="strickler",
                                      manning=1.0 / 0.03, q_peak=q_peak,
                                      hydrograph=(self.t_knots, base))
         self.info = info
         g = info["geometry"]
         a = self.case.a
-        rng = np.random.default_rng(seed + 1)
         self.n_zones = n_zones
         self.zone = kmeans_zones(a["cell_x"], a["cell_y"], n_zones, seed + 2)
-        self.n_true = rng.uniform(0.015, 0.06, n_zones)
         self.n_default = np.full(n_zones, 0.03)
         self.Q_default = base
-        # truth: the same limb with a seeded +-30 % random-walk wiggle
-        wig = np.cumsum(rng.standard_normal(n_knots))
-        wig = (wig - wig.mean()) / (np.abs(wig - wig.mean()).max() + 1e-12)
-        self.Q_true = base * (1.0 + 0.3 * wig)
+        self.n_true, self.Q_true = self.sample_controls(seed + 1 if truth_seed is None else truth_seed)
         self.west_len = info["west_len"]
-        # initial state: a free surface sloping down to the east at half the
-        # bed slope, at rest -- water moves everywhere from the start, so the
+        # reference level from the analytic bed (mesh independent); initial
+        # state: a free surface sloping down to the east at half the bed
+        # slope, at rest -- water moves everywhere from the start, so the
         # roughness of every zone shapes the gauge series; dry where the
         # bumps emerge and near the west edge, where the inflow wets the bed
-        zs = info["z0"] + 0.5e-3 * (5000.0 - a["cell_x"])
+        bumps = info["bumps"]
+        gx, gy = np.meshgrid(np.linspace(0, L_DOMAIN, 256), np.linspace(0, L_DOMAIN, 256))
+        self.z0 = float(np.quantile(bed(bumps, gx.ravel(), gy.ravel()), level_quantile))
+        zs = self.z0 + 0.5e-3 * (L_DOMAIN - a["cell_x"])
         a["h0"] = np.maximum(zs - a["cell_zb"], 0.0)
         a["qx0"] = np.zeros(self.case.n_cell)
         a["qy0"] = np.zeros(self.case.n_cell)
-        # gauges: a third at seeded points near the inflow (x < 300 m), the
-        # rest at seeded wet points anywhere
-        h0 = a["h0"]
-        cells = []
+        # gauge points: 2/5 near the inflow (x < 300 m), the rest where the
+        # analytic start depth exceeds 0.2 m
+        rng = np.random.default_rng(seed + 3)
+        pts = []
         n_west = n_gauges * 2 // 5
-        while len(cells) < n_gauges:
-            west = len(cells) < n_west
-            x = rng.uniform(10.0, 300.0) if west else rng.uniform(0.0, 5000.0)
-            y = rng.uniform(0.0, 5000.0)
-            c = int(locate_cells(g, [x], [y])[0])
-            if c >= 0 and (west or h0[c] > 0.2) and c not in cells:
-                cells.append(c)
-        self.gauge_cells = np.array(cells, dtype=np.int64)
+        while len(pts) < n_gauges:
+            west = len(pts) < n_west
+            x = rng.uniform(10.0, 300.0) if west else rng.uniform(0.0, L_DOMAIN)
+            y = rng.uniform(0.0, L_DOMAIN)
+            if west or (self.z0 + 0.5e-3 * (L_DOMAIN - x) - bed(bumps, x, y)) > 0.2:
+                pts.append((x, y))
+        self.gauge_xy = np.array(pts)
+        self.gauge_cells = locate_cells(g, self.gauge_xy[:, 0], self.gauge_xy[:, 1])
         self.obs_t = np.arange(obs_dt, self.window + 1e-9, obs_dt)
         self.series = 1  # west inflow series (boundary code 2)
+
+    def sample_controls(self, seed):
+        """Seeded controls of the problem's prior: n per zone uniform in
+        [0.015, 0.06]; the default hydrograph times a +-30 % random walk."""
+        rng = np.random.default_rng(seed)
+        n = rng.uniform(0.015, 0.06, self.n_zones)
+        wig = np.cumsum(rng.standard_normal(len(self.t_knots)))
+        wig = (wig - wig.mean()) / (np.abs(wig - wig.mean()).max() + 1e-12)
+        return n, self.Q_default * (1.0 + 0.3 * wig)
 
     def K(self, n_zone):
         return 1.0 / np.asarray(n_zone, dtype=np.float64)[self.zone]
@@ -137,3 +167,108 @@ class Calibration:
         W = interp_weights(self.t_knots, t_steps)
         gQ = (W.T @ G["bc"][:, self.series]) / self.west_len
         return gn, gQ
+
+    # -- twin experiment and 4D-Var -------------------------------------------
+    def observe(self, solver, n_zone, Q, max_steps=200000):
+        """Gauge series [n_obs, n_gauges] of a run at the given controls."""
+        from .assim import Assim
+        self.apply_hydrograph(solver, Q)
+        K = self.K(n_zone)
+        A = Assim(solver, self.obs_t, gauge_cells=self.gauge_cells,
+                  gauge_h=np.zeros((len(self.obs_t), len(self.gauge_cells))), max_steps=max_steps)
+        A.record(K1=K, K2=K)
+        return A.gauge_series()
+
+    def fourdvar(self, solver, obs, n0, Q0, J_tol=0.0, max_iter=60, max_steps=200000, m=6,
+                 n_bounds=(0.01, 0.1), Q_bounds=(0.0, 5000.0), max_ls=30, time_limit=None,
+                 n_tiles=1):
+        """Projected L-BFGS (memory m) on the 68 controls from (n0, Q0)
+        against the gauge series `obs`, organised like the reference's
+        minimiser (assim/swSpE_minimize.cpp:398-511): a gradient evaluation
+        runs the dry run at the controls (fixing the time-step schedule) and
+        the adjoint; the line-search probes evaluate the cost on that frozen
+        schedule (replay only).  Controls are scaled to O(1) (n / 0.03,
+        Q / max Q_default) and kept in their bounds by projection.  Stops
+        when J <= J_tol, after max_iter iterations or time_limit seconds.
+        Returns (n, Q, trace) with trace = [(wall s, J, kind)] for every
+        evaluation, kind 'grad' or 'ls'."""
+        from .assim import Assim, interp_weights
+        A = Assim(solver, self.obs_t, gauge_cells=self.gauge_cells, gauge_h=obs,
+                  max_steps=max_steps, n_tiles=n_tiles)
+        nz = self.n_zones
+        sn, sq = 0.03, max(float(np.max(self.Q_default)), 1.0)
+        lo = np.concatenate([np.full(nz, n_bounds[0] / sn), np.full(len(Q0), Q_bounds[0] / sq)])
+        hi = np.concatenate([np.full(nz, n_bounds[1] / sn), np.full(len(Q0), Q_bounds[1] / sq)])
+        trace = []
+        t0 = time.perf_counter()
+        state = {}
+
+        def controls(z):
+            return z[:nz] * sn, z[nz:] * sq
+
+        def grad(z):
+            n, Q = controls(z)
+            self.apply_hydrograph(solver, Q)
+            K = self.K(n)
+            G = A.gradient(K1=K, K2=K)
+            rec = A.get_record()
+            gn, gQ = self.grad_controls(G, n, rec["t"][:-1])
+            state["bc"] = rec["bc"].copy()
+            state["W"] = interp_weights(self.t_knots, rec["t"][:-1])
+            trace.append((time.perf_counter() - t0, G["J"], "grad"))
+            return G["J"], np.concatenate([gn * sn, gQ * sq])
+
+        def cost(z):  # frozen schedule of the last gradient evaluation
+            n, Q = controls(z)
+            bc = state["bc"].copy()
+            bc[:, self.series] = state["W"] @ Q / self.west_len
+            A.set_bc_seq(bc)
+            K = self.K(n)
+            J = A.cost(K1=K, K2=K)
+            trace.append((time.perf_counter() - t0, J, "ls"))
+            return J
+
+        z = np.clip(np.concatenate([np.asarray(n0) / sn, np.asarray(Q0) / sq]), lo, hi)
+        J, g = grad(z)
+        S, Y = [], []
+        for it in range(max_iter):
+            if J <= J_tol or (time_limit and time.perf_counter() - t0 > time_limit):
+                break
+            free = ~(((z <= lo) & (g > 0)) | ((z >= hi) & (g < 0)))
+            q = np.where(free, g, 0.0)
+            alphas = []
+            for s_, y_ in reversed(list(zip(S, Y))):
+                a = np.dot(s_, q) / np.dot(y_, s_)
+                alphas.append(a)
+                q = q - a * y_
+            if S:
+                q = q * (np.dot(S[-1], Y[-1]) / np.dot(Y[-1], Y[-1]))
+            for (s_, y_), a in zip(zip(S, Y), reversed(alphas)):
+                b = np.dot(y_, q) / np.dot(y_, s_)
+                q = q + s_ * (a - b)
+            d = -np.where(free, q, 0.0)
+            if np.dot(d, g) >= 0.0:  # not a descent direction: restart from steepest descent
+                S, Y = [], []
+                d = -np.where(free, g, 0.0)
+            step = 1.0 if S else min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
+            accepted = False
+            for _ in range(max_ls):
+                zn = np.clip(z + step * d, lo, hi)
+                Jn = cost(zn)
+                if Jn <= J + 1e-4 * np.dot(g, zn - z):
+                    accepted = True
+                    break
+                step *= 0.5
+            if not accepted:
+                break
+            Jn, gn = grad(zn)
+            sk, yk = zn - z, gn - g
+            if np.dot(sk, yk) > 1e-12 * np.dot(sk, sk):
+                S.append(sk)
+                Y.append(yk)
+                if len(S) > m:
+                    S.pop(0)
+                    Y.pop(0)
+            z, J, g = zn, Jn, gn
+        n, Q = controls(z)
+        return n, Q, trace

@@ -164,11 +164,13 @@ def scale_case(folder, px=500, py=500, L=5000.0, seed=1234, t_max=3600.0, scheme
     ny_ = g.array("node_y")
     # bed: slope 1e-3 (down towards the east) + 20 Gaussian bumps
     z = 1e-3 * (L - nx_)
+    bumps = []
     for _ in range(20):
         cx, cy = rng.uniform(0, L, 2)
         amp = rng.uniform(0.0, 3.0)
         sig = rng.uniform(50.0, 300.0)
         z = z + amp * np.exp(-((nx_ - cx) ** 2 + (ny_ - cy) ** 2) / (2.0 * sig * sig))
+        bumps.append((cx, cy, amp, sig))
     g.set_node_z(z)
     # boundary codes: west edge (x = 0) is the inflow (code 2), the rest walls (code 1)
     bxm = g.array("bnd_xm")
@@ -200,7 +202,7 @@ def scale_case(folder, px=500, py=500, L=5000.0, seed=1234, t_max=3600.0, scheme
     nc, nb, ni, nn = g.sizes()
     info = {"n_cell": nc, "n_bnd": nb, "n_int": ni, "n_node": nn,
             "flips": getattr(g, "n_flips", 0), "west_len": west_len, "z0": z0,
-            "wet_fraction": float(np.mean(h0 > 0)), "geometry": g}
+            "wet_fraction": float(np.mean(h0 > 0)), "geometry": g, "bumps": bumps}
     bc_v = np.stack([np.zeros_like(tq), qunit])
     if folder is None:
         # same inputs, kept in memory (a written case reads back bit-identical: %.17e)
