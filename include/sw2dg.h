@@ -169,6 +169,35 @@ int sw2dg_assim_profile(sw2dg_ctx *ctx, const double *phi, const double *K1, con
 /* State at the end of the recorded run (reference order). */
 int sw2dg_assim_final_state(sw2dg_ctx *ctx, double *h, double *qx, double *qy);
 
+/* ---- Batched ensemble: many differentiable-forward runs over uneven meshes
+ * in one persistent launch.  The reference runs one simulation per process
+ * (the _E dry run of build_checkpoint_records, swSpE_checkpoint.cpp:97-455,
+ * one sw2d_enzyme <case> forward call each); run r of an ensemble gives the
+ * same result as sw2dg_assim_record on its own ctx.  Godunov, no head loss. ---- */
+typedef struct sw2dg_ens sw2dg_ens;
+/* meshes[n_runs] in reference order; params shared by every run (t0, t_max,
+ * dt_max, cfl_max, h_min, phi_min, friction). */
+int sw2dg_ens_create(int32_t n_runs, const sw2dg_mesh *meshes, const sw2dg_params *params,
+                     int32_t device, sw2dg_ens **out);
+/* Controls, boundary series, initial state and gauges of one run (reference
+ * order; may be called again to replace them). */
+int sw2dg_ens_set_run(sw2dg_ens *ens, int32_t run, const double *phi, const double *K1,
+                      const double *K2, const double *fric_alpha_deg, int32_t n_series,
+                      int32_t n_times, const int32_t *type, const double *t, const double *v,
+                      const double *h, const double *qx, const double *qy, int32_t n_gauges,
+                      const int64_t *gauge_cell);
+/* Runs every run from its initial state to t_max (or max_steps steps) with
+ * workgroups of `block` threads (256, 512, 1024), landing on the observation
+ * times obs_t; gauge_out [n_runs][n_obs][max n_gauges] (0 where not reached),
+ * steps_out [n_runs], device_ms = duration of the launch.  Any output may be NULL. */
+int sw2dg_ens_run(sw2dg_ens *ens, int32_t n_obs, const double *obs_t, int32_t block,
+                  int64_t max_steps, double *gauge_out, int64_t *steps_out, double *device_ms);
+/* Final state of one run after sw2dg_ens_run (reference order). */
+int sw2dg_ens_get_state(sw2dg_ens *ens, int32_t run, double *h, double *qx, double *qy);
+int64_t sw2dg_ens_device_bytes(sw2dg_ens *ens);
+const char *sw2dg_ens_last_error(sw2dg_ens *ens);
+void sw2dg_ens_destroy(sw2dg_ens *ens);
+
 /* Bytes of device memory held by the ctx. */
 int64_t sw2dg_device_bytes(sw2dg_ctx *ctx);
 const char *sw2dg_last_error(sw2dg_ctx *ctx);

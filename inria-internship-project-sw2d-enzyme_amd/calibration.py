@@ -17,6 +17,7 @@ and seed only, so meshes of different resolution of the same domain carry
 the same problem: the batched ensemble runs coarse meshes of it, the 4D-Var
 the fine one.
 """
+import functools
 import time
 
 import numpy as np
@@ -27,6 +28,7 @@ from .cases import scale_case
 L_DOMAIN = 5000.0
 
 
+@functools.lru_cache(maxsize=16)
 def zone_centres(k, seed, L=L_DOMAIN, iters=25, n_fit=50000):
     """Centres of k Manning zones: Lloyd's k-means of seeded uniform points of
     the L x L domain, ordered by x."""
@@ -57,7 +59,9 @@ def locate_cells(geom, px, py):
     cx, cy = nx_[tri].mean(axis=1), ny_[tri].mean(axis=1)
     out = []
     for x, y in zip(np.atleast_1d(px), np.atleast_1d(py)):
-        cand = np.argsort((cx - x) ** 2 + (cy - y) ** 2)[:16]
+        d2 = (cx - x) ** 2 + (cy - y) ** 2
+        cand = np.argpartition(d2, 16)[:16]
+        cand = cand[np.argsort(d2[cand])]
         found = -1
         for c in cand:
             a, b, d = tri[c]

@@ -1,0 +1,612 @@
+// sw2dg_ensemble.hip -- batched ensemble of differentiable-forward (_E) runs
+// over uneven meshes, in one persistent launch.
+//
+// Each workgroup owns one run (its own mesh, controls, hydrograph, clock)
+// and steps it to t_max: edge fluxes -> block-level CFL minimum with the
+// observation landing of build_checkpoint_records
+// (assim/swSpE_checkpoint.cpp:224-366) -> cell update -> gauge sampling,
+// phases separated by workgroup barriers.  No grid-wide synchronisation is
+// needed, so the launch holds every run; runs are dispatched largest first
+// (workgroup b takes the b-th largest run) and the hardware dispatcher hands
+// the remaining runs to CUs as they free up.  The per-step operations are
+// those of the single-run dry run (k_e_edges / k_e_cfl / k_e_dt / k_e_update
+// in sw2dg_adjoint.hip), in the same order, so each run reproduces
+// sw2dg_assim_record on its mesh bit for bit.
+//
+// Layout: all runs' arrays concatenated (cells, internal edges, boundary
+// edges, incidences, series), each run's in the device order of
+// build_tables (Hilbert cells, edges by lower cell), indices local to the run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/sw2dg.h"
+#include "sp_adjoint.h"
+#include "sp_physics.h"
+#include "sw2dg_internal.h"
+
+using namespace sw2dg_detail;
+using sp::kSmall;
+
+namespace {
+
+constexpr int kMaxSeries = 8;
+
+struct RunDesc {
+    int nc, ni, nb;      // sizes
+    int c0, e0, b0;      // offsets of cells, internal edges, boundary edges
+    int ip0;             // offset of the run's int_ptr / bnd_ptr (nc + 1 entries each)
+    int ns, s0;          // series count and offset (types)
+    int nt, t0;          // knots count and offset (times); values at s0 * nt_max? see v0
+    int v0;              // offset of the run's series values [ns][nt]
+    int ng, g0;          // gauges
+};
+
+struct EnsDev {
+    const RunDesc *runs;
+    // cells
+    const double *zb, *area, *phi, *beta, *K1, *K2, *fcos, *fsin;
+    double *h, *qx, *qy;
+    const int *int_ptr, *int_inc, *bnd_ptr, *bnd_inc;
+    // internal edges
+    const int *eL, *eR;
+    const double *elen, *enx, *eny;
+    double *F, *sw;  // F: 5 per internal edge, then 3 per boundary edge (offset Fb)
+    size_t Fb;
+    // boundary edges
+    const int *bcell, *bser;
+    const double *blen, *bnx, *bny;
+    // series
+    const int *stype;
+    const double *st, *sv;
+    // gauges
+    const int *gcell;
+};
+
+__device__ double interp_e_(const double *tt, const double *vv, int n, double t) {
+    if (n == 0) return 0.0;
+    if (n == 1 || t <= tt[0] - 1.0e-12) return vv[0];
+    if (t >= tt[n - 1]) return vv[n - 1];
+    int it = 0;
+    while (it < n - 1 && t > tt[it + 1]) it++;
+    const double t1 = tt[it], t2 = tt[it + 1], v1 = vv[it], v2 = vv[it + 1];
+    const double den = t2 - t1;
+    if (den < 1e-12) return v1;
+    return v1 + (v2 - v1) * (t - t1) / den;
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, double t_start,
+                                           double t_max, double dt_max, double cfl, int n_obs,
+                                           const double *__restrict__ obs_t, int ng_max,
+                                           double *__restrict__ gser, int64_t max_steps,
+                                           int64_t *steps_out) {
+    const int r = order[blockIdx.x];
+    const RunDesc R = d.runs[r];
+    const int tid = (int)threadIdx.x;
+    const int nc = R.nc, ni = R.ni, nb = R.nb;
+    // run-local views
+    const double *zb = d.zb + R.c0, *area = d.area + R.c0, *phi = d.phi + R.c0;
+    const double *beta = d.beta + R.c0, *K1 = d.K1 + R.c0, *K2 = d.K2 + R.c0;
+    const double *fcos = d.fcos + R.c0, *fsin = d.fsin + R.c0;
+    double *h = d.h + R.c0, *qx = d.qx + R.c0, *qy = d.qy + R.c0;
+    const int *iptr = d.int_ptr + R.ip0, *bptr = d.bnd_ptr + R.ip0;
+    const int *iinc = d.int_inc + 2 * (size_t)R.e0, *binc = d.bnd_inc + R.b0;
+    const int *eL = d.eL + R.e0, *eR = d.eR + R.e0;
+    const double *elen = d.elen + R.e0, *enx = d.enx + R.e0, *eny = d.eny + R.e0;
+    double *F = d.F + 5 * (size_t)R.e0, *Fb = d.F + d.Fb + 3 * (size_t)R.b0;
+    double *sw = d.sw + 2 * (size_t)R.e0;
+    const int *bcell = d.bcell + R.b0, *bser = d.bser + R.b0;
+    const double *blen = d.blen + R.b0, *bnx = d.bnx + R.b0, *bny = d.bny + R.b0;
+    const int *stype = d.stype + R.s0;
+    const double *st = d.st + R.t0, *sv = d.sv + R.v0;
+    const int *gcell = d.gcell + R.g0;
+    double *gout = gser + (size_t)r * n_obs * ng_max;
+
+    __shared__ double red[B / 64];
+    __shared__ double s_bcv[kMaxSeries];
+    __shared__ double s_dt;
+    __shared__ int s_trig, s_oi;
+    double t = t_start;
+    int oi = 0;
+    int64_t step = 0;
+    while (t < t_max && step < max_steps) {
+        if (tid < R.ns) s_bcv[tid] = interp_e_(st, sv + (size_t)tid * R.nt, R.nt, t);
+        __syncthreads();
+        // ---- edge fluxes (k_e_edges)
+        for (int e = tid; e < ni + nb; e += B) {
+            if (e >= ni) {
+                const int j = e - ni;
+                const int i = bcell[j];
+                const spe::Side sd = spe::lr_fwd(h[i], qx[i], qy[i], zb[i], beta[i], bnx[j], bny[j], o.h_min);
+                const int ser = bser[j];
+                double f[3];
+                spe::bnd_fwd(o.h_min, o.phi_min, beta[i], phi[i], zb[i], sd.z, sd.q, sd.r, sd.lp,
+                             stype[ser], s_bcv[ser], f);
+                double *fb = Fb + 3 * (size_t)j;
+                fb[0] = f[0];
+                fb[1] = f[1];
+                fb[2] = f[2];
+                continue;
+            }
+            const int L = eL[e], Rr = eR[e];
+            const double nx = enx[e], ny = eny[e];
+            const double hL = h[L], hR = h[Rr];
+            const spe::Side sL = spe::lr_fwd(hL, qx[L], qy[L], zb[L], beta[L], nx, ny, o.h_min);
+            const spe::Side sR = spe::lr_fwd(hR, qx[Rr], qy[Rr], zb[Rr], beta[Rr], nx, ny, o.h_min);
+            double f[5];
+            spe::hllc_fwd(o.h_min, o.phi_min, beta[L], beta[Rr], phi[L], phi[Rr], zb[L], zb[Rr], sL.z, sL.q,
+                          sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+            double *Fe = F + 5 * (size_t)e;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) Fe[k] = f[k];
+            double m1 = fabs(sL.lm), v = fabs(sL.lp);
+            if (m1 < v) m1 = v;
+            if (m1 < (v = fabs(sR.lm))) m1 = v;
+            if (m1 < (v = fabs(sR.lp))) m1 = v;
+            const double len = elen[e];
+            sw[2 * e] = (hL > o.h_min) ? m1 * len : 0.0;
+            sw[2 * e + 1] = (hR > o.h_min) ? m1 * len : 0.0;
+        }
+        __syncthreads();
+        // ---- CFL (k_e_cfl): per-cell candidate, block minimum
+        double cand = dt_max;
+        for (int i = tid; i < nc; i += B) {
+            const double hi = h[i];
+            double swept = 0.0;
+            if (hi > o.h_min) {
+                for (int k = bptr[i]; k < bptr[i + 1]; ++k) {
+                    const int j = binc[k];
+                    const spe::Side s = spe::lr_fwd(hi, qx[i], qy[i], zb[i], beta[i], bnx[j], bny[j], o.h_min);
+                    const double lm = (fabs(s.lp) < fabs(s.lm)) ? fabs(s.lm) : fabs(s.lp);
+                    swept += lm * blen[j];
+                }
+            }
+            for (int k = iptr[i]; k < iptr[i + 1]; ++k) {
+                const int inc = iinc[k];
+                swept += sw[2 * (size_t)(inc >> 1) + (inc & 1)];
+            }
+            const double den = (swept < kSmall) ? kSmall : swept;
+            const double x = cfl * (area[i] / den);
+            if (x < cand) cand = x;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o2 = __shfl_xor(cand, off, 64);
+            cand = (o2 < cand) ? o2 : cand;
+        }
+        if ((tid & 63) == 0) red[tid >> 6] = cand;
+        __syncthreads();
+        // ---- step schedule (k_e_dt)
+        if (tid == 0) {
+            double dtc = red[0];
+            for (int w = 1; w < B / 64; ++w) dtc = (red[w] < dtc) ? red[w] : dtc;
+            if ((t + dtc) > t_max) dtc = t_max - t;
+            if (dtc < 1e-6) dtc = 1e-6;
+            double target = dt_max;
+            int oo = oi;
+            if (oo < n_obs) {
+                const double to = obs_t[oo] - t;
+                if (to - 10e-6 > 0.0 && to < target) target = to;
+            }
+            double dt = (dtc < target) ? dtc : target;
+            if ((t + dt) > t_max) dt = t_max - t;
+            if (dt < 1e-6) dt = 1e-6;
+            const double tn = t + dt;
+            while (oo < n_obs && tn > obs_t[oo] + 1e-6) oo++;
+            int tr = -1;
+            if (oo < n_obs && fabs(tn - obs_t[oo]) < 1e-6) tr = oo++;
+            s_dt = dt;
+            s_trig = tr;
+            s_oi = oo;
+        }
+        __syncthreads();
+        const double dt = s_dt;
+        // ---- cell update (k_e_update): balance, clean-up, clamp, friction
+        for (int i = tid; i < nc; i += B) {
+            double sh = h[i], sx = qx[i], sy = qy[i];
+            const double ph = phi[i], ar = area[i];
+            for (int k = bptr[i]; k < bptr[i + 1]; ++k) {
+                const int j = binc[k];
+                const double nx = bnx[j], ny = bny[j];
+                const double *f = Fb + 3 * (size_t)j;
+                const double co = spe::edge_coeff(blen[j], ar, ph, o.phi_min);
+                sh += (+co) * f[0] * dt;
+                sx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
+                sy += ((+co * ny) * f[1] + (+co * ny) * 0.0 + (+co * nx) * f[2]) * dt;
+            }
+            for (int k = iptr[i]; k < iptr[i + 1]; ++k) {
+                const int inc = iinc[k];
+                const int e = inc >> 1, side = inc & 1;
+                const double co = spe::edge_coeff(elen[e], ar, ph, o.phi_min);
+                double a, b, c;
+                spe::side_contrib(side, co, enx[e], eny[e], F + 5 * (size_t)e, dt, a, b, c);
+                sh += a;
+                sx += b;
+                sy += c;
+            }
+            if (fabs(sh) < kSmall) sh = 0.0;
+            if (fabs(sx) < kSmall) sx = 0.0;
+            if (fabs(sy) < kSmall) sy = 0.0;
+            if (sh < 0.) {
+                sh = 0.;
+                sx = 0.;
+                sy = 0.;
+            }
+            if (o.friction != sp::FRIC_NONE)
+                spe::fric_fwd(o.h_min, sh, sx, sy, fcos[i], fsin[i], K1[i], K2[i], dt);
+            h[i] = sh;
+            qx[i] = sx;
+            qy[i] = sy;
+        }
+        __syncthreads();
+        // ---- gauges of an observation step
+        const int tr = s_trig;
+        if (tr >= 0)
+            for (int g = tid; g < R.ng; g += B) gout[(size_t)tr * ng_max + g] = h[gcell[g]];
+        t = t + dt;
+        oi = s_oi;
+        ++step;
+        __syncthreads();
+    }
+    if (tid == 0) steps_out[r] = step;
+}
+
+}  // namespace
+
+struct sw2dg_ens {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    sw2dg_params prm{};
+    Opts o{};
+    int n_runs = 0;
+    std::vector<MeshTables> T;
+    std::vector<RunDesc> runs;
+    int64_t NC = 0, NI = 0, NB = 0, NIP = 0;
+    std::vector<char> run_set;
+    // host staging of the per-run inputs (device order); series and gauges
+    // per run, concatenated at launch
+    std::vector<double> phi, beta, K1, K2, fcos, fsin, h0, qx0, qy0;
+    std::vector<std::vector<int>> r_type, r_gauge;
+    std::vector<std::vector<double>> r_t, r_v;
+    EnsDev d{};
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    template <class T_>
+    T_ *dalloc(size_t n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T_)) != hipSuccess) return nullptr;
+        allocs.push_back(p);
+        bytes += (int64_t)(std::max<size_t>(n, 1) * sizeof(T_));
+        return (T_ *)p;
+    }
+    template <class T_>
+    T_ *upload(const std::vector<T_> &v) {
+        T_ *p = dalloc<T_>(v.size());
+        if (p && !v.empty()) hipMemcpy(p, v.data(), v.size() * sizeof(T_), hipMemcpyHostToDevice);
+        return p;
+    }
+    ~sw2dg_ens() {
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+int efail(sw2dg_ens *e, int code, const std::string &msg) {
+    if (e) e->err = msg;
+    return code;
+}
+#define EHIP(e, x)                                                                      \
+    do {                                                                                \
+        hipError_t err_ = (x);                                                          \
+        if (err_ != hipSuccess) return efail(e, SW2DG_ERR_HIP, hipGetErrorString(err_)); \
+    } while (0)
+}  // namespace
+
+extern "C" {
+
+int sw2dg_ens_create(int32_t n_runs, const sw2dg_mesh *meshes, const sw2dg_params *prm,
+                     int32_t device, sw2dg_ens **out) {
+    if (!out) return SW2DG_ERR_ARG;
+    *out = nullptr;
+    if (n_runs <= 0 || !meshes || !prm) return SW2DG_ERR_ARG;
+    if (prm->scheme != SW2DG_GODUNOV || prm->head_loss) return SW2DG_ERR_ARG;
+    int64_t NC = 0, NI = 0, NB = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        const sw2dg_mesh *m = meshes + r;
+        if (m->n_cell <= 0 || m->n_int < 0 || m->n_bnd < 0) return SW2DG_ERR_ARG;
+        for (int64_t e = 0; e < m->n_bnd; ++e)
+            if (m->bnd_cell[e] < 0 || m->bnd_cell[e] >= m->n_cell || m->bnd_series[e] < 0 ||
+                m->bnd_series[e] >= kMaxSeries)
+                return SW2DG_ERR_ARG;
+        for (int64_t e = 0; e < m->n_int; ++e)
+            if (m->int_L[e] < 0 || m->int_L[e] >= m->n_cell || m->int_R[e] < 0 ||
+                m->int_R[e] >= m->n_cell || m->int_L[e] == m->int_R[e])
+                return SW2DG_ERR_ARG;
+        NC += m->n_cell;
+        NI += m->n_int;
+        NB += m->n_bnd;
+    }
+    if (NC > INT32_MAX / 2 || NI > INT32_MAX / 8) return SW2DG_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SW2DG_ERR_NO_DEVICE;
+    auto *E = new sw2dg_ens();
+    if (device < 0) hipGetDevice(&E->device);
+    else E->device = device;
+    if (hipSetDevice(E->device) != hipSuccess || hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete E;
+        return SW2DG_ERR_HIP;
+    }
+    E->prm = *prm;
+    E->o.scheme = prm->scheme;
+    E->o.topo = prm->topo;
+    E->o.friction = prm->friction;
+    E->o.head_loss = 0;
+    E->o.h_min = prm->h_min;
+    E->o.phi_min = prm->phi_min;
+    E->n_runs = n_runs;
+    E->NC = NC;
+    E->NI = NI;
+    E->NB = NB;
+    E->T.resize(n_runs);
+    E->runs.resize(n_runs);
+    E->run_set.assign(n_runs, 0);
+    E->r_type.resize(n_runs);
+    E->r_gauge.resize(n_runs);
+    E->r_t.resize(n_runs);
+    E->r_v.resize(n_runs);
+    std::vector<double> zb(NC), area(NC), elen(NI), enx(NI), eny(NI), blen(NB), bnx(NB), bny(NB);
+    std::vector<int> iptr, bptr, iinc(2 * NI), binc(NB), eL(NI), eR(NI), bcell(NB), bser(NB);
+    int64_t c0 = 0, e0 = 0, b0 = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        const sw2dg_mesh *m = meshes + r;
+        MeshTables &T = E->T[r];
+        build_tables(m, T);
+        RunDesc &R = E->runs[r];
+        R.nc = (int)m->n_cell;
+        R.ni = (int)m->n_int;
+        R.nb = (int)m->n_bnd;
+        R.c0 = (int)c0;
+        R.e0 = (int)e0;
+        R.b0 = (int)b0;
+        R.ip0 = (int)iptr.size();
+        for (int64_t k = 0; k < m->n_cell; ++k) {
+            zb[c0 + k] = m->cell_zb[T.perm[k]];
+            area[c0 + k] = m->cell_area[T.perm[k]];
+        }
+        iptr.insert(iptr.end(), T.iptr.begin(), T.iptr.end());
+        bptr.insert(bptr.end(), T.bptr.begin(), T.bptr.end());
+        std::copy(T.iinc.begin(), T.iinc.end(), iinc.begin() + 2 * e0);
+        std::copy(T.binc.begin(), T.binc.end(), binc.begin() + b0);
+        std::copy(T.eL.begin(), T.eL.end(), eL.begin() + e0);
+        std::copy(T.eR.begin(), T.eR.end(), eR.begin() + e0);
+        std::copy(T.elen.begin(), T.elen.end(), elen.begin() + e0);
+        std::copy(T.enx.begin(), T.enx.end(), enx.begin() + e0);
+        std::copy(T.eny.begin(), T.eny.end(), eny.begin() + e0);
+        std::copy(T.blen.begin(), T.blen.end(), blen.begin() + b0);
+        std::copy(T.bnx.begin(), T.bnx.end(), bnx.begin() + b0);
+        std::copy(T.bny.begin(), T.bny.end(), bny.begin() + b0);
+        std::copy(T.bser.begin(), T.bser.begin() + m->n_bnd, bser.begin() + b0);
+        std::copy(T.bcell.begin(), T.bcell.begin() + m->n_bnd, bcell.begin() + b0);
+        c0 += m->n_cell;
+        e0 += m->n_int;
+        b0 += m->n_bnd;
+    }
+    E->NIP = (int64_t)iptr.size();
+    E->phi.assign(NC, 1.0);
+    E->beta.assign(NC, 1.0);
+    E->K1.assign(NC, 0.0);
+    E->K2.assign(NC, 0.0);
+    E->fcos.assign(NC, 1.0);
+    E->fsin.assign(NC, 0.0);
+    E->h0.assign(NC, 0.0);
+    E->qx0.assign(NC, 0.0);
+    E->qy0.assign(NC, 0.0);
+    EnsDev &d = E->d;
+    d.zb = E->upload(zb);
+    d.area = E->upload(area);
+    d.int_ptr = E->upload(iptr);
+    d.bnd_ptr = E->upload(bptr);
+    d.int_inc = E->upload(iinc);
+    d.bnd_inc = E->upload(binc);
+    d.eL = E->upload(eL);
+    d.eR = E->upload(eR);
+    d.elen = E->upload(elen);
+    d.enx = E->upload(enx);
+    d.eny = E->upload(eny);
+    d.blen = E->upload(blen);
+    d.bnx = E->upload(bnx);
+    d.bny = E->upload(bny);
+    d.bser = E->upload(bser);
+    d.bcell = E->upload(bcell);
+    d.h = E->dalloc<double>(NC);
+    d.qx = E->dalloc<double>(NC);
+    d.qy = E->dalloc<double>(NC);
+    d.Fb = 5 * (size_t)std::max<int64_t>(NI, 1);
+    d.F = E->dalloc<double>(d.Fb + 3 * (size_t)std::max<int64_t>(NB, 1));
+    d.sw = E->dalloc<double>(2 * (size_t)std::max<int64_t>(NI, 1));
+    for (void *p : E->allocs)
+        if (!p) {
+            delete E;
+            return SW2DG_ERR_HIP;
+        }
+    *out = E;
+    return SW2DG_OK;
+}
+
+int sw2dg_ens_set_run(sw2dg_ens *E, int32_t run, const double *phi, const double *K1,
+                      const double *K2, const double *fric_alpha_deg, int32_t n_series,
+                      int32_t n_times, const int32_t *type, const double *t, const double *v,
+                      const double *h, const double *qx, const double *qy, int32_t n_gauges,
+                      const int64_t *gauge_cell) {
+    if (!E || run < 0 || run >= E->n_runs) return SW2DG_ERR_ARG;
+    if (!phi || !K1 || !K2 || !h || !qx || !qy) return efail(E, SW2DG_ERR_ARG, "missing run input");
+    if (n_series < 0 || n_series > kMaxSeries || (n_series > 0 && (n_times < 1 || !type || !t || !v)))
+        return efail(E, SW2DG_ERR_ARG, "bad boundary series");
+    if (n_gauges < 0 || (n_gauges > 0 && !gauge_cell)) return efail(E, SW2DG_ERR_ARG, "bad gauges");
+    const MeshTables &T = E->T[run];
+    RunDesc &R = E->runs[run];
+    for (int64_t e = 0; e < T.nb; ++e)
+        if (T.bser[e] >= n_series) return efail(E, SW2DG_ERR_ARG, "a boundary edge refers to a missing series");
+    for (int k = 0; k < n_series; ++k)
+        if (type[k] < 0 || type[k] > 3) return efail(E, SW2DG_ERR_BC, "BC type not handled");
+    for (int g = 0; g < n_gauges; ++g)
+        if (gauge_cell[g] < 0 || gauge_cell[g] >= T.nc) return efail(E, SW2DG_ERR_ARG, "gauge cell out of range");
+    const double rad = std::atan(1.0) / 45.0;
+    for (int64_t k = 0; k < T.nc; ++k) {
+        const int src = T.perm[k];
+        const size_t q = R.c0 + k;
+        E->phi[q] = phi[src];
+        E->K1[q] = K1[src];
+        E->K2[q] = K2[src];
+        const double a = fric_alpha_deg ? fric_alpha_deg[src] * rad : 0.0;
+        E->fcos[q] = std::cos(a);
+        E->fsin[q] = std::sin(a);
+        E->h0[q] = h[src];
+        E->qx0[q] = qx[src];
+        E->qy0[q] = qy[src];
+    }
+    R.ns = n_series;
+    R.nt = n_series > 0 ? n_times : 0;
+    R.ng = n_gauges;
+    E->r_type[run].assign(type, type + n_series);
+    E->r_t[run].assign(t, t + R.nt);
+    E->r_v[run].assign(v, v + (size_t)n_series * R.nt);
+    E->r_gauge[run].resize(n_gauges);
+    for (int g = 0; g < n_gauges; ++g) E->r_gauge[run][g] = T.inv[gauge_cell[g]];
+    E->run_set[run] = 1;
+    return SW2DG_OK;
+}
+
+int sw2dg_ens_run(sw2dg_ens *E, int32_t n_obs, const double *obs_t, int32_t block,
+                  int64_t max_steps, double *gauge_out, int64_t *steps_out, double *device_ms) {
+    if (!E || n_obs < 0 || (n_obs > 0 && !obs_t) || max_steps <= 0) return SW2DG_ERR_ARG;
+    if (block != 256 && block != 512 && block != 1024) return efail(E, SW2DG_ERR_ARG, "block must be 256, 512 or 1024");
+    for (int r = 0; r < E->n_runs; ++r)
+        if (!E->run_set[r]) return efail(E, SW2DG_ERR_ARG, "sw2dg_ens_set_run not called for every run");
+    EHIP(E, hipSetDevice(E->device));
+    EnsDev &d = E->d;
+    // concatenate the runs' series and gauges
+    std::vector<int> c_type, c_gauge;
+    std::vector<double> c_t, c_v;
+    int ng_max = 0;
+    for (int r = 0; r < E->n_runs; ++r) {
+        RunDesc &R = E->runs[r];
+        R.s0 = (int)c_type.size();
+        c_type.insert(c_type.end(), E->r_type[r].begin(), E->r_type[r].end());
+        R.t0 = (int)c_t.size();
+        c_t.insert(c_t.end(), E->r_t[r].begin(), E->r_t[r].end());
+        R.v0 = (int)c_v.size();
+        c_v.insert(c_v.end(), E->r_v[r].begin(), E->r_v[r].end());
+        R.g0 = (int)c_gauge.size();
+        c_gauge.insert(c_gauge.end(), E->r_gauge[r].begin(), E->r_gauge[r].end());
+        ng_max = std::max(ng_max, R.ng);
+    }
+    std::vector<void *> tmp;
+    auto up = [&](auto &v) {
+        using T_ = typename std::decay_t<decltype(v)>::value_type;
+        T_ *p = nullptr;
+        if (hipMalloc((void **)&p, std::max<size_t>(v.size(), 1) * sizeof(T_)) != hipSuccess) return (T_ *)nullptr;
+        tmp.push_back(p);
+        if (!v.empty()) hipMemcpy(p, v.data(), v.size() * sizeof(T_), hipMemcpyHostToDevice);
+        return p;
+    };
+    d.phi = up(E->phi);
+    d.beta = up(E->beta);
+    d.K1 = up(E->K1);
+    d.K2 = up(E->K2);
+    d.fcos = up(E->fcos);
+    d.fsin = up(E->fsin);
+    d.stype = up(c_type);
+    d.st = up(c_t);
+    d.sv = up(c_v);
+    d.gcell = up(c_gauge);
+    d.runs = up(E->runs);
+    std::vector<double> ot(obs_t, obs_t + n_obs);
+    double *d_obs = up(ot);
+    // largest runs first
+    std::vector<int> order(E->n_runs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return E->runs[a].nc + E->runs[a].ni > E->runs[b].nc + E->runs[b].ni; });
+    int *d_order = up(order);
+    const size_t ng = (size_t)E->n_runs * std::max(1, n_obs) * std::max(1, ng_max);
+    double *d_g = nullptr;
+    int64_t *d_steps = nullptr;
+    bool ok = hipMalloc((void **)&d_g, ng * sizeof(double)) == hipSuccess;
+    if (ok) tmp.push_back(d_g);
+    ok = ok && hipMalloc((void **)&d_steps, E->n_runs * sizeof(int64_t)) == hipSuccess;
+    if (ok) tmp.push_back(d_steps);
+    for (void *p : tmp)
+        if (!p) ok = false;
+    int st = SW2DG_OK;
+    if (!ok) st = efail(E, SW2DG_ERR_HIP, "out of device memory for the ensemble inputs");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st) {
+        hipMemsetAsync(d_g, 0, ng * sizeof(double), E->stream);
+        hipMemcpyAsync(d.h, E->h0.data(), E->NC * sizeof(double), hipMemcpyHostToDevice, E->stream);
+        hipMemcpyAsync(d.qx, E->qx0.data(), E->NC * sizeof(double), hipMemcpyHostToDevice, E->stream);
+        hipMemcpyAsync(d.qy, E->qy0.data(), E->NC * sizeof(double), hipMemcpyHostToDevice, E->stream);
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, E->stream);
+        const sw2dg_params &p = E->prm;
+        const dim3 grid(E->n_runs);
+        if (block == 256)
+            hipLaunchKernelGGL(k_ens<256>, grid, dim3(256), 0, E->stream, d, d_order, E->o, p.t0, p.t_max,
+                               p.dt_max, p.cfl_max, n_obs, d_obs, std::max(1, ng_max), d_g, max_steps, d_steps);
+        else if (block == 512)
+            hipLaunchKernelGGL(k_ens<512>, grid, dim3(512), 0, E->stream, d, d_order, E->o, p.t0, p.t_max,
+                               p.dt_max, p.cfl_max, n_obs, d_obs, std::max(1, ng_max), d_g, max_steps, d_steps);
+        else
+            hipLaunchKernelGGL(k_ens<1024>, grid, dim3(1024), 0, E->stream, d, d_order, E->o, p.t0, p.t_max,
+                               p.dt_max, p.cfl_max, n_obs, d_obs, std::max(1, ng_max), d_g, max_steps, d_steps);
+        hipEventRecord(e1, E->stream);
+        hipError_t err = hipStreamSynchronize(E->stream);
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err != hipSuccess) st = efail(E, SW2DG_ERR_HIP, hipGetErrorString(err));
+        float ms = 0.f;
+        if (!st) hipEventElapsedTime(&ms, e0, e1);
+        if (device_ms) *device_ms = ms;
+        if (!st && gauge_out)
+            if (hipMemcpy(gauge_out, d_g, ng * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                st = efail(E, SW2DG_ERR_HIP, "copy of the gauge series failed");
+        if (!st && steps_out)
+            if (hipMemcpy(steps_out, d_steps, E->n_runs * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
+                st = efail(E, SW2DG_ERR_HIP, "copy of the step counts failed");
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    for (void *p : tmp) (void)hipFree(p);
+    return st;
+}
+
+int sw2dg_ens_get_state(sw2dg_ens *E, int32_t run, double *h, double *qx, double *qy) {
+    if (!E || run < 0 || run >= E->n_runs) return SW2DG_ERR_ARG;
+    EHIP(E, hipSetDevice(E->device));
+    const RunDesc &R = E->runs[run];
+    const MeshTables &T = E->T[run];
+    std::vector<double> v(R.nc);
+    double *dst[3] = {h, qx, qy};
+    double *src[3] = {E->d.h, E->d.qx, E->d.qy};
+    for (int f = 0; f < 3; ++f) {
+        if (!dst[f]) continue;
+        EHIP(E, hipMemcpy(v.data(), src[f] + R.c0, R.nc * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < R.nc; ++k) dst[f][T.perm[k]] = v[k];
+    }
+    return SW2DG_OK;
+}
+
+int64_t sw2dg_ens_device_bytes(sw2dg_ens *E) { return E ? E->bytes : 0; }
+const char *sw2dg_ens_last_error(sw2dg_ens *E) { return E ? E->err.c_str() : "null ensemble"; }
+void sw2dg_ens_destroy(sw2dg_ens *E) { delete E; }
+
+}  // extern "C"

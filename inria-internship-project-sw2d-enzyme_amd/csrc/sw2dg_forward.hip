@@ -584,6 +584,121 @@ const char *err_text(int st) {
 
 }  // namespace
 
+namespace sw2dg_detail {
+// Renumbering and device-order tables of one mesh (shared by the single-run
+// ctx and the ensemble): Hilbert order of the cell centres, internal edges by
+// (lower, upper) renumbered cell, boundary edges by cell, incidence lists in
+// the reference's order (swParent.cpp:469-486).
+void build_tables(const sw2dg_mesh *mesh, MeshTables &T) {
+    const int64_t nc = mesh->n_cell, nb = mesh->n_bnd, ni = mesh->n_int;
+    T.nc = nc;
+    T.nb = nb;
+    T.ni = ni;
+    // ---- Hilbert renumbering of cells
+    double xmin = mesh->cell_x[0], xmax = xmin, ymin = mesh->cell_y[0], ymax = ymin;
+    for (int64_t i = 1; i < nc; ++i) {
+        xmin = std::min(xmin, mesh->cell_x[i]);
+        xmax = std::max(xmax, mesh->cell_x[i]);
+        ymin = std::min(ymin, mesh->cell_y[i]);
+        ymax = std::max(ymax, mesh->cell_y[i]);
+    }
+    const double sc = 65535.0 / std::max({xmax - xmin, ymax - ymin, 1e-30});
+    std::vector<uint32_t> key(nc);
+    for (int64_t i = 0; i < nc; ++i) {
+        const uint32_t hx = (uint32_t)std::min(65535.0, std::max(0.0, (mesh->cell_x[i] - xmin) * sc));
+        const uint32_t hy = (uint32_t)std::min(65535.0, std::max(0.0, (mesh->cell_y[i] - ymin) * sc));
+        key[i] = hilbert_d(hx, hy);
+    }
+    T.perm.resize(nc);
+    std::iota(T.perm.begin(), T.perm.end(), 0);
+    std::stable_sort(T.perm.begin(), T.perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    T.inv.resize(nc);
+    for (int64_t k = 0; k < nc; ++k) T.inv[T.perm[k]] = (int)k;
+    // ---- internal edges sorted by (lower, upper) renumbered cell
+    T.eperm.resize(ni);
+    std::iota(T.eperm.begin(), T.eperm.end(), 0);
+    std::vector<int64_t> ekey(ni);
+    for (int64_t e = 0; e < ni; ++e) {
+        const int64_t a = T.inv[mesh->int_L[e]], b = T.inv[mesh->int_R[e]];
+        ekey[e] = std::min(a, b) * (int64_t)nc + std::max(a, b);
+    }
+    std::stable_sort(T.eperm.begin(), T.eperm.end(), [&](int a, int b) { return ekey[a] < ekey[b]; });
+    std::vector<int> einv(ni);
+    for (int64_t k = 0; k < ni; ++k) einv[T.eperm[k]] = (int)k;
+    // ---- boundary edges sorted by renumbered cell (stable: keeps reference order)
+    T.bperm.resize(nb);
+    std::iota(T.bperm.begin(), T.bperm.end(), 0);
+    std::stable_sort(T.bperm.begin(), T.bperm.end(),
+                     [&](int a, int b) { return T.inv[mesh->bnd_cell[a]] < T.inv[mesh->bnd_cell[b]]; });
+    T.binv.resize(nb);
+    for (int64_t k = 0; k < nb; ++k) T.binv[T.bperm[k]] = (int)k;
+    // ---- incidence lists in reference order (swParent.cpp:469-486)
+    auto &iptr = T.iptr, &bptr = T.bptr;
+    iptr.assign(nc + 1, 0);
+    bptr.assign(nc + 1, 0);
+    for (int64_t e = 0; e < ni; ++e) {
+        iptr[T.inv[mesh->int_L[e]] + 1]++;
+        iptr[T.inv[mesh->int_R[e]] + 1]++;
+    }
+    for (int64_t e = 0; e < nb; ++e) bptr[T.inv[mesh->bnd_cell[e]] + 1]++;
+    for (int64_t i = 0; i < nc; ++i) {
+        iptr[i + 1] += iptr[i];
+        bptr[i + 1] += bptr[i];
+    }
+    auto &iinc = T.iinc, &binc = T.binc;
+    iinc.assign(2 * ni, 0);
+    binc.assign(nb, 0);
+    std::vector<int> fill_i(iptr.begin(), iptr.end() - 1), fill_b(bptr.begin(), bptr.end() - 1);
+    for (int64_t e = 0; e < ni; ++e) {  // ascending reference edge index
+        const int ne = einv[e];
+        iinc[fill_i[T.inv[mesh->int_L[e]]]++] = (ne << 1) | 0;
+        iinc[fill_i[T.inv[mesh->int_R[e]]]++] = (ne << 1) | 1;
+    }
+    for (int64_t e = 0; e < nb; ++e) binc[fill_b[T.inv[mesh->bnd_cell[e]]]++] = T.binv[e];
+    // ---- gather arrays into the new order
+    auto &eL = T.eL, &eR = T.eR;
+    auto &elen = T.elen, &enx = T.enx, &eny = T.eny, &exm = T.exm, &eym = T.eym;
+    eL.assign(ni, 0);
+    eR.assign(ni, 0);
+    elen.assign(ni, 0.0);
+    enx.assign(ni, 0.0);
+    eny.assign(ni, 0.0);
+    exm.assign(ni, 0.0);
+    eym.assign(ni, 0.0);
+    for (int64_t k = 0; k < ni; ++k) {
+        const int e = T.eperm[k];
+        eL[k] = T.inv[mesh->int_L[e]];
+        eR[k] = T.inv[mesh->int_R[e]];
+        elen[k] = mesh->int_len[e];
+        enx[k] = mesh->int_nx[e];
+        eny[k] = mesh->int_ny[e];
+        exm[k] = mesh->int_xm[e];
+        eym[k] = mesh->int_ym[e];
+    }
+    auto &blen = T.blen, &bnx = T.bnx, &bny = T.bny;
+    auto &bser = T.bser, &bcell = T.bcell, &bcells = T.bcells;
+    blen.assign(nb, 0.0);
+    bnx.assign(nb, 0.0);
+    bny.assign(nb, 0.0);
+    bser.assign(nb, 0);
+    bcell.assign(std::max<int64_t>(nb, 1), 0);
+    bcells.clear();
+    for (int64_t k = 0; k < nb; ++k) {
+        const int e = T.bperm[k];
+        blen[k] = mesh->bnd_len[e];
+        bnx[k] = mesh->bnd_nx[e];
+        bny[k] = mesh->bnd_ny[e];
+        bser[k] = mesh->bnd_series[e];
+        bcell[k] = T.inv[mesh->bnd_cell[e]];
+    }
+    for (int64_t i = 0; i < nc; ++i)
+        if (bptr[i + 1] > bptr[i]) bcells.push_back((int)i);
+    T.n_bcells = (int)bcells.size();
+    if (bcells.empty()) bcells.push_back(0);
+}
+
+}  // namespace sw2dg_detail
+
 extern "C" {
 
 const char *sw2dg_version(void) { return "sw2dg 0.1 (gfx950, FP64, fp-contract off)"; }
@@ -630,95 +745,24 @@ int sw2dg_ctx_create(const sw2dg_mesh *mesh, const sw2dg_params *prm, int32_t de
     c->nc = nc;
     c->nb = nb;
     c->ni = ni;
-    // ---- Hilbert renumbering of cells
-    double xmin = mesh->cell_x[0], xmax = xmin, ymin = mesh->cell_y[0], ymax = ymin;
-    for (int64_t i = 1; i < nc; ++i) {
-        xmin = std::min(xmin, mesh->cell_x[i]);
-        xmax = std::max(xmax, mesh->cell_x[i]);
-        ymin = std::min(ymin, mesh->cell_y[i]);
-        ymax = std::max(ymax, mesh->cell_y[i]);
-    }
-    const double sc = 65535.0 / std::max({xmax - xmin, ymax - ymin, 1e-30});
-    std::vector<uint32_t> key(nc);
-    for (int64_t i = 0; i < nc; ++i) {
-        const uint32_t hx = (uint32_t)std::min(65535.0, std::max(0.0, (mesh->cell_x[i] - xmin) * sc));
-        const uint32_t hy = (uint32_t)std::min(65535.0, std::max(0.0, (mesh->cell_y[i] - ymin) * sc));
-        key[i] = hilbert_d(hx, hy);
-    }
-    c->perm.resize(nc);
-    std::iota(c->perm.begin(), c->perm.end(), 0);
-    std::stable_sort(c->perm.begin(), c->perm.end(), [&](int a, int b) { return key[a] < key[b]; });
-    c->inv.resize(nc);
-    for (int64_t k = 0; k < nc; ++k) c->inv[c->perm[k]] = (int)k;
-    // ---- internal edges sorted by (lower, upper) renumbered cell
-    c->eperm.resize(ni);
-    std::iota(c->eperm.begin(), c->eperm.end(), 0);
-    std::vector<int64_t> ekey(ni);
-    for (int64_t e = 0; e < ni; ++e) {
-        const int64_t a = c->inv[mesh->int_L[e]], b = c->inv[mesh->int_R[e]];
-        ekey[e] = std::min(a, b) * (int64_t)nc + std::max(a, b);
-    }
-    std::stable_sort(c->eperm.begin(), c->eperm.end(), [&](int a, int b) { return ekey[a] < ekey[b]; });
-    std::vector<int> einv(ni);
-    for (int64_t k = 0; k < ni; ++k) einv[c->eperm[k]] = (int)k;
-    // ---- boundary edges sorted by renumbered cell (stable: keeps reference order)
-    c->bperm.resize(nb);
-    std::iota(c->bperm.begin(), c->bperm.end(), 0);
-    std::stable_sort(c->bperm.begin(), c->bperm.end(),
-                     [&](int a, int b) { return c->inv[mesh->bnd_cell[a]] < c->inv[mesh->bnd_cell[b]]; });
-    c->binv.resize(nb);
-    for (int64_t k = 0; k < nb; ++k) c->binv[c->bperm[k]] = (int)k;
-    // ---- incidence lists in reference order (swParent.cpp:469-486)
-    std::vector<int> iptr(nc + 1, 0), bptr(nc + 1, 0);
-    for (int64_t e = 0; e < ni; ++e) {
-        iptr[c->inv[mesh->int_L[e]] + 1]++;
-        iptr[c->inv[mesh->int_R[e]] + 1]++;
-    }
-    for (int64_t e = 0; e < nb; ++e) bptr[c->inv[mesh->bnd_cell[e]] + 1]++;
-    for (int64_t i = 0; i < nc; ++i) {
-        iptr[i + 1] += iptr[i];
-        bptr[i + 1] += bptr[i];
-    }
-    std::vector<int> iinc(2 * ni), binc(nb), fill_i(iptr.begin(), iptr.end() - 1),
-        fill_b(bptr.begin(), bptr.end() - 1);
-    for (int64_t e = 0; e < ni; ++e) {  // ascending reference edge index
-        const int ne = einv[e];
-        iinc[fill_i[c->inv[mesh->int_L[e]]]++] = (ne << 1) | 0;
-        iinc[fill_i[c->inv[mesh->int_R[e]]]++] = (ne << 1) | 1;
-    }
-    for (int64_t e = 0; e < nb; ++e) binc[fill_b[c->inv[mesh->bnd_cell[e]]]++] = c->binv[e];
-    // ---- gather arrays into the new order
+    MeshTables T;
+    build_tables(mesh, T);
+    c->perm = T.perm;
+    c->inv = T.inv;
+    c->eperm = T.eperm;
+    c->bperm = T.bperm;
+    c->binv = T.binv;
+    auto &iptr = T.iptr, &bptr = T.bptr, &iinc = T.iinc, &binc = T.binc;
+    auto &eL = T.eL, &eR = T.eR;
+    auto &elen = T.elen, &enx = T.enx, &eny = T.eny, &exm = T.exm, &eym = T.eym;
+    auto &blen = T.blen, &bnx = T.bnx, &bny = T.bny;
+    auto &bser = T.bser, &bcell = T.bcell, &bcells = T.bcells;
+    const int n_bcells = T.n_bcells;
     auto cellv = [&](const double *src) {
         std::vector<double> v(nc);
         for (int64_t k = 0; k < nc; ++k) v[k] = src[c->perm[k]];
         return v;
     };
-    std::vector<int> eL(ni), eR(ni);
-    std::vector<double> elen(ni), enx(ni), eny(ni), exm(ni), eym(ni);
-    for (int64_t k = 0; k < ni; ++k) {
-        const int e = c->eperm[k];
-        eL[k] = c->inv[mesh->int_L[e]];
-        eR[k] = c->inv[mesh->int_R[e]];
-        elen[k] = mesh->int_len[e];
-        enx[k] = mesh->int_nx[e];
-        eny[k] = mesh->int_ny[e];
-        exm[k] = mesh->int_xm[e];
-        eym[k] = mesh->int_ym[e];
-    }
-    std::vector<double> blen(nb), bnx(nb), bny(nb);
-    std::vector<int> bser(nb), bcell(std::max<int64_t>(nb, 1), 0), bcells;
-    for (int64_t k = 0; k < nb; ++k) {
-        const int e = c->bperm[k];
-        blen[k] = mesh->bnd_len[e];
-        bnx[k] = mesh->bnd_nx[e];
-        bny[k] = mesh->bnd_ny[e];
-        bser[k] = mesh->bnd_series[e];
-        bcell[k] = c->inv[mesh->bnd_cell[e]];
-    }
-    for (int64_t i = 0; i < nc; ++i)
-        if (bptr[i + 1] > bptr[i]) bcells.push_back((int)i);
-    const int n_bcells = (int)bcells.size();
-    if (bcells.empty()) bcells.push_back(0);
     c->bser_host.assign(mesh->bnd_series, mesh->bnd_series + nb);
     MeshDev &m = c->m;
     m.n_cell = (int)nc;

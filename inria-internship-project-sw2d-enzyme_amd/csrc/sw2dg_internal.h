@@ -83,6 +83,19 @@ __device__ __forceinline__ int xcd_block() { return xcd_remap((int)blockIdx.x, (
 
 inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
+// host-side device-order tables of one mesh
+struct MeshTables {
+    int64_t nc = 0, nb = 0, ni = 0;
+    std::vector<int> perm, inv, eperm, bperm, binv;  // perm[new] = reference index
+    std::vector<int> iptr, bptr, iinc, binc;
+    std::vector<int> eL, eR;
+    std::vector<double> elen, enx, eny, exm, eym;
+    std::vector<double> blen, bnx, bny;
+    std::vector<int> bser, bcell, bcells;
+    int n_bcells = 0;
+};
+void build_tables(const sw2dg_mesh *mesh, MeshTables &T);
+
 struct AssimState;  // 4D-Var records and buffers (sw2dg_adjoint.hip)
 void assim_destroy(AssimState *a);
 
