@@ -198,6 +198,30 @@ int64_t sw2dg_ens_device_bytes(sw2dg_ens *ens);
 const char *sw2dg_ens_last_error(sw2dg_ens *ens);
 void sw2dg_ens_destroy(sw2dg_ens *ens);
 
+/* ---- Learned warm start: a dense MLP (GELU between layers) trained and run
+ * with BF16 MFMA kernels, FP32 accumulation, FP32 master weights.  No
+ * reference counterpart (the reference starts its minimiser from a
+ * porosity_ebauche_map, input.solv; swSpE_minimize.cpp:267-1073). ---- */
+typedef struct sw2dg_mlp sw2dg_mlp;
+/* dims[n_layers + 1]: input width, hidden widths, output width. */
+int sw2dg_mlp_create(int32_t n_layers, const int32_t *dims, int32_t max_batch, int32_t device,
+                     sw2dg_mlp **out);
+/* Parameters flattened layer by layer: W_l [dims[l] x dims[l+1]] row-major, then b_l. */
+int64_t sw2dg_mlp_n_params(sw2dg_mlp *mlp);
+int sw2dg_mlp_set_params(sw2dg_mlp *mlp, const float *params);  /* also resets Adam */
+int sw2dg_mlp_get_params(sw2dg_mlp *mlp, float *params);
+/* y [batch x dims[L]] = MLP(x [batch x dims[0]]) (inputs rounded to BF16). */
+int sw2dg_mlp_forward(sw2dg_mlp *mlp, int32_t batch, const float *x, float *y);
+/* Mean squared error against t and its gradient (flattened like the
+ * parameters; may be NULL). */
+int sw2dg_mlp_grad(sw2dg_mlp *mlp, int32_t batch, const float *x, const float *t, float *loss,
+                   float *grads);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on a batch; loss may be NULL. */
+int sw2dg_mlp_train_step(sw2dg_mlp *mlp, int32_t batch, const float *x, const float *t, float lr,
+                         float *loss);
+const char *sw2dg_mlp_last_error(sw2dg_mlp *mlp);
+void sw2dg_mlp_destroy(sw2dg_mlp *mlp);
+
 /* Bytes of device memory held by the ctx. */
 int64_t sw2dg_device_bytes(sw2dg_ctx *ctx);
 const char *sw2dg_last_error(sw2dg_ctx *ctx);

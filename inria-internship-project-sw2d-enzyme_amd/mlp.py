@@ -1,0 +1,161 @@
+"""The learned warm start: a dense MLP on the GPU (include/sw2dg.h,
+sw2dg_mlp_*), trained with the project's BF16 MFMA kernels, and the mapping
+between gauge series / calibration controls and its normalised inputs /
+outputs.
+
+``MLP(dims)`` holds the network; ``train`` runs Adam over minibatches;
+``WarmStart`` wraps the normalisation of the calibration problem: inputs are
+the gauge series minus the gauges' initial depth, scaled; outputs the 8 zone
+roughnesses (log n) and the 60 hydrograph knots relative to the default
+hydrograph.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+_f = C.POINTER(C.c_float)
+_DECL = {
+    "sw2dg_mlp_create": ([C.c_int32, N._i32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
+    "sw2dg_mlp_n_params": ([C.c_void_p], C.c_int64),
+    "sw2dg_mlp_set_params": ([C.c_void_p, _f], C.c_int),
+    "sw2dg_mlp_get_params": ([C.c_void_p, _f], C.c_int),
+    "sw2dg_mlp_forward": ([C.c_void_p, C.c_int32, _f, _f], C.c_int),
+    "sw2dg_mlp_grad": ([C.c_void_p, C.c_int32, _f, _f, _f, _f], C.c_int),
+    "sw2dg_mlp_train_step": ([C.c_void_p, C.c_int32, _f, _f, C.c_float, _f], C.c_int),
+    "sw2dg_mlp_last_error": ([C.c_void_p], C.c_char_p),
+    "sw2dg_mlp_destroy": ([C.c_void_p], None),
+}
+MLP_SYMBOLS = tuple(_DECL)
+
+
+def _lib():
+    lib = N.gpu_lib()
+    if not getattr(lib, "_mlp_ready", False):
+        for k, (a, r) in _DECL.items():
+            f = getattr(lib, k)
+            f.argtypes = a
+            f.restype = r
+        lib._mlp_ready = True
+    return lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(_f)
+
+
+def init_params(dims, seed):
+    """He-style init, flattened layer by layer (W [in x out] then b)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for a, b in zip(dims[:-1], dims[1:]):
+        out.append((rng.standard_normal((a, b)) * np.sqrt(2.0 / a)).astype(np.float32).ravel())
+        out.append(np.zeros(b, dtype=np.float32))
+    return np.concatenate(out)
+
+
+class MLP:
+    def __init__(self, dims, max_batch=1024, seed=0, device=-1):
+        self.lib = _lib()
+        self.dims = [int(d) for d in dims]
+        self.max_batch = int(max_batch)
+        d = np.ascontiguousarray(self.dims, dtype=np.int32)
+        h = C.c_void_p()
+        rc = self.lib.sw2dg_mlp_create(len(self.dims) - 1, N.i32ptr(d), self.max_batch, device, C.byref(h))
+        if rc != 0:
+            raise N.SW2DError(rc, "sw2dg_mlp_create failed")
+        self.h = h
+        self.n_params = int(self.lib.sw2dg_mlp_n_params(h))
+        self.set_params(init_params(self.dims, seed))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise N.SW2DError(rc, self.lib.sw2dg_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.sw2dg_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self._check(self.lib.sw2dg_mlp_set_params(self.h, _fp(p)))
+
+    def params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        self._check(self.lib.sw2dg_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = []
+        for i in range(0, len(x), self.max_batch):
+            xb = np.ascontiguousarray(x[i: i + self.max_batch])
+            y = np.empty((len(xb), self.dims[-1]), dtype=np.float32)
+            self._check(self.lib.sw2dg_mlp_forward(self.h, len(xb), _fp(xb), _fp(y)))
+            out.append(y)
+        return np.concatenate(out)
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.sw2dg_mlp_grad(self.h, len(x), _fp(x), _fp(t), C.byref(loss), _fp(g)))
+        return float(loss.value), g
+
+    def train(self, x, t, epochs=50, batch=256, lr=1e-3, seed=0, lr_decay=1.0):
+        """Adam over shuffled minibatches; returns the mean loss of every epoch."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        rng = np.random.default_rng(seed)
+        hist = []
+        loss = C.c_float()
+        for ep in range(epochs):
+            idx = rng.permutation(len(x))
+            tot, nb = 0.0, 0
+            for i in range(0, len(x) - batch + 1, batch):
+                sel = idx[i: i + batch]
+                xb = np.ascontiguousarray(x[sel])
+                tb = np.ascontiguousarray(t[sel])
+                self._check(self.lib.sw2dg_mlp_train_step(self.h, len(sel), _fp(xb), _fp(tb),
+                                                          float(lr), C.byref(loss)))
+                tot += loss.value
+                nb += 1
+            hist.append(tot / max(nb, 1))
+            lr *= lr_decay
+        return hist
+
+
+class WarmStart:
+    """Normalisation of the calibration problem around the MLP.
+
+    x: gauge depths [n_obs, n_gauges] minus the gauges' initial depth,
+       divided by 0.5 m, flattened gauge-major;
+    y: log(n / 0.03) per zone (8) and Q / Q_default - 1 per knot (60)."""
+
+    def __init__(self, h_gauge0, Q_default, n_ref=0.03):
+        self.h0 = np.asarray(h_gauge0, dtype=np.float64)
+        self.Qd = np.asarray(Q_default, dtype=np.float64)
+        self.n_ref = n_ref
+
+    def encode_x(self, gauges):
+        g = np.asarray(gauges, dtype=np.float64)  # [..., n_obs, n_gauges]
+        return ((g - self.h0) / 0.5).swapaxes(-1, -2).reshape(g.shape[:-2] + (-1,)).astype(np.float32)
+
+    def encode_y(self, n_zone, Q):
+        return np.concatenate([np.log(np.asarray(n_zone) / self.n_ref),
+                               np.asarray(Q) / self.Qd - 1.0], axis=-1).astype(np.float32)
+
+    def decode_y(self, y):
+        y = np.asarray(y, dtype=np.float64)
+        nz = y.shape[-1] - len(self.Qd)
+        return self.n_ref * np.exp(y[..., :nz]), self.Qd * (1.0 + y[..., nz:])

@@ -36,9 +36,9 @@ def test_host_library_exports_host_symbols():
 
 def test_python_bindings_cover_the_headers(sw):
     N = sw._native
-    from sw2d_amd import assim, ensemble
+    from sw2d_amd import assim, ensemble, mlp
     assert set(declared(HEADERS[0])) <= (set(N.GPU_SYMBOLS) | set(assim.ASSIM_SYMBOLS)
-                                         | set(ensemble.ENS_SYMBOLS))
+                                         | set(ensemble.ENS_SYMBOLS) | set(mlp.MLP_SYMBOLS))
     assert set(declared(HEADERS[1])) <= set(N.HOST_SYMBOLS)
 
 
