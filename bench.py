@@ -231,6 +231,76 @@ def cpu_baseline(res, args):
     }
 
 
+def time_to_misfit_bench(sw, args):
+    """The learned-vs-plain 4D-Var experiment: ensemble training set -> MLP
+    -> 4D-Var on the px-lattice mesh from the network's prediction and from
+    the default controls, both to the same gauge-misfit tolerance."""
+    from sw2d_amd import calibration, warmstart
+    lo, hi = (float(v) for v in args.train_cells.split(","))
+    if args.ttm_stage in ("all", "train"):
+        ts = warmstart.training_set(args.train_runs, launch_runs=args.launch_runs,
+                                    n_meshes=args.train_meshes, cells=(lo, hi),
+                                    window=args.ttm_window, log=log)
+        ws = warmstart.WarmStart(ts["Q_default"])
+        t0 = time.perf_counter()
+        hist = ws.fit(ts["gauges"], ts["h0"], ts["n"], ts["Q"], epochs=args.epochs, log=log)
+        stats = {"ens_runs_per_launch": ts["runs_per_launch"], "ens_launches": ts["launches"],
+                 "ens_block": ts["block"], "ens_device_s": ts["device_ms"] / 1e3,
+                 "ens_cell_steps_per_s": ts["cell_steps"] / (ts["device_ms"] * 1e-3),
+                 "ens_wall_s": ts["wall_s"], "fit_s": time.perf_counter() - t0,
+                 "train_loss": hist[-1][0], "validation_loss": hist[-1][1],
+                 "train_runs": args.train_runs, "train_cells_lo": lo, "train_cells_hi": hi}
+        if args.ttm_cache:
+            ws.save(args.ttm_cache, **stats)
+        if args.ttm_stage == "train":
+            return {"leg": "learned warm start: training", **stats}
+    else:
+        ws, d = warmstart.WarmStart.load(args.ttm_cache)
+        stats = {k: float(d[k]) for k in d if k.startswith(("ens_", "fit_", "train", "valid"))}
+    cal = calibration.Calibration(px=args.px, window=args.ttm_window, level_quantile=0.9)
+    s = sw.SpSolver(cal.case, device=0)
+    t0 = time.perf_counter()
+    obs = cal.observe(s, cal.n_true, cal.Q_true, max_steps=args.ttm_max_steps)
+    log(f"observed the truth on {cal.case.n_cell} cells in {time.perf_counter() - t0:.1f} s")
+    h0g = cal.case.a["h0"][cal.gauge_cells]
+    t0 = time.perf_counter()
+    n_l, Q_l = ws.predict(obs, h0g)
+    pred_s = time.perf_counter() - t0
+    n_obs, n_g = obs.shape
+    J_tol = args.misfit_rms ** 2 * n_obs * n_g
+    log(f"learned start: max |n/n_true - 1| = {np.max(np.abs(n_l / cal.n_true - 1)):.3f} "
+        f"(default {np.max(np.abs(cal.n_default / cal.n_true - 1)):.3f}); J_tol = {J_tol:.3e}")
+    starts = {"learned": (n_l, Q_l, pred_s), "plain": (cal.n_default, cal.Q_default, 0.0)}
+    res = warmstart.time_to_misfit(cal, s, obs, starts, J_tol, max_iter=args.ttm_max_iter,
+                                   time_limit=args.ttm_time_limit,
+                                   max_steps=args.ttm_max_steps, n_tiles=args.ttm_tiles,
+                                   adj_cut=args.adj_cut, log=log)
+    s.close()
+    tl, tp = res["learned"]["seconds"], res["plain"]["seconds"]
+    return {
+        "metric": METRIC,
+        "leg": "4D-Var time-to-misfit, learned vs plain",
+        "value": (tp / tl) if (tl and tp) else None,
+        "unit": "plain / learned time to the misfit tolerance",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "f64 (4D-Var), bf16 MFMA / fp32 accumulate (network)",
+        "data": "synthetic: seeded meshes, prior-drawn controls, twin observations",
+        "config": {
+            "workload": f"4D-Var on the {cal.case.n_cell}-cell mesh, {args.ttm_window:.0f} s window "
+                        f"in {args.ttm_tiles} tile(s), adjoint thin-film cut {args.adj_cut} m, "
+                        f"68 controls (8 Manning zones + 60-knot "
+                        f"hydrograph), 20 gauges every 30 s; tolerance: RMS gauge misfit "
+                        f"{args.misfit_rms} m (J <= {J_tol:.4g})",
+            "network": [n_obs * n_g, *warmstart.HIDDEN, len(n_l) + len(Q_l)],
+        },
+        "training": stats,
+        "learned": {k: v for k, v in res["learned"].items() if k != "trace"},
+        "plain": {k: v for k, v in res["plain"].items() if k != "trace"},
+        "traces": {k: v["trace"] for k, v in res.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,12 +313,33 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forward", action="store_true")
+    ap.add_argument("--time-to-misfit", action="store_true",
+                    help="run the learned-vs-plain 4D-Var experiment instead")
+    ap.add_argument("--train-runs", type=int, default=16384)
+    ap.add_argument("--launch-runs", type=int, default=1024)
+    ap.add_argument("--train-meshes", type=int, default=64)
+    ap.add_argument("--train-cells", default="2e4,1e5")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--ttm-window", type=float, default=1200.0)
+    ap.add_argument("--ttm-max-steps", type=int, default=30000)
+    ap.add_argument("--ttm-max-iter", type=int, default=200)
+    ap.add_argument("--ttm-time-limit", type=float, default=420.0)
+    ap.add_argument("--misfit-rms", type=float, default=0.01)
+    ap.add_argument("--ttm-tiles", type=int, default=1)
+    ap.add_argument("--adj-cut", type=float, default=1e-3,
+                    help="thin-film cut of the 4D-Var adjoint (m; 0 = exact adjoint)")
+    ap.add_argument("--ttm-stage", choices=("all", "train", "fourdvar"), default="all")
+    ap.add_argument("--ttm-cache", default=None,
+                    help="npz of the trained network (written by 'train', read by 'fourdvar')")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
     sw = load_pkg()
+    if args.time_to_misfit:
+        print(json.dumps(time_to_misfit_bench(sw, args)))
+        return
     res = fwd_adj_bench(sw, args)
     roof, kernels = roofline(res, args)
     case = res["case"]

@@ -159,6 +159,15 @@ int sw2dg_assim_gradient(sw2dg_ctx *ctx, const double *phi, const double *K1, co
  * series a twin experiment observes.  No reference counterpart (the
  * reference reads observations from files, swSpE_ad.cpp:47-154). */
 int sw2dg_assim_gauge_series(sw2dg_ctx *ctx, double *out);
+/* Thin-film cut of the reverse sweep: the adjoint of a cell's state after a
+ * step is dropped where that state's depth is below h_cut (m), so no
+ * sensitivity is carried back through films that are all but dry.  0 (the
+ * default) keeps the exact adjoint of the _E scheme, i.e. the reference's
+ * Enzyme gradient (swSpE_checkpoint.cpp:722-1281).  No reference
+ * counterpart: over long windows with wet/dry fronts the exact adjoint
+ * grows without bound through such films while the misfit itself varies
+ * smoothly; the cut gradient follows finite differences of the misfit. */
+int sw2dg_assim_set_adjoint_cut(sw2dg_ctx *ctx, double h_cut);
 /* One sw2dg_assim_gradient at the given controls with an event pair around
  * every kernel launch on the ctx stream: average device duration (ms) and
  * launch count per kernel (names are static strings), up to k_max entries.
@@ -194,6 +203,13 @@ int sw2dg_ens_run(sw2dg_ens *ens, int32_t n_obs, const double *obs_t, int32_t bl
                   int64_t max_steps, double *gauge_out, int64_t *steps_out, double *device_ms);
 /* Final state of one run after sw2dg_ens_run (reference order). */
 int sw2dg_ens_get_state(sw2dg_ens *ens, int32_t run, double *h, double *qx, double *qy);
+/* Upper bound on the workgroups one run may get when the runs fit the GPU
+ * at once (cooperative launch, runs split into clusters; default 64).  1:
+ * one workgroup per run always.  Results do not depend on it. */
+int sw2dg_ens_set_cluster_limit(sw2dg_ens *ens, int32_t max_wg_per_run);
+/* Shape of the last sw2dg_ens_run launch: workgroups launched and the
+ * largest cluster.  Either output may be NULL. */
+int sw2dg_ens_last_launch(sw2dg_ens *ens, int32_t *workgroups, int32_t *max_wg_per_run);
 int64_t sw2dg_ens_device_bytes(sw2dg_ens *ens);
 const char *sw2dg_ens_last_error(sw2dg_ens *ens);
 void sw2dg_ens_destroy(sw2dg_ens *ens);

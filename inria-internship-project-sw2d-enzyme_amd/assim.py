@@ -40,6 +40,7 @@ _DECL = {
                              C.c_int),
     "sw2dg_assim_final_state": ([C.c_void_p, N._d, N._d, N._d], C.c_int),
     "sw2dg_assim_gauge_series": ([C.c_void_p, N._d], C.c_int),
+    "sw2dg_assim_set_adjoint_cut": ([C.c_void_p, C.c_double], C.c_int),
     "sw2dg_assim_profile": ([C.c_void_p, N._d, N._d, N._d, C.c_int32, N._d, N._i64,
                              C.POINTER(C.c_char_p), C.POINTER(C.c_int32)], C.c_int),
 }
@@ -58,7 +59,7 @@ class Assim:
 
     def __init__(self, solver, obs_t, gauge_cells=None, gauge_h=None, gauge_w=None, frames=None,
                  frame_w=None, normalize=False, stride=0, n_tiles=1, t_window=-1e300,
-                 max_steps=100000):
+                 max_steps=100000, adj_cut=0.0):
         self.s = solver
         self.lib = solver.lib
         _declare(self.lib)
@@ -86,7 +87,10 @@ class Assim:
         self.n_obs = o.n_obs
         self.s._check(self.lib.sw2dg_assim_setup(solver.ctx, C.byref(o), int(stride), int(n_tiles),
                                                  float(t_window), self.max_steps))
+        # the cut is a property of the solver context: set it on every Assim
+        self.s._check(self.lib.sw2dg_assim_set_adjoint_cut(solver.ctx, float(adj_cut)))
         self.n_steps = 0
+        self._tmp = []
 
     def _hold(self, a):
         a = np.ascontiguousarray(a)
@@ -94,12 +98,17 @@ class Assim:
         return a
 
     def _ctl(self, x):
+        """Per-call control array (the library copies it before returning, so
+        it lives until the next call only)."""
         if x is None:
             return None
-        return N.dptr(self._hold(np.broadcast_to(np.asarray(x, dtype=np.float64),
-                                                 (self.s.case.n_cell,)).copy()))
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64),
+                                                 (self.s.case.n_cell,)))
+        self._tmp.append(a)
+        return N.dptr(a)
 
     def record(self, phi=None, K1=None, K2=None):
+        self._tmp = []
         n = C.c_int64()
         self.s._check(self.lib.sw2dg_assim_record(self.s.ctx, self._ctl(phi), self._ctl(K1),
                                                   self._ctl(K2), C.byref(n)))
@@ -118,10 +127,11 @@ class Assim:
         return {"t": t, "dt": dt[:n], "bc": bc[: n * ns].reshape(n, ns), "trig": trig[:n]}
 
     def set_bc_seq(self, bc):
-        bc = self._hold(np.asarray(bc, dtype=np.float64).reshape(-1))
+        bc = np.ascontiguousarray(np.asarray(bc, dtype=np.float64).reshape(-1))
         self.s._check(self.lib.sw2dg_assim_set_bc_seq(self.s.ctx, N.dptr(bc)))
 
     def cost(self, phi=None, K1=None, K2=None):
+        self._tmp = []
         J = C.c_double()
         self.s._check(self.lib.sw2dg_assim_cost(self.s.ctx, self._ctl(phi), self._ctl(K1),
                                                 self._ctl(K2), C.byref(J)))
@@ -129,6 +139,7 @@ class Assim:
 
     def gradient(self, phi=None, K1=None, K2=None):
         """Dry run at the controls, then the adjoint.  Returns J and dJ/d(phi, K1, K2, bc)."""
+        self._tmp = []
         nc = self.s.case.n_cell
         ns = max(self.s.case.n_series, 1)
         J = C.c_double()
@@ -145,6 +156,7 @@ class Assim:
 
     def profile(self, phi=None, K1=None, K2=None, k_max=32):
         """One gradient with per-launch HIP events: {kernel: (avg_ms, launches)}."""
+        self._tmp = []
         avg = np.zeros(k_max)
         cnt = np.zeros(k_max, dtype=np.int64)
         names = (C.c_char_p * k_max)()

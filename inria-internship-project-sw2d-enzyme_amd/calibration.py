@@ -181,24 +181,37 @@ class Calibration:
         A = Assim(solver, self.obs_t, gauge_cells=self.gauge_cells,
                   gauge_h=np.zeros((len(self.obs_t), len(self.gauge_cells))), max_steps=max_steps)
         A.record(K1=K, K2=K)
+        self._check_window(A.get_record(), max_steps)
         return A.gauge_series()
+
+    def _check_window(self, rec, max_steps):
+        if rec["t"][-1] < self.obs_t[-1] - 1e-9 * max(1.0, self.obs_t[-1]):
+            raise RuntimeError(f"the run stopped at t = {rec['t'][-1]:.3f} s, before the last "
+                               f"observation at {self.obs_t[-1]:.3f} s: raise max_steps "
+                               f"(now {max_steps})")
 
     def fourdvar(self, solver, obs, n0, Q0, J_tol=0.0, max_iter=60, max_steps=200000, m=6,
                  n_bounds=(0.01, 0.1), Q_bounds=(0.0, 5000.0), max_ls=30, time_limit=None,
-                 n_tiles=1):
+                 n_tiles=1, ls_cost="auto", adj_cut=0.0):
         """Projected L-BFGS (memory m) on the 68 controls from (n0, Q0)
         against the gauge series `obs`, organised like the reference's
         minimiser (assim/swSpE_minimize.cpp:398-511): a gradient evaluation
         runs the dry run at the controls (fixing the time-step schedule) and
         the adjoint; the line-search probes evaluate the cost on that frozen
-        schedule (replay only).  Controls are scaled to O(1) (n / 0.03,
-        Q / max Q_default) and kept in their bounds by projection.  Stops
-        when J <= J_tol, after max_iter iterations or time_limit seconds.
-        Returns (n, Q, trace) with trace = [(wall s, J, kind)] for every
-        evaluation, kind 'grad' or 'ls'."""
+        schedule (replay only, ls_cost "frozen").  With n_tiles > 1 the
+        gradient is that of the tiled cost (each tile restarts from the dry
+        run's state at its top), which drifts from the true misfit away from
+        the recorded trajectory; the probes then run a dry run at the trial
+        controls and take the true misfit of its gauge series (ls_cost
+        "record", the default for tiles).  adj_cut > 0 drops the adjoint
+        of thin films (sw2dg_assim_set_adjoint_cut).  Controls are scaled to O(1)
+        (n / 0.03, Q / max Q_default) and kept in their bounds by projection.
+        Stops when J <= J_tol, after max_iter iterations or time_limit
+        seconds.  Returns (n, Q, trace) with trace = [(wall s, J, kind)] for
+        every evaluation, kind 'grad' or 'ls'."""
         from .assim import Assim, interp_weights
         A = Assim(solver, self.obs_t, gauge_cells=self.gauge_cells, gauge_h=obs,
-                  max_steps=max_steps, n_tiles=n_tiles)
+                  max_steps=max_steps, n_tiles=n_tiles, adj_cut=adj_cut)
         nz = self.n_zones
         sn, sq = 0.03, max(float(np.max(self.Q_default)), 1.0)
         lo = np.concatenate([np.full(nz, n_bounds[0] / sn), np.full(len(Q0), Q_bounds[0] / sq)])
@@ -216,14 +229,28 @@ class Calibration:
             K = self.K(n)
             G = A.gradient(K1=K, K2=K)
             rec = A.get_record()
+            self._check_window(rec, max_steps)
             gn, gQ = self.grad_controls(G, n, rec["t"][:-1])
             state["bc"] = rec["bc"].copy()
             state["W"] = interp_weights(self.t_knots, rec["t"][:-1])
             trace.append((time.perf_counter() - t0, G["J"], "grad"))
             return G["J"], np.concatenate([gn * sn, gQ * sq])
 
-        def cost(z):  # frozen schedule of the last gradient evaluation
+        if ls_cost == "auto":
+            ls_cost = "frozen" if n_tiles == 1 else "record"
+        obs_arr = np.asarray(obs, dtype=np.float64)
+
+        def cost(z):
             n, Q = controls(z)
+            if ls_cost == "record":  # true misfit of a dry run at z
+                self.apply_hydrograph(solver, Q)
+                K = self.K(n)
+                A.record(K1=K, K2=K)
+                self._check_window(A.get_record(), max_steps)
+                J = float(np.sum((A.gauge_series() - obs_arr) ** 2))
+                trace.append((time.perf_counter() - t0, J, "ls"))
+                return J
+            # frozen schedule of the last gradient evaluation
             bc = state["bc"].copy()
             bc[:, self.series] = state["W"] @ Q / self.west_len
             A.set_bc_seq(bc)
@@ -254,15 +281,20 @@ class Calibration:
             if np.dot(d, g) >= 0.0:  # not a descent direction: restart from steepest descent
                 S, Y = [], []
                 d = -np.where(free, g, 0.0)
-            step = 1.0 if S else min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
             accepted = False
-            for _ in range(max_ls):
-                zn = np.clip(z + step * d, lo, hi)
-                Jn = cost(zn)
-                if Jn <= J + 1e-4 * np.dot(g, zn - z):
-                    accepted = True
+            for attempt in range(2):  # the quasi-Newton direction, then steepest descent
+                step = 1.0 if S else min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
+                for _ in range(max_ls):
+                    zn = np.clip(z + step * d, lo, hi)
+                    Jn = cost(zn)
+                    if Jn <= J + 1e-4 * np.dot(g, zn - z):
+                        accepted = True
+                        break
+                    step *= 0.5
+                if accepted or not S:
                     break
-                step *= 0.5
+                S, Y = [], []
+                d = -np.where(free, g, 0.0)
             if not accepted:
                 break
             Jn, gn = grad(zn)

@@ -518,6 +518,11 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
         s.qy = 0.;
     }
     const CellSt after_clamp = s;
+    if (s.h < o.adj_cut) {  // thin film: its adjoint is not carried back (adj_cut = 0: never)
+        ah = 0.0;
+        ax = 0.0;
+        ay = 0.0;
+    }
     if (o.head_loss) {
         if (o.friction != sp::FRIC_NONE)
             spe::fric_fwd(o.h_min, s.h, s.qx, s.qy, m.fcos[i], m.fsin[i], cs.K1[i], cs.K2[i], dt);
@@ -881,6 +886,12 @@ int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int
 }  // namespace
 
 extern "C" {
+
+int sw2dg_assim_set_adjoint_cut(sw2dg_ctx *c, double h_cut) {
+    if (!c || !(h_cut >= 0.0)) return SW2DG_ERR_ARG;
+    c->o.adj_cut = h_cut;
+    return SW2DG_OK;
+}
 
 int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_t n_tiles,
                       double t_window_start, int64_t max_steps) {

@@ -1,17 +1,31 @@
 // sw2dg_ensemble.hip -- batched ensemble of differentiable-forward (_E) runs
 // over uneven meshes, in one persistent launch.
 //
-// Each workgroup owns one run (its own mesh, controls, hydrograph, clock)
-// and steps it to t_max: edge fluxes -> block-level CFL minimum with the
+// Every run (its own mesh, controls, hydrograph, clock) is stepped to t_max
+// by a cluster of c >= 1 workgroups: edge fluxes -> CFL minimum with the
 // observation landing of build_checkpoint_records
-// (assim/swSpE_checkpoint.cpp:224-366) -> cell update -> gauge sampling,
-// phases separated by workgroup barriers.  No grid-wide synchronisation is
-// needed, so the launch holds every run; runs are dispatched largest first
-// (workgroup b takes the b-th largest run) and the hardware dispatcher hands
-// the remaining runs to CUs as they free up.  The per-step operations are
-// those of the single-run dry run (k_e_edges / k_e_cfl / k_e_dt / k_e_update
-// in sw2dg_adjoint.hip), in the same order, so each run reproduces
-// sw2dg_assim_record on its mesh bit for bit.
+// (assim/swSpE_checkpoint.cpp:224-366) -> cell update -> gauge sampling.
+// The per-step operations are those of the single-run dry run (k_e_edges /
+// k_e_cfl / k_e_dt / k_e_update in sw2dg_adjoint.hip), element by element
+// and in the same order, so each run reproduces sw2dg_assim_record on its
+// mesh bit for bit whatever its cluster size (the CFL minimum is exact in
+// any order; every workgroup of a cluster derives the same dt from the same
+// partial minima).
+//
+// Two launch modes:
+//  * more runs than co-resident workgroups: one workgroup per run (c = 1),
+//    phases separated by workgroup barriers, runs dispatched largest first;
+//    the hardware dispatcher hands the remaining runs to CUs as they free up.
+//  * otherwise: a cooperative launch that fills the GPU; every run gets at
+//    least one workgroup and the remaining ones go to the runs with the most
+//    work per workgroup (cells^1.5 ~ cells x steps), so the largest meshes no
+//    longer set the pace alone.  A cluster splits its run's edges and cells
+//    into contiguous (Hilbert-ordered) ranges and meets at three barriers per
+//    step (after the fluxes, the CFL partials and the update): agent-scope
+//    release / acquire around a per-run arrival counter.  Clusters are packed
+//    XCD by XCD (workgroup w runs on XCD w mod 8), so a run's workgroups
+//    share one L2.  A barrier wait has a spin limit; reaching it raises an
+//    abort flag that every workgroup polls, so the launch always drains.
 //
 // Layout: all runs' arrays concatenated (cells, internal edges, boundary
 // edges, incidences, series), each run's in the device order of
@@ -23,6 +37,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -81,16 +96,63 @@ __device__ double interp_e_(const double *tt, const double *vv, int n, double t)
     return v1 + (v2 - v1) * (t - t1) / den;
 }
 
+// workgroup -> (run, rank in the run's cluster); per-run cluster size,
+// arrival counter and CFL partial minima
+struct Clusters {
+    const int *wg_run, *wg_rank, *run_c;
+    unsigned *arrive;
+    double *part;  // [run][cmax]
+    int cmax;
+    int *abort;
+};
+
+constexpr unsigned kSpinLimit = 1u << 24;  // ~1 s of s_sleep 2 per wait
+
+// barrier of the c workgroups of one run (c == 1: the workgroup barrier)
+__device__ __forceinline__ bool run_sync(const Clusters &cl, int r, int c, unsigned &target, int *s_flag) {
+    __syncthreads();
+    if (c == 1) return true;
+    if (threadIdx.x == 0) {
+        target += (unsigned)c;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(cl.arrive + r, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(cl.arrive + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (__hip_atomic_load(cl.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(cl.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
 template <int B>
-__global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, double t_start,
+__global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double t_start,
                                            double t_max, double dt_max, double cfl, int n_obs,
                                            const double *__restrict__ obs_t, int ng_max,
                                            double *__restrict__ gser, int64_t max_steps,
                                            int64_t *steps_out) {
-    const int r = order[blockIdx.x];
+    const int r = cl.wg_run[blockIdx.x];
+    if (r < 0) return;  // unused slot of the XCD packing
+    const int rank = cl.wg_rank[blockIdx.x], c = cl.run_c[r];
     const RunDesc R = d.runs[r];
     const int tid = (int)threadIdx.x;
     const int nc = R.nc, ni = R.ni, nb = R.nb;
+    // this workgroup's contiguous share of the run's edges and cells
+    const int e_lo = (int)((int64_t)(ni + nb) * rank / c), e_hi = (int)((int64_t)(ni + nb) * (rank + 1) / c);
+    const int c_lo = (int)((int64_t)nc * rank / c), c_hi = (int)((int64_t)nc * (rank + 1) / c);
+    double *part = cl.part + (size_t)r * cl.cmax;
     // run-local views
     const double *zb = d.zb + R.c0, *area = d.area + R.c0, *phi = d.phi + R.c0;
     const double *beta = d.beta + R.c0, *K1 = d.K1 + R.c0, *K2 = d.K2 + R.c0;
@@ -112,15 +174,16 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, d
     __shared__ double red[B / 64];
     __shared__ double s_bcv[kMaxSeries];
     __shared__ double s_dt;
-    __shared__ int s_trig, s_oi;
+    __shared__ int s_trig, s_oi, s_flag;
     double t = t_start;
     int oi = 0;
     int64_t step = 0;
+    unsigned target = 0;
     while (t < t_max && step < max_steps) {
         if (tid < R.ns) s_bcv[tid] = interp_e_(st, sv + (size_t)tid * R.nt, R.nt, t);
         __syncthreads();
         // ---- edge fluxes (k_e_edges)
-        for (int e = tid; e < ni + nb; e += B) {
+        for (int e = e_lo + tid; e < e_hi; e += B) {
             if (e >= ni) {
                 const int j = e - ni;
                 const int i = bcell[j];
@@ -154,10 +217,10 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, d
             sw[2 * e] = (hL > o.h_min) ? m1 * len : 0.0;
             sw[2 * e + 1] = (hR > o.h_min) ? m1 * len : 0.0;
         }
-        __syncthreads();
-        // ---- CFL (k_e_cfl): per-cell candidate, block minimum
+        if (!run_sync(cl, r, c, target, &s_flag)) break;
+        // ---- CFL (k_e_cfl): per-cell candidate, cluster minimum
         double cand = dt_max;
-        for (int i = tid; i < nc; i += B) {
+        for (int i = c_lo + tid; i < c_hi; i += B) {
             const double hi = h[i];
             double swept = 0.0;
             if (hi > o.h_min) {
@@ -183,10 +246,16 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, d
         }
         if ((tid & 63) == 0) red[tid >> 6] = cand;
         __syncthreads();
-        // ---- step schedule (k_e_dt)
         if (tid == 0) {
-            double dtc = red[0];
-            for (int w = 1; w < B / 64; ++w) dtc = (red[w] < dtc) ? red[w] : dtc;
+            double m = red[0];
+            for (int w = 1; w < B / 64; ++w) m = (red[w] < m) ? red[w] : m;
+            part[rank] = m;
+        }
+        if (!run_sync(cl, r, c, target, &s_flag)) break;
+        // ---- step schedule (k_e_dt), replicated in every workgroup of the run
+        if (tid == 0) {
+            double dtc = part[0];
+            for (int w = 1; w < c; ++w) dtc = (part[w] < dtc) ? part[w] : dtc;
             if ((t + dtc) > t_max) dtc = t_max - t;
             if (dtc < 1e-6) dtc = 1e-6;
             double target = dt_max;
@@ -209,7 +278,7 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, d
         __syncthreads();
         const double dt = s_dt;
         // ---- cell update (k_e_update): balance, clean-up, clamp, friction
-        for (int i = tid; i < nc; i += B) {
+        for (int i = c_lo + tid; i < c_hi; i += B) {
             double sh = h[i], sx = qx[i], sy = qy[i];
             const double ph = phi[i], ar = area[i];
             for (int k = bptr[i]; k < bptr[i + 1]; ++k) {
@@ -245,17 +314,16 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, const int *order, Opts o, d
             qx[i] = sx;
             qy[i] = sy;
         }
-        __syncthreads();
-        // ---- gauges of an observation step
         const int tr = s_trig;
-        if (tr >= 0)
-            for (int g = tid; g < R.ng; g += B) gout[(size_t)tr * ng_max + g] = h[gcell[g]];
         t = t + dt;
         oi = s_oi;
         ++step;
-        __syncthreads();
+        if (!run_sync(cl, r, c, target, &s_flag)) break;
+        // ---- gauges of an observation step (the cluster's first workgroup)
+        if (tr >= 0 && rank == 0)
+            for (int g = tid; g < R.ng; g += B) gout[(size_t)tr * ng_max + g] = h[gcell[g]];
     }
-    if (tid == 0) steps_out[r] = step;
+    if (tid == 0 && rank == 0) steps_out[r] = step;
 }
 
 }  // namespace
@@ -277,6 +345,8 @@ struct sw2dg_ens {
     std::vector<std::vector<int>> r_type, r_gauge;
     std::vector<std::vector<double>> r_t, r_v;
     EnsDev d{};
+    int max_cluster = 64;  // workgroups per run in the cooperative mode (1: one per run)
+    int last_wgs = 0, last_cmax = 1;
     std::vector<void *> allocs;
     int64_t bytes = 0;
     template <class T_>
@@ -309,6 +379,95 @@ int efail(sw2dg_ens *e, int code, const std::string &msg) {
         hipError_t err_ = (x);                                                          \
         if (err_ != hipSuccess) return efail(e, SW2DG_ERR_HIP, hipGetErrorString(err_)); \
     } while (0)
+
+// Cluster plan: workgroups per run and the workgroup -> (run, rank) tables.
+// capacity = co-resident workgroups of the launch (0: one workgroup per run,
+// largest first).  In the cooperative mode every run gets one workgroup, the
+// rest go one by one to the run with the largest work per workgroup
+// (cells^1.5), at most max_c per run and at least 2 B cells per workgroup;
+// clusters are then packed into the 8 XCDs (first fit, largest first;
+// workgroup w sits on XCD w mod 8) and unused slots get run -1.
+struct Plan {
+    std::vector<int> wg_run, wg_rank, run_c;
+    int cmax = 1;
+};
+
+Plan make_plan(const std::vector<RunDesc> &runs, int capacity, int max_c, int B) {
+    const int n = (int)runs.size();
+    Plan p;
+    p.run_c.assign(n, 1);
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return runs[a].nc + runs[a].ni > runs[b].nc + runs[b].ni; });
+    if (capacity <= 0 || n > capacity) {
+        p.wg_run = order;
+        p.wg_rank.assign(n, 0);
+        return p;
+    }
+    std::vector<double> w(n);
+    std::vector<int> cap(n);
+    for (int r = 0; r < n; ++r) {
+        w[r] = std::pow((double)runs[r].nc, 1.5);
+        cap[r] = std::max(1, std::min(max_c, runs[r].nc / (2 * B)));
+    }
+    std::priority_queue<std::pair<double, int>> q;
+    for (int r = 0; r < n; ++r)
+        if (cap[r] > 1) q.push({w[r], r});
+    int used = n;
+    while (used < capacity && !q.empty()) {
+        const int r = q.top().second;
+        q.pop();
+        p.run_c[r] += 1;
+        ++used;
+        if (p.run_c[r] < cap[r]) q.push({w[r] / p.run_c[r], r});
+    }
+    // XCD packing
+    const int nx = 8, per = capacity / nx;
+    std::vector<std::vector<std::pair<int, int>>> bins(nx);  // (run, rank)
+    std::vector<int> order_c(n);
+    std::iota(order_c.begin(), order_c.end(), 0);
+    std::stable_sort(order_c.begin(), order_c.end(), [&](int a, int b) { return p.run_c[a] > p.run_c[b]; });
+    for (int r : order_c) {
+        int c = p.run_c[r], k = 0;
+        int fit = -1;
+        for (int x = 0; x < nx; ++x)
+            if ((int)bins[x].size() + c <= per) {
+                fit = x;
+                break;
+            }
+        while (k < c) {
+            int x = fit;
+            if (x < 0) {  // split over the emptiest bins
+                x = 0;
+                for (int y = 1; y < nx; ++y)
+                    if (bins[y].size() < bins[x].size()) x = y;
+            }
+            bins[x].push_back({r, k++});
+            if (fit >= 0 && (int)bins[fit].size() >= per) fit = -1;
+        }
+        p.cmax = std::max(p.cmax, c);
+    }
+    size_t longest = 0;
+    for (auto &b : bins) longest = std::max(longest, b.size());
+    const int G = (int)(longest * nx);
+    p.wg_run.assign(G, -1);
+    p.wg_rank.assign(G, 0);
+    for (int x = 0; x < nx; ++x)
+        for (size_t s_ = 0; s_ < bins[x].size(); ++s_) {
+            p.wg_run[s_ * nx + x] = bins[x][s_].first;
+            p.wg_rank[s_ * nx + x] = bins[x][s_].second;
+        }
+    return p;
+}
+
+template <int B>
+int co_resident(sw2dg_ens *E) {
+    int per_cu = 0, n_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ens<B>, B, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device) != hipSuccess) return 0;
+    return per_cu * n_cu;
+}
 }  // namespace
 
 extern "C" {
@@ -532,12 +691,24 @@ int sw2dg_ens_run(sw2dg_ens *E, int32_t n_obs, const double *obs_t, int32_t bloc
     d.runs = up(E->runs);
     std::vector<double> ot(obs_t, obs_t + n_obs);
     double *d_obs = up(ot);
-    // largest runs first
-    std::vector<int> order(E->n_runs);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return E->runs[a].nc + E->runs[a].ni > E->runs[b].nc + E->runs[b].ni; });
-    int *d_order = up(order);
+    int capacity = 0;
+    if (E->max_cluster > 1)
+        capacity = block == 256 ? co_resident<256>(E) : block == 512 ? co_resident<512>(E) : co_resident<1024>(E);
+    const Plan plan = make_plan(E->runs, capacity, E->max_cluster, block);
+    const bool coop = capacity > 0 && E->n_runs <= capacity;
+    E->last_wgs = (int)plan.wg_run.size();
+    E->last_cmax = plan.cmax;
+    Clusters cl{};
+    cl.wg_run = up(plan.wg_run);
+    cl.wg_rank = up(plan.wg_rank);
+    cl.run_c = up(plan.run_c);
+    cl.cmax = plan.cmax;
+    std::vector<unsigned> zeros_u(E->n_runs, 0u);
+    cl.arrive = up(zeros_u);
+    std::vector<double> part((size_t)E->n_runs * plan.cmax, 0.0);
+    cl.part = up(part);
+    std::vector<int> abort_v(1, 0);
+    cl.abort = up(abort_v);
     const size_t ng = (size_t)E->n_runs * std::max(1, n_obs) * std::max(1, ng_max);
     double *d_g = nullptr;
     int64_t *d_steps = nullptr;
@@ -559,20 +730,24 @@ int sw2dg_ens_run(sw2dg_ens *E, int32_t n_obs, const double *obs_t, int32_t bloc
         hipEventCreate(&e1);
         hipEventRecord(e0, E->stream);
         const sw2dg_params &p = E->prm;
-        const dim3 grid(E->n_runs);
-        if (block == 256)
-            hipLaunchKernelGGL(k_ens<256>, grid, dim3(256), 0, E->stream, d, d_order, E->o, p.t0, p.t_max,
-                               p.dt_max, p.cfl_max, n_obs, d_obs, std::max(1, ng_max), d_g, max_steps, d_steps);
-        else if (block == 512)
-            hipLaunchKernelGGL(k_ens<512>, grid, dim3(512), 0, E->stream, d, d_order, E->o, p.t0, p.t_max,
-                               p.dt_max, p.cfl_max, n_obs, d_obs, std::max(1, ng_max), d_g, max_steps, d_steps);
-        else
-            hipLaunchKernelGGL(k_ens<1024>, grid, dim3(1024), 0, E->stream, d, d_order, E->o, p.t0, p.t_max,
-                               p.dt_max, p.cfl_max, n_obs, d_obs, std::max(1, ng_max), d_g, max_steps, d_steps);
+        const dim3 grid((unsigned)plan.wg_run.size());
+        double t0 = p.t0, tm = p.t_max, dtm = p.dt_max, cf = p.cfl_max;
+        int ngm = std::max(1, ng_max);
+        int64_t ms_ = max_steps;
+        Opts o = E->o;
+        void *args[] = {&d, &cl, &o, &t0, &tm, &dtm, &cf, &n_obs, &d_obs, &ngm, &d_g, &ms_, &d_steps};
+        const void *fn = block == 256 ? (const void *)k_ens<256> : block == 512 ? (const void *)k_ens<512>
+                                                                                 : (const void *)k_ens<1024>;
+        hipError_t err = coop ? hipLaunchCooperativeKernel(fn, grid, dim3(block), args, 0, E->stream)
+                              : hipLaunchKernel(fn, grid, dim3(block), args, 0, E->stream);
         hipEventRecord(e1, E->stream);
-        hipError_t err = hipStreamSynchronize(E->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(E->stream);
         if (err == hipSuccess) err = hipGetLastError();
         if (err != hipSuccess) st = efail(E, SW2DG_ERR_HIP, hipGetErrorString(err));
+        int aborted = 0;
+        if (!st && hipMemcpy(&aborted, cl.abort, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            st = efail(E, SW2DG_ERR_HIP, "copy of the abort flag failed");
+        if (!st && aborted) st = efail(E, SW2DG_ERR_HIP, "a run's workgroups did not meet at a barrier (spin limit)");
         float ms = 0.f;
         if (!st) hipEventElapsedTime(&ms, e0, e1);
         if (device_ms) *device_ms = ms;
@@ -602,6 +777,19 @@ int sw2dg_ens_get_state(sw2dg_ens *E, int32_t run, double *h, double *qx, double
         EHIP(E, hipMemcpy(v.data(), src[f] + R.c0, R.nc * sizeof(double), hipMemcpyDeviceToHost));
         for (int k = 0; k < R.nc; ++k) dst[f][T.perm[k]] = v[k];
     }
+    return SW2DG_OK;
+}
+
+int sw2dg_ens_set_cluster_limit(sw2dg_ens *E, int32_t max_wg_per_run) {
+    if (!E || max_wg_per_run < 1 || max_wg_per_run > 1024) return SW2DG_ERR_ARG;
+    E->max_cluster = max_wg_per_run;
+    return SW2DG_OK;
+}
+
+int sw2dg_ens_last_launch(sw2dg_ens *E, int32_t *workgroups, int32_t *max_wg_per_run) {
+    if (!E) return SW2DG_ERR_ARG;
+    if (workgroups) *workgroups = E->last_wgs;
+    if (max_wg_per_run) *max_wg_per_run = E->last_cmax;
     return SW2DG_OK;
 }
 

@@ -820,6 +820,7 @@ int sw2dg_ctx_create(const sw2dg_mesh *mesh, const sw2dg_params *prm, int32_t de
     c->o.phi_min = prm->phi_min;
     c->o.v_max = prm->v_max;
     c->o.fr_max = prm->fr_max;
+    c->o.adj_cut = 0.0;
     // a default "no series" BC table (every boundary edge must then have series 0 set later)
     for (void *p : c->allocs)
         if (!p) {

@@ -70,6 +70,7 @@ struct MeshDev {
 struct Opts {
     int scheme, topo, friction, head_loss, neg_det;
     double h_min, phi_min, v_max, fr_max;
+    double adj_cut;  // reverse sweep: adjoint zeroed at cells shallower than this (0: exact)
 };
 
 // Workgroup b is dispatched to XCD b % 8.  Give each XCD a contiguous range

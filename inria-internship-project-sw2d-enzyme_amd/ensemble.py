@@ -27,6 +27,8 @@ _DECL = {
     "sw2dg_ens_run": ([C.c_void_p, C.c_int32, N._d, C.c_int32, C.c_int64, N._d, N._i64,
                        C.POINTER(C.c_double)], C.c_int),
     "sw2dg_ens_get_state": ([C.c_void_p, C.c_int32, N._d, N._d, N._d], C.c_int),
+    "sw2dg_ens_set_cluster_limit": ([C.c_void_p, C.c_int32], C.c_int),
+    "sw2dg_ens_last_launch": ([C.c_void_p, N._i32, N._i32], C.c_int),
     "sw2dg_ens_device_bytes": ([C.c_void_p], C.c_int64),
     "sw2dg_ens_last_error": ([C.c_void_p], C.c_char_p),
     "sw2dg_ens_destroy": ([C.c_void_p], None),
@@ -133,6 +135,16 @@ class Ensemble:
     def device_bytes(self):
         return self.lib.sw2dg_ens_device_bytes(self.h)
 
+    def set_cluster_limit(self, max_wg_per_run):
+        """Workgroups a run may get when all runs fit the GPU at once (1: one each)."""
+        self._check(self.lib.sw2dg_ens_set_cluster_limit(self.h, int(max_wg_per_run)))
+
+    def last_launch(self):
+        """(workgroups, largest cluster) of the last run()."""
+        w, c = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
+        self._check(self.lib.sw2dg_ens_last_launch(self.h, N.i32ptr(w), N.i32ptr(c)))
+        return int(w[0]), int(c[0])
+
 
 class Autotuner:
     """Picks the workgroup size of the ensemble launch: every candidate runs
@@ -188,11 +200,13 @@ class EnsembleProblem:
     problem's prior run in one launch and return the gauge series."""
 
     def __init__(self, n_runs=256, seed=7, window=1200.0, level_quantile=0.9, sizes=None,
-                 device=-1):
+                 device=-1, replicate=1):
         from .calibration import Calibration
         px = log_uniform_sizes(n_runs, seed) if sizes is None else np.asarray(sizes)
-        self.cals = [Calibration(px=int(p), window=window, level_quantile=level_quantile)
-                     for p in px]
+        meshes = [Calibration(px=int(p), window=window, level_quantile=level_quantile)
+                  for p in px]
+        # `replicate` runs per mesh (distinct controls, one device copy each)
+        self.cals = [c for c in meshes for _ in range(int(replicate))]
         self.ens = Ensemble([c.case for c in self.cals], device=device)
         c0 = self.cals[0]
         self.obs_t = c0.obs_t

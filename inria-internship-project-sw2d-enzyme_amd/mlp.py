@@ -1,13 +1,10 @@
-"""The learned warm start: a dense MLP on the GPU (include/sw2dg.h,
-sw2dg_mlp_*), trained with the project's BF16 MFMA kernels, and the mapping
-between gauge series / calibration controls and its normalised inputs /
-outputs.
+"""The network of the learned warm start: a dense MLP on the GPU
+(include/sw2dg.h, sw2dg_mlp_*), trained and evaluated with the project's
+BF16 MFMA kernels (FP32 accumulation, FP32 master weights).
 
-``MLP(dims)`` holds the network; ``train`` runs Adam over minibatches;
-``WarmStart`` wraps the normalisation of the calibration problem: inputs are
-the gauge series minus the gauges' initial depth, scaled; outputs the 8 zone
-roughnesses (log n) and the 60 hydrograph knots relative to the default
-hydrograph.
+``MLP(dims)`` holds the network; ``forward`` evaluates it, ``grad`` returns
+the mean-squared-error loss and its gradient, ``train`` runs Adam over
+shuffled minibatches.
 """
 import ctypes as C
 
@@ -134,28 +131,3 @@ class MLP:
             lr *= lr_decay
         return hist
 
-
-class WarmStart:
-    """Normalisation of the calibration problem around the MLP.
-
-    x: gauge depths [n_obs, n_gauges] minus the gauges' initial depth,
-       divided by 0.5 m, flattened gauge-major;
-    y: log(n / 0.03) per zone (8) and Q / Q_default - 1 per knot (60)."""
-
-    def __init__(self, h_gauge0, Q_default, n_ref=0.03):
-        self.h0 = np.asarray(h_gauge0, dtype=np.float64)
-        self.Qd = np.asarray(Q_default, dtype=np.float64)
-        self.n_ref = n_ref
-
-    def encode_x(self, gauges):
-        g = np.asarray(gauges, dtype=np.float64)  # [..., n_obs, n_gauges]
-        return ((g - self.h0) / 0.5).swapaxes(-1, -2).reshape(g.shape[:-2] + (-1,)).astype(np.float32)
-
-    def encode_y(self, n_zone, Q):
-        return np.concatenate([np.log(np.asarray(n_zone) / self.n_ref),
-                               np.asarray(Q) / self.Qd - 1.0], axis=-1).astype(np.float32)
-
-    def decode_y(self, y):
-        y = np.asarray(y, dtype=np.float64)
-        nz = y.shape[-1] - len(self.Qd)
-        return self.n_ref * np.exp(y[..., :nz]), self.Qd * (1.0 + y[..., nz:])

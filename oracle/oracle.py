@@ -127,6 +127,8 @@ def _e_decl():
     L.orc_e_gradient_frames.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d,
                                         _d, _i32, C.c_int32, _d, _d, _d, C.c_int32, _d, _d, _d, _d]
     L.orc_e_gradient_frames.restype = C.c_double
+    L.orc_e_set_adjoint_cut.argtypes = [C.c_double]
+    L.orc_e_set_adjoint_cut.restype = None
     L._e_ready = True
     return L
 
@@ -185,9 +187,10 @@ class OracleE:
 
 
 def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=None,
-                       normalize=False, n_obs=None, bc=None, gauge_w=None):
+                       normalize=False, n_obs=None, bc=None, gauge_w=None, adj_cut=0.0):
     """Gauge cost of the frozen replay and its gradient (reverse sweep of the
-    oracle's own replay): {"J", "phi", "K1", "K2", "bc" [n_steps, n_series]}."""
+    oracle's own replay): {"J", "phi", "K1", "K2", "bc" [n_steps, n_series]}.
+    adj_cut > 0: thin-film cut of the reverse sweep (orc_e_set_adjoint_cut)."""
     a = self.case.a
     phi = self._ctl(phi, a["phi"])
     K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
@@ -203,10 +206,15 @@ def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=N
     nob = int(n_obs if n_obs is not None else (trig.max() + 1 if len(trig) else 0))
     gphi, gK1, gK2 = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
     gbc = np.empty(max(n, 1) * self.ns)
-    J = _e_decl().orc_e_gradient(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
-                                 _dp(qy0), n, _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32), nob,
-                                 len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
-                                 1 if normalize else 0, _dp(gphi), _dp(gK1), _dp(gK2), _dp(gbc))
+    L = _e_decl()
+    L.orc_e_set_adjoint_cut(float(adj_cut))
+    try:
+        J = L.orc_e_gradient(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+                             _dp(qy0), n, _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32), nob,
+                             len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
+                             1 if normalize else 0, _dp(gphi), _dp(gK1), _dp(gK2), _dp(gbc))
+    finally:
+        L.orc_e_set_adjoint_cut(0.0)
     return {"J": J, "phi": gphi, "K1": gK1, "K2": gK2, "bc": gbc[: n * self.ns].reshape(n, self.ns)}
 
 

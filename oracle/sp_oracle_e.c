@@ -908,6 +908,13 @@ static void fac_dphi(const orc_case *c, const double *phi, int64_t cell, double 
 /* Gradient of the frozen-replay gauge cost (orc_e_replay) with respect to
  * porosity, the two Strickler coefficients per cell and the boundary value of
  * every step and series.  Returns J. */
+/* Thin-film cut of the reverse sweep (no reference counterpart; 0 = the
+ * exact adjoint): the adjoint of U^{n+1} is dropped at cells whose depth
+ * h^{n+1} is below the cut, before the reverse of step n.  Restates the
+ * library's sw2dg_assim_set_adjoint_cut for parity. */
+static double g_adj_cut = 0.0;
+void orc_e_set_adjoint_cut(double h_cut) { g_adj_cut = h_cut; }
+
 static double e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
                          const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
                          const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
@@ -981,6 +988,9 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
                 lam[2 * nc + i] += 2.0 * wq * (Un1[2 * nc + i] - fqy[i]) / norm;
             }
         }
+        if (g_adj_cut > 0.0)
+            for (int64_t i = 0; i < nc; i++)
+                if (Un1[i] < g_adj_cut) lam[i] = lam[nc + i] = lam[2 * nc + i] = 0.0;
         /* recompute the step's intermediates */
         memcpy(s.h, Un, sizeof(double) * nc);
         memcpy(s.qx, Un + nc, sizeof(double) * nc);

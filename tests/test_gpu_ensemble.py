@@ -2,7 +2,8 @@
 launch over uneven meshes reproduces the single-run dry run of the 4D-Var
 layer (sw2dg_assim_record, itself checked against the reference's _E
 physics) bit for bit -- gauge series, step count and final state -- for
-every workgroup size the autotuner may pick."""
+every workgroup size the autotuner may pick, with runs split over clusters
+of workgroups (cooperative launch) and with one workgroup per run."""
 import numpy as np
 import pytest
 
@@ -12,7 +13,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def problem(sw):
     from sw2d_amd import ensemble
-    P = ensemble.EnsembleProblem(n_runs=4, seed=3, window=120.0, sizes=[12, 20, 16, 28])
+    P = ensemble.EnsembleProblem(n_runs=4, seed=3, window=120.0, sizes=[40, 60, 50, 80])
     return P
 
 
@@ -25,9 +26,17 @@ def test_gpu_ensemble_runs_match_single_runs_bitwise(sw, problem):
     outs = {}
     for block in (256, 512, 1024):
         outs[block] = P.ens.run(P.obs_t, block=block)
-        print(f"[gpu] block {block}: steps {outs[block][1]}, {outs[block][2]:.2f} ms", flush=True)
+        wgs, cmax = P.ens.last_launch()
+        print(f"[gpu] block {block}: {wgs} workgroups, clusters up to {cmax}; steps {outs[block][1]}, "
+              f"{outs[block][2]:.2f} ms", flush=True)
+        if block == 256:
+            assert cmax > 1  # the runs were split over clusters
+    P.ens.set_cluster_limit(1)
+    outs["single"] = P.ens.run(P.obs_t, block=256)
+    assert P.ens.last_launch() == (len(P.cals), 1)
+    P.ens.set_cluster_limit(64)
     g, steps, _ = outs[256]
-    for block in (512, 1024):
+    for block in (512, 1024, "single"):
         np.testing.assert_array_equal(outs[block][0], g)
         np.testing.assert_array_equal(outs[block][1], steps)
     for r, c in enumerate(P.cals):
