@@ -301,6 +301,55 @@ def time_to_misfit_bench(sw, args):
     }
 
 
+def ensemble_bench(sw, args):
+    """The batched-ensemble leg: 256 runs of the calibration problem on
+    meshes of 20k-250k cells (log-uniform), 1200 s each, one launch, the
+    workgroup size picked by the autotuner."""
+    from sw2d_amd import ensemble
+    t0 = time.perf_counter()
+    P = ensemble.EnsembleProblem(n_runs=args.ens_runs, seed=7, window=1200.0)
+    log(f"ensemble: {len(P.cals)} runs, {P.n_cells} cells ({P.ens.device_bytes() / 1e9:.2f} GB), "
+        f"built in {time.perf_counter() - t0:.1f} s")
+    n, Q = P.sample(1)
+    P.set_controls(n, Q)
+    tuner = ensemble.Autotuner(probe_steps=30)
+    block = tuner.pick(P.ens, P.obs_t)
+    log(f"autotune (ms per step of the slowest run): {tuner.last} -> {block}")
+    t0 = time.perf_counter()
+    g, steps, ms = P.ens.run(P.obs_t, block=block)
+    wall = time.perf_counter() - t0
+    wgs, cmax = P.ens.last_launch()
+    cells = np.array([c.case.n_cell for c in P.cals])
+    cs = int(np.sum(cells * steps))
+    log(f"ensemble run: {ms / 1e3:.2f} s device ({wall:.2f} s wall), {wgs} workgroups, clusters up "
+        f"to {cmax}; {cs / (ms * 1e-3):.3e} cell-steps/s")
+    return {"metric": METRIC, "leg": "batched ensemble (_E forward, gauge series)",
+            "value": cs / (ms * 1e-3), "unit": "cell-steps/s", "higher_is_better": True,
+            "n_gpus": 1, "dtype": "f64",
+            "data": "synthetic: seeded meshes, prior-drawn controls",
+            "config": {"workload": f"{len(P.cals)} runs of the calibration problem, meshes "
+                                   f"{int(cells.min())}-{int(cells.max())} cells, 1200 s each, "
+                                   "20 gauges x 40 observations per run",
+                       "total_cells": int(cells.sum()), "cell_steps": cs,
+                       "steps_min_max": [int(steps.min()), int(steps.max())]},
+            "device_s": ms / 1e3, "wall_s": wall, "block": block, "workgroups": wgs,
+            "max_cluster": cmax, "autotune_ms_per_step": {str(k): v for k, v in tuner.last.items()}}
+
+
+def copy_calibration(args):
+    """A plain device copy of known size (torch's elementwise copy kernel,
+    2 x 2^28 doubles read and written once) for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of a profiled run to bytes."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +362,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forward", action="store_true")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the batched-ensemble leg instead")
+    ap.add_argument("--ens-runs", type=int, default=256)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain copy kernel of known size (counter scaling)")
     ap.add_argument("--time-to-misfit", action="store_true",
                     help="run the learned-vs-plain 4D-Var experiment instead")
     ap.add_argument("--train-runs", type=int, default=16384)
@@ -336,9 +390,15 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args)))
+        return
     sw = load_pkg()
     if args.time_to_misfit:
         print(json.dumps(time_to_misfit_bench(sw, args)))
+        return
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(sw, args)))
         return
     res = fwd_adj_bench(sw, args)
     roof, kernels = roofline(res, args)
