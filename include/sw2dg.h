@@ -159,6 +159,19 @@ int sw2dg_assim_gradient(sw2dg_ctx *ctx, const double *phi, const double *K1, co
  * series a twin experiment observes.  No reference counterpart (the
  * reference reads observations from files, swSpE_ad.cpp:47-154). */
 int sw2dg_assim_gauge_series(sw2dg_ctx *ctx, double *out);
+/* Adjoint explosion guard of the reverse sweep (swSpE_checkpoint.cpp:
+ * 1035-1117): after each segment, ||adjoint of h|| of its start state is
+ * compared with the first non-zero one; above `ratio` times it, the gradient
+ * returns to the last watermark (taken after milestone segments -- every
+ * 50th, the first and last, those with observations -- or, tiled, before
+ * each tile) and the sweep stops.  Later sw2dg_assim_cost calls then start
+ * at the first kept segment, as the reference's replay does (:1355-1400),
+ * and the gradient's J is that truncated cost.  Default 1e8 (the
+ * reference's); ratio <= 0 turns the guard off. */
+int sw2dg_assim_set_explosion_guard(sw2dg_ctx *ctx, double ratio);
+/* Truncation of the last gradient: first kept segment and its first step
+ * (both 0 when the whole window was kept). */
+int sw2dg_assim_truncation(sw2dg_ctx *ctx, int64_t *first_kept_step, int64_t *first_kept_segment);
 /* Thin-film cut of the reverse sweep: the adjoint of a cell's state after a
  * step is dropped where that state's depth is below h_cut (m), so no
  * sensitivity is carried back through films that are all but dry.  0 (the

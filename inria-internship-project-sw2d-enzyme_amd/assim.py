@@ -41,6 +41,8 @@ _DECL = {
     "sw2dg_assim_final_state": ([C.c_void_p, N._d, N._d, N._d], C.c_int),
     "sw2dg_assim_gauge_series": ([C.c_void_p, N._d], C.c_int),
     "sw2dg_assim_set_adjoint_cut": ([C.c_void_p, C.c_double], C.c_int),
+    "sw2dg_assim_set_explosion_guard": ([C.c_void_p, C.c_double], C.c_int),
+    "sw2dg_assim_truncation": ([C.c_void_p, N._i64, N._i64], C.c_int),
     "sw2dg_assim_profile": ([C.c_void_p, N._d, N._d, N._d, C.c_int32, N._d, N._i64,
                              C.POINTER(C.c_char_p), C.POINTER(C.c_int32)], C.c_int),
 }
@@ -59,7 +61,7 @@ class Assim:
 
     def __init__(self, solver, obs_t, gauge_cells=None, gauge_h=None, gauge_w=None, frames=None,
                  frame_w=None, normalize=False, stride=0, n_tiles=1, t_window=-1e300,
-                 max_steps=100000, adj_cut=0.0):
+                 max_steps=100000, adj_cut=0.0, explosion_ratio=1e8):
         self.s = solver
         self.lib = solver.lib
         _declare(self.lib)
@@ -89,6 +91,8 @@ class Assim:
                                                  float(t_window), self.max_steps))
         # the cut is a property of the solver context: set it on every Assim
         self.s._check(self.lib.sw2dg_assim_set_adjoint_cut(solver.ctx, float(adj_cut)))
+        self.s._check(self.lib.sw2dg_assim_set_explosion_guard(solver.ctx, float(explosion_ratio)))
+        self.stride = int(stride) if stride else int(np.ceil(np.sqrt(self.max_steps)))
         self.n_steps = 0
         self._tmp = []
 
@@ -153,6 +157,13 @@ class Assim:
         self.n_steps = n.value
         return {"J": J.value, "phi": gphi, "K1": gK1, "K2": gK2,
                 "bc": gbc[: n.value * ns].reshape(n.value, ns), "n_steps": n.value}
+
+    def truncation(self):
+        """(first kept step, first kept segment) of the last gradient: (0, 0)
+        unless the explosion guard stopped the reverse sweep."""
+        st, sg = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+        self.s._check(self.lib.sw2dg_assim_truncation(self.s.ctx, N.i64ptr(st), N.i64ptr(sg)))
+        return int(st[0]), int(sg[0])
 
     def profile(self, phi=None, K1=None, K2=None, k_max=32):
         """One gradient with per-launch HIP events: {kernel: (avg_ms, launches)}."""

@@ -83,6 +83,12 @@ struct AssimState {
     double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
     double *d_gphi = nullptr, *d_gK1 = nullptr, *d_gK2 = nullptr, *d_gbce = nullptr,
            *d_gbc = nullptr;
+    // explosion guard (swSpE_checkpoint.cpp:1035-1117): ratio (<= 0: off),
+    // gradient snapshot at the last milestone, truncation of the last gradient
+    double expl_ratio = 1e8;
+    int64_t trunc_seg = -1;
+    double *d_wm = nullptr;     // 3 nc: gphi | gK1 | gK2 at the watermark
+    double *d_nrm = nullptr;    // block partials of ||lam_h||^2, then the sum
     // controls (device order)
     double *d_phi = nullptr, *d_K1 = nullptr, *d_K2 = nullptr;
     // series -> boundary edges (renumbered) in reference order
@@ -450,6 +456,29 @@ __global__ __launch_bounds__(kBlock) void k_cost_frame_part(const double *__rest
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// ||x||^2 of n values: per-block tree sums, then one thread sums the blocks
+// in block order (deterministic)
+__global__ __launch_bounds__(kBlock) void k_sq_part(const double *__restrict__ x, int n,
+                                                    double *part) {
+    __shared__ double red[kBlock];
+    const int i = blockIdx.x * kBlock + (int)threadIdx.x;
+    const double v = (i < n) ? x[i] : 0.0;
+    red[threadIdx.x] = v * v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sq_sum(const double *part, int nparts, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nparts; ++b) s += part[b];
+    *out = s;
+}
+
 __global__ void k_cost_frame_sum(const double *part, int nparts, double norm, double *J) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double s = 0.0;
@@ -563,6 +592,9 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
 //           the adjoint restarts from 0; the porosity terms still count)
 //   seed    + dJ_obs/dU^n when U^n is observed
 //   CELL    reverse of the cell part of step n-1 (else: write lam = adj U^n)
+// gather: -1 nothing to gather (the last step, or a tile top: the future
+// adjoint is dropped), 1 gather part + edge sides, 2 take lam (the gather
+// already done by the CELL = false launch that closed the next segment).
 // Blocks past n_cell_blocks sum the boundary-value adjoints of step n per
 // series (one block per series, fixed order).
 template <bool CELL>
@@ -593,7 +625,11 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
     const int nc = m.n_cell;
     if (i >= nc) return;
     double ah = 0.0, ax = 0.0, ay = 0.0, gp = 0.0;
-    if (gather >= 0) {
+    if (gather == 2) {  // gathered at the end of the previous segment
+        ah = lam[i];
+        ax = lam[nc + i];
+        ay = lam[2 * nc + i];
+    } else if (gather >= 0) {
         const double a0 = part[i], b0 = part[nc + i], c0 = part[2 * nc + i];
         double a = a0, b = b0, c = c0;
         for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
@@ -611,14 +647,20 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
         }
     }
     seed_cell(sd, nc, i, ah, ax, ay);
+    // the gathered porosity terms and those of the cell reverse are added
+    // one after the other, so the result does not depend on whether a
+    // segment boundary splits them over two launches (stride invariance)
+    double g = gphi[i] + gp;
     if (CELL) {
-        cell_reverse(m, o, cs, P, dt, i, ah, ax, ay, lams, part, gp, gK1, gK2);
+        double gc_ = 0.0;
+        cell_reverse(m, o, cs, P, dt, i, ah, ax, ay, lams, part, gc_, gK1, gK2);
+        g += gc_;
     } else {
         lam[i] = ah;
         lam[nc + i] = ax;
         lam[2 * nc + i] = ay;
     }
-    gphi[i] += gp;
+    gphi[i] = g;
 }
 
 // per boundary cell: reverse of its boundary fluxes (hypBoundaryFlux_E and
@@ -848,6 +890,21 @@ Plan make_plan(const AssimState *a, const std::vector<double> &T) {
         p.tile_start.push_back(s_t);
         p.tile_top[e_t] = 1;
     }
+    // after an adjoint explosion the cost starts where the gradient stopped
+    // (replay_trajectory_cost, swSpE_checkpoint.cpp:1311-1400): tiles that
+    // start before the kept segments are dropped, one tile otherwise
+    if (a->trunc_seg >= 0 && a->trunc_seg + 1 < p.K) {
+        const int64_t s0 = a->trunc_seg + 1;
+        if (a->n_tiles > 1) {
+            std::vector<int64_t> kept;
+            for (int64_t st : p.tile_start)
+                if (st >= s0) kept.push_back(st);
+            p.tile_start = kept.empty() ? std::vector<int64_t>{s0} : kept;
+        } else {
+            p.tile_start = {s0};
+        }
+        p.first = std::max(p.first, s0);
+    }
     return p;
 }
 
@@ -886,6 +943,20 @@ int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int
 }  // namespace
 
 extern "C" {
+
+int sw2dg_assim_set_explosion_guard(sw2dg_ctx *c, double ratio) {
+    if (!c || !c->assim || !(ratio == ratio)) return SW2DG_ERR_ARG;
+    c->assim->expl_ratio = ratio;
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_truncation(sw2dg_ctx *c, int64_t *first_kept_step, int64_t *first_kept_segment) {
+    if (!c || !c->assim) return SW2DG_ERR_ARG;
+    const AssimState *a = c->assim;
+    if (first_kept_segment) *first_kept_segment = a->trunc_seg + 1;
+    if (first_kept_step) *first_kept_step = (a->trunc_seg + 1) * a->stride;
+    return SW2DG_OK;
+}
 
 int sw2dg_assim_set_adjoint_cut(sw2dg_ctx *c, double h_cut) {
     if (!c || !(h_cut >= 0.0)) return SW2DG_ERR_ARG;
@@ -995,6 +1066,8 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_gK2 = a->alloc<double>(nc);
     a->d_gbce = a->alloc<double>(std::max<int64_t>(nb, 1));
     a->d_gbc = a->alloc<double>((size_t)max_steps * ns);
+    a->d_wm = a->alloc<double>(3 * nc);
+    a->d_nrm = a->alloc<double>(nblocks(nc) + 1);
     a->d_phi = a->alloc<double>(nc);
     a->d_K1 = a->alloc<double>(nc);
     a->d_K2 = a->alloc<double>(nc);
@@ -1145,6 +1218,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     std::vector<double> T, dt;
     std::vector<int> trig;
     if ((st = fetch_record(c, a, T, trig, dt))) return st;
+    a->trunc_seg = -1;  // reset from any previous gradient (swSpE_checkpoint.cpp:731)
     const Plan p = make_plan(a, T);
     const int64_t nc = c->nc;
     const int ns = std::max(c->n_series, 1);
@@ -1177,20 +1251,39 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
                     c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
                     a->d_lams, a->d_part, a->d_gphi, a->d_gbce);
     };
+    // explosion guard (swSpE_checkpoint.cpp:1035-1117): ||adjoint of h|| at
+    // each segment start against the first non-zero one; a gradient snapshot
+    // (watermark) at milestones (every 50th, first and last segment, segments
+    // with observations) or, tiled, at each tile top; on explosion the
+    // gradient returns to the watermark and the reverse sweep stops
+    const bool guard = a->expl_ratio > 0.0;
+    const bool tiled = p.tile_start.size() > 1;
+    const int nbk = nblocks(nc);
+    double ref = -1.0;
+    int64_t wm_seg = p.K - 1, wm_kept = p.K, wm_step = a->n_steps;
+    auto snapshot = [&](int64_t from_step) -> int {
+        HIPCHK(c, hipMemcpyAsync(a->d_wm, a->d_gphi, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a->d_wm + nc, a->d_gK1, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(a->d_wm + 2 * nc, a->d_gK2, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        wm_step = from_step;
+        return SW2DG_OK;
+    };
     bool started = false;
-    int64_t n_last = -1;  // first step of the last reversed segment
     for (int64_t k = p.K - 1; k >= p.first; --k) {
-        // forward recompute of segment k from its checkpoint (with its costs);
-        // the boundary n1 between segments k and k+1 is crossed after it
+        if (guard && tiled && p.tile_top[k]) {  // clean state before a new tile
+            if ((st = snapshot((k + 1) * a->stride))) return st;
+            wm_kept = k + 1;
+        }
+        // forward recompute of segment k from its checkpoint (with its costs)
         if ((st = replay_segment(c, a, k, trig, true, nullptr, true))) return st;
         const int64_t n0 = k * a->stride, n1 = std::min(a->n_steps, n0 + a->stride);
         {
             const int64_t nt = n1 - 1;
             const SeedArgs sd = seed_args(c, a, trig[nt], seg_state(a, c, (int)(n1 - n0)));
-            if (!started)
-                gc(true, -1, -1, sd, seg_pre(a, c, (int)(nt - n0)), dt[nt]);
-            else  // tile top: the adjoint restarts from 0
-                gc(true, p.tile_top[k] ? 0 : 1, n1, sd, seg_pre(a, c, (int)(nt - n0)), dt[nt]);
+            // the adjoint of the segment's end state was gathered when segment
+            // k + 1 closed (into lam); a tile top drops it
+            gc(true, (!started || p.tile_top[k]) ? -1 : 2, -1, sd, seg_pre(a, c, (int)(nt - n0)),
+               dt[nt]);
             started = true;
             bnd(nt, (int)(nt - n0));
         }
@@ -1204,9 +1297,41 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
                 bnd(n - 1, j - 1);
             }
         }
-        n_last = n0;
+        // close the segment: adjoint of its start state (no seed: an
+        // observation of that state belongs to segment k - 1), the porosity
+        // terms of its first step and the boundary-value sums of step n0
+        gc(false, 1, n0, seed_args(c, a, -1, nullptr), nullptr, 0.0);
+        if (!guard) continue;
+        KLAUNCH(a, k_sq_part, dim3(nbk), blk, 0, c->stream, a->d_lam, (int)nc, a->d_nrm);
+        KLAUNCH(a, k_sq_sum, dim3(1), dim3(64), 0, c->stream, a->d_nrm, nbk, a->d_nrm + nbk);
+        double nrm2 = 0.0;
+        HIPCHK(c, hipMemcpyAsync(&nrm2, a->d_nrm + nbk, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double nrm = std::sqrt(nrm2);
+        if (ref < 0.0 && nrm > 0.0) ref = nrm;
+        if (!tiled) {
+            bool has_obs = false;
+            for (int64_t n = n0; n < n1; ++n) has_obs = has_obs || trig[n] >= 0;
+            if (k % 50 == 0 || k == p.K - 1 || k == 0 || has_obs) {
+                if ((st = snapshot(n0))) return st;
+                wm_seg = k;
+            }
+        }
+        if (ref > 0.0 && nrm > a->expl_ratio * ref) {
+            HIPCHK(c, hipMemcpyAsync(a->d_gphi, a->d_wm, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(a->d_gK1, a->d_wm + nc, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(a->d_gK2, a->d_wm + 2 * nc, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            if (wm_step > 0)
+                zero(c, a->d_gbc, (size_t)std::min(wm_step, a->n_steps) * ns);
+            a->trunc_seg = (tiled ? wm_kept : wm_seg) - 1;
+            // the cost covers what the gradient kept: observations of the
+            // dropped segments leave the total
+            const int64_t keep_from = (a->trunc_seg + 1) * a->stride;
+            for (int64_t n = 0; n < std::min(keep_from, a->n_steps); ++n)
+                if (trig[n] >= 0) zero(c, a->d_J + 1 + trig[n], 1);
+            break;
+        }
     }
-    if (started) gc(false, 1, n_last, seed_args(c, a, -1, nullptr), nullptr, 0.0);
     KLAUNCH(a, k_cost_total, dim3(1), dim3(64), 0, c->stream, a->d_J, a->n_obs);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));

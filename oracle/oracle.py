@@ -129,6 +129,10 @@ def _e_decl():
     L.orc_e_gradient_frames.restype = C.c_double
     L.orc_e_set_adjoint_cut.argtypes = [C.c_double]
     L.orc_e_set_adjoint_cut.restype = None
+    L.orc_e_set_explosion_guard.argtypes = [C.c_double, C.c_int64]
+    L.orc_e_set_explosion_guard.restype = None
+    L.orc_e_truncation.argtypes = []
+    L.orc_e_truncation.restype = C.c_int64
     L._e_ready = True
     return L
 
@@ -187,10 +191,13 @@ class OracleE:
 
 
 def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=None,
-                       normalize=False, n_obs=None, bc=None, gauge_w=None, adj_cut=0.0):
+                       normalize=False, n_obs=None, bc=None, gauge_w=None, adj_cut=0.0,
+                       explosion=None):
     """Gauge cost of the frozen replay and its gradient (reverse sweep of the
-    oracle's own replay): {"J", "phi", "K1", "K2", "bc" [n_steps, n_series]}.
-    adj_cut > 0: thin-film cut of the reverse sweep (orc_e_set_adjoint_cut)."""
+    oracle's own replay): {"J", "phi", "K1", "K2", "bc" [n_steps, n_series],
+    "first_kept_step"}.  adj_cut > 0: thin-film cut of the reverse sweep
+    (orc_e_set_adjoint_cut); explosion = (ratio, stride): the explosion guard
+    over segments of `stride` steps (orc_e_set_explosion_guard)."""
     a = self.case.a
     phi = self._ctl(phi, a["phi"])
     K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
@@ -208,14 +215,19 @@ def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=N
     gbc = np.empty(max(n, 1) * self.ns)
     L = _e_decl()
     L.orc_e_set_adjoint_cut(float(adj_cut))
+    if explosion is not None:
+        L.orc_e_set_explosion_guard(float(explosion[0]), int(explosion[1]))
     try:
         J = L.orc_e_gradient(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
                              _dp(qy0), n, _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32), nob,
                              len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
                              1 if normalize else 0, _dp(gphi), _dp(gK1), _dp(gK2), _dp(gbc))
+        kept = int(L.orc_e_truncation())
     finally:
         L.orc_e_set_adjoint_cut(0.0)
-    return {"J": J, "phi": gphi, "K1": gK1, "K2": gK2, "bc": gbc[: n * self.ns].reshape(n, self.ns)}
+        L.orc_e_set_explosion_guard(0.0, 0)
+    return {"J": J, "phi": gphi, "K1": gK1, "K2": gK2, "bc": gbc[: n * self.ns].reshape(n, self.ns),
+            "first_kept_step": kept}
 
 
 OracleE.gradient = _oracle_e_gradient

@@ -915,6 +915,24 @@ static void fac_dphi(const orc_case *c, const double *phi, int64_t cell, double 
 static double g_adj_cut = 0.0;
 void orc_e_set_adjoint_cut(double h_cut) { g_adj_cut = h_cut; }
 
+/* Adjoint explosion guard (get_spatial_gradient_checkpointed,
+ * assim/swSpE_checkpoint.cpp:1035-1117, untiled): segments of `stride`
+ * steps; after the reverse of segment k, ||lambda_h|| of its start state
+ * (before that state's own observation seed) against the first non-zero
+ * one; watermark = the gradient after milestone segments (k % 50 == 0,
+ * first, last, segments with an observation); above ratio x ref the
+ * gradient returns to the watermark, boundary-value gradients of earlier
+ * steps are zeroed and the cost keeps only observations from the
+ * watermark segment on.  ratio <= 0 or stride <= 0: off.
+ * orc_e_truncation() gives the first kept step of the last gradient. */
+static double g_expl_ratio = 0.0;
+static int64_t g_expl_stride = 0, g_first_kept = 0;
+void orc_e_set_explosion_guard(double ratio, int64_t stride) {
+    g_expl_ratio = ratio;
+    g_expl_stride = stride;
+}
+int64_t orc_e_truncation(void) { return g_first_kept; }
+
 static double e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
                          const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
                          const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
@@ -934,6 +952,8 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
     memcpy(U + nc, qx0, sizeof(double) * nc);
     memcpy(U + 2 * nc, qy0, sizeof(double) * nc);
     double J = 0.0;
+    double *Jstep = (double *)calloc((size_t)(n_steps > 0 ? n_steps : 1), sizeof(double));
+    g_first_kept = 0;
     for (int64_t n = 0; n < n_steps; n++) {
         e_step(&s, dt_seq[n], bc_seq + n * ns);
         double *Un = U + 3 * (size_t)nc * (size_t)(n + 1);
@@ -949,6 +969,7 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
             }
             if (normalize && n_obs > 0) Jo /= (double)n_obs;
             J += Jo;
+            Jstep[n] += Jo;
         }
         if (oi >= 0 && oi < n_obs && frames) {
             const double *fh = frames + (int64_t)oi * 3 * nc, *fqx = fh + nc, *fqy = fh + 2 * nc;
@@ -960,8 +981,15 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
             }
             if (normalize && n_obs > 0) Jo /= (double)n_obs;
             J += Jo;
+            Jstep[n] += Jo;
         }
     }
+    const int guard = g_expl_ratio > 0.0 && g_expl_stride > 0;
+    const int64_t S = g_expl_stride > 0 ? g_expl_stride : 1;
+    const int64_t K = (n_steps + S - 1) / S;
+    double ref = -1.0;
+    int64_t wm_seg = K - 1;
+    double *wm = guard ? (double *)calloc(3 * (size_t)nc, sizeof(double)) : NULL;
     /* reverse */
     double *lam = (double *)calloc(3 * (size_t)nc, sizeof(double));  /* adjoint of U^{n+1} */
     double *lams = (double *)calloc(3 * (size_t)nc, sizeof(double)); /* adjoint after balance */
@@ -1116,7 +1144,35 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
             lr_adj(Un[R], Un[nc + R], Un[2 * nc + R], c->beta[R], nx, ny, c->h_min, d[3], d[4], d[5],
                    d[7], d[9], &lam[R], &lam[nc + R], &lam[2 * nc + R]);
         }
+        if (guard && n % S == 0) { /* segment k = n / S reversed: lam = adjoint of U^n */
+            const int64_t k = n / S;
+            double q = 0.0;
+            for (int64_t i = 0; i < nc; i++) q += lam[i] * lam[i];
+            const double nrm = sqrt(q);
+            if (ref < 0.0 && nrm > 0.0) ref = nrm;
+            int has_obs = 0;
+            for (int64_t m = n; m < n + S && m < n_steps; m++) has_obs |= obs_trig[m] >= 0;
+            if (k % 50 == 0 || k == K - 1 || k == 0 || has_obs) {
+                memcpy(wm, g_phi, sizeof(double) * nc);
+                memcpy(wm + nc, g_K1, sizeof(double) * nc);
+                memcpy(wm + 2 * nc, g_K2, sizeof(double) * nc);
+                wm_seg = k;
+            }
+            if (ref > 0.0 && nrm > g_expl_ratio * ref) {
+                memcpy(g_phi, wm, sizeof(double) * nc);
+                memcpy(g_K1, wm + nc, sizeof(double) * nc);
+                memcpy(g_K2, wm + 2 * nc, sizeof(double) * nc);
+                g_first_kept = wm_seg * S;
+                for (int64_t m = 0; m < g_first_kept * (ns > 0 ? ns : 1); m++) g_bc[m] = 0.0;
+                J = 0.0;
+                for (int64_t m = g_first_kept; m < n_steps; m++)
+                    if (obs_trig[m] >= 0) J += Jstep[m];
+                break;
+            }
+        }
     }
+    free(wm);
+    free(Jstep);
     free(U);
     free(lam);
     free(lams);
