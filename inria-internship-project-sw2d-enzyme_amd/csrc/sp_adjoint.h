@@ -46,29 +46,61 @@ SPD Side lr_fwd(double h, double qx, double qy, double zb, double b, double nx, 
     return s;
 }
 
-// adjoint of lr_fwd: (dz, dq, dr, dlm, dlp) -> accumulate into (dh, dqx, dqy)
-SPD void lr_vjp(double h, double qx, double qy, double b, double nx, double ny, double h_min,
-                double dz, double dq, double dr, double dlm, double dlp, double &dh, double &dqx,
-                double &dqy) {
-    const double q = qx * nx + qy * ny;
-    const double u = (h > h_min) ? q / h : 0.0;
+// the intermediates of lr_fwd its adjoint needs (normal discharge q,
+// velocity u, celerity c = sqrt(g h), s = sqrt((b - 1) b u^2 + c^2), 1/h)
+struct LrAux {
+    double q, u, c, s, inv_h;
+};
+
+// lr_fwd keeping its intermediates: the same operations, so the returned Side
+// is bit-identical to lr_fwd's (lambda_minus / lambda_plus share one root)
+SPD Side lr_fwd_x(double h, double qx, double qy, double zb, double b, double nx, double ny,
+                  double h_min, LrAux &x) {
+    Side s;
+    s.z = h + zb;
+    s.q = qx * nx + qy * ny;
+    s.r = -qx * ny + qy * nx;
+    const bool wet = h > h_min;
+    const double u = wet ? s.q / h : 0.0;
     const double c = sqrt(kG * h);
-    const double A = (b - 1.0) * b * u * u + c * c;
-    const double s = sqrt(A);
+    const double root = sqrt((b - 1.0) * b * u * u + c * c);
+    s.lm = b * u - root;
+    s.lp = b * u + root;
+    x.q = s.q;
+    x.u = u;
+    x.c = c;
+    x.s = root;
+    x.inv_h = wet ? 1.0 / h : 0.0;
+    return s;
+}
+
+// adjoint of lr_fwd from its intermediates: (dz, dq, dr, dlm, dlp) ->
+// accumulate into (dh, dqx, dqy)
+SPD void lr_vjp_x(const LrAux &x, double b, double nx, double ny, double dz, double dq, double dr,
+                  double dlm, double dlp, double &dh, double &dqx, double &dqy) {
     double du = b * (dlm + dlp);
     const double ds = dlp - dlm;
-    const double dA = dsqrt_arg(s, ds);
-    du += dA * (b - 1.0) * b * 2.0 * u;
-    const double dc = dA * 2.0 * c;
-    double dhh = dz + dsqrt_arg(c, dc) * kG;
+    const double dA = dsqrt_arg(x.s, ds);
+    du += dA * (b - 1.0) * b * 2.0 * x.u;
+    const double dc = dA * 2.0 * x.c;
+    double dhh = dz + dsqrt_arg(x.c, dc) * kG;
     double dqq = dq;
-    if (h > h_min) {
-        dqq += du / h;
-        dhh += -du * q / (h * h);
+    if (x.inv_h != 0.0) {  // h > h_min
+        dqq += du * x.inv_h;
+        dhh += -du * x.q * (x.inv_h * x.inv_h);
     }
     dh += dhh;
     dqx += dqq * nx - dr * ny;
     dqy += dqq * ny + dr * nx;
+}
+
+// adjoint of lr_fwd: (dz, dq, dr, dlm, dlp) -> accumulate into (dh, dqx, dqy)
+SPD void lr_vjp(double h, double qx, double qy, double b, double nx, double ny, double h_min,
+                double dz, double dq, double dr, double dlm, double dlp, double &dh, double &dqx,
+                double &dqy) {
+    LrAux x;
+    (void)lr_fwd_x(h, qx, qy, 0.0, b, nx, ny, h_min, x);
+    lr_vjp_x(x, b, nx, ny, dz, dq, dr, dlm, dlp, dh, dqx, dqy);
 }
 
 // ---------------------------------------------------------------------------
@@ -154,7 +186,8 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
     if (reg == R_BOTH) {
         const double hLs = (hL > kSmall) ? hL : kSmall;
         const double hRs = (hR > kSmall) ? hR : kSmall;
-        const double uL = qL / hLs, uR = qR / hRs;
+        const double ihL = 1.0 / hLs, ihR = 1.0 / hRs;
+        const double uL = qL * ihL, uR = qR * ihR;
         int s1 = (lmL < lmR) ? 1 : 2;
         double L1 = (s1 == 1) ? lmL : lmR;
         if (L1 > 0.0) {
@@ -170,22 +203,23 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
         const double den0 = L2 - L1;
         const bool den_free = !(den0 < kSmall);
         const double den = den_free ? den0 : kSmall;
+        const double iden = 1.0 / den, iden2 = iden * iden;
         const double dphi = phiL - phiR;
         const double sq = sqrt(dphi * dphi + kPhiEps * kPhiEps);
         const double phiLR = 0.5 * (phiL + phiR - sq);
         const double A1 = L2 * qL - L1 * qR + L1 * L2 * (zR - zL);
         const double N1 = phiLR * A1;
-        const double f1 = N1 / den;
+        const double f1 = N1 * iden;  // sign of N1 / den
         const double fact = 0.5 * phiLR * (hL + hR);
         const double s2a = (phiR * hR * hR - phiL * hL * hL) * 0.5 + fact * (zL - zR);
         const double s2b = s2a * kG;
-        const double s2 = s2b / den;
+        const double s2 = s2b * iden;
         const double sl = -L1 * s2, sr = L2 * s2;
         const double FL = sp::specific_force(bL, phiL, hL, uL);
         const double FR = sp::specific_force(bR, phiR, hR, uR);
         const double P = phiR * qR - phiL * qL;
         const double A2 = L2 * (FL + sl) - L1 * (FR - sr) + L1 * L2 * P;
-        const double vL = rL / hLs, vR = rR / hRs;
+        const double vL = rL * ihL, vR = rR * ihR;
         double df1 = df[0];
         const double df2 = df[1], df3 = df[2];
         double dsl = df[3], dsr = df[4];
@@ -195,17 +229,17 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
         if (f1 >= 0.0) {
             df1 += df3 * bL * vL;
             const double dvL = df3 * bL * f1;
-            d[2] += dvL / hLs;
-            dhLs += -dvL * rL / (hLs * hLs);
+            d[2] += dvL * ihL;
+            dhLs += -dvL * rL * (ihL * ihL);
         } else {
             df1 += df3 * bR * vR;
             const double dvR = df3 * bR * f1;
-            d[5] += dvR / hRs;
-            dhRs += -dvR * rR / (hRs * hRs);
+            d[5] += dvR * ihR;
+            dhRs += -dvR * rR * (ihR * ihR);
         }
         // f2 = A2 / den
-        const double dA2 = df2 / den;
-        dden += -df2 * A2 / (den * den);
+        const double dA2 = df2 * iden;
+        dden += -df2 * A2 * iden2;
         dL2 += dA2 * (FL + sl) + dA2 * L1 * P;
         const double dFL = dA2 * L2;
         dsl += dA2 * L2;
@@ -228,8 +262,8 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
         dL1 += -dsl * s2;
         dL2 += dsr * s2;
         const double ds2 = -dsl * L1 + dsr * L2;
-        const double ds2a = ds2 * kG / den;
-        dden += -ds2 * s2b / (den * den);
+        const double ds2a = ds2 * kG * iden;
+        dden += -ds2 * s2b * iden2;
         d[11] += ds2a * 0.5 * hR * hR;
         dhR += ds2a * phiR * hR;
         d[10] += -ds2a * 0.5 * hL * hL;
@@ -241,8 +275,8 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
         dhL += dfact * 0.5 * phiLR;
         dhR += dfact * 0.5 * phiLR;
         // f1 = phiLR A1 / den
-        const double dN1 = df1 / den;
-        dden += -df1 * N1 / (den * den);
+        const double dN1 = df1 * iden;
+        dden += -df1 * N1 * iden2;
         dphiLR += dN1 * A1;
         const double dA1 = dN1 * phiLR;
         dL2 += dA1 * (qL + L1 * (zR - zL));
@@ -268,10 +302,10 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
         if (s2s == 1) d[8] += dL2;
         if (s2s == 2) d[9] += dL2;
         // velocities
-        d[1] += duL / hLs;
-        dhLs += -duL * qL / (hLs * hLs);
-        d[4] += duR / hRs;
-        dhRs += -duR * qR / (hRs * hRs);
+        d[1] += duL * ihL;
+        dhLs += -duL * qL * (ihL * ihL);
+        d[4] += duR * ihR;
+        dhRs += -duR * qR * (ihR * ihR);
         if (hL > kSmall) dhL += dhLs;
         if (hR > kSmall) dhR += dhRs;
     } else if (reg == R_RHS) {
@@ -494,7 +528,8 @@ SPD void fric_vjp(double h_min, double h, double qx, double qy, double ca, doubl
         return;
     }
     const double hs = h + kSmall;
-    const double u = qx / hs, v = qy / hs;
+    const double ihs = 1.0 / hs;
+    const double u = qx * ihs, v = qy * ihs;
     const double ul = u * ca + v * sa;
     const double vl = -u * sa + v * ca;
     const double un = sqrt(u * u + v * v + 1e-16);
@@ -503,27 +538,31 @@ SPD void fric_vjp(double h_min, double h, double qx, double qy, double ca, doubl
     const double cb = cbrt(h + 1e-16);
     const double h43 = h * cb;
     const double D1 = K1 * K1 * h43 + 1e-10, D2 = K2 * K2 * h43 + 1e-10;
-    const double d1 = 1.0 + (dt * kG * un) / D1;
-    const double d2 = 1.0 + (dt * kG * un) / D2;
-    const double ul2 = ul / d1, vl2 = vl / d2;
+    const double iD1 = 1.0 / D1, iD2 = 1.0 / D2;
+    const double gdt = dt * kG;
+    const double d1 = 1.0 + gdt * un * iD1;
+    const double d2 = 1.0 + gdt * un * iD2;
+    const double id1 = 1.0 / d1, id2 = 1.0 / d2;
+    const double ul2 = ul * id1, vl2 = vl * id2;
     const double T1 = ul2 * ca - vl2 * sa, T2 = ul2 * sa + vl2 * ca;
     dh += dqx * T1 + dqy * T2;
     const double dT1 = dqx * h, dT2 = dqy * h;
     const double dul2 = dT1 * ca + dT2 * sa;
     const double dvl2 = -dT1 * sa + dT2 * ca;
-    const double dul = dul2 / d1, dvl = dvl2 / d2;
-    const double dd1 = -dul2 * ul / (d1 * d1), dd2 = -dvl2 * vl / (d2 * d2);
-    const double dun = dd1 * dt * kG / D1 + dd2 * dt * kG / D2;
-    const double dD1 = -dd1 * dt * kG * un / (D1 * D1), dD2 = -dd2 * dt * kG * un / (D2 * D2);
+    const double dul = dul2 * id1, dvl = dvl2 * id2;
+    const double dd1 = -dul2 * ul * (id1 * id1), dd2 = -dvl2 * vl * (id2 * id2);
+    const double dun = (dd1 * iD1 + dd2 * iD2) * gdt;
+    const double dD1 = -dd1 * gdt * un * (iD1 * iD1), dD2 = -dd2 * gdt * un * (iD2 * iD2);
     if (K1in > 1e-10) dK1 += dD1 * 2.0 * K1 * h43;
     if (K2in > 1e-10) dK2 += dD2 * 2.0 * K2 * h43;
     const double dh43 = dD1 * K1 * K1 + dD2 * K2 * K2;
     dh += dh43 * (cb + h * cb / (3.0 * (h + 1e-16)));
-    double du = dun * u / un + dul * ca - dvl * sa;
-    double dv = dun * v / un + dul * sa + dvl * ca;
-    dqx = du / hs;
-    dqy = dv / hs;
-    dh += -du * qx / (hs * hs) - dv * qy / (hs * hs);
+    const double iun = 1.0 / un;
+    double du = dun * u * iun + dul * ca - dvl * sa;
+    double dv = dun * v * iun + dul * sa + dvl * ca;
+    dqx = du * ihs;
+    dqy = dv * ihs;
+    dh += -(du * qx + dv * qy) * (ihs * ihs);
 }
 
 // ---------------------------------------------------------------------------

@@ -472,11 +472,19 @@ __global__ __launch_bounds__(kBlock) void k_sq_part(const double *__restrict__ x
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_sq_sum(const double *part, int nparts, double *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one block of kBlock threads: thread t sums blocks t, t + kBlock, ... in
+// order, then a fixed tree (deterministic)
+__global__ __launch_bounds__(kBlock) void k_sq_sum(const double *part, int nparts, double *out) {
+    __shared__ double red[kBlock];
     double s = 0.0;
-    for (int b = 0; b < nparts; ++b) s += part[b];
-    *out = s;
+    for (int b = (int)threadIdx.x; b < nparts; b += kBlock) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 __global__ void k_cost_frame_sum(const double *part, int nparts, double norm, double *J) {
@@ -728,14 +736,15 @@ __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls c
     const double hL = U[L], qxL = U[nc + L], qyL = U[2 * nc + L];
     const double hR = U[R], qxR = U[nc + R], qyR = U[2 * nc + R];
     const double bL = m.beta[L], bR = m.beta[R];
-    const spe::Side sL = spe::lr_fwd(hL, qxL, qyL, m.zb[L], bL, nx, ny, o.h_min);
-    const spe::Side sR = spe::lr_fwd(hR, qxR, qyR, m.zb[R], bR, nx, ny, o.h_min);
+    spe::LrAux xL, xR;
+    const spe::Side sL = spe::lr_fwd_x(hL, qxL, qyL, m.zb[L], bL, nx, ny, o.h_min, xL);
+    const spe::Side sR = spe::lr_fwd_x(hR, qxR, qyR, m.zb[R], bR, nx, ny, o.h_min, xR);
     double d[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     spe::hllc_vjp(o.h_min, o.phi_min, bL, bR, phiL, phiR, m.zb[L], m.zb[R], sL.z, sL.q, sL.r, sR.z,
                   sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, dF, d);
     double dhL = 0, dqxL = 0, dqyL = 0, dhR = 0, dqxR = 0, dqyR = 0;
-    spe::lr_vjp(hL, qxL, qyL, bL, nx, ny, o.h_min, d[0], d[1], d[2], d[6], d[8], dhL, dqxL, dqyL);
-    spe::lr_vjp(hR, qxR, qyR, bR, nx, ny, o.h_min, d[3], d[4], d[5], d[7], d[9], dhR, dqxR, dqyR);
+    spe::lr_vjp_x(xL, bL, nx, ny, d[0], d[1], d[2], d[6], d[8], dhL, dqxL, dqyL);
+    spe::lr_vjp_x(xR, bR, nx, ny, d[3], d[4], d[5], d[7], d[9], dhR, dqxR, dqyR);
     double *so = side_out + 8 * (size_t)e;
     so[0] = dhL;
     so[1] = dqxL;
@@ -1303,7 +1312,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         gc(false, 1, n0, seed_args(c, a, -1, nullptr), nullptr, 0.0);
         if (!guard) continue;
         KLAUNCH(a, k_sq_part, dim3(nbk), blk, 0, c->stream, a->d_lam, (int)nc, a->d_nrm);
-        KLAUNCH(a, k_sq_sum, dim3(1), dim3(64), 0, c->stream, a->d_nrm, nbk, a->d_nrm + nbk);
+        KLAUNCH(a, k_sq_sum, dim3(1), blk, 0, c->stream, a->d_nrm, nbk, a->d_nrm + nbk);
         double nrm2 = 0.0;
         HIPCHK(c, hipMemcpyAsync(&nrm2, a->d_nrm + nbk, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
