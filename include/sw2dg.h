@@ -159,6 +159,21 @@ int sw2dg_assim_gradient(sw2dg_ctx *ctx, const double *phi, const double *K1, co
  * series a twin experiment observes.  No reference counterpart (the
  * reference reads observations from files, swSpE_ad.cpp:47-154). */
 int sw2dg_assim_gauge_series(sw2dg_ctx *ctx, double *out);
+/* Change of variable of the porosity control (phi_correct / dphi_dm /
+ * update_phi_norm_bounds, swSpE.h:59-107, swSpE_core.cpp:229-266): with mode
+ * != 0 the `phi` argument of record / cost / gradient is the control m and
+ * phi = correct(m): 1 clamp to [0, 1.05], 2 logistic 1 / (1 + exp(-lambda m)),
+ * 3 normalise (m - lo) / (hi - lo) with lo / hi = min / max of m over
+ * free_cells (all cells if n_free = 0), taken at each dry run and frozen for
+ * later costs; dJ_dphi is then dJ/dm.  mask_wall != 0 zeroes dJ/dm where
+ * phi <= phi_min (mask_wall_gradient, swSpE_checkpoint.cpp:1206-1215).
+ * Mode 0 (default): phi is used as given. */
+int sw2dg_assim_set_phi_transform(sw2dg_ctx *ctx, int32_t mode, double lambda,
+                                  const int64_t *free_cells, int64_t n_free, int32_t mask_wall);
+/* The same change of variable on host arrays (no device needed): phi and
+ * dphi/dm of n controls for the given bounds; either output may be NULL. */
+int sw2dg_phi_transform_eval(int32_t mode, double lambda, double lo, double hi, int64_t n,
+                             const double *m, double *phi, double *dphi);
 /* Adjoint explosion guard of the reverse sweep (swSpE_checkpoint.cpp:
  * 1035-1117): after each segment, ||adjoint of h|| of its start state is
  * compared with the first non-zero one; above `ratio` times it, the gradient

@@ -42,11 +42,29 @@ _DECL = {
     "sw2dg_assim_gauge_series": ([C.c_void_p, N._d], C.c_int),
     "sw2dg_assim_set_adjoint_cut": ([C.c_void_p, C.c_double], C.c_int),
     "sw2dg_assim_set_explosion_guard": ([C.c_void_p, C.c_double], C.c_int),
+    "sw2dg_assim_set_phi_transform": ([C.c_void_p, C.c_int32, C.c_double, N._i64, C.c_int64,
+                                       C.c_int32], C.c_int),
     "sw2dg_assim_truncation": ([C.c_void_p, N._i64, N._i64], C.c_int),
     "sw2dg_assim_profile": ([C.c_void_p, N._d, N._d, N._d, C.c_int32, N._d, N._i64,
                              C.POINTER(C.c_char_p), C.POINTER(C.c_int32)], C.c_int),
 }
-ASSIM_SYMBOLS = tuple(_DECL)
+ASSIM_SYMBOLS = tuple(_DECL) + ("sw2dg_phi_transform_eval",)
+PHI_MODES = {None: 0, "none": 0, "clamp": 1, "tanh": 2, "normalize": 3}
+
+
+def phi_transform_eval(mode, m, lam=1.0, lo=0.0, hi=1.0):
+    """phi_correct and dphi/dm of the library (host code, no device): (phi, dphi)."""
+    lib = N.gpu_lib()
+    f = lib.sw2dg_phi_transform_eval
+    f.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int64, N._d, N._d, N._d]
+    f.restype = C.c_int
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    phi, dphi = np.empty_like(m), np.empty_like(m)
+    rc = f(PHI_MODES[mode], float(lam), float(lo), float(hi), len(m), N.dptr(m), N.dptr(phi),
+           N.dptr(dphi))
+    if rc != 0:
+        raise N.SW2DError(rc, "sw2dg_phi_transform_eval failed")
+    return phi, dphi
 
 
 def _declare(lib):
@@ -61,7 +79,8 @@ class Assim:
 
     def __init__(self, solver, obs_t, gauge_cells=None, gauge_h=None, gauge_w=None, frames=None,
                  frame_w=None, normalize=False, stride=0, n_tiles=1, t_window=-1e300,
-                 max_steps=100000, adj_cut=0.0, explosion_ratio=1e8):
+                 max_steps=100000, adj_cut=0.0, explosion_ratio=1e8, phi_transform=None,
+                 free_cells=None, mask_wall=False):
         self.s = solver
         self.lib = solver.lib
         _declare(self.lib)
@@ -93,6 +112,15 @@ class Assim:
         self.s._check(self.lib.sw2dg_assim_set_adjoint_cut(solver.ctx, float(adj_cut)))
         self.s._check(self.lib.sw2dg_assim_set_explosion_guard(solver.ctx, float(explosion_ratio)))
         self.stride = int(stride) if stride else int(np.ceil(np.sqrt(self.max_steps)))
+        # phi_transform: None or (mode, lambda), mode "clamp" | "tanh" | "normalize";
+        # the phi passed to record / cost / gradient is then the control m
+        mode, lam = (None, 1.0) if phi_transform is None else (tuple(phi_transform) + (1.0,))[:2]
+        code = PHI_MODES[mode]
+        fc = np.ascontiguousarray([] if free_cells is None else free_cells, dtype=np.int64)
+        self._free = fc
+        self.s._check(self.lib.sw2dg_assim_set_phi_transform(
+            solver.ctx, code, float(lam), N.i64ptr(fc) if len(fc) else None, len(fc),
+            1 if mask_wall else 0))
         self.n_steps = 0
         self._tmp = []
 

@@ -20,6 +20,8 @@
 // All gathers run in fixed reference order: no atomics, deterministic.
 #include <hip/hip_runtime.h>
 
+#include <limits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -83,6 +85,15 @@ struct AssimState {
     double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
     double *d_gphi = nullptr, *d_gK1 = nullptr, *d_gK2 = nullptr, *d_gbce = nullptr,
            *d_gbc = nullptr;
+    // change of variable of the porosity control (phi_correct / dphi_dm,
+    // swSpE.h:59-107): 0 none, 1 clamp [0, 1.05], 2 logistic (lambda), 3
+    // normalise (m - lo) / (hi - lo) with lo / hi over the free cells, frozen
+    // at each dry run (update_phi_norm_bounds, swSpE_core.cpp:229-266)
+    int phi_mode = 0;
+    double phi_lambda = 1.0, norm_lo = 0.0, norm_scale = 1.0;
+    std::vector<int64_t> free_idx;
+    bool mask_wall = false;
+    std::vector<double> m_last;  // the control passed with the last dry run / cost (reference order)
     // explosion guard (swSpE_checkpoint.cpp:1035-1117): ratio (<= 0: off),
     // gradient snapshot at the last milestone, truncation of the last gradient
     double expl_ratio = 1e8;
@@ -770,8 +781,62 @@ int zero(sw2dg_ctx *c, double *p, size_t n) {
     return SW2DG_OK;
 }
 
-int upload_ctl(sw2dg_ctx *c, AssimState *a, const double *phi, const double *K1, const double *K2) {
+double phi_correct(int mode, double lam, double lo, double scale, double m) {
+    switch (mode) {
+        case 1: return (m < 0.0) ? 0.0 : ((m > 1.05) ? 1.05 : m);
+        case 2: return 1.0 / (1.0 + std::exp(-lam * m));
+        case 3: return (m - lo) * scale;
+        default: return m;
+    }
+}
+
+double dphi_dm(int mode, double lam, double scale, double m) {
+    switch (mode) {
+        case 1: return (m < 0.0 || m > 1.05) ? 0.0 : 1.0;
+        case 2: {
+            const double s = 1.0 / (1.0 + std::exp(-lam * m));
+            return lam * s * (1.0 - s);
+        }
+        case 3: return scale;
+        default: return 1.0;
+    }
+}
+
+// lo / hi of m over the free cells (all cells when none are listed); a
+// (nearly) uniform map disables the normalisation (phi = m)
+void update_norm_bounds(AssimState *a, const double *m, int64_t nc) {
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    auto see = [&](int64_t i) {
+        lo = std::min(lo, m[i]);
+        hi = std::max(hi, m[i]);
+    };
+    if (!a->free_idx.empty())
+        for (int64_t i : a->free_idx) see(i);
+    else
+        for (int64_t i = 0; i < nc; ++i) see(i);
+    if (hi - lo < 1e-12) {
+        a->norm_lo = 0.0;
+        a->norm_scale = 1.0;
+    } else {
+        a->norm_lo = lo;
+        a->norm_scale = 1.0 / (hi - lo);
+    }
+}
+
+int upload_ctl(sw2dg_ctx *c, AssimState *a, const double *phi, const double *K1, const double *K2,
+               bool new_bounds = false) {
     const int64_t nc = c->nc;
+    std::vector<double> phi_t;
+    if (phi && a->phi_mode != 0) {  // phi is the control m: apply phi_correct on the host
+        a->m_last.assign(phi, phi + nc);
+        if (a->phi_mode == 3 && new_bounds) update_norm_bounds(a, phi, nc);
+        phi_t.resize(nc);
+        for (int64_t i = 0; i < nc; ++i)
+            phi_t[i] = phi_correct(a->phi_mode, a->phi_lambda, a->norm_lo, a->norm_scale, phi[i]);
+        phi = phi_t.data();
+    } else {
+        a->m_last.clear();
+    }
     auto put = [&](const double *src, double *dst, const double *dflt) -> int {
         if (src) {
             std::vector<double> v = c->to_dev_cells(src);
@@ -953,6 +1018,35 @@ int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int
 
 extern "C" {
 
+int sw2dg_assim_set_phi_transform(sw2dg_ctx *c, int32_t mode, double lambda, const int64_t *free_cells,
+                                  int64_t n_free, int32_t mask_wall) {
+    if (!c || !c->assim || mode < 0 || mode > 3 || !(lambda > 0.0) || n_free < 0 ||
+        (n_free > 0 && !free_cells))
+        return SW2DG_ERR_ARG;
+    for (int64_t k = 0; k < n_free; ++k)
+        if (free_cells[k] < 0 || free_cells[k] >= c->nc) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    a->phi_mode = mode;
+    a->phi_lambda = lambda;
+    a->free_idx.assign(free_cells, free_cells + n_free);
+    a->mask_wall = mask_wall != 0;
+    a->norm_lo = 0.0;
+    a->norm_scale = 1.0;
+    return SW2DG_OK;
+}
+
+int sw2dg_phi_transform_eval(int32_t mode, double lambda, double lo, double hi, int64_t n,
+                             const double *m, double *phi, double *dphi) {
+    if (mode < 0 || mode > 3 || n < 0 || (n > 0 && !m)) return SW2DG_ERR_ARG;
+    const double scale = (hi - lo < 1e-12) ? 1.0 : 1.0 / (hi - lo);
+    const double l0 = (hi - lo < 1e-12) ? 0.0 : lo;
+    for (int64_t i = 0; i < n; ++i) {
+        if (phi) phi[i] = phi_correct(mode, lambda, l0, scale, m[i]);
+        if (dphi) dphi[i] = dphi_dm(mode, lambda, scale, m[i]);
+    }
+    return SW2DG_OK;
+}
+
 int sw2dg_assim_set_explosion_guard(sw2dg_ctx *c, double ratio) {
     if (!c || !c->assim || !(ratio == ratio)) return SW2DG_ERR_ARG;
     c->assim->expl_ratio = ratio;
@@ -1103,7 +1197,7 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
     if (!c || !c->assim) return SW2DG_ERR_ARG;
     AssimState *a = c->assim;
     HIPCHK(c, hipSetDevice(c->device));
-    int st = upload_ctl(c, a, phi, K1, K2);
+    int st = upload_ctl(c, a, phi, K1, K2, /*new_bounds=*/true);
     if (st) return st;
     const int64_t nc = c->nc;
     // initial state -> checkpoint 0 and the dry-run buffers
@@ -1356,6 +1450,22 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         return SW2DG_OK;
     };
     if ((st = pull(a->d_gphi, dJ_dphi))) return st;
+    if (dJ_dphi && !a->m_last.empty()) {  // dJ/dm = dJ/dphi dphi/dm
+        for (int64_t i = 0; i < nc; ++i) {
+            const double m = a->m_last[i];
+            dJ_dphi[i] *= dphi_dm(a->phi_mode, a->phi_lambda, a->norm_scale, m);
+            // optional wall mask (mask_wall_gradient, swSpE_checkpoint.cpp:1206-1215)
+            if (a->mask_wall &&
+                !(phi_correct(a->phi_mode, a->phi_lambda, a->norm_lo, a->norm_scale, m) > c->o.phi_min))
+                dJ_dphi[i] = 0.0;
+        }
+    } else if (dJ_dphi && a->mask_wall) {
+        std::vector<double> ph(nc);
+        HIPCHK(c, hipMemcpy(v.data(), a->d_phi, nc * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < nc; ++q) ph[c->perm[q]] = v[q];
+        for (int64_t i = 0; i < nc; ++i)
+            if (!(ph[i] > c->o.phi_min)) dJ_dphi[i] = 0.0;
+    }
     if ((st = pull(a->d_gK1, dJ_dK1))) return st;
     if ((st = pull(a->d_gK2, dJ_dK2))) return st;
     if (dJ_dbc && a->n_steps && c->n_series)
