@@ -74,6 +74,73 @@ SPD Side lr_fwd_x(double h, double qx, double qy, double zb, double b, double nx
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// MUSCL-EVR _E (swSpE_muscl.cpp:22-187) with the Kurganov-Petrova
+// desingularised velocity (swSpDesingVelocity.h:28-39)
+// ---------------------------------------------------------------------------
+constexpr double kHDesing = 5e-3;
+
+SPD double desing(double q, double h) {
+    const double h2 = h * h;
+    const double h4 = h2 * h2;
+    const double e4 = kHDesing * kHDesing * kHDesing * kHDesing;
+    const double den = sqrt(h4 + ((h4 < e4) ? e4 : h4));
+    return 1.4142135623730951 * h * q / den;
+}
+
+// boundary edge: first order, desingularised velocity
+SPD Side lr_bnd_muscl(double h, double qx, double qy, double zb, double b, double nx, double ny) {
+    Side s;
+    s.z = h + zb;
+    s.q = qx * nx + qy * ny;
+    s.r = -qx * ny + qy * nx;
+    const double u = desing(s.q, h);
+    const double c = sqrt(kG * h);
+    s.lm = sp::lambda_minus(b, c, u);
+    s.lp = sp::lambda_plus(b, c, u);
+    return s;
+}
+
+// internal edge, one side: characteristic-offset reconstruction of the free
+// surface with the cell's limited slope (sx, sy), dxm / dym = edge midpoint
+// minus cell centre, dt = the dt the reconstruction reads
+SPD Side recon_muscl(int side, double h, double qx, double qy, double zb, double b, double sx,
+                     double sy, double nx, double ny, double dxm, double dym, double dt) {
+    const double zc = h + zb;
+    const double q_local = qx * nx + qy * ny;
+    const double r_local = qy * nx - qx * ny;
+    const double c_local = sqrt(kG * ((kSmall < h) ? h : kSmall));
+    const double u_local = desing(q_local, h);
+    const double v_local = desing(r_local, h);
+    double l1 = sp::lambda_minus(b, c_local, u_local);
+    double l2 = sp::lambda_plus(b, c_local, u_local);
+    if (side == 0) {
+        l1 = (0.0 < l1) ? l1 : 0.0;
+        l2 = (0.0 < l2) ? l2 : 0.0;
+    } else {
+        l1 = (l1 < 0.0) ? l1 : 0.0;
+        l2 = (l2 < 0.0) ? l2 : 0.0;
+    }
+    const double dx1 = dxm - 0.5 * l1 * dt * nx;
+    const double dy1 = dym - 0.5 * l1 * dt * ny;
+    const double z1 = zc + dx1 * sx + dy1 * sy;
+    const double h1 = (0.0 < z1 - zb) ? (z1 - zb) : 0.0;
+    const double dx2 = dxm - 0.5 * l2 * dt * nx;
+    const double dy2 = dym - 0.5 * l2 * dt * ny;
+    const double z2 = zc + dx2 * sx + dy2 * sy;
+    const double h2 = (0.0 < z2 - zb) ? (z2 - zb) : 0.0;
+    const double hr = 0.5 * (h1 + h2);
+    Side s;
+    s.z = 0.5 * (h1 + h2) + zb;
+    s.q = 0.5 * (h1 + h2) * u_local;
+    s.r = 0.5 * (h1 + h2) * v_local;
+    const double u = desing(s.q, hr);
+    const double c = sqrt(kG * hr);
+    s.lm = sp::lambda_minus(b, c, u);
+    s.lp = sp::lambda_plus(b, c, u);
+    return s;
+}
+
 // adjoint of lr_fwd from its intermediates: (dz, dq, dr, dlm, dlp) ->
 // accumulate into (dh, dqx, dqy)
 SPD void lr_vjp_x(const LrAux &x, double b, double nx, double ny, double dz, double dq, double dr,

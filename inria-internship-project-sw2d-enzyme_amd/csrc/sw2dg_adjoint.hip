@@ -46,6 +46,8 @@ struct EClock {
     double t_max, dt_max, cfl;
 };
 
+constexpr bool kMusclAdjoint = false;
+
 struct Controls {
     const double *phi, *K1, *K2;
 };
@@ -66,6 +68,7 @@ struct AssimState {
     double t_window = -1e300;
     int64_t max_steps = 0, n_steps = 0;
     double *d_T = nullptr, *d_dt = nullptr, *d_bc = nullptr;
+    std::vector<double> h_dt;  // host copy of the recorded dt sequence (replay launches read it)
     int *d_trig = nullptr;
     EClock *d_ck = nullptr;
     bool recorded = false;
@@ -78,6 +81,7 @@ struct AssimState {
     double *d_Fseg = nullptr;  // stride x the same: fluxes of every step of the replayed segment
     double *d_Pseg = nullptr;  // stride x 6 nc: balanced state + internal increments per step
     double *d_sw = nullptr;    // 2 n_int
+    double *d_sxy = nullptr;   // 2 nc: limited free-surface slopes of the step (MUSCL-EVR)
     double *d_partial = nullptr;
     double *d_J = nullptr;     // [1 + n_obs]: total, then the cost of each observation
     double *d_cpart = nullptr; // frame-cost block partials
@@ -176,16 +180,78 @@ __device__ double interp_e(const double *tt, const double *vv, int n, double t) 
     return v1 + (v2 - v1) * (t - t1) / den;
 }
 
+// MUSCL-EVR: per cell, free-surface gradient over the cell's neighbours and
+// the reference's in-place limiter replayed over the cell's internal edges
+// in ascending order (computeMusclSlopes, swSpHypMusclEvr.cpp:11-115; a
+// cell's slope is only changed by its own edges, so the per-cell replay is
+// exact).  sxy = [dz/dx | dz/dy].
+__global__ __launch_bounds__(kBlock) void k_e_slopes(MeshDev m, const double *__restrict__ U,
+                                                     double *__restrict__ sxy, const double *T,
+                                                     int64_t n, double t_max) {
+    if (T && !(T[n] < t_max)) return;
+    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+    const int nc = m.n_cell;
+    if (i >= nc) return;
+    const double zi = U[i] + m.zb[i];
+    const double xi = m.cx[i], yi = m.cy[i];
+    const int b = m.int_ptr[i], e_end = m.int_ptr[i + 1];
+    double sx = 0.0, sy = 0.0;
+    if (e_end - b > 1) {
+        double wsum = 0.0;
+        for (int k = b; k < e_end; ++k) {
+            const int inc = m.int_inc[k];
+            const int e = inc >> 1;
+            const int j = (inc & 1) ? m.eL[e] : m.eR[e];
+            const double zj = U[j] + m.zb[j];
+            const double dx = m.cx[j] - xi;
+            const double dy = m.cy[j] - yi;
+            const double dist = sqrt(dx * dx + dy * dy);
+            const double w = 1.0 / dist;
+            sx += (zj - zi) * dx / dist / dist * w;
+            sy += (zj - zi) * dy / dist / dist * w;
+            wsum += w;
+        }
+        sx /= wsum;
+        sy /= wsum;
+    }
+    for (int k = b; k < e_end; ++k) {
+        const int e = m.int_inc[k] >> 1;
+        const int L = m.eL[e], R = m.eR[e];
+        const double zL = U[L] + m.zb[L];
+        const double zR = U[R] + m.zb[R];
+        const double fmx = sp::smax(zL, zR);
+        const double fmn = sp::smin(zL, zR);
+        const double fr = zi + (m.exm[e] - xi) * sx + (m.eym[e] - yi) * sy;
+        if (fr < fmn) {
+            const double sf = (fmn - zi) / (fr - zi);
+            sx *= sf;
+            sy *= sf;
+        }
+        if (fr > fmx) {
+            const double sf = (fmx - zi) / (fr - zi);
+            sx *= sf;
+            sy *= sf;
+        }
+    }
+    sxy[i] = sx;
+    sxy[nc + i] = sy;
+}
+
 // Edge fluxes of one _E step.  Threads [0, n_int): internal edges, Godunov _E
 // states + HLLC_E -> F[5e..5e+4] (+ CFL swept terms in the dry run).
 // Threads [n_int, n_int + n_bnd): boundary edges, hypBoundaryFlux_E ->
 // F[5 n_int + 3j ..] with the step's boundary value (bcv, or interpolated at
 // T[n] in the dry run, as build_checkpoint_records does).
+// MUSCL (template): MUSCL-EVR _E states from the step's slopes sxy, the
+// reconstruction reading dt_recon (*dtp when given: the dry run's previous
+// step, 0 before the first; else dtv: the replay's current step).
+template <bool MUSCL>
 __global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls cs, BcDev bc,
                                                     const double *__restrict__ U,
                                                     double *__restrict__ F, double *__restrict__ sw,
                                                     const double *T, int64_t n, double t_max,
-                                                    const double *bcv) {
+                                                    const double *bcv, const double *__restrict__ sxy,
+                                                    const double *dtp, double dtv) {
     if (T && !(T[n] < t_max)) return;
     const int e = xcd_block() * kBlock + (int)threadIdx.x;
     const int nc = m.n_cell;
@@ -194,8 +260,10 @@ __global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls 
         if (j >= m.n_bnd) return;
         const int i = m.bcell[j];
         const double h = U[i];
-        const spe::Side sd = spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j],
-                                         m.bny[j], o.h_min);
+        const spe::Side sd =
+            MUSCL ? spe::lr_bnd_muscl(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j])
+                  : spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j],
+                                o.h_min);
         const int ser = m.bser[j];
         const double v = bcv ? bcv[ser]
                              : interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n]);
@@ -211,8 +279,18 @@ __global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls 
     const int L = m.eL[e], R = m.eR[e];
     const double nx = m.enx[e], ny = m.eny[e];
     const double hL = U[L], hR = U[R];
-    const spe::Side sL = spe::lr_fwd(hL, U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], nx, ny, o.h_min);
-    const spe::Side sR = spe::lr_fwd(hR, U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], nx, ny, o.h_min);
+    spe::Side sL, sR;
+    if (MUSCL) {
+        const double dt = dtp ? *dtp : dtv;
+        const double xm = m.exm[e], ym = m.eym[e];
+        sL = spe::recon_muscl(0, hL, U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], sxy[L], sxy[nc + L],
+                              nx, ny, xm - m.cx[L], ym - m.cy[L], dt);
+        sR = spe::recon_muscl(1, hR, U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], sxy[R], sxy[nc + R],
+                              nx, ny, xm - m.cx[R], ym - m.cy[R], dt);
+    } else {
+        sL = spe::lr_fwd(hL, U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], nx, ny, o.h_min);
+        sR = spe::lr_fwd(hR, U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], nx, ny, o.h_min);
+    }
     double f[5];
     spe::hllc_fwd(o.h_min, o.phi_min, m.beta[L], m.beta[R], cs.phi[L], cs.phi[R], m.zb[L], m.zb[R],
                   sL.z, sL.q, sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
@@ -241,6 +319,7 @@ __device__ __forceinline__ double wave_min(double v) {
 
 // dry-run CFL (hypComputeTimeStep_E, swSpE_hyp.cpp:578-635) and the step
 // schedule of build_checkpoint_records (swSpE_checkpoint.cpp:224-366)
+template <bool MUSCL>
 __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const double *__restrict__ U,
                                                   const double *__restrict__ sw, const EClock *ck,
                                                   double *partial, const double *T, int64_t n) {
@@ -257,8 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const doubl
         if (h > o.h_min) {
             for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
                 const int j = m.bnd_inc[k];
-                const spe::Side s = spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i],
-                                                m.bnx[j], m.bny[j], o.h_min);
+                const spe::Side s =
+                    MUSCL ? spe::lr_bnd_muscl(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j])
+                          : spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j],
+                                        o.h_min);
                 const double lm = (fabs(s.lp) < fabs(s.lm)) ? fabs(s.lm) : fabs(s.lp);
                 swept += lm * m.blen[j];
             }
@@ -876,12 +957,28 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
     const dim3 blk(kBlock);
     const Controls cs = ctl(a);
     const int ns = std::max(c->n_series, 1);
-    KLAUNCH(a, k_e_edges, dim3(nblocks(c->ni + c->nb)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
-            Uin, F, dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max,
-            dry ? nullptr : a->d_bc + (size_t)n * ns);
+    const bool muscl = c->o.scheme == 1;
+    const dim3 ge(nblocks(c->ni + c->nb));
+    const double *bcv = dry ? nullptr : a->d_bc + (size_t)n * ns;
+    if (muscl) {
+        // the dry run reconstructs with the previous step's dt (0 before the
+        // first step), the replay with the step's own dt (see k_e_edges)
+        KLAUNCH(a, k_e_slopes, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, Uin, a->d_sxy,
+                dry ? a->d_T : nullptr, n, c->prm.t_max);
+        KLAUNCH(a, k_e_edges<true>, ge, blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin, F,
+                dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max, bcv, a->d_sxy,
+                (dry && n > 0) ? a->d_dt + (n - 1) : nullptr, dry ? 0.0 : a->h_dt[n]);
+    } else {
+        KLAUNCH(a, k_e_edges<false>, ge, blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin, F,
+                dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max, bcv, nullptr, nullptr, 0.0);
+    }
     if (dry) {
-        KLAUNCH(a, k_e_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
-                a->d_ck, a->d_partial, a->d_T, n);
+        if (muscl)
+            KLAUNCH(a, k_e_cfl<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
+                    a->d_ck, a->d_partial, a->d_T, n);
+        else
+            KLAUNCH(a, k_e_cfl<false>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
+                    a->d_ck, a->d_partial, a->d_T, n);
         KLAUNCH(a, k_e_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
                 a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
     }
@@ -992,6 +1089,7 @@ int fetch_record(sw2dg_ctx *c, AssimState *a, std::vector<double> &T, std::vecto
         HIPCHK(c, hipMemcpy(trig.data(), a->d_trig, trig.size() * sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(dt.data(), a->d_dt, dt.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
+    a->h_dt = dt;
     return SW2DG_OK;
 }
 
@@ -1071,8 +1169,6 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
                       double t_window_start, int64_t max_steps) {
     if (!c || !obs || max_steps <= 0 || n_tiles < 1 || stride < 0) return SW2DG_ERR_ARG;
     if (!c->state_set) return fail(c, SW2DG_ERR_ARG, "sw2dg_set_state must be called first");
-    if (c->o.scheme != SW2DG_GODUNOV)
-        return fail(c, SW2DG_ERR_ARG, "the adjoint covers the Godunov reconstruction");
     if (obs->n_obs < 0 || obs->n_gauges < 0 || (obs->n_obs > 0 && !obs->t)) return SW2DG_ERR_ARG;
     if (obs->n_gauges > 0 && (!obs->gauge_cell || !obs->gauge_h)) return SW2DG_ERR_ARG;
     for (int g = 0; g < obs->n_gauges; ++g)
@@ -1157,6 +1253,7 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_Fseg = a->alloc<double>((size_t)a->stride * flux_len(c));
     a->d_Pseg = a->alloc<double>((size_t)a->stride * 6 * (size_t)nc);
     a->d_sw = a->alloc<double>(2 * (size_t)std::max<int64_t>(ni, 1));
+    a->d_sxy = a->alloc<double>(c->o.scheme == 1 ? 2 * (size_t)nc : 1);
     a->d_partial = a->alloc<double>(nblocks(nc));
     a->d_cpart = a->alloc<double>(nblocks(nc));
     a->d_J = a->alloc<double>(1 + (size_t)std::max(1, obs->n_obs));
@@ -1313,6 +1410,9 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
                          double *dJ_dbc, int64_t *n_steps) {
     if (n_steps) *n_steps = 0;
     if (!c || !c->assim) return SW2DG_ERR_ARG;
+    if (c->o.scheme == 1 && !kMusclAdjoint) {
+        return fail(c, SW2DG_ERR_ARG, "the MUSCL-EVR adjoint is not built: the gradient needs num_scheme godunov");
+    }
     AssimState *a = c->assim;
     HIPCHK(c, hipSetDevice(c->device));
     int64_t nst = 0;

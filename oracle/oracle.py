@@ -114,6 +114,9 @@ def _e_decl():
     L.orc_e_record.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int32, _d, C.c_int64,
                                _d, _d, _i32]
     L.orc_e_record.restype = C.c_int64
+    L.orc_e_record_states.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int32, _d,
+                                      C.c_int64, _d, _d, _i32, _d]
+    L.orc_e_record_states.restype = C.c_int64
     L.orc_e_replay.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d, _d, _i32,
                                C.c_int32, C.c_int32, _i64, _d, _d, C.c_int32, _d, _d, _d]
     L.orc_e_replay.restype = C.c_double
@@ -152,7 +155,7 @@ class OracleE:
         self._keep.append(v)
         return v
 
-    def record(self, obs_t, phi=None, K1=None, K2=None, max_steps=100000):
+    def record(self, obs_t, phi=None, K1=None, K2=None, max_steps=100000, states=False):
         a = self.case.a
         phi = self._ctl(phi, a["phi"])
         K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
@@ -162,11 +165,16 @@ class OracleE:
         bc = np.empty(max_steps * self.ns)
         trig = np.empty(max_steps, dtype=np.int32)
         h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
-        n = _e_decl().orc_e_record(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
-                                   _dp(qy0), len(obs_t), _dp(obs_t), max_steps, _dp(dt), _dp(bc),
-                                   trig.ctypes.data_as(_i32))
-        return {"dt": dt[:n].copy(), "bc": bc[: n * self.ns].reshape(n, self.ns).copy(),
-                "trig": trig[:n].copy()}
+        S = np.empty((max_steps + 1) * 3 * self.nc) if states else None
+        n = _e_decl().orc_e_record_states(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
+                                          _dp(qx0), _dp(qy0), len(obs_t), _dp(obs_t), max_steps,
+                                          _dp(dt), _dp(bc), trig.ctypes.data_as(_i32),
+                                          _dp(S) if states else None)
+        rec = {"dt": dt[:n].copy(), "bc": bc[: n * self.ns].reshape(n, self.ns).copy(),
+               "trig": trig[:n].copy()}
+        if states:  # the dry run's states [step][h | qx | qy]
+            rec["states"] = S[: (n + 1) * 3 * self.nc].reshape(n + 1, 3, self.nc).copy()
+        return rec
 
     def replay(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=None, normalize=False,
                n_obs=None, bc=None):

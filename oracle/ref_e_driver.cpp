@@ -32,7 +32,8 @@ struct Driver : public swSpE {
         phi_ptr = phi.data();
         computeEdgeFactors();
         t = t_0;
-        dt = dt_max;
+        dt = 0.0;  // the solver's dt member before any step (swSp.h:155); the MUSCL-EVR
+                   // reconstruction of the first step reads it
         size_t oi = 0;
         size_t n = 0;
         while (t < t_max) {

@@ -36,6 +36,12 @@ typedef struct {
     double *swept;
     double *fcos, *fsin, *hcos, *hsin;
     const double *phi, *K1, *K2;
+    /* MUSCL-EVR (_E): free surface, limited slopes, neighbour lists and the
+     * dt its reconstruction reads (the dry run's previous step, the replay's
+     * current step) */
+    double *zc, *dzdx, *dzdy;
+    int64_t *nb_ptr, *nb_idx;
+    double dt_recon;
 } esolver;
 
 static double lam_m(double b, double c, double u) { return b * u - sqrt((b - 1.0) * b * u * u + c * c); }
@@ -110,6 +116,132 @@ static void lr(esolver *s) {
         if (m < (v = fabs(s->lpi[2 * e + 1]))) m = v;
         s->lmaxi[e] = m;
     }
+}
+
+/* Kurganov-Petrova desingularised velocity -- assim/swSpDesingVelocity.h:28-39 */
+#define H_DESING 5e-3
+static double desing(double q, double h, double he) {
+    double h2 = h * h;
+    double h4 = h2 * h2;
+    double e4 = he * he * he * he;
+    double den = sqrt(h4 + ((h4 > e4) ? h4 : e4));
+    return 1.4142135623730951 * h * q / den;
+}
+
+/* computeMusclSlopes on the free surface -- swSpHypMusclEvr.cpp:11-115 */
+static void e_slopes(esolver *s) {
+    const orc_case *c = s->c;
+    for (int64_t i = 0; i < c->n_cell; i++) {
+        s->dzdx[i] = 0.0;
+        s->dzdy[i] = 0.0;
+        double wsum = 0.0;
+        if (s->nb_ptr[i + 1] - s->nb_ptr[i] > 1) {
+            for (int64_t k = s->nb_ptr[i]; k < s->nb_ptr[i + 1]; k++) {
+                int64_t j = s->nb_idx[k];
+                double dx = c->cell_x[j] - c->cell_x[i];
+                double dy = c->cell_y[j] - c->cell_y[i];
+                double dist = sqrt(dx * dx + dy * dy);
+                double weight = 1.0 / dist;
+                s->dzdx[i] += (s->zc[j] - s->zc[i]) * dx / dist / dist * weight;
+                s->dzdy[i] += (s->zc[j] - s->zc[i]) * dy / dist / dist * weight;
+                wsum += weight;
+            }
+            s->dzdx[i] /= wsum;
+            s->dzdy[i] /= wsum;
+        }
+    }
+    for (int64_t e = 0; e < c->n_int; e++) {
+        int64_t L = c->int_L[e], R = c->int_R[e];
+        double fmx = (s->zc[L] < s->zc[R]) ? s->zc[R] : s->zc[L];
+        double fmn = (s->zc[R] < s->zc[L]) ? s->zc[R] : s->zc[L];
+        for (int side = 0; side < 2; side++) {
+            int64_t k = side ? R : L;
+            double fr = s->zc[k] + (c->int_xm[e] - c->cell_x[k]) * s->dzdx[k] +
+                        (c->int_ym[e] - c->cell_y[k]) * s->dzdy[k];
+            if (fr < fmn) {
+                double sf = (fmn - s->zc[k]) / (fr - s->zc[k]);
+                s->dzdx[k] *= sf;
+                s->dzdy[k] *= sf;
+            }
+            if (fr > fmx) {
+                double sf = (fmx - s->zc[k]) / (fr - s->zc[k]);
+                s->dzdx[k] *= sf;
+                s->dzdy[k] *= sf;
+            }
+        }
+    }
+}
+
+/* hypMusclEvrComputeLRStates_E -- assim/swSpE_muscl.cpp:22-187 */
+static void lr_muscl(esolver *s) {
+    const orc_case *c = s->c;
+    const double dt = s->dt_recon;
+    for (int64_t i = 0; i < c->n_cell; i++) s->zc[i] = s->h[i] + c->cell_zb[i];
+    e_slopes(s);
+    for (int64_t e = 0; e < c->n_bnd; e++) {
+        double nx = c->bnd_nx[e], ny = c->bnd_ny[e];
+        int64_t id = c->bnd_cell[e];
+        double b = c->beta[id], q = s->qx[id], r = s->qy[id], hR = s->h[id];
+        s->zrb[e] = s->h[id] + c->cell_zb[id];
+        s->qrb[e] = q * nx + r * ny;
+        s->rrb[e] = -q * ny + r * nx;
+        double uR = desing(s->qrb[e], hR, H_DESING);
+        double cR = sqrt(G * hR);
+        double lm = lam_m(b, cR, uR), lp = lam_p(b, cR, uR);
+        s->lpb[e] = lp;
+        s->lmaxb[e] = (fabs(lm) < fabs(lp)) ? fabs(lp) : fabs(lm);
+    }
+    for (int64_t e = 0; e < c->n_int; e++) {
+        double nx = c->int_nx[e], ny = c->int_ny[e];
+        for (int sd = 0; sd < 2; sd++) {
+            int64_t id = sd ? c->int_R[e] : c->int_L[e];
+            double b = c->beta[id], zb = c->cell_zb[id];
+            double h_local = s->h[id];
+            double q_local = s->qx[id] * nx + s->qy[id] * ny;
+            double r_local = s->qy[id] * nx - s->qx[id] * ny;
+            double c_local = sqrt(G * ((SMALL < h_local) ? h_local : SMALL));
+            double u_local = desing(q_local, h_local, H_DESING);
+            double v_local = desing(r_local, h_local, H_DESING);
+            double l1 = lam_m(b, c_local, u_local), l2 = lam_p(b, c_local, u_local);
+            if (sd == 0) {
+                l1 = (0.0 < l1) ? l1 : 0.0;
+                l2 = (0.0 < l2) ? l2 : 0.0;
+            } else {
+                l1 = (l1 < 0.0) ? l1 : 0.0;
+                l2 = (l2 < 0.0) ? l2 : 0.0;
+            }
+            double dx1 = c->int_xm[e] - c->cell_x[id] - 0.5 * l1 * dt * nx;
+            double dy1 = c->int_ym[e] - c->cell_y[id] - 0.5 * l1 * dt * ny;
+            double z1 = s->zc[id] + dx1 * s->dzdx[id] + dy1 * s->dzdy[id];
+            double h1 = (0.0 < z1 - zb) ? (z1 - zb) : 0.0;
+            double dx2 = c->int_xm[e] - c->cell_x[id] - 0.5 * l2 * dt * nx;
+            double dy2 = c->int_ym[e] - c->cell_y[id] - 0.5 * l2 * dt * ny;
+            double z2 = s->zc[id] + dx2 * s->dzdx[id] + dy2 * s->dzdy[id];
+            double h2 = (0.0 < z2 - zb) ? (z2 - zb) : 0.0;
+            double hr = 0.5 * (h1 + h2);
+            s->zri[2 * e + sd] = 0.5 * (h1 + h2) + zb;
+            s->qri[2 * e + sd] = 0.5 * (h1 + h2) * u_local;
+            s->rri[2 * e + sd] = 0.5 * (h1 + h2) * v_local;
+            double uu = desing(s->qri[2 * e + sd], hr, H_DESING);
+            double cc = sqrt(G * hr);
+            s->lmi[2 * e + sd] = lam_m(b, cc, uu);
+            s->lpi[2 * e + sd] = lam_p(b, cc, uu);
+        }
+        double m1 = fabs(s->lmi[2 * e]), v = fabs(s->lpi[2 * e]);
+        if (m1 < v) m1 = v;
+        double m2 = fabs(s->lmi[2 * e + 1]);
+        v = fabs(s->lpi[2 * e + 1]);
+        if (m2 < v) m2 = v;
+        s->lmaxi[e] = (m1 < m2) ? m2 : m1;
+    }
+}
+
+/* hypComputeLRStates_E -- swSpE_hyp.cpp:487-498: Godunov or MUSCL-EVR */
+static void lr_any(esolver *s) {
+    if (s->c->scheme == ORC_MUSCL_EVR)
+        lr_muscl(s);
+    else
+        lr(s);
 }
 
 /* hypComputeTimeStep_E -- swSpE_hyp.cpp:578-635 (without the t_max clamp and
@@ -356,7 +488,8 @@ static void sources(esolver *s, double dt) {
 /* one _E step with a given dt and per-series BC values (ckpt_segment_E body) */
 static void e_step(esolver *s, double dt, const double *bc_step) {
     const orc_case *c = s->c;
-    lr(s);
+    s->dt_recon = dt; /* ckpt_segment_E sets solver->dt first (swSpE_checkpoint.cpp:658-659) */
+    lr_any(s);
     fluxes(s, bc_step);
     balance(s, dt);
     for (int64_t n = 0; n < c->n_cell; ++n) {
@@ -386,7 +519,26 @@ static void e_init(esolver *s, const orc_case *c, const double *phi, const doubl
     A(s->lmaxi, ni);
     A(s->F1b, nb); A(s->F2b, nb); A(s->F3b, nb);
     A(s->F1i, ni); A(s->F2i, ni); A(s->F3i, ni); A(s->sLi, ni); A(s->sRi, ni);
+    A(s->zc, nc); A(s->dzdx, nc); A(s->dzdy, nc);
 #undef A
+    /* neighbour lists in ascending internal-edge order -- swParent.cpp:476-486 */
+    s->nb_ptr = (int64_t *)calloc((size_t)nc + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < c->n_int; e++) {
+        s->nb_ptr[c->int_L[e] + 1]++;
+        s->nb_ptr[c->int_R[e] + 1]++;
+    }
+    for (int64_t i = 0; i < nc; i++) s->nb_ptr[i + 1] += s->nb_ptr[i];
+    s->nb_idx = (int64_t *)calloc((size_t)(2 * ni + 1), sizeof(int64_t));
+    {
+        int64_t *fill = (int64_t *)calloc((size_t)nc, sizeof(int64_t));
+        for (int64_t e = 0; e < c->n_int; e++) {
+            int64_t L = c->int_L[e], R = c->int_R[e];
+            s->nb_idx[s->nb_ptr[L] + fill[L]++] = R;
+            s->nb_idx[s->nb_ptr[R] + fill[R]++] = L;
+        }
+        free(fill);
+    }
+    s->dt_recon = 0.0; /* the solver's dt member before any step (swSp.h:155) */
     memcpy(s->h, h0, sizeof(double) * (size_t)nc);
     memcpy(s->qx, qx0, sizeof(double) * (size_t)nc);
     memcpy(s->qy, qy0, sizeof(double) * (size_t)nc);
@@ -405,8 +557,10 @@ static void e_free(esolver *s) {
     double *p[] = {s->h, s->qx, s->qy, s->swept, s->fcos, s->fsin, s->hcos, s->hsin, s->fac_b,
                    s->fac_L, s->fac_R, s->zrb, s->qrb, s->rrb, s->lpb, s->lmaxb, s->zri, s->qri,
                    s->rri, s->lmi, s->lpi, s->lmaxi, s->F1b, s->F2b, s->F3b, s->F1i, s->F2i,
-                   s->F3i, s->sLi, s->sRi};
+                   s->F3i, s->sLi, s->sRi, s->zc, s->dzdx, s->dzdy};
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
+    free(s->nb_ptr);
+    free(s->nb_idx);
 }
 
 /* Dry run (build_checkpoint_records, swSpE_checkpoint.cpp:210-401): steps from
@@ -414,12 +568,35 @@ static void e_free(esolver *s) {
  * t_max and floored at 1e-6; records dt, the per-series BC values at the step
  * start and the observation index triggered by the step (-1 if none).
  * Returns the number of steps (<= max_steps). */
+int64_t orc_e_record_states(const orc_case *c, const double *phi, const double *K1,
+                            const double *K2, const double *h0, const double *qx0,
+                            const double *qy0, int32_t n_obs, const double *obs_t,
+                            int64_t max_steps, double *dt_seq, double *bc_seq, int32_t *obs_trig,
+                            double *states);
+
 int64_t orc_e_record(const orc_case *c, const double *phi, const double *K1, const double *K2,
                      const double *h0, const double *qx0, const double *qy0, int32_t n_obs,
                      const double *obs_t, int64_t max_steps, double *dt_seq, double *bc_seq,
                      int32_t *obs_trig) {
+    return orc_e_record_states(c, phi, K1, K2, h0, qx0, qy0, n_obs, obs_t, max_steps, dt_seq,
+                               bc_seq, obs_trig, NULL);
+}
+
+/* As orc_e_record; states (optional, (max_steps + 1) x 3 n_cell): the dry
+ * run's state [h | qx | qy] after every step, the initial state first. */
+int64_t orc_e_record_states(const orc_case *c, const double *phi, const double *K1,
+                            const double *K2, const double *h0, const double *qx0,
+                            const double *qy0, int32_t n_obs, const double *obs_t,
+                            int64_t max_steps, double *dt_seq, double *bc_seq, int32_t *obs_trig,
+                            double *states) {
     esolver s;
     e_init(&s, c, phi, K1, K2, h0, qx0, qy0);
+    const int64_t ncs = c->n_cell;
+    if (states) {
+        memcpy(states, h0, sizeof(double) * ncs);
+        memcpy(states + ncs, qx0, sizeof(double) * ncs);
+        memcpy(states + 2 * ncs, qy0, sizeof(double) * ncs);
+    }
     double t = c->t0;
     int32_t oi = 0;
     int64_t n = 0;
@@ -432,13 +609,17 @@ int64_t orc_e_record(const orc_case *c, const double *phi, const double *K1, con
             double to = obs_t[oi] - t;
             if (to - 10e-6 > 0.0 && to < target) target = to;
         }
-        lr(&s);
+        /* the reconstruction reads the solver's dt, still the previous step's
+         * here: build_checkpoint_records sets it after the CFL step
+         * (swSpE_checkpoint.cpp:244-251) */
+        lr_any(&s);
         double dtc = cfl_dt(&s);
         if ((t + dtc) > c->t_max) dtc = c->t_max - t;
         if (dtc < 1e-6) dtc = 1e-6;
         double dt = (dtc < target) ? dtc : target;
         if ((t + dt) > c->t_max) dt = c->t_max - t;
         if (dt < 1e-6) dt = 1e-6;
+        s.dt_recon = dt;
         fluxes(&s, bc);
         balance(&s, dt);
         for (int64_t i = 0; i < c->n_cell; ++i)
@@ -452,6 +633,12 @@ int64_t orc_e_record(const orc_case *c, const double *phi, const double *K1, con
         for (int k = 0; k < c->n_series; k++) bc_seq[n * c->n_series + k] = bc[k];
         obs_trig[n] = trig;
         n++;
+        if (states) {
+            double *St = states + 3 * ncs * n;
+            memcpy(St, s.h, sizeof(double) * ncs);
+            memcpy(St + ncs, s.qx, sizeof(double) * ncs);
+            memcpy(St + 2 * ncs, s.qy, sizeof(double) * ncs);
+        }
     }
     free(bc);
     e_free(&s);

@@ -83,6 +83,22 @@ def e_golden_cases():
         "e_mixed_godunov_manning_nohl": ("mixed_case", dict(scheme="godunov", friction="manning",
                                                             t_max=20.0, head_loss=False),
                                          [5.0, 10.0, 15.0, 20.0]),
+        "e_scale_muscl_strickler": ("scale_case", dict(px=30, py=30, L=300.0, scheme="muscl_evr",
+                                                       t_max=120.0, friction="strickler",
+                                                       manning=30.0), [30.0, 60.0, 90.0, 120.0]),
+        "e_mixed_muscl_strickler": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
+                                                       t_max=20.0), [5.0, 10.0, 15.0, 20.0]),
+        "e_scale_muscl_nofriction": ("scale_case", dict(px=30, py=30, L=300.0, scheme="muscl_evr",
+                                                        t_max=120.0, friction="none"),
+                                     [30.0, 60.0, 90.0, 120.0]),
+        # closed boxes, no friction: no transcendental function on the path
+        # (walls give no critical-depth inflow), so the GPU can match exactly
+        "e_dambreak_muscl": ("dambreak_case", dict(nx=40, ny=20, Lx=80.0, Ly=40.0, x_dam=40.0,
+                                                   scheme="muscl_evr", friction="none", t_max=10.0),
+                             [2.5, 5.0, 7.5, 10.0]),
+        "e_drybreak_muscl": ("dambreak_case", dict(nx=40, ny=20, Lx=80.0, Ly=40.0, x_dam=30.0,
+                                                   h_right=0.0, scheme="muscl_evr", friction="none",
+                                                   t_max=10.0), [2.5, 5.0, 7.5, 10.0]),
     }
 
 
@@ -103,10 +119,12 @@ def run_ref_e(folder, obs):
     return np.array(dt), np.array(trig, dtype=np.int32), S[:, 0], S[:, 1], S[:, 2]
 
 
-def main_e(sw):
+def main_e(sw, only=None):
     if not os.path.exists(REF_E):
         sys.exit(f"{REF_E} missing: run `make -C {os.path.join(REPO, 'oracle')} ref` first")
     for name, (gen, kw, obs) in e_golden_cases().items():
+        if only and name not in only:
+            continue
         tmp = tempfile.mkdtemp(prefix="sw2d_golden_e_")
         try:
             make_case(sw, tmp, gen, kw)
@@ -176,4 +194,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 2 and sys.argv[1] == "--e-only":  # regenerate named _E fixtures only
+        sys.path.insert(0, REPO)
+        import bench
+        main_e(bench.load_pkg(), only=set(sys.argv[2:]))
+    else:
+        main()
