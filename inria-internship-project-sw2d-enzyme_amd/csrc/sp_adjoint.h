@@ -10,6 +10,8 @@
 //   hypComputeBalance_E          assim/swSpE_hyp.cpp:336-394
 //   sourceFrictionStrickler_E    assim/swSpE_friction.cpp:19-76
 //   sourceSingularHeadLoss_E     assim/swSpE_friction.cpp:79-126
+//   hypMusclEvrComputeLRStates_E assim/swSpE_muscl.cpp:22-187
+//   computeMusclSlopes           swSpHypMusclEvr.cpp:11-115 (free-surface slopes)
 // Derivative conventions: branch decisions (wet/dry regimes, min/max
 // selections, clamps) are taken from the primal values and held fixed; the
 // derivative of sqrt at 0 is taken as 0.
@@ -139,6 +141,210 @@ SPD Side recon_muscl(int side, double h, double qx, double qy, double zb, double
     s.lm = sp::lambda_minus(b, c, u);
     s.lp = sp::lambda_plus(b, c, u);
     return s;
+}
+
+// d desing(q, h) / dq and / dh; the max picks its branch from the primal h
+SPD void desing_d(double q, double h, double &dq, double &dh) {
+    const double h2 = h * h;
+    const double h4 = h2 * h2;
+    const double e4 = kHDesing * kHDesing * kHDesing * kHDesing;
+    const bool big = !(h4 < e4);
+    const double D2 = h4 + (big ? h4 : e4);
+    const double rD = 1.0 / sqrt(D2);
+    // d(h q / D)/dh = q / D (1 - h dD2/dh / (2 D2)), dD2/dh = (big ? 8 : 4) h^3
+    dq = 1.4142135623730951 * h * rD;
+    dh = 1.4142135623730951 * q * rD * (1.0 - (big ? 4.0 : 2.0) * h4 / D2);
+}
+
+// adjoint of (lm, lp) = b u -/+ sqrt((b - 1) b u^2 + c^2): adds to (du, dc)
+SPD void lam_vjp(double b, double c, double u, double dlm, double dlp, double &du, double &dc) {
+    const double root = sqrt((b - 1.0) * b * u * u + c * c);
+    const double dA = dsqrt_arg(root, dlp - dlm);
+    du += b * (dlm + dlp) + dA * (b - 1.0) * b * 2.0 * u;
+    dc += dA * 2.0 * c;
+}
+
+// adjoint of lr_bnd_muscl: (dz, dq, dr, dlp) -> accumulate into (dh, dqx, dqy)
+SPD void lr_bnd_muscl_vjp(double h, double qx, double qy, double b, double nx, double ny, double dz,
+                          double dq, double dr, double dlp, double &dh, double &dqx, double &dqy) {
+    const double q = qx * nx + qy * ny;
+    const double u = desing(q, h);
+    const double c = sqrt(kG * h);
+    double du = 0.0, dc = 0.0, gq, gh;
+    lam_vjp(b, c, u, 0.0, dlp, du, dc);
+    desing_d(q, h, gq, gh);
+    const double dqq = dq + du * gq;
+    dh += dz + dsqrt_arg(c, dc * kG) + du * gh;
+    dqx += dqq * nx - dr * ny;
+    dqy += dqq * ny + dr * nx;
+}
+
+// adjoint of recon_muscl: (dz, dq, dr, dlm, dlp) of the side -> accumulate
+// into the cell's (dh, dqx, dqy) (its free surface h + zb included) and its
+// limited slope (dsx, dsy).  Clipped eigenvalues and dry offsets pass no
+// derivative.
+SPD void recon_muscl_vjp(int side, double h, double qx, double qy, double zb, double b, double sx,
+                         double sy, double nx, double ny, double dxm, double dym, double dt,
+                         double dz, double dq, double dr, double dlm, double dlp, double &dh,
+                         double &dqx, double &dqy, double &dsx, double &dsy) {
+    const double zc = h + zb;
+    const double ql = qx * nx + qy * ny;
+    const double rl = qy * nx - qx * ny;
+    const double cl = sqrt(kG * ((kSmall < h) ? h : kSmall));
+    const double ul = desing(ql, h);
+    const double vl = desing(rl, h);
+    double l1 = sp::lambda_minus(b, cl, ul);
+    double l2 = sp::lambda_plus(b, cl, ul);
+    const bool p1 = side ? (l1 < 0.0) : (0.0 < l1);
+    const bool p2 = side ? (l2 < 0.0) : (0.0 < l2);
+    if (!p1) l1 = 0.0;
+    if (!p2) l2 = 0.0;
+    const double dx1 = dxm - 0.5 * l1 * dt * nx, dy1 = dym - 0.5 * l1 * dt * ny;
+    const double dx2 = dxm - 0.5 * l2 * dt * nx, dy2 = dym - 0.5 * l2 * dt * ny;
+    const double z1 = zc + dx1 * sx + dy1 * sy;
+    const double z2 = zc + dx2 * sx + dy2 * sy;
+    const bool w1 = 0.0 < z1 - zb, w2 = 0.0 < z2 - zb;
+    const double hr = 0.5 * ((w1 ? (z1 - zb) : 0.0) + (w2 ? (z2 - zb) : 0.0));
+    const double qo = hr * ul;
+    const double cc = sqrt(kG * hr);
+    const double uu = desing(qo, hr);
+    // eigenvalues of the reconstructed state
+    double duu = 0.0, dcc = 0.0, gq, gh;
+    lam_vjp(b, cc, uu, dlm, dlp, duu, dcc);
+    desing_d(qo, hr, gq, gh);
+    const double dqo = dq + duu * gq;
+    const double dhr = dz + dsqrt_arg(cc, dcc * kG) + duu * gh + dqo * ul + dr * vl;
+    double dul = dqo * hr;
+    const double dvl = dr * hr;
+    // the two characteristic-offset free surfaces
+    const double dz1 = w1 ? 0.5 * dhr : 0.0;
+    const double dz2 = w2 ? 0.5 * dhr : 0.0;
+    dsx += dz1 * dx1 + dz2 * dx2;
+    dsy += dz1 * dy1 + dz2 * dy2;
+    const double proj = -0.5 * dt * (nx * sx + ny * sy);
+    // eigenvalues of the cell state
+    double dcl = 0.0;
+    lam_vjp(b, cl, ul, p1 ? proj * dz1 : 0.0, p2 ? proj * dz2 : 0.0, dul, dcl);
+    double dhh = dz1 + dz2 + ((kSmall < h) ? dsqrt_arg(cl, dcl * kG) : 0.0);
+    desing_d(ql, h, gq, gh);
+    const double dql = dul * gq;
+    dhh += dul * gh;
+    desing_d(rl, h, gq, gh);
+    const double drl = dvl * gq;
+    dhh += dvl * gh;
+    dh += dhh;
+    dqx += dql * nx - drl * ny;
+    dqy += dql * ny + drl * nx;
+}
+
+// ---------------------------------------------------------------------------
+// Free-surface slope of one cell (computeMusclSlopes, swSpHypMusclEvr.cpp:
+// 11-115): inverse-distance weighted gradient over the neighbours, then the
+// in-place limiter visits of the cell's internal edges in ascending edge
+// order (a cell's slope changes only at its own edges' visits, so the
+// per-cell replay is the reference's global loop restricted to the cell).
+// Nb gives, for incidence k = 0 .. deg-1 in that order: zj(k), xj(k), yj(k)
+// of the neighbour, xm(k), ym(k) of the edge midpoint and side(k) (1: the
+// cell is the edge's right cell).
+// ---------------------------------------------------------------------------
+template <class Nb>
+SPD void slope_raw(const Nb &nb, int deg, double zi, double xi, double yi, double &sx, double &sy,
+                   double &wsum) {
+    sx = 0.0;
+    sy = 0.0;
+    wsum = 0.0;
+    if (deg <= 1) return;
+    for (int k = 0; k < deg; ++k) {
+        const double zj = nb.zj(k);
+        const double dx = nb.xj(k) - xi;
+        const double dy = nb.yj(k) - yi;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const double w = 1.0 / dist;
+        sx += (zj - zi) * dx / dist / dist * w;
+        sy += (zj - zi) * dy / dist / dist * w;
+        wsum += w;
+    }
+    sx /= wsum;
+    sy /= wsum;
+}
+
+// limiter visit of incidence k: scales (sx, sy) when the reconstruction at
+// the edge midpoint leaves [min, max] of the two free surfaces.  Returns the
+// cell whose surface bounds it (0 this cell, 1 the neighbour; -1 no
+// scaling), with D = fr - zi and the scale sf.
+template <class Nb>
+SPD int slope_visit(const Nb &nb, int k, double zi, double xi, double yi, double &sx, double &sy,
+                    double &D, double &sf) {
+    const double zj = nb.zj(k);
+    const int s = nb.side(k);
+    const double zL = s ? zj : zi;
+    const double zR = s ? zi : zj;
+    const double fmx = sp::smax(zL, zR);
+    const double fmn = sp::smin(zL, zR);
+    const double fr = zi + (nb.xm(k) - xi) * sx + (nb.ym(k) - yi) * sy;
+    int src = -1;
+    if (fr < fmn) {
+        sf = (fmn - zi) / (fr - zi);
+        sx *= sf;
+        sy *= sf;
+        D = fr - zi;
+        src = ((zR < zL) == (s == 0)) ? 1 : 0;  // fmn is zR when zR < zL
+    }
+    if (fr > fmx) {
+        sf = (fmx - zi) / (fr - zi);
+        sx *= sf;
+        sy *= sf;
+        D = fr - zi;
+        src = ((zL < zR) == (s == 0)) ? 1 : 0;  // fmx is zR when zL < zR
+    }
+    return src;
+}
+
+template <class Nb>
+SPD void slope_fwd(const Nb &nb, int deg, double zi, double xi, double yi, double &sx, double &sy) {
+    double wsum, D, sf;
+    slope_raw(nb, deg, zi, xi, yi, sx, sy, wsum);
+    for (int k = 0; k < deg; ++k) (void)slope_visit(nb, k, zi, xi, yi, sx, sy, D, sf);
+}
+
+// adjoint of slope_fwd: (gx, gy) = adjoint of the limited slope -> adds the
+// adjoint of this cell's free surface to dzi and gives that of the neighbour
+// across incidence k through out.set(k, v) then out.add(k, v).  The slope
+// each visit saw is replayed from the raw slope (deg (deg + 1) / 2 visits).
+// Each scaling sf = (z_f - zi) / D with D = a . slope (independent of zi).
+template <class Nb, class Out>
+SPD void slope_vjp(const Nb &nb, int deg, double zi, double xi, double yi, double gx, double gy,
+                   double &dzi, Out &out) {
+    double sx0, sy0, wsum;
+    slope_raw(nb, deg, zi, xi, yi, sx0, sy0, wsum);
+    for (int k = deg - 1; k >= 0; --k) {
+        double sx = sx0, sy = sy0, D = 1.0, sf = 1.0;
+        for (int q = 0; q < k; ++q) (void)slope_visit(nb, q, zi, xi, yi, sx, sy, D, sf);
+        const double px = sx, py = sy;
+        const int src = slope_visit(nb, k, zi, xi, yi, sx, sy, D, sf);
+        double zn = 0.0;
+        if (src >= 0) {
+            const double gs = gx * px + gy * py;
+            const double c = gs * sf / D;
+            gx = sf * gx - c * (nb.xm(k) - xi);
+            gy = sf * gy - c * (nb.ym(k) - yi);
+            if (src == 1) {  // bounded by the neighbour's surface (else d/dzi cancels)
+                zn = gs / D;
+                dzi -= zn;
+            }
+        }
+        out.set(k, zn);
+    }
+    if (deg <= 1) return;
+    for (int k = 0; k < deg; ++k) {
+        const double dx = nb.xj(k) - xi;
+        const double dy = nb.yj(k) - yi;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const double w = 1.0 / dist;
+        const double a = (gx * dx + gy * dy) / dist / dist * w / wsum;
+        out.add(k, a);
+        dzi -= a;
+    }
 }
 
 // adjoint of lr_fwd from its intermediates: (dz, dq, dr, dlm, dlp) ->

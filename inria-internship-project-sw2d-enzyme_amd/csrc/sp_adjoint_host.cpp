@@ -76,4 +76,64 @@ void sw2dt_hl_vjp(const double *p, double h_min, const double *dout, double *d) 
     d[2] = dqy;
 }
 
+// p = {h, qx, qy, zb, b, sx, sy, nx, ny, dxm, dym, dt}; out = {z, q, r, lm, lp}
+void sw2dt_recon_fwd(const double *p, int side, double *out) {
+    const spe::Side s = spe::recon_muscl(side, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8],
+                                         p[9], p[10], p[11]);
+    out[0] = s.z;
+    out[1] = s.q;
+    out[2] = s.r;
+    out[3] = s.lm;
+    out[4] = s.lp;
+}
+// dout[5] -> d[5] = adjoints of h, qx, qy, sx, sy
+void sw2dt_recon_vjp(const double *p, int side, const double *dout, double *d) {
+    for (int k = 0; k < 5; ++k) d[k] = 0.0;
+    spe::recon_muscl_vjp(side, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10],
+                         p[11], dout[0], dout[1], dout[2], dout[3], dout[4], d[0], d[1], d[2], d[3],
+                         d[4]);
+}
+
+// p = {h, qx, qy, zb, b, nx, ny}; out = {z, q, r, lp}
+void sw2dt_bnd_muscl_fwd(const double *p, double *out) {
+    const spe::Side s = spe::lr_bnd_muscl(p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
+    out[0] = s.z;
+    out[1] = s.q;
+    out[2] = s.r;
+    out[3] = s.lp;
+}
+// dout[4] -> d[3] = adjoints of h, qx, qy
+void sw2dt_bnd_muscl_vjp(const double *p, const double *dout, double *d) {
+    d[0] = d[1] = d[2] = 0.0;
+    spe::lr_bnd_muscl_vjp(p[0], p[1], p[2], p[4], p[5], p[6], dout[0], dout[1], dout[2], dout[3],
+                          d[0], d[1], d[2]);
+}
+
+// one cell's slope over deg incidences: nb = deg x {zj, xj, yj, xm, ym, side}
+struct HostNb {
+    const double *v;
+    double zj(int k) const { return v[6 * k]; }
+    double xj(int k) const { return v[6 * k + 1]; }
+    double yj(int k) const { return v[6 * k + 2]; }
+    double xm(int k) const { return v[6 * k + 3]; }
+    double ym(int k) const { return v[6 * k + 4]; }
+    int side(int k) const { return (int)v[6 * k + 5]; }
+};
+struct HostOut {
+    double *d;
+    void set(int k, double x) const { d[k] = x; }
+    void add(int k, double x) const { d[k] += x; }
+};
+// c = {zi, xi, yi}; out = {sx, sy}
+void sw2dt_slope_fwd(const double *c, const double *nb, int deg, double *out) {
+    spe::slope_fwd(HostNb{nb}, deg, c[0], c[1], c[2], out[0], out[1]);
+}
+// g = adjoint of (sx, sy) -> d[0] = adjoint of zi, d[1 + k] = adjoint of zj(k)
+void sw2dt_slope_vjp(const double *c, const double *nb, int deg, const double *g, double *d) {
+    double dzi = 0.0;
+    HostOut o{d + 1};
+    spe::slope_vjp(HostNb{nb}, deg, c[0], c[1], c[2], g[0], g[1], dzi, o);
+    d[0] = dzi;
+}
+
 }  // extern "C"

@@ -46,8 +46,6 @@ struct EClock {
     double t_max, dt_max, cfl;
 };
 
-constexpr bool kMusclAdjoint = false;
-
 struct Controls {
     const double *phi, *K1, *K2;
 };
@@ -82,11 +80,14 @@ struct AssimState {
     double *d_Pseg = nullptr;  // stride x 6 nc: balanced state + internal increments per step
     double *d_sw = nullptr;    // 2 n_int
     double *d_sxy = nullptr;   // 2 nc: limited free-surface slopes of the step (MUSCL-EVR)
+    double *d_Sseg = nullptr;  // stride x 2 nc: the slopes of every step of the replayed segment
     double *d_partial = nullptr;
     double *d_J = nullptr;     // [1 + n_obs]: total, then the cost of each observation
     double *d_cpart = nullptr; // frame-cost block partials
     // adjoint
     double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
+    double *d_sd = nullptr;    // MUSCL-EVR: 4 n_int slope adjoints per edge side
+    double *d_zn = nullptr;    // MUSCL-EVR: 2 n_int neighbour free-surface adjoints
     double *d_gphi = nullptr, *d_gK1 = nullptr, *d_gK2 = nullptr, *d_gbce = nullptr,
            *d_gbc = nullptr;
     // change of variable of the porosity control (phi_correct / dphi_dm,
@@ -180,6 +181,28 @@ __device__ double interp_e(const double *tt, const double *vv, int n, double t) 
     return v1 + (v2 - v1) * (t - t1) / den;
 }
 
+// the internal-edge incidences of cell i in its reference order, for the
+// slope functions of sp_adjoint.h (free surface z = h + zb of the state U)
+struct NbDev {
+    const MeshDev &m;
+    const double *U;
+    int b;
+    __device__ int inc(int k) const { return m.int_inc[b + k]; }
+    __device__ int nbr(int k) const {
+        const int c = inc(k);
+        return (c & 1) ? m.eL[c >> 1] : m.eR[c >> 1];
+    }
+    __device__ double zj(int k) const {
+        const int j = nbr(k);
+        return U[j] + m.zb[j];
+    }
+    __device__ double xj(int k) const { return m.cx[nbr(k)]; }
+    __device__ double yj(int k) const { return m.cy[nbr(k)]; }
+    __device__ double xm(int k) const { return m.exm[inc(k) >> 1]; }
+    __device__ double ym(int k) const { return m.eym[inc(k) >> 1]; }
+    __device__ int side(int k) const { return inc(k) & 1; }
+};
+
 // MUSCL-EVR: per cell, free-surface gradient over the cell's neighbours and
 // the reference's in-place limiter replayed over the cell's internal edges
 // in ascending order (computeMusclSlopes, swSpHypMusclEvr.cpp:11-115; a
@@ -192,47 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_e_slopes(MeshDev m, const double *__
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     const int nc = m.n_cell;
     if (i >= nc) return;
-    const double zi = U[i] + m.zb[i];
-    const double xi = m.cx[i], yi = m.cy[i];
-    const int b = m.int_ptr[i], e_end = m.int_ptr[i + 1];
-    double sx = 0.0, sy = 0.0;
-    if (e_end - b > 1) {
-        double wsum = 0.0;
-        for (int k = b; k < e_end; ++k) {
-            const int inc = m.int_inc[k];
-            const int e = inc >> 1;
-            const int j = (inc & 1) ? m.eL[e] : m.eR[e];
-            const double zj = U[j] + m.zb[j];
-            const double dx = m.cx[j] - xi;
-            const double dy = m.cy[j] - yi;
-            const double dist = sqrt(dx * dx + dy * dy);
-            const double w = 1.0 / dist;
-            sx += (zj - zi) * dx / dist / dist * w;
-            sy += (zj - zi) * dy / dist / dist * w;
-            wsum += w;
-        }
-        sx /= wsum;
-        sy /= wsum;
-    }
-    for (int k = b; k < e_end; ++k) {
-        const int e = m.int_inc[k] >> 1;
-        const int L = m.eL[e], R = m.eR[e];
-        const double zL = U[L] + m.zb[L];
-        const double zR = U[R] + m.zb[R];
-        const double fmx = sp::smax(zL, zR);
-        const double fmn = sp::smin(zL, zR);
-        const double fr = zi + (m.exm[e] - xi) * sx + (m.eym[e] - yi) * sy;
-        if (fr < fmn) {
-            const double sf = (fmn - zi) / (fr - zi);
-            sx *= sf;
-            sy *= sf;
-        }
-        if (fr > fmx) {
-            const double sf = (fmx - zi) / (fr - zi);
-            sx *= sf;
-            sy *= sf;
-        }
-    }
+    const NbDev nb{m, U, m.int_ptr[i]};
+    double sx, sy;
+    spe::slope_fwd(nb, m.int_ptr[i + 1] - nb.b, U[i] + m.zb[i], m.cx[i], m.cy[i], sx, sy);
     sxy[i] = sx;
     sxy[nc + i] = sy;
 }
@@ -693,8 +678,10 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
 //   seed    + dJ_obs/dU^n when U^n is observed
 //   CELL    reverse of the cell part of step n-1 (else: write lam = adj U^n)
 // gather: -1 nothing to gather (the last step, or a tile top: the future
-// adjoint is dropped), 1 gather part + edge sides, 2 take lam (the gather
-// already done by the CELL = false launch that closed the next segment).
+// adjoint is dropped), 1 gather part + edge sides (+ zn, MUSCL-EVR: the
+// neighbours' slope adjoints through this cell's free surface), 2 take lam
+// (the gather already done by the CELL = false launch that closed the next
+// segment).
 // Blocks past n_cell_blocks sum the boundary-value adjoints of step n per
 // series (one block per series, fixed order).
 template <bool CELL>
@@ -706,7 +693,8 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
                                                  double *__restrict__ lam, double *gphi,
                                                  double *gK1, double *gK2, int n_cell_blocks,
                                                  const int *ser_ptr, const int *ser_edges,
-                                                 const double *gbce, double *gbc_n) {
+                                                 const double *gbce, double *gbc_n,
+                                                 const double *__restrict__ zn) {
     if ((int)blockIdx.x >= n_cell_blocks) {
         __shared__ double red[kBlock];
         const int k = (int)blockIdx.x - n_cell_blocks;
@@ -739,6 +727,7 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
             b += sv[1];
             c += sv[2];
             gp += sv[3];
+            if (zn) a += zn[inc ^ 1];  // the neighbour's slope through this cell's surface
         }
         if (gather > 0) {  // not a tile top
             ah = a;
@@ -764,8 +753,9 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
 }
 
 // per boundary cell: reverse of its boundary fluxes (hypBoundaryFlux_E and
-// the Godunov _E states) and of their factors, in reference order, added to
+// the Godunov or MUSCL-EVR _E boundary states) and of their factors, in reference order, added to
 // the cell part of the adjoint; boundary-value adjoint per edge
+template <bool MUSCL>
 __global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs, BcDev bc,
                                                   const double *__restrict__ U,
                                                   const double *__restrict__ F, double dt,
@@ -786,7 +776,8 @@ __global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs
         const int j = m.bnd_inc[k];
         const double nx = m.bnx[j], ny = m.bny[j];
         const double *f = Fb + 3 * (size_t)j;
-        const spe::Side sd = spe::lr_fwd(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny, o.h_min);
+        const spe::Side sd = MUSCL ? spe::lr_bnd_muscl(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny)
+                                   : spe::lr_fwd(h0, qx0, qy0, m.zb[i], m.beta[i], nx, ny, o.h_min);
         const int ser = m.bser[j];
         const double co = spe::edge_coeff(m.blen[j], area, phi, o.phi_min);
         double df[3];
@@ -798,7 +789,10 @@ __global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs
         double d[6] = {0, 0, 0, 0, 0, 0};
         spe::bnd_vjp(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp,
                      bc.type[ser], bcv[ser], df, d);
-        spe::lr_vjp(h0, qx0, qy0, m.beta[i], nx, ny, o.h_min, d[0], d[1], d[2], 0.0, d[3], ph, px, py);
+        if (MUSCL)
+            spe::lr_bnd_muscl_vjp(h0, qx0, qy0, m.beta[i], nx, ny, d[0], d[1], d[2], d[3], ph, px, py);
+        else
+            spe::lr_vjp(h0, qx0, qy0, m.beta[i], nx, ny, o.h_min, d[0], d[1], d[2], 0.0, d[3], ph, px, py);
         gp += d[4];
         gbce[j] = d[5];
     }
@@ -808,11 +802,17 @@ __global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs
     gphi[i] += gp;
 }
 
-// per internal edge: flux adjoints from both sides, HLLC_E and Godunov _E reverse
+// per internal edge: flux adjoints from both sides, HLLC_E reverse, then the
+// reverse of the Godunov _E states, or (MUSCL) of the MUSCL-EVR
+// reconstruction of each side from the step's slopes sxy with the step's dt,
+// whose slope adjoints go to sd[4e ..] = {L: dsx, dsy, R: dsx, dsy}
+template <bool MUSCL>
 __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls cs,
                                                    const double *__restrict__ U, double dt,
                                                    const double *__restrict__ lams,
-                                                   double *__restrict__ side_out) {
+                                                   double *__restrict__ side_out,
+                                                   const double *__restrict__ sxy,
+                                                   double *__restrict__ sd) {
     const int e = xcd_block() * kBlock + (int)threadIdx.x;
     if (e >= m.n_int) return;
     const int nc = m.n_cell;
@@ -828,15 +828,35 @@ __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls c
     const double hL = U[L], qxL = U[nc + L], qyL = U[2 * nc + L];
     const double hR = U[R], qxR = U[nc + R], qyR = U[2 * nc + R];
     const double bL = m.beta[L], bR = m.beta[R];
-    spe::LrAux xL, xR;
-    const spe::Side sL = spe::lr_fwd_x(hL, qxL, qyL, m.zb[L], bL, nx, ny, o.h_min, xL);
-    const spe::Side sR = spe::lr_fwd_x(hR, qxR, qyR, m.zb[R], bR, nx, ny, o.h_min, xR);
     double d[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    spe::hllc_vjp(o.h_min, o.phi_min, bL, bR, phiL, phiR, m.zb[L], m.zb[R], sL.z, sL.q, sL.r, sR.z,
-                  sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, dF, d);
     double dhL = 0, dqxL = 0, dqyL = 0, dhR = 0, dqxR = 0, dqyR = 0;
-    spe::lr_vjp_x(xL, bL, nx, ny, d[0], d[1], d[2], d[6], d[8], dhL, dqxL, dqyL);
-    spe::lr_vjp_x(xR, bR, nx, ny, d[3], d[4], d[5], d[7], d[9], dhR, dqxR, dqyR);
+    if (MUSCL) {
+        const double xm = m.exm[e], ym = m.eym[e];
+        const double sxL = sxy[L], syL = sxy[nc + L], sxR = sxy[R], syR = sxy[nc + R];
+        const double dxL = xm - m.cx[L], dyL = ym - m.cy[L], dxR = xm - m.cx[R], dyR = ym - m.cy[R];
+        const spe::Side sL = spe::recon_muscl(0, hL, qxL, qyL, m.zb[L], bL, sxL, syL, nx, ny, dxL, dyL, dt);
+        const spe::Side sR = spe::recon_muscl(1, hR, qxR, qyR, m.zb[R], bR, sxR, syR, nx, ny, dxR, dyR, dt);
+        spe::hllc_vjp(o.h_min, o.phi_min, bL, bR, phiL, phiR, m.zb[L], m.zb[R], sL.z, sL.q, sL.r, sR.z,
+                      sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, dF, d);
+        double gxL = 0, gyL = 0, gxR = 0, gyR = 0;
+        spe::recon_muscl_vjp(0, hL, qxL, qyL, m.zb[L], bL, sxL, syL, nx, ny, dxL, dyL, dt, d[0], d[1],
+                             d[2], d[6], d[8], dhL, dqxL, dqyL, gxL, gyL);
+        spe::recon_muscl_vjp(1, hR, qxR, qyR, m.zb[R], bR, sxR, syR, nx, ny, dxR, dyR, dt, d[3], d[4],
+                             d[5], d[7], d[9], dhR, dqxR, dqyR, gxR, gyR);
+        double *g = sd + 4 * (size_t)e;
+        g[0] = gxL;
+        g[1] = gyL;
+        g[2] = gxR;
+        g[3] = gyR;
+    } else {
+        spe::LrAux xL, xR;
+        const spe::Side sL = spe::lr_fwd_x(hL, qxL, qyL, m.zb[L], bL, nx, ny, o.h_min, xL);
+        const spe::Side sR = spe::lr_fwd_x(hR, qxR, qyR, m.zb[R], bR, nx, ny, o.h_min, xR);
+        spe::hllc_vjp(o.h_min, o.phi_min, bL, bR, phiL, phiR, m.zb[L], m.zb[R], sL.z, sL.q, sL.r, sR.z,
+                      sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, dF, d);
+        spe::lr_vjp_x(xL, bL, nx, ny, d[0], d[1], d[2], d[6], d[8], dhL, dqxL, dqyL);
+        spe::lr_vjp_x(xR, bR, nx, ny, d[3], d[4], d[5], d[7], d[9], dhR, dqxR, dqyR);
+    }
     double *so = side_out + 8 * (size_t)e;
     so[0] = dhL;
     so[1] = dqxL;
@@ -846,6 +866,40 @@ __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls c
     so[5] = dqxR;
     so[6] = dqyR;
     so[7] = d[11];
+}
+
+// the slope adjoints of one cell's incidences go to zn[incidence]: the term
+// of the neighbour across that edge, gathered by the neighbour as zn[inc ^ 1]
+struct ZnOut {
+    double *zn;
+    const int *inc;
+    int b;
+    __device__ void set(int k, double v) const { zn[inc[b + k]] = v; }
+    __device__ void add(int k, double v) const { zn[inc[b + k]] += v; }
+};
+
+// MUSCL-EVR, per cell: gather the adjoint of its limited slope from its
+// edges' sides (sd), reverse the slope and its limiter (spe::slope_vjp): the
+// term of its own free surface is added to the h part of the adjoint of U^n,
+// those of its neighbours go to zn
+__global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__restrict__ U,
+                                                     const double *__restrict__ sd,
+                                                     double *__restrict__ part,
+                                                     double *__restrict__ zn) {
+    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+    if (i >= m.n_cell) return;
+    const int b = m.int_ptr[i], deg = m.int_ptr[i + 1] - b;
+    double gx = 0.0, gy = 0.0;
+    for (int k = 0; k < deg; ++k) {
+        const int inc = m.int_inc[b + k];
+        gx += sd[2 * (size_t)inc];
+        gy += sd[2 * (size_t)inc + 1];
+    }
+    const NbDev nb{m, U, b};
+    const ZnOut out{zn, m.int_inc, b};
+    double dzi = 0.0;
+    spe::slope_vjp(nb, deg, U[i] + m.zb[i], m.cx[i], m.cy[i], gx, gy, dzi, out);
+    part[i] += dzi;
 }
 
 __global__ void k_zero(double *p, size_t n) {
@@ -949,11 +1003,12 @@ size_t flux_len(const sw2dg_ctx *c) {
 
 double *seg_flux(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Fseg + (size_t)j * flux_len(c); }
 double *seg_pre(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Pseg + (size_t)j * 6 * c->nc; }
+double *seg_slopes(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Sseg + (size_t)j * 2 * c->nc; }
 
 // one _E step U_in -> U_out; the edge fluxes go to F (kept for the reverse
 // sweep when F is a slot of the segment flux buffer)
 void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout, int64_t n,
-                   bool dry, double *F, double *P = nullptr) {
+                   bool dry, double *F, double *P = nullptr, double *S = nullptr) {
     const dim3 blk(kBlock);
     const Controls cs = ctl(a);
     const int ns = std::max(c->n_series, 1);
@@ -963,10 +1018,11 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
     if (muscl) {
         // the dry run reconstructs with the previous step's dt (0 before the
         // first step), the replay with the step's own dt (see k_e_edges)
-        KLAUNCH(a, k_e_slopes, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, Uin, a->d_sxy,
+        double *sxy = S ? S : a->d_sxy;
+        KLAUNCH(a, k_e_slopes, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, Uin, sxy,
                 dry ? a->d_T : nullptr, n, c->prm.t_max);
         KLAUNCH(a, k_e_edges<true>, ge, blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin, F,
-                dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max, bcv, a->d_sxy,
+                dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max, bcv, sxy,
                 (dry && n > 0) ? a->d_dt + (n - 1) : nullptr, dry ? 0.0 : a->h_dt[n]);
     } else {
         KLAUNCH(a, k_e_edges<false>, ge, blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin, F,
@@ -1106,7 +1162,8 @@ int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int
     for (int64_t n = n0; n < n1; ++n) {
         const int j = (int)(n - n0);
         launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), n, false, seg_flux(a, c, j),
-                      store ? seg_pre(a, c, j) : nullptr);
+                      store ? seg_pre(a, c, j) : nullptr,
+                      (store && c->o.scheme == 1) ? seg_slopes(a, c, j) : nullptr);
         if (with_cost && trig[n] >= 0) launch_cost(c, a, seg_state(a, c, j + 1), trig[n]);
     }
     return SW2DG_OK;
@@ -1253,7 +1310,11 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_Fseg = a->alloc<double>((size_t)a->stride * flux_len(c));
     a->d_Pseg = a->alloc<double>((size_t)a->stride * 6 * (size_t)nc);
     a->d_sw = a->alloc<double>(2 * (size_t)std::max<int64_t>(ni, 1));
-    a->d_sxy = a->alloc<double>(c->o.scheme == 1 ? 2 * (size_t)nc : 1);
+    const bool muscl = c->o.scheme == 1;
+    a->d_sxy = a->alloc<double>(muscl ? 2 * (size_t)nc : 1);
+    a->d_Sseg = a->alloc<double>(muscl ? (size_t)a->stride * 2 * (size_t)nc : 1);
+    a->d_sd = a->alloc<double>(muscl ? 4 * (size_t)std::max<int64_t>(ni, 1) : 1);
+    a->d_zn = a->alloc<double>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
     a->d_partial = a->alloc<double>(nblocks(nc));
     a->d_cpart = a->alloc<double>(nblocks(nc));
     a->d_J = a->alloc<double>(1 + (size_t)std::max(1, obs->n_obs));
@@ -1410,9 +1471,6 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
                          double *dJ_dbc, int64_t *n_steps) {
     if (n_steps) *n_steps = 0;
     if (!c || !c->assim) return SW2DG_ERR_ARG;
-    if (c->o.scheme == 1 && !kMusclAdjoint) {
-        return fail(c, SW2DG_ERR_ARG, "the MUSCL-EVR adjoint is not built: the gradient needs num_scheme godunov");
-    }
     AssimState *a = c->assim;
     HIPCHK(c, hipSetDevice(c->device));
     int64_t nst = 0;
@@ -1434,6 +1492,8 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     const Controls cs = ctl(a);
     const dim3 blk(kBlock);
     const int ncb = nblocks(nc);
+    const bool muscl = c->o.scheme == 1;
+    const double *zn = muscl ? a->d_zn : nullptr;
     // fused gather (step n) + cell reverse (step n-1) [+ boundary-value sums of step n]
     auto gc = [&](bool cell, int gather, int64_t bc_step, const SeedArgs &sd, const double *P,
                   double dtv) {
@@ -1442,17 +1502,34 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         if (cell)
             KLAUNCH(a, k_r_gc<true>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
-                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn);
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn);
         else
             KLAUNCH(a, k_r_gc<false>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
-                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn);
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn);
     };
     auto bnd = [&](int64_t n, int j) {
-        if (c->m.n_bnd_cells > 0)
-            KLAUNCH(a, k_r_bnd, dim3(nblocks(c->m.n_bnd_cells)), blk, 0, c->stream, c->m, c->o, cs,
-                    c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
+        if (c->m.n_bnd_cells <= 0) return;
+        if (muscl)
+            KLAUNCH(a, k_r_bnd<true>, dim3(nblocks(c->m.n_bnd_cells)), blk, 0, c->stream, c->m, c->o,
+                    cs, c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
                     a->d_lams, a->d_part, a->d_gphi, a->d_gbce);
+        else
+            KLAUNCH(a, k_r_bnd<false>, dim3(nblocks(c->m.n_bnd_cells)), blk, 0, c->stream, c->m, c->o,
+                    cs, c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
+                    a->d_lams, a->d_part, a->d_gphi, a->d_gbce);
+    };
+    // edge reverse of step n (segment slot j); MUSCL-EVR: then the slopes
+    auto edges = [&](int64_t n, int j) {
+        if (muscl) {
+            KLAUNCH(a, k_r_edge<true>, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs,
+                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side, seg_slopes(a, c, j), a->d_sd);
+            KLAUNCH(a, k_r_slopes, dim3(ncb), blk, 0, c->stream, c->m, seg_state(a, c, j), a->d_sd,
+                    a->d_part, a->d_zn);
+        } else {
+            KLAUNCH(a, k_r_edge<false>, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs,
+                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side, nullptr, nullptr);
+        }
     };
     // explosion guard (swSpE_checkpoint.cpp:1035-1117): ||adjoint of h|| at
     // each segment start against the first non-zero one; a gradient snapshot
@@ -1492,8 +1569,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         }
         for (int64_t n = n1 - 1; n >= n0; --n) {
             const int j = (int)(n - n0);
-            KLAUNCH(a, k_r_edge, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs,
-                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side);
+            edges(n, j);
             if (n > n0) {
                 const SeedArgs sd = seed_args(c, a, trig[n - 1], seg_state(a, c, j));
                 gc(true, 1, n, sd, seg_pre(a, c, j - 1), dt[n - 1]);

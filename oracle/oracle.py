@@ -136,6 +136,12 @@ def _e_decl():
     L.orc_e_set_explosion_guard.restype = None
     L.orc_e_truncation.argtypes = []
     L.orc_e_truncation.restype = C.c_int64
+    L.orc_e_set_plant.argtypes = [C.c_int]
+    L.orc_e_set_plant.restype = None
+    L.orc_e_set_checkpoints.argtypes = [C.c_int64, _d]
+    L.orc_e_set_checkpoints.restype = None
+    L.orc_e_dry_states.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int64, _d, _d, _d]
+    L.orc_e_dry_states.restype = None
     L._e_ready = True
     return L
 
@@ -200,12 +206,14 @@ class OracleE:
 
 def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=None,
                        normalize=False, n_obs=None, bc=None, gauge_w=None, adj_cut=0.0,
-                       explosion=None):
+                       explosion=None, checkpoints=None):
     """Gauge cost of the frozen replay and its gradient (reverse sweep of the
     oracle's own replay): {"J", "phi", "K1", "K2", "bc" [n_steps, n_series],
     "first_kept_step"}.  adj_cut > 0: thin-film cut of the reverse sweep
     (orc_e_set_adjoint_cut); explosion = (ratio, stride): the explosion guard
-    over segments of `stride` steps (orc_e_set_explosion_guard)."""
+    over segments of `stride` steps (orc_e_set_explosion_guard); checkpoints =
+    (stride, states): segments replayed from the dry run's states
+    [n_steps + 1, 3, n_cell] (orc_e_set_checkpoints)."""
     a = self.case.a
     phi = self._ctl(phi, a["phi"])
     K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
@@ -225,6 +233,11 @@ def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=N
     L.orc_e_set_adjoint_cut(float(adj_cut))
     if explosion is not None:
         L.orc_e_set_explosion_guard(float(explosion[0]), int(explosion[1]))
+    if checkpoints is not None:
+        ck = np.ascontiguousarray(checkpoints[1], dtype=np.float64)
+        if ck.size != (n + 1) * 3 * self.nc:
+            raise ValueError("checkpoint states must be [n_steps + 1, 3, n_cell]")
+        L.orc_e_set_checkpoints(int(checkpoints[0]), _dp(ck))
     try:
         J = L.orc_e_gradient(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
                              _dp(qy0), n, _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32), nob,
@@ -234,11 +247,31 @@ def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=N
     finally:
         L.orc_e_set_adjoint_cut(0.0)
         L.orc_e_set_explosion_guard(0.0, 0)
+        L.orc_e_set_checkpoints(0, None)
     return {"J": J, "phi": gphi, "K1": gK1, "K2": gK2, "bc": gbc[: n * self.ns].reshape(n, self.ns),
             "first_kept_step": kept}
 
 
 OracleE.gradient = _oracle_e_gradient
+
+
+def _oracle_e_dry_states(self, rec, phi=None, K1=None, K2=None, bc=None):
+    """The dry run's states on the schedule rec (reconstruction with the
+    previous step's dt): [n_steps + 1, 3, n_cell] (orc_e_dry_states)."""
+    a = self.case.a
+    phi = self._ctl(phi, a["phi"])
+    K1 = self._ctl(K1, a["fc1"] if len(a["fc1"]) else np.zeros(self.nc))
+    K2 = self._ctl(K2, a["fc2"] if len(a["fc2"]) else np.zeros(self.nc))
+    dt = np.ascontiguousarray(rec["dt"], dtype=np.float64)
+    bcs = np.ascontiguousarray(rec["bc"] if bc is None else bc, dtype=np.float64).reshape(-1)
+    h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
+    S = np.empty((len(dt) + 1) * 3 * self.nc)
+    _e_decl().orc_e_dry_states(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+                               _dp(qy0), len(dt), _dp(dt), _dp(bcs), _dp(S))
+    return S.reshape(len(dt) + 1, 3, self.nc)
+
+
+OracleE.dry_states = _oracle_e_dry_states
 
 
 def _oracle_e_frame_cost(self, rec, frames, phi=None, K1=None, K2=None, w_h=None, w_q=None,

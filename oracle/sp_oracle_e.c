@@ -12,8 +12,12 @@
  *   assim/swSpE.h            interpolateTimeSeries_E (138-153)
  *   assim/swSpE_checkpoint.cpp build_checkpoint_records (97-455, the dry run
  *                            that fixes dt / BC / observation triggers) and
- *                            ckpt_segment_E (584-715, the frozen replay).
- * Godunov reconstruction only.  Costs: gauge misfit (swSpE_gauges.cpp:166-179)
+ *                            ckpt_segment_E (584-715, the frozen replay)
+ *   assim/swSpE_muscl.cpp    hypMusclEvrComputeLRStates_E (22-187) with the
+ *                            slopes of computeMusclSlopes
+ *                            (swSpHypMusclEvr.cpp:11-115)
+ * Godunov and MUSCL-EVR reconstructions; the reverse sweep below covers both
+ * (the slope limiter included).  Costs: gauge misfit (swSpE_gauges.cpp:166-179)
  * and full-frame misfit (swSpE_ad.cpp:273-308), optional normalisation by the
  * number of observations.
  */
@@ -679,6 +683,41 @@ double orc_e_replay(const orc_case *c, const double *phi, const double *K1, cons
     return J;
 }
 
+/* The dry run's states on a given schedule: as orc_e_replay, but the
+ * reconstruction reads the previous step's dt (0 before the first), as
+ * build_checkpoint_records does (swSpE_checkpoint.cpp:244-251); on the dry
+ * run's own schedule these are its states.  states: (n_steps + 1) x 3
+ * n_cell [h | qx | qy], the initial state first. */
+void orc_e_dry_states(const orc_case *c, const double *phi, const double *K1, const double *K2,
+                      const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
+                      const double *dt_seq, const double *bc_seq, double *states) {
+    esolver s;
+    e_init(&s, c, phi, K1, K2, h0, qx0, qy0);
+    const int64_t nc = c->n_cell;
+    const size_t b = sizeof(double) * (size_t)nc;
+    memcpy(states, h0, b);
+    memcpy(states + nc, qx0, b);
+    memcpy(states + 2 * nc, qy0, b);
+    double prev = 0.0;
+    for (int64_t n = 0; n < n_steps; n++) {
+        const double dt = dt_seq[n];
+        s.dt_recon = prev;
+        lr_any(&s);
+        s.dt_recon = dt;
+        fluxes(&s, bc_seq + n * c->n_series);
+        balance(&s, dt);
+        for (int64_t i = 0; i < nc; ++i)
+            if (s.h[i] < 0.) { s.h[i] = 0.; s.qx[i] = 0.; s.qy[i] = 0.; }
+        sources(&s, dt);
+        prev = dt;
+        double *St = states + 3 * nc * (n + 1);
+        memcpy(St, s.h, b);
+        memcpy(St + nc, s.qx, b);
+        memcpy(St + 2 * nc, s.qy, b);
+    }
+    e_free(&s);
+}
+
 /* ------------------------------------------------------------------------
  * Reverse sweep of the frozen replay: the derivative that the reference
  * obtains from Enzyme (get_spatial_gradient_checkpointed,
@@ -722,6 +761,177 @@ static void lr_adj(double h, double qx, double qy, double b, double nx, double n
     *ah += ahh;
     *aqx += aq * nx - ar * ny;
     *aqy += aq * ny + ar * nx;
+}
+
+/* d desing(q, h) / dq and / dh (swSpDesingVelocity.h:28-39): the max picks
+ * its branch from the primal values */
+static void desing_grad(double q, double h, double he, double *dq, double *dh) {
+    double h2 = h * h, h4 = h2 * h2, e4 = he * he * he * he;
+    int big = h4 > e4;
+    double D2 = h4 + (big ? h4 : e4), D = sqrt(D2);
+    double dD2 = (big ? 8.0 : 4.0) * h2 * h; /* d(h4 + max(h4, e4))/dh */
+    *dq = 1.4142135623730951 * h / D;
+    *dh = 1.4142135623730951 * q / D - 0.5 * 1.4142135623730951 * h * q * dD2 / (D2 * D);
+}
+
+/* adjoint of a boundary side of hypMusclEvrComputeLRStates_E
+ * (swSpE_muscl.cpp:48-64, desingularised velocity): (z, q, r, lp) ->
+ * accumulates into (ah, aqx, aqy) */
+static void lr_bnd_muscl_adj(double h, double qx, double qy, double b, double nx, double ny,
+                             double az, double aq, double ar, double alp, double *ah, double *aqx,
+                             double *aqy) {
+    double q = qx * nx + qy * ny;
+    double u = desing(q, h, H_DESING);
+    double c = sqrt(G * h);
+    double dup, dcp, dq, dh;
+    lam_grad(b, c, u, 1, &dup, &dcp);
+    desing_grad(q, h, H_DESING, &dq, &dh);
+    double au = alp * dup;
+    double ahh = az + alp * dcp * G * dsq(G * h) + au * dh;
+    aq += au * dq;
+    *ah += ahh;
+    *aqx += aq * nx - ar * ny;
+    *aqy += aq * ny + ar * nx;
+}
+
+/* adjoint of one side (sd 0 = L, 1 = R) of the internal-edge reconstruction
+ * of hypMusclEvrComputeLRStates_E (swSpE_muscl.cpp:66-180): (z, q, r, lm,
+ * lp) of the side -> accumulates into the cell's (ah, aqx, aqy) (its free
+ * surface z = h + zb included) and its limited slope (asx, asy) */
+static void recon_adj(int sd, double h, double qx, double qy, double zb, double b, double sx,
+                      double sy, double nx, double ny, double dxm, double dym, double dt, double az,
+                      double aq, double ar, double alm, double alp, double *ah, double *aqx,
+                      double *aqy, double *asx, double *asy) {
+    /* forward intermediates */
+    double zc = h + zb;
+    double ql = qx * nx + qy * ny, rl = qy * nx - qx * ny;
+    double cl = sqrt(G * ((SMALL < h) ? h : SMALL));
+    double ul = desing(ql, h, H_DESING), vl = desing(rl, h, H_DESING);
+    double l1 = lam_m(b, cl, ul), l2 = lam_p(b, cl, ul);
+    int p1 = sd ? (l1 < 0.0) : (0.0 < l1), p2 = sd ? (l2 < 0.0) : (0.0 < l2);
+    if (!p1) l1 = 0.0;
+    if (!p2) l2 = 0.0;
+    double dx1 = dxm - 0.5 * l1 * dt * nx, dy1 = dym - 0.5 * l1 * dt * ny;
+    double dx2 = dxm - 0.5 * l2 * dt * nx, dy2 = dym - 0.5 * l2 * dt * ny;
+    double z1 = zc + dx1 * sx + dy1 * sy, z2 = zc + dx2 * sx + dy2 * sy;
+    int w1 = 0.0 < z1 - zb, w2 = 0.0 < z2 - zb;
+    double hr = 0.5 * ((w1 ? z1 - zb : 0.0) + (w2 ? z2 - zb : 0.0));
+    double qo = hr * ul;
+    double cc = sqrt(G * hr);
+    double uu = desing(qo, hr, H_DESING);
+    /* reverse: eigenvalues of the reconstructed state */
+    double dum, dcm, dup, dcp, dq, dh;
+    lam_grad(b, cc, uu, 0, &dum, &dcm);
+    lam_grad(b, cc, uu, 1, &dup, &dcp);
+    double auu = alm * dum + alp * dup, acc = alm * dcm + alp * dcp;
+    desing_grad(qo, hr, H_DESING, &dq, &dh);
+    double aqo = aq + auu * dq;
+    double ahr = az + acc * G * dsq(G * hr) + auu * dh + aqo * ul + ar * vl;
+    double aul = aqo * hr, avl = ar * hr;
+    /* the two characteristic-offset free surfaces */
+    double az1 = w1 ? 0.5 * ahr : 0.0, az2 = w2 ? 0.5 * ahr : 0.0;
+    *asx += az1 * dx1 + az2 * dx2;
+    *asy += az1 * dy1 + az2 * dy2;
+    double al1 = p1 ? -0.5 * dt * (nx * sx + ny * sy) * az1 : 0.0;
+    double al2 = p2 ? -0.5 * dt * (nx * sx + ny * sy) * az2 : 0.0;
+    /* eigenvalues of the cell state */
+    lam_grad(b, cl, ul, 0, &dum, &dcm);
+    lam_grad(b, cl, ul, 1, &dup, &dcp);
+    aul += al1 * dum + al2 * dup;
+    double acl = al1 * dcm + al2 * dcp;
+    double ahh = az1 + az2 + ((SMALL < h) ? acl * G * dsq(G * h) : 0.0);
+    desing_grad(ql, h, H_DESING, &dq, &dh);
+    double aql = aul * dq;
+    ahh += aul * dh;
+    desing_grad(rl, h, H_DESING, &dq, &dh);
+    double arl = avl * dq;
+    ahh += avl * dh;
+    *ah += ahh;
+    *aqx += aql * nx - arl * ny;
+    *aqy += aql * ny + arl * nx;
+}
+
+/* adjoint of e_slopes (computeMusclSlopes, swSpHypMusclEvr.cpp:11-115):
+ * (asx, asy) = adjoint of the limited slopes -> adds the adjoint of the free
+ * surface zc to azc.  The limiter visits are replayed forward (keeping the
+ * slope each visit saw), then reversed in the opposite edge / side order. */
+static void e_slopes_adj(esolver *s, double *asx, double *asy, double *azc) {
+    const orc_case *c = s->c;
+    const int64_t nc = c->n_cell, ni = c->n_int;
+    double *sx = (double *)calloc((size_t)nc, sizeof(double));
+    double *sy = (double *)calloc((size_t)nc, sizeof(double));
+    double *ws = (double *)calloc((size_t)nc, sizeof(double));
+    double *tx = (double *)calloc((size_t)(2 * ni + 1), sizeof(double));
+    double *ty = (double *)calloc((size_t)(2 * ni + 1), sizeof(double));
+    const double *z = s->zc;
+    for (int64_t i = 0; i < nc; i++) {
+        if (s->nb_ptr[i + 1] - s->nb_ptr[i] <= 1) continue;
+        for (int64_t k = s->nb_ptr[i]; k < s->nb_ptr[i + 1]; k++) {
+            int64_t j = s->nb_idx[k];
+            double dx = c->cell_x[j] - c->cell_x[i], dy = c->cell_y[j] - c->cell_y[i];
+            double dist = sqrt(dx * dx + dy * dy), w = 1.0 / dist;
+            sx[i] += (z[j] - z[i]) * dx / dist / dist * w;
+            sy[i] += (z[j] - z[i]) * dy / dist / dist * w;
+            ws[i] += w;
+        }
+        sx[i] /= ws[i];
+        sy[i] /= ws[i];
+    }
+    for (int64_t e = 0; e < ni; e++) {
+        int64_t L = c->int_L[e], R = c->int_R[e];
+        double fmx = (z[L] < z[R]) ? z[R] : z[L], fmn = (z[R] < z[L]) ? z[R] : z[L];
+        for (int sd = 0; sd < 2; sd++) {
+            int64_t k = sd ? R : L;
+            tx[2 * e + sd] = sx[k];
+            ty[2 * e + sd] = sy[k];
+            double fr = z[k] + (c->int_xm[e] - c->cell_x[k]) * sx[k] + (c->int_ym[e] - c->cell_y[k]) * sy[k];
+            double f = (fr < fmn) ? fmn : (fr > fmx) ? fmx : fr;
+            if (f != fr) {
+                double sf = (f - z[k]) / (fr - z[k]);
+                sx[k] *= sf;
+                sy[k] *= sf;
+            }
+        }
+    }
+    for (int64_t e = ni - 1; e >= 0; e--) {
+        int64_t L = c->int_L[e], R = c->int_R[e];
+        for (int sd = 1; sd >= 0; sd--) {
+            int64_t k = sd ? R : L;
+            double ax = c->int_xm[e] - c->cell_x[k], ay = c->int_ym[e] - c->cell_y[k];
+            double px = tx[2 * e + sd], py = ty[2 * e + sd];
+            double fr = z[k] + ax * px + ay * py;
+            int64_t fsrc; /* the cell whose free surface bounds the reconstruction */
+            if (fr < ((z[R] < z[L]) ? z[R] : z[L]))
+                fsrc = (z[R] < z[L]) ? R : L;
+            else if (fr > ((z[L] < z[R]) ? z[R] : z[L]))
+                fsrc = (z[L] < z[R]) ? R : L;
+            else
+                continue;
+            double D = fr - z[k], sf = (z[fsrc] - z[k]) / D;
+            double gs = asx[k] * px + asy[k] * py; /* adjoint of sf, times D / (f - z_k) */
+            asx[k] = sf * asx[k] - gs * sf / D * ax;
+            asy[k] = sf * asy[k] - gs * sf / D * ay;
+            /* sf = (z_f - z_k) / D with D = a . slope independent of z_k */
+            azc[fsrc] += gs / D;
+            azc[k] -= gs / D;
+        }
+    }
+    for (int64_t i = 0; i < nc; i++) {
+        if (s->nb_ptr[i + 1] - s->nb_ptr[i] <= 1) continue;
+        for (int64_t k = s->nb_ptr[i]; k < s->nb_ptr[i + 1]; k++) {
+            int64_t j = s->nb_idx[k];
+            double dx = c->cell_x[j] - c->cell_x[i], dy = c->cell_y[j] - c->cell_y[i];
+            double dist = sqrt(dx * dx + dy * dy), w = 1.0 / dist;
+            double a = (asx[i] * dx + asy[i] * dy) / dist / dist * w / ws[i];
+            azc[j] += a;
+            azc[i] -= a;
+        }
+    }
+    free(sx);
+    free(sy);
+    free(ws);
+    free(tx);
+    free(ty);
 }
 
 /* adjoint of hypHllc_E: df[5] -> d[12] = d(zL qL rL zR qR rR lmL lmR lpL lpR phiL phiR) */
@@ -1120,6 +1330,28 @@ void orc_e_set_explosion_guard(double ratio, int64_t stride) {
 }
 int64_t orc_e_truncation(void) { return g_first_kept; }
 
+/* Checkpoint restarts of the replay under the reverse sweep
+ * (get_spatial_gradient_checkpointed, swSpE_checkpoint.cpp:985-1030): each
+ * segment of `stride` steps is replayed from the dry run's state at its
+ * first step (states, as orc_e_dry_states writes them, kept by the caller),
+ * and the adjoint of that start state is handed to the previous segment's
+ * end state.  The cost sums the segments' observations.  Where the dry run
+ * and the replay differ (MUSCL-EVR: the reconstruction's dt) the result
+ * depends on the stride; otherwise it is the plain reverse sweep.  stride
+ * <= 0 or states NULL: off. */
+static int64_t g_ck_stride = 0;
+static const double *g_ck_states = NULL;
+
+/* Planted mistakes, for the tests that show the checks fail on them (0 =
+ * none): 1 the slopes' adjoint is dropped (the limited slope held fixed),
+ * 2 the reverse reconstructs with the previous step's dt (the dry run's). */
+static int g_plant = 0;
+void orc_e_set_plant(int code) { g_plant = code; }
+void orc_e_set_checkpoints(int64_t stride, const double *states) {
+    g_ck_stride = stride;
+    g_ck_states = states;
+}
+
 static double e_gradient(const orc_case *c, const double *phi, const double *K1, const double *K2,
                          const double *h0, const double *qx0, const double *qy0, int64_t n_steps,
                          const double *dt_seq, const double *bc_seq, const int32_t *obs_trig,
@@ -1141,7 +1373,14 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
     double J = 0.0;
     double *Jstep = (double *)calloc((size_t)(n_steps > 0 ? n_steps : 1), sizeof(double));
     g_first_kept = 0;
+    const int restart = g_ck_stride > 0 && g_ck_states != NULL;
     for (int64_t n = 0; n < n_steps; n++) {
+        if (restart && n > 0 && n % g_ck_stride == 0) { /* segment start: the dry run's state */
+            const double *ck = g_ck_states + 3 * (size_t)nc * (size_t)n;
+            memcpy(s.h, ck, sizeof(double) * nc);
+            memcpy(s.qx, ck + nc, sizeof(double) * nc);
+            memcpy(s.qy, ck + 2 * nc, sizeof(double) * nc);
+        }
         e_step(&s, dt_seq[n], bc_seq + n * ns);
         double *Un = U + 3 * (size_t)nc * (size_t)(n + 1);
         memcpy(Un, s.h, sizeof(double) * nc);
@@ -1181,12 +1420,19 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
     double *lam = (double *)calloc(3 * (size_t)nc, sizeof(double));  /* adjoint of U^{n+1} */
     double *lams = (double *)calloc(3 * (size_t)nc, sizeof(double)); /* adjoint after balance */
     double *bh = (double *)malloc(sizeof(double) * 3 * (size_t)nc);
+    const int muscl = c->scheme == ORC_MUSCL_EVR;
+    double *asx = (double *)calloc((size_t)nc, sizeof(double)); /* adjoint of the limited slopes */
+    double *asy = (double *)calloc((size_t)nc, sizeof(double));
     for (int64_t i = 0; i < nc; i++) g_phi[i] = g_K1[i] = g_K2[i] = 0.0;
     for (int64_t k = 0; k < n_steps * (ns > 0 ? ns : 1); k++) g_bc[k] = 0.0;
     for (int64_t n = n_steps - 1; n >= 0; n--) {
         const double dt = dt_seq[n];
         const double *bcn = bc_seq + n * ns;
-        const double *Un = U + 3 * (size_t)nc * (size_t)n;
+        /* start state of step n (a segment start: the checkpoint), end state
+         * of step n (the replayed one, which its observation sees) */
+        const double *Un = (restart && n > 0 && n % g_ck_stride == 0)
+                               ? g_ck_states + 3 * (size_t)nc * (size_t)n
+                               : U + 3 * (size_t)nc * (size_t)n;
         const double *Un1 = U + 3 * (size_t)nc * (size_t)(n + 1);
         int32_t oi = obs_trig[n];
         if (oi >= 0 && oi < n_obs && n_gauges > 0)
@@ -1206,11 +1452,13 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
         if (g_adj_cut > 0.0)
             for (int64_t i = 0; i < nc; i++)
                 if (Un1[i] < g_adj_cut) lam[i] = lam[nc + i] = lam[2 * nc + i] = 0.0;
-        /* recompute the step's intermediates */
+        /* recompute the step's intermediates (the replay reconstructs with
+         * the step's own dt) */
         memcpy(s.h, Un, sizeof(double) * nc);
         memcpy(s.qx, Un + nc, sizeof(double) * nc);
         memcpy(s.qy, Un + 2 * nc, sizeof(double) * nc);
-        lr(&s);
+        s.dt_recon = (g_plant == 2) ? (n > 0 ? dt_seq[n - 1] : 0.0) : dt;
+        lr_any(&s);
         fluxes(&s, bcn);
         memcpy(bh, Un, sizeof(double) * 3 * nc);
         {
@@ -1297,8 +1545,12 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
                     s.rrb[e], s.lpb[e], c->series_type[k], bcn[k], df, d);
             g_phi[id] += d[4];
             g_bc[n * ns + k] += d[5];
-            lr_adj(Un[id], Un[nc + id], Un[2 * nc + id], c->beta[id], nx, ny, c->h_min, d[0], d[1],
-                   d[2], 0.0, d[3], &lam[id], &lam[nc + id], &lam[2 * nc + id]);
+            if (muscl)
+                lr_bnd_muscl_adj(Un[id], Un[nc + id], Un[2 * nc + id], c->beta[id], nx, ny, d[0],
+                                 d[1], d[2], d[3], &lam[id], &lam[nc + id], &lam[2 * nc + id]);
+            else
+                lr_adj(Un[id], Un[nc + id], Un[2 * nc + id], c->beta[id], nx, ny, c->h_min, d[0],
+                       d[1], d[2], 0.0, d[3], &lam[id], &lam[nc + id], &lam[2 * nc + id]);
         }
         for (int64_t e = 0; e < c->n_int; e++) {
             int64_t L = c->int_L[e], R = c->int_R[e];
@@ -1326,10 +1578,26 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
                      s.lpi[2 * e + 1], df, d);
             g_phi[L] += d[10];
             g_phi[R] += d[11];
-            lr_adj(Un[L], Un[nc + L], Un[2 * nc + L], c->beta[L], nx, ny, c->h_min, d[0], d[1], d[2],
-                   d[6], d[8], &lam[L], &lam[nc + L], &lam[2 * nc + L]);
-            lr_adj(Un[R], Un[nc + R], Un[2 * nc + R], c->beta[R], nx, ny, c->h_min, d[3], d[4], d[5],
-                   d[7], d[9], &lam[R], &lam[nc + R], &lam[2 * nc + R]);
+            if (muscl) {
+                recon_adj(0, Un[L], Un[nc + L], Un[2 * nc + L], c->cell_zb[L], c->beta[L], s.dzdx[L],
+                          s.dzdy[L], nx, ny, c->int_xm[e] - c->cell_x[L], c->int_ym[e] - c->cell_y[L],
+                          s.dt_recon, d[0], d[1], d[2], d[6], d[8], &lam[L], &lam[nc + L], &lam[2 * nc + L],
+                          &asx[L], &asy[L]);
+                recon_adj(1, Un[R], Un[nc + R], Un[2 * nc + R], c->cell_zb[R], c->beta[R], s.dzdx[R],
+                          s.dzdy[R], nx, ny, c->int_xm[e] - c->cell_x[R], c->int_ym[e] - c->cell_y[R],
+                          s.dt_recon, d[3], d[4], d[5], d[7], d[9], &lam[R], &lam[nc + R], &lam[2 * nc + R],
+                          &asx[R], &asy[R]);
+            } else {
+                lr_adj(Un[L], Un[nc + L], Un[2 * nc + L], c->beta[L], nx, ny, c->h_min, d[0], d[1],
+                       d[2], d[6], d[8], &lam[L], &lam[nc + L], &lam[2 * nc + L]);
+                lr_adj(Un[R], Un[nc + R], Un[2 * nc + R], c->beta[R], nx, ny, c->h_min, d[3], d[4],
+                       d[5], d[7], d[9], &lam[R], &lam[nc + R], &lam[2 * nc + R]);
+            }
+        }
+        if (muscl) { /* slopes: free surface zc = h + zb of the cell and its neighbours */
+            if (g_plant != 1) e_slopes_adj(&s, asx, asy, lam);
+            memset(asx, 0, sizeof(double) * nc);
+            memset(asy, 0, sizeof(double) * nc);
         }
         if (guard && n % S == 0) { /* segment k = n / S reversed: lam = adjoint of U^n */
             const int64_t k = n / S;
@@ -1359,6 +1627,8 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
         }
     }
     free(wm);
+    free(asx);
+    free(asy);
     free(Jstep);
     free(U);
     free(lam);

@@ -1,34 +1,55 @@
-"""MUSCL-EVR _E on the GPU (k_e_slopes + k_e_edges<true>): the dry run
-against the reference's own _E physics is in test_gpu_adjoint.py (the
-e_*_muscl_* fixtures); here the frozen replay -- which reconstructs with the
-step's own dt, as the reference's taped segment does
-(swSpE_checkpoint.cpp:658-659), while the dry run used the previous step's --
-against the CPU oracle's replay on the recorded schedule: cost within 1e-12
-where no transcendental function is on the path (closed boxes without
-friction), within 1e-4 where the inflow's exp/log and the friction's cbrt
-differ from glibc's by an ulp and the second-order scheme amplifies it
-(CHAOTIC in test_gpu_adjoint.py)."""
+"""MUSCL-EVR _E on the GPU: forward (k_e_slopes + k_e_edges<true>) and its
+adjoint (k_r_edge<true> + k_r_slopes + k_r_bnd<true>).
+
+The dry run against the reference's own _E physics is in test_gpu_adjoint.py
+(the e_*_muscl_* fixtures).  Here:
+- the frozen replay -- which reconstructs with the step's own dt, as the
+  reference's taped segment does (swSpE_checkpoint.cpp:658-659), while the
+  dry run used the previous step's -- against the CPU oracle's replay on the
+  recorded schedule: cost within 1e-12 where no transcendental function is on
+  the path (closed boxes without friction), within 1e-4 where the inflow's
+  exp/log and the friction's cbrt differ from glibc's by an ulp and the
+  second-order scheme amplifies it (CHAOTIC in test_gpu_adjoint.py);
+- the gradient against the oracle's reverse sweep (itself checked against
+  central differences on the CPU, slope limiter included) on the same
+  recorded schedule, every control kind; for the CHAOTIC cases the gauges
+  observe the first 8 s, before the 1-ulp differences have grown;
+- the gradient against central differences of the GPU's own replay cost
+  (one segment covers the window: the gradient is then that of the
+  continuous replay the cost runs);
+- over several segments, each replayed from its dry-run checkpoint, against
+  the oracle with the same restarts.
+"""
 import numpy as np
 import pytest
 
-from conftest import e_golden_names, regen_case, rel_err
+from conftest import e_golden_names, regen_case
 from test_gpu_adjoint import CHAOTIC
 
 pytestmark = pytest.mark.gpu
 MUSCL = [n for n in e_golden_names() if "muscl" in n]
 
 
+def _gauges(case, n_obs):
+    h0 = case.a["h0"]
+    cells = np.nonzero(h0 > 0.1)[0][::5][:8].astype(np.int64)
+    obs = np.array([[h0[c] * (1.0 + 0.1 * (k + 1)) for c in cells] for k in range(n_obs)])
+    return cells, obs
+
+
+def _open(sw, tmp_path, name):
+    g = regen_case(sw, name, str(tmp_path))
+    case = sw.Case(str(tmp_path))
+    return g, case, sw.SpSolver(case)
+
+
 @pytest.mark.parametrize("name", MUSCL)
 def test_gpu_muscl_replay_cost_matches_oracle(sw, oracle, tmp_path, name):
     from sw2d_amd import assim
-    g = regen_case(sw, name, str(tmp_path))
-    case = sw.Case(str(tmp_path))
-    s = sw.SpSolver(case)
+    g, case, s = _open(sw, tmp_path, name)
     print(f"[gpu] muscl _E replay {name}", flush=True)
     obs_t = list(g["obs_t"])
-    h0 = case.a["h0"]
-    cells = np.nonzero(h0 > 0.1)[0][::5][:8].astype(np.int64)
-    obs = np.array([[h0[c] * (1.0 + 0.1 * (k + 1)) for c in cells] for k in range(len(obs_t))])
+    cells, obs = _gauges(case, len(obs_t))
     A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs)
     A.record()
     rec = A.get_record()
@@ -38,7 +59,109 @@ def test_gpu_muscl_replay_cost_matches_oracle(sw, oracle, tmp_path, name):
                                obs=obs, n_obs=len(obs_t))
     tol = 1e-4 if name in CHAOTIC else 1e-12
     assert abs(J - Jo) <= tol * abs(Jo), (J, Jo)
-    # the gradient refuses MUSCL-EVR until its adjoint is built
-    with pytest.raises(Exception):
-        A.gradient()
+    s.close()
+
+
+@pytest.mark.parametrize("name", MUSCL)
+def test_gpu_muscl_gradient_matches_oracle_reverse_sweep(sw, oracle, tmp_path, name):
+    """max |gpu - oracle| / max |oracle| <= 1e-10 per control kind (the two
+    differ in summation order and, on the CHAOTIC cases, by 1-ulp
+    transcendentals)."""
+    from sw2d_amd import assim
+    g, case, s = _open(sw, tmp_path, name)
+    print(f"[gpu] muscl gradient vs oracle {name}", flush=True)
+    obs_t = [2.0, 4.0, 6.0, 8.0] if name in CHAOTIC else list(g["obs_t"])
+    cells, obs = _gauges(case, len(obs_t))
+    A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs, normalize=True)
+    G = A.gradient()
+    rec = A.get_record()
+    E = oracle.OracleE(case)
+    O = E.gradient({"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}, gauges=cells, obs=obs,
+                   normalize=True, n_obs=len(obs_t))
+    print(f"[gpu]   J {G['J']!r} oracle {O['J']!r}", flush=True)
+    assert abs(G["J"] - O["J"]) <= 1e-12 * abs(O["J"])
+    for key in ("phi", "K1", "K2", "bc"):
+        scale = np.max(np.abs(O[key])) if O[key].size else 0.0
+        if scale == 0.0:
+            assert not np.any(G[key])
+            continue
+        e = np.max(np.abs(G[key] - O[key])) / scale
+        print(f"[gpu]   {key} rel {e:.3e}", flush=True)
+        assert e <= 1e-10, (key, e)
+    s.close()
+
+
+def test_gpu_muscl_gradient_vs_central_fd(sw, tmp_path):
+    """<dJ/dx, v> against central differences of the GPU replay cost on the
+    recorded schedule: porosity, both Strickler coefficients and each
+    boundary series (q inflow, fr outlet, h and z levels)."""
+    from sw2d_amd import assim
+    g, case, s = _open(sw, tmp_path, "e_mixed_muscl_strickler")
+    print("[gpu] muscl gradient vs fd", flush=True)
+    obs_t = list(g["obs_t"])
+    cells, obs = _gauges(case, len(obs_t))
+    A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs, normalize=True)
+    G = A.gradient()
+    # central differences carry ~ 1e-16 |J| / e of round-off
+    floor = lambda e: 4e-16 * abs(G["J"]) / e
+    rng = np.random.default_rng(3)
+    base = {"phi": case.a["phi"], "K1": case.a["fc1"], "K2": case.a["fc2"]}
+    open_ = case.a["phi"] > 0.05
+    for key, x0 in base.items():
+        v = np.where(open_, rng.standard_normal(case.n_cell), 0.0) * np.maximum(np.abs(x0), 1e-3)
+        e = 1e-7
+        fd = (A.cost(**{key: x0 + e * v}) - A.cost(**{key: x0 - e * v})) / (2 * e)
+        ad = float(np.dot(G[key], v))
+        print(f"[gpu]   {key} fd {fd!r} adj {ad!r}", flush=True)
+        assert abs(fd - ad) <= 1e-6 * abs(ad) + floor(e), (key, fd, ad)
+    bc0 = A.get_record()["bc"].copy()
+    for series in (1, 2, 3, 4):
+        v = np.zeros_like(bc0)
+        v[:, series] = np.linspace(0.0, 1.0, len(bc0))
+        e = 1e-6
+        A.set_bc_seq(bc0 + e * v)
+        Jp = A.cost()
+        A.set_bc_seq(bc0 - e * v)
+        Jm = A.cost()
+        A.set_bc_seq(bc0)
+        fd = (Jp - Jm) / (2 * e)
+        ad = float(np.sum(G["bc"] * v))
+        print(f"[gpu]   bc{series} fd {fd!r} adj {ad!r}", flush=True)
+        assert abs(fd - ad) <= 1e-6 * abs(ad) + floor(e), (series, fd, ad)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["e_dambreak_muscl", "e_mixed_muscl_strickler"])
+def test_gpu_muscl_gradient_segments_restart_from_dry_run_states(sw, oracle, tmp_path, name):
+    """With several segments each one is replayed from its dry-run checkpoint
+    (reconstructed with the previous step's dt), so the MUSCL-EVR gradient
+    depends on the stride, as the reference's does: stride 5 against the
+    oracle's reverse sweep with the same restarts (orc_e_set_checkpoints, the
+    dry run's states on the GPU's schedule); and deterministic."""
+    from sw2d_amd import assim
+    g, case, s = _open(sw, tmp_path, name)
+    print(f"[gpu] muscl gradient over segments {name}", flush=True)
+    obs_t = list(g["obs_t"])
+    cells, obs = _gauges(case, len(obs_t))
+    A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs, normalize=True, stride=5)
+    G = A.gradient()
+    G2 = A.gradient()
+    for key in ("phi", "K1", "K2", "bc"):
+        np.testing.assert_array_equal(G[key], G2[key])
+    rec = A.get_record()
+    rec = {"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}
+    E = oracle.OracleE(case)
+    S = E.dry_states(rec)
+    O = E.gradient(rec, gauges=cells, obs=obs, normalize=True, n_obs=len(obs_t), checkpoints=(5, S))
+    O1 = E.gradient(rec, gauges=cells, obs=obs, normalize=True, n_obs=len(obs_t))
+    print(f"[gpu]   J {G['J']!r} oracle {O['J']!r} (one segment {O1['J']!r})", flush=True)
+    assert abs(G["J"] - O["J"]) <= 1e-12 * abs(O["J"])
+    for key in ("phi", "K1", "K2", "bc"):
+        scale = np.max(np.abs(O[key])) if O[key].size else 0.0
+        if scale == 0.0:
+            continue
+        e = np.max(np.abs(G[key] - O[key])) / scale
+        moved = np.max(np.abs(O1[key] - O[key])) / scale
+        print(f"[gpu]   {key} rel {e:.3e} (restarts move it by {moved:.3e})", flush=True)
+        assert e <= 1e-10, (key, e)
     s.close()
