@@ -20,6 +20,13 @@ All inputs are resident in HBM when the timed region starts; each call
 copies the 3 x n_cell control arrays in and the gradients out (<0.1 % of
 the time).
 
+The same gradient with the MUSCL-EVR _E forward (limited free-surface
+slopes, desingularised velocities) and its adjoint is reported under
+"fwd_adj_muscl_evr" (or the Godunov one there with --scheme muscl_evr).
+The MUSCL-EVR adjoint grows through this window's wet/dry fronts until the
+reference's explosion guard (ratio 1e8) would stop the reverse sweep part
+way; that line is timed with the guard off so that every time step is
+reversed, and the guarded gradient's first kept step is reported beside it.
 The native (non-differentiable) MUSCL-EVR forward on the same mesh is timed
 as well and reported under "forward".
 
@@ -38,6 +45,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(REPO, "inria-internship-project-sw2d-enzyme_amd")
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E vendor peak
 METRIC = "FP64 cell-steps/s fwd+adj (%HBM roofline); 4D-Var time-to-misfit learned vs plain"
+SCHEME_TEXT = {"godunov": "Godunov/HLLC", "muscl_evr": "MUSCL-EVR (limited free-surface slopes)/HLLC"}
 
 
 def load_pkg():
@@ -67,16 +75,24 @@ def pass_bytes(nc, ni, nb, scheme):
     return 80 * nc + 32 * ni + 32 * nb, 120 * nc + 32 * ni + 32 * nb
 
 
-def kernel_bytes(name, nc, ni, nb, n_gauges):
+def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
     """Compulsory bytes of one launch of each kernel of the gradient (every
     array element the launch touches, read once / written once; int32
-    indices, FP64 values; friction on, head loss off)."""
+    indices, FP64 values; friction on, head loss off).  <true> variants are
+    the MUSCL-EVR ones."""
+    muscl = scheme == "muscl_evr"
     table = {
         # cells U, zb, beta, phi; internal edges L, R, nx, ny, F out (+ length and
         # swept out in the dry run); boundary edges cell, nx, ny, series, F out
-        "k_e_edges": 48 * nc + 64 * ni + 48 * nb,
+        "k_e_edges<false>": 48 * nc + 64 * ni + 48 * nb,
+        # + cell slopes and centroids, edge midpoints
+        "k_e_edges<true>": 80 * nc + 80 * ni + 48 * nb,
+        # cells U(h), zb, centroid, CSR ptr, slopes out; incidences, edge cells
+        # and midpoints
+        "k_e_slopes": 52 * nc + 32 * ni,
         # cells U, zb, beta, area, CSR ptrs; incidences + swept; boundary edges
-        "k_e_cfl": 56 * nc + 24 * ni + 28 * nb,
+        "k_e_cfl<false>": 56 * nc + 24 * ni + 28 * nb,
+        "k_e_cfl<true>": 56 * nc + 24 * ni + 28 * nb,
         "k_e_dt": 8 * ((nc + 255) // 256),
         # cells U in/out, phi, area, K1, K2, cos, sin, ptrs; incidences, F, edge
         # geometry; boundary incidences, geometry and F
@@ -84,13 +100,20 @@ def kernel_bytes(name, nc, ni, nb, n_gauges):
         # + balanced state and internal increments out
         "k_e_update<true>": 152 * nc + 72 * ni + 52 * nb,
         # cells part in/out, ptrs, P in, phi, K1, K2, cos, sin, lams out,
-        # gphi/gK1/gK2 read-modify-write; incidences + edge sides
-        "k_r_gc<true>": 216 * nc + 72 * ni,
-        "k_r_gc<false>": 80 * nc + 72 * ni,
+        # gphi/gK1/gK2 read-modify-write; incidences + edge sides (+ the
+        # neighbours' slope terms, MUSCL)
+        "k_r_gc<true>": 216 * nc + (88 if muscl else 72) * ni,
+        "k_r_gc<false>": 80 * nc + (88 if muscl else 72) * ni,
         # boundary cells: state, statics, lams, part, gphi; edges geometry, F, gbce
-        "k_r_bnd": 160 * nb,
+        "k_r_bnd<false>": 160 * nb,
+        "k_r_bnd<true>": 160 * nb,
         # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, length; sides out
-        "k_r_edge": 80 * nc + 96 * ni,
+        "k_r_edge<false>": 80 * nc + 96 * ni,
+        # + cell slopes and centroids, edge midpoints, slope adjoints out
+        "k_r_edge<true>": 112 * nc + 144 * ni,
+        # cells U(h), zb, centroid, ptr, part(h) in/out; incidences, slope
+        # adjoints in, neighbour terms out, edge cells and midpoints
+        "k_r_slopes": 52 * nc + 80 * ni,
         "k_gauge_sample": 16 * n_gauges,
         "k_cost_gauges": 32 * n_gauges,
         "k_cost_total": 64,
@@ -98,13 +121,16 @@ def kernel_bytes(name, nc, ni, nb, n_gauges):
     return table.get(name)
 
 
-def fwd_adj_bench(sw, args):
+def fwd_adj_bench(sw, args, scheme, explosion_ratio=1e8):
+    """Gradients on the calibration workload; explosion_ratio = 0 times the
+    whole reverse sweep even where the guard would stop it (the guarded
+    gradient's first kept step is reported as well)."""
     from sw2d_amd import assim, calibration
     t0 = time.perf_counter()
-    cal = calibration.Calibration(px=args.px, window=args.window)
+    cal = calibration.Calibration(px=args.px, window=args.window, scheme=scheme)
     case = cal.case
     nc, ni, nb = case.n_cell, case.n_int, case.n_bnd
-    log(f"calibration mesh {nc} cells, {ni} internal edges, {nb} boundary edges "
+    log(f"{scheme}: calibration mesh {nc} cells, {ni} internal edges, {nb} boundary edges "
         f"(built in {time.perf_counter() - t0:.1f} s)")
     s = sw.SpSolver(case, device=0)
     # twin run at the true controls -> gauge observations
@@ -118,8 +144,12 @@ def fwd_adj_bench(sw, args):
     # gradient at the default controls
     cal.apply_hydrograph(s, cal.Q_default)
     K0 = cal.K(cal.n_default)
+    guarded = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=obs,
+                          max_steps=args.max_steps)
+    guarded.gradient(K1=K0, K2=K0)
+    kept = guarded.truncation()[0]
     A = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=obs,
-                    max_steps=args.max_steps)
+                    max_steps=args.max_steps, explosion_ratio=explosion_ratio)
     for _ in range(args.warmup):
         G = A.gradient(K1=K0, K2=K0)
     t0 = time.perf_counter()
@@ -129,16 +159,24 @@ def fwd_adj_bench(sw, args):
     n_steps = G["n_steps"]
     rec = A.get_record()
     gn, gQ = cal.grad_controls(G, cal.n_default, rec["t"][:-1])
-    log(f"{args.steps} gradients x {n_steps} time steps in {elapsed:.3f} s; J = {G['J']:.6e}, "
+    if A.truncation()[0] != 0:
+        raise RuntimeError("the timed gradients did not reverse every time step")
+    log(f"{scheme}: guarded gradient keeps steps {kept}..{n_steps - 1} (explosion guard, ratio 1e8)")
+    log(f"{scheme}: {args.steps} gradients x {n_steps} time steps in {elapsed:.3f} s; J = {G['J']:.6e}, "
         f"|dJ/dn| = {np.linalg.norm(gn):.3e}, |dJ/dQ| = {np.linalg.norm(gQ):.3e}")
-    prof = A.profile(K1=K0, K2=K0)
-    out = dict(cal=cal, case=case, solver=s, assim=A, n_steps=n_steps, elapsed=elapsed,
+    prof = None
+    if not args.no_profile:
+        log(f"{scheme}: per-launch timing of one gradient (HIP events)")
+        prof = A.profile(K1=K0, K2=K0)
+    out = dict(scheme=scheme, guard_first_kept_step=kept, explosion_ratio=explosion_ratio,
+               cal=cal, case=case, solver=s, assim=A, n_steps=n_steps, elapsed=elapsed,
                J=G["J"], prof=prof, rec=rec, obs=obs, K0=K0, gn=gn, gQ=gQ,
                stride=int(np.ceil(np.sqrt(args.max_steps))))
     return out
 
 
 def roofline(res, args):
+    scheme = res["scheme"]
     case = res["case"]
     nc, ni, nb = case.n_cell, case.n_int, case.n_bnd
     n_steps = res["n_steps"]
@@ -147,20 +185,20 @@ def roofline(res, args):
     total_ms = 0.0
     for name, (avg, cnt) in res["prof"].items():
         total_ms += avg * cnt
-        b = kernel_bytes(name, nc, ni, nb, ng)
+        b = kernel_bytes(name, nc, ni, nb, ng, scheme)
         kernels[name] = {"avg_ms": avg, "launches": cnt, "bytes_per_launch": b,
                          "gbs": (b / (avg * 1e-3) / 1e9) if (b and avg > 0) else None}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     d = kernels[dom]
     achieved = d["gbs"]
-    Bf, Ba = pass_bytes(nc, ni, nb, "godunov")
+    Bf, Ba = pass_bytes(nc, ni, nb, scheme)
     step_ms = total_ms / n_steps  # device time per time step (fwd + recompute + adj)
     step_gbs = (Bf + Ba) / (step_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(REPO, "profiles", "traffic_fwd_adj_1m.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get(scheme, {}).get(dom)
     return {
         "bound": "hbm",
         "kernel": dom,
@@ -362,6 +400,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forward", action="store_true")
+    ap.add_argument("--scheme", choices=("muscl_evr", "godunov"), default="godunov",
+                    help="reconstruction of the headline fwd+adj line")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the per-launch event timing and the roofline (for counter runs)")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the fwd+adj line of the other reconstruction")
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched-ensemble leg instead")
     ap.add_argument("--ens-runs", type=int, default=256)
@@ -400,7 +444,12 @@ def main():
     if args.ensemble:
         print(json.dumps(ensemble_bench(sw, args)))
         return
-    res = fwd_adj_bench(sw, args)
+    ratio = lambda sc: 0.0 if sc == "muscl_evr" else 1e8
+    res = fwd_adj_bench(sw, args, args.scheme, ratio(args.scheme))
+    if args.no_profile:  # timing only (runs under a counter profiler)
+        print(json.dumps({"scheme": args.scheme, "value": res["case"].n_cell * res["n_steps"] * args.steps
+                          / res["elapsed"], "unit": "fwd+adj cell-steps/s"}))
+        return
     roof, kernels = roofline(res, args)
     case = res["case"]
     nc = case.n_cell
@@ -423,16 +472,32 @@ def main():
         "config": {
             "workload": "adjoint gradient on the 1M mesh: 8 Manning zones + 60-knot inflow "
                         "hydrograph (68 controls), 20 gauges every 30 s, 90 s window, "
-                        "sqrt(N) checkpointing, Godunov/HLLC _E forward",
+                        "sqrt(N) checkpointing, " + SCHEME_TEXT[args.scheme] + " _E forward",
+            "scheme": args.scheme,
             "n_cell": nc, "n_int": case.n_int, "n_bnd": case.n_bnd,
             "time_steps_per_gradient": n_steps,
             "checkpoint_stride": res["stride"],
+            "explosion_guard_ratio": res["explosion_ratio"],
+            "guarded_gradient_first_kept_step": res["guard_first_kept_step"],
             "cost_J": res["J"],
         },
         "roofline": roof,
         "kernels": kernels,
     }
     res["solver"].close()
+    if not args.no_other:  # the other reconstruction on the same workload
+        other = "godunov" if args.scheme == "muscl_evr" else "muscl_evr"
+        r2 = fwd_adj_bench(sw, args, other, ratio(other))
+        roof2, k2 = roofline(r2, args)
+        line["fwd_adj_" + other] = {
+            "value": r2["case"].n_cell * r2["n_steps"] * args.steps / r2["elapsed"],
+            "unit": "fwd+adj cell-steps/s", "ms_per_step": r2["elapsed"] * 1e3 / args.steps,
+            "time_steps_per_gradient": r2["n_steps"], "cost_J": r2["J"],
+            "explosion_guard_ratio": r2["explosion_ratio"],
+            "guarded_gradient_first_kept_step": r2["guard_first_kept_step"],
+            "workload": "the same, " + SCHEME_TEXT[other] + " _E forward",
+            "roofline": roof2, "kernels": k2}
+        r2["solver"].close()
     if not args.no_forward:
         line["forward"] = forward_bench(sw, args)
     if not args.no_cpu:

@@ -92,14 +92,15 @@ class Calibration:
     t_knots, turned into unit discharge on the west edges)."""
 
     def __init__(self, px=707, seed=1234, n_zones=8, n_knots=60, n_gauges=20, obs_dt=30.0,
-                 window=90.0, q_peak=500.0, t_event=3600.0, truth_seed=None, level_quantile=0.4):
+                 window=90.0, q_peak=500.0, t_event=3600.0, truth_seed=None, level_quantile=0.4,
+                 scheme="godunov"):
         self.window = float(window)
         self.t_knots = np.linspace(0.0, self.window, n_knots)
         # default hydrograph: the rising limb of the event's triangular
         # hydrograph (0 -> q_peak over t_event / 2) on the knots
         base = q_peak * np.minimum(self.t_knots / (0.5 * t_event), 1.0) + 5.0
         self.case, info = scale_case(None, px=px, py=px, L=L_DOMAIN, seed=seed, t_max=self.window,
-                                     scheme="godunov", friction="strickler",
+                                     scheme=scheme, friction="strickler",
                                      manning=1.0 / 0.03, q_peak=q_peak,
                                      hydrograph=(self.t_knots, base))
         self.info = info

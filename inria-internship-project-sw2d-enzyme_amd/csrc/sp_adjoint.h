@@ -247,41 +247,29 @@ SPD void recon_muscl_vjp(int side, double h, double qx, double qy, double zb, do
 // of the neighbour, xm(k), ym(k) of the edge midpoint and side(k) (1: the
 // cell is the edge's right cell).
 // ---------------------------------------------------------------------------
-template <class Nb>
-SPD void slope_raw(const Nb &nb, int deg, double zi, double xi, double yi, double &sx, double &sy,
-                   double &wsum) {
-    sx = 0.0;
-    sy = 0.0;
-    wsum = 0.0;
-    if (deg <= 1) return;
-    for (int k = 0; k < deg; ++k) {
-        const double zj = nb.zj(k);
-        const double dx = nb.xj(k) - xi;
-        const double dy = nb.yj(k) - yi;
-        const double dist = sqrt(dx * dx + dy * dy);
-        const double w = 1.0 / dist;
-        sx += (zj - zi) * dx / dist / dist * w;
-        sy += (zj - zi) * dy / dist / dist * w;
-        wsum += w;
-    }
-    sx /= wsum;
-    sy /= wsum;
+// one neighbour's term of the raw slope (dx, dy = neighbour centre minus
+// the cell's)
+SPD void slope_raw_term(double zi, double zj, double dx, double dy, double &sx, double &sy,
+                        double &wsum) {
+    const double dist = sqrt(dx * dx + dy * dy);
+    const double w = 1.0 / dist;
+    sx += (zj - zi) * dx / dist / dist * w;
+    sy += (zj - zi) * dy / dist / dist * w;
+    wsum += w;
 }
 
-// limiter visit of incidence k: scales (sx, sy) when the reconstruction at
-// the edge midpoint leaves [min, max] of the two free surfaces.  Returns the
-// cell whose surface bounds it (0 this cell, 1 the neighbour; -1 no
-// scaling), with D = fr - zi and the scale sf.
-template <class Nb>
-SPD int slope_visit(const Nb &nb, int k, double zi, double xi, double yi, double &sx, double &sy,
-                    double &D, double &sf) {
-    const double zj = nb.zj(k);
-    const int s = nb.side(k);
+// limiter visit of one incidence (s = the cell's side, ax / ay = edge
+// midpoint minus the cell's centre): scales (sx, sy) when the
+// reconstruction at the midpoint leaves [min, max] of the two free
+// surfaces.  Returns the cell whose surface bounds it (0 this cell, 1 the
+// neighbour; -1 no scaling), with D = fr - zi and the scale sf.
+SPD int slope_visit_term(double zi, double zj, int s, double ax, double ay, double &sx, double &sy,
+                         double &D, double &sf) {
     const double zL = s ? zj : zi;
     const double zR = s ? zi : zj;
     const double fmx = sp::smax(zL, zR);
     const double fmn = sp::smin(zL, zR);
-    const double fr = zi + (nb.xm(k) - xi) * sx + (nb.ym(k) - yi) * sy;
+    const double fr = zi + ax * sx + ay * sy;
     int src = -1;
     if (fr < fmn) {
         sf = (fmn - zi) / (fr - zi);
@@ -298,6 +286,24 @@ SPD int slope_visit(const Nb &nb, int k, double zi, double xi, double yi, double
         src = ((zL < zR) == (s == 0)) ? 1 : 0;  // fmx is zR when zL < zR
     }
     return src;
+}
+
+template <class Nb>
+SPD void slope_raw(const Nb &nb, int deg, double zi, double xi, double yi, double &sx, double &sy,
+                   double &wsum) {
+    sx = 0.0;
+    sy = 0.0;
+    wsum = 0.0;
+    if (deg <= 1) return;
+    for (int k = 0; k < deg; ++k) slope_raw_term(zi, nb.zj(k), nb.xj(k) - xi, nb.yj(k) - yi, sx, sy, wsum);
+    sx /= wsum;
+    sy /= wsum;
+}
+
+template <class Nb>
+SPD int slope_visit(const Nb &nb, int k, double zi, double xi, double yi, double &sx, double &sy,
+                    double &D, double &sf) {
+    return slope_visit_term(zi, nb.zj(k), nb.side(k), nb.xm(k) - xi, nb.ym(k) - yi, sx, sy, D, sf);
 }
 
 template <class Nb>
@@ -345,6 +351,78 @@ SPD void slope_vjp(const Nb &nb, int deg, double zi, double xi, double yi, doubl
         out.add(k, a);
         dzi -= a;
     }
+}
+
+// slope_vjp for cells of at most MAXD incidences, with the incidences' data
+// and the slope each visit saw held in registers (one forward replay); each
+// neighbour's term is passed once, through out.put(k, v).  Returns false,
+// doing nothing, when deg > MAXD.
+template <int MAXD, class Nb, class Out>
+SPD bool slope_vjp_reg(const Nb &nb, int deg, double zi, double xi, double yi, double gx, double gy,
+                       double &dzi, const Out &out) {
+    if (deg > MAXD) return false;
+    double zj[MAXD], dx[MAXD], dy[MAXD], ax[MAXD], ay[MAXD], px[MAXD], py[MAXD], D[MAXD], sf[MAXD];
+    int side[MAXD], src[MAXD];
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k) {
+        if (k < deg) {
+            zj[k] = nb.zj(k);
+            dx[k] = nb.xj(k) - xi;
+            dy[k] = nb.yj(k) - yi;
+            ax[k] = nb.xm(k) - xi;
+            ay[k] = nb.ym(k) - yi;
+            side[k] = nb.side(k);
+        }
+    }
+    double sx = 0.0, sy = 0.0, wsum = 0.0;
+    if (deg > 1) {
+#pragma unroll
+        for (int k = 0; k < MAXD; ++k)
+            if (k < deg) slope_raw_term(zi, zj[k], dx[k], dy[k], sx, sy, wsum);
+        sx /= wsum;
+        sy /= wsum;
+    }
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k) {
+        if (k < deg) {
+            px[k] = sx;
+            py[k] = sy;
+            D[k] = 1.0;
+            sf[k] = 1.0;
+            src[k] = slope_visit_term(zi, zj[k], side[k], ax[k], ay[k], sx, sy, D[k], sf[k]);
+        }
+    }
+    double zn[MAXD];
+#pragma unroll
+    for (int k = MAXD - 1; k >= 0; --k) {
+        if (k < deg) {
+            zn[k] = 0.0;
+            if (src[k] >= 0) {
+                const double gs = gx * px[k] + gy * py[k];
+                const double c = gs * sf[k] / D[k];
+                gx = sf[k] * gx - c * ax[k];
+                gy = sf[k] * gy - c * ay[k];
+                if (src[k] == 1) {
+                    zn[k] = gs / D[k];
+                    dzi -= zn[k];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k) {
+        if (k < deg) {
+            if (deg > 1) {
+                const double dist = sqrt(dx[k] * dx[k] + dy[k] * dy[k]);
+                const double w = 1.0 / dist;
+                const double a = (gx * dx[k] + gy * dy[k]) / dist / dist * w / wsum;
+                zn[k] += a;
+                dzi -= a;
+            }
+            out.put(k, zn[k]);
+        }
+    }
+    return true;
 }
 
 // adjoint of lr_fwd from its intermediates: (dz, dq, dr, dlm, dlp) ->
@@ -570,10 +648,11 @@ SPD void hllc_vjp(double h_min, double phi_min, double bL, double bR, double phi
             dL2 += dden;
             dL1 -= dden;
         }
-        if (s1 == 1) d[6] += dL1;
-        if (s1 == 2) d[7] += dL1;
-        if (s2s == 1) d[8] += dL2;
-        if (s2s == 2) d[9] += dL2;
+        // selected by value, not by index, so d stays in registers
+        d[6] += (s1 == 1) ? dL1 : 0.0;
+        d[7] += (s1 == 2) ? dL1 : 0.0;
+        d[8] += (s2s == 1) ? dL2 : 0.0;
+        d[9] += (s2s == 2) ? dL2 : 0.0;
         // velocities
         d[1] += duL * ihL;
         dhLs += -duL * qL * (ihL * ihL);

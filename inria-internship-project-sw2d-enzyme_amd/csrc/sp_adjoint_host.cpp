@@ -123,6 +123,7 @@ struct HostOut {
     double *d;
     void set(int k, double x) const { d[k] = x; }
     void add(int k, double x) const { d[k] += x; }
+    void put(int k, double x) const { d[k] = x; }
 };
 // c = {zi, xi, yi}; out = {sx, sy}
 void sw2dt_slope_fwd(const double *c, const double *nb, int deg, double *out) {
@@ -134,6 +135,15 @@ void sw2dt_slope_vjp(const double *c, const double *nb, int deg, const double *g
     HostOut o{d + 1};
     spe::slope_vjp(HostNb{nb}, deg, c[0], c[1], c[2], g[0], g[1], dzi, o);
     d[0] = dzi;
+}
+// the same through the register variant (cells of at most 6 incidences);
+// returns 0 when deg is beyond that
+int sw2dt_slope_vjp_reg(const double *c, const double *nb, int deg, const double *g, double *d) {
+    double dzi = 0.0;
+    const HostOut o{d + 1};
+    const bool ok = spe::slope_vjp_reg<6>(HostNb{nb}, deg, c[0], c[1], c[2], g[0], g[1], dzi, o);
+    d[0] = dzi;
+    return ok ? 1 : 0;
 }
 
 }  // extern "C"

@@ -87,7 +87,8 @@ struct AssimState {
     // adjoint
     double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
     double *d_sd = nullptr;    // MUSCL-EVR: 4 n_int slope adjoints per edge side
-    double *d_zn = nullptr;    // MUSCL-EVR: 2 n_int neighbour free-surface adjoints
+    double *d_zn = nullptr;    // MUSCL-EVR: 2 n_int neighbour free-surface adjoints (CSR slots)
+    int *d_mirror = nullptr;   // MUSCL-EVR: CSR slot of the other cell's incidence on the same edge
     double *d_gphi = nullptr, *d_gK1 = nullptr, *d_gK2 = nullptr, *d_gbce = nullptr,
            *d_gbc = nullptr;
     // change of variable of the porosity control (phi_correct / dphi_dm,
@@ -678,8 +679,8 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
 //   seed    + dJ_obs/dU^n when U^n is observed
 //   CELL    reverse of the cell part of step n-1 (else: write lam = adj U^n)
 // gather: -1 nothing to gather (the last step, or a tile top: the future
-// adjoint is dropped), 1 gather part + edge sides (+ zn, MUSCL-EVR: the
-// neighbours' slope adjoints through this cell's free surface), 2 take lam
+// adjoint is dropped), 1 gather part + edge sides (+ zn[mirror], MUSCL-EVR:
+// the neighbours' slope adjoints through this cell's free surface), 2 take lam
 // (the gather already done by the CELL = false launch that closed the next
 // segment).
 // Blocks past n_cell_blocks sum the boundary-value adjoints of step n per
@@ -694,7 +695,8 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
                                                  double *gK1, double *gK2, int n_cell_blocks,
                                                  const int *ser_ptr, const int *ser_edges,
                                                  const double *gbce, double *gbc_n,
-                                                 const double *__restrict__ zn) {
+                                                 const double *__restrict__ zn,
+                                                 const int *__restrict__ mirror) {
     if ((int)blockIdx.x >= n_cell_blocks) {
         __shared__ double red[kBlock];
         const int k = (int)blockIdx.x - n_cell_blocks;
@@ -727,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
             b += sv[1];
             c += sv[2];
             gp += sv[3];
-            if (zn) a += zn[inc ^ 1];  // the neighbour's slope through this cell's surface
+            if (zn) a += zn[mirror[k]];  // the neighbour's slope through this cell's surface
         }
         if (gather > 0) {  // not a tile top
             ah = a;
@@ -802,12 +804,21 @@ __global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs
     gphi[i] += gp;
 }
 
+// an opaque copy of x: expressions of the copy are not merged with the same
+// expressions of x evaluated earlier (an empty asm statement, no instruction)
+__device__ __forceinline__ double fresh(double x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // per internal edge: flux adjoints from both sides, HLLC_E reverse, then the
 // reverse of the Godunov _E states, or (MUSCL) of the MUSCL-EVR
 // reconstruction of each side from the step's slopes sxy with the step's dt,
 // whose slope adjoints go to sd[4e ..] = {L: dsx, dsy, R: dsx, dsy}
+// register budgets: 4 waves per SIMD (128 VGPRs) fit the Godunov reverse
+// without spills, the MUSCL-EVR one needs 144 (3 waves)
 template <bool MUSCL>
-__global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls cs,
+__global__ __launch_bounds__(kBlock, MUSCL ? 3 : 4) void k_r_edge(MeshDev m, Opts o, Controls cs,
                                                    const double *__restrict__ U, double dt,
                                                    const double *__restrict__ lams,
                                                    double *__restrict__ side_out,
@@ -839,10 +850,15 @@ __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls c
         spe::hllc_vjp(o.h_min, o.phi_min, bL, bR, phiL, phiR, m.zb[L], m.zb[R], sL.z, sL.q, sL.r, sR.z,
                       sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, dF, d);
         double gxL = 0, gyL = 0, gxR = 0, gyR = 0;
-        spe::recon_muscl_vjp(0, hL, qxL, qyL, m.zb[L], bL, sxL, syL, nx, ny, dxL, dyL, dt, d[0], d[1],
-                             d[2], d[6], d[8], dhL, dqxL, dqyL, gxL, gyL);
-        spe::recon_muscl_vjp(1, hR, qxR, qyR, m.zb[R], bR, sxR, syR, nx, ny, dxR, dyR, dt, d[3], d[4],
-                             d[5], d[7], d[9], dhR, dqxR, dqyR, gxR, gyR);
+        // the VJPs recompute the reconstruction from fresh copies of its
+        // inputs instead of keeping its intermediates live across hllc_vjp
+        // (216 -> 144 VGPRs, 2 -> 3 waves per SIMD)
+        spe::recon_muscl_vjp(0, fresh(hL), fresh(qxL), fresh(qyL), fresh(m.zb[L]), bL, fresh(sxL),
+                             fresh(syL), nx, ny, fresh(dxL), fresh(dyL), dt, d[0], d[1], d[2], d[6],
+                             d[8], dhL, dqxL, dqyL, gxL, gyL);
+        spe::recon_muscl_vjp(1, fresh(hR), fresh(qxR), fresh(qyR), fresh(m.zb[R]), bR, fresh(sxR),
+                             fresh(syR), nx, ny, fresh(dxR), fresh(dyR), dt, d[3], d[4], d[5], d[7],
+                             d[9], dhR, dqxR, dqyR, gxR, gyR);
         double *g = sd + 4 * (size_t)e;
         g[0] = gxL;
         g[1] = gyL;
@@ -868,20 +884,25 @@ __global__ __launch_bounds__(kBlock) void k_r_edge(MeshDev m, Opts o, Controls c
     so[7] = d[11];
 }
 
-// the slope adjoints of one cell's incidences go to zn[incidence]: the term
-// of the neighbour across that edge, gathered by the neighbour as zn[inc ^ 1]
+// the slope adjoint's term for the neighbour across a cell's incidence k
+// goes to zn[b + k], the incidence's own CSR slot (contiguous per cell); the
+// neighbour reads it through the mirror index of its own incidence
 struct ZnOut {
     double *zn;
-    const int *inc;
     int b;
-    __device__ void set(int k, double v) const { zn[inc[b + k]] = v; }
-    __device__ void add(int k, double v) const { zn[inc[b + k]] += v; }
+    __device__ void set(int k, double v) const { zn[b + k] = v; }
+    __device__ void add(int k, double v) const { zn[b + k] += v; }
+    __device__ void put(int k, double v) const { zn[b + k] = v; }
 };
+
+// cells of at most this many internal edges (every triangle or quad) keep
+// their incidences in registers in the slope reverse
+constexpr int kSlopeRegDeg = 4;
 
 // MUSCL-EVR, per cell: gather the adjoint of its limited slope from its
 // edges' sides (sd), reverse the slope and its limiter (spe::slope_vjp): the
 // term of its own free surface is added to the h part of the adjoint of U^n,
-// those of its neighbours go to zn
+// those of its neighbours go to zn (CSR slots of this cell)
 __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__restrict__ U,
                                                      const double *__restrict__ sd,
                                                      double *__restrict__ part,
@@ -896,9 +917,11 @@ __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__
         gy += sd[2 * (size_t)inc + 1];
     }
     const NbDev nb{m, U, b};
-    const ZnOut out{zn, m.int_inc, b};
+    const ZnOut out{zn, b};
+    const double zi = U[i] + m.zb[i], xi = m.cx[i], yi = m.cy[i];
     double dzi = 0.0;
-    spe::slope_vjp(nb, deg, U[i] + m.zb[i], m.cx[i], m.cy[i], gx, gy, dzi, out);
+    if (!spe::slope_vjp_reg<kSlopeRegDeg>(nb, deg, zi, xi, yi, gx, gy, dzi, out))
+        spe::slope_vjp(nb, deg, zi, xi, yi, gx, gy, dzi, out);
     part[i] += dzi;
 }
 
@@ -1315,6 +1338,14 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_Sseg = a->alloc<double>(muscl ? (size_t)a->stride * 2 * (size_t)nc : 1);
     a->d_sd = a->alloc<double>(muscl ? 4 * (size_t)std::max<int64_t>(ni, 1) : 1);
     a->d_zn = a->alloc<double>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
+    a->d_mirror = a->alloc<int>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
+    if (muscl && ni > 0) {  // mirror[p] = slot of (e, 1 - side) for the slot p of (e, side)
+        std::vector<int> inc(2 * ni), slot(2 * ni), mir(2 * ni);
+        HIPCHK(c, hipMemcpy(inc.data(), c->m.int_inc, inc.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < 2 * ni; ++p) slot[inc[p]] = (int)p;
+        for (int64_t p = 0; p < 2 * ni; ++p) mir[p] = slot[inc[p] ^ 1];
+        HIPCHK(c, hipMemcpy(a->d_mirror, mir.data(), mir.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     a->d_partial = a->alloc<double>(nblocks(nc));
     a->d_cpart = a->alloc<double>(nblocks(nc));
     a->d_J = a->alloc<double>(1 + (size_t)std::max(1, obs->n_obs));
@@ -1502,11 +1533,11 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         if (cell)
             KLAUNCH(a, k_r_gc<true>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
-                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn);
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn, a->d_mirror);
         else
             KLAUNCH(a, k_r_gc<false>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
-                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn);
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn, a->d_mirror);
     };
     auto bnd = [&](int64_t n, int j) {
         if (c->m.n_bnd_cells <= 0) return;

@@ -271,3 +271,26 @@ def test_muscl_slope_planted_error_is_detected():
 
         worst = max(worst, check(f, g_bad, z, list(range(5)), 2, rng, eps=1e-7, trials=2))
     assert worst > 1e-2, worst
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 6])
+def test_muscl_slope_register_variant_equals_replay(deg):
+    """slope_vjp_reg (incidences and visit slopes kept in registers, the
+    kernels' path up to 4 internal edges) against slope_vjp (visits
+    replayed): the same adjoints to round-off, and none beyond its bound."""
+    rng = np.random.default_rng(40 + deg)
+    for _ in range(30):
+        nb, z = _slope_cell(rng, deg)
+        w = rng.standard_normal(2)
+        d1 = _slope_g(nb, deg)(z, w)
+        n2 = nb.copy()
+        n2[:, 0] = z[1:]
+        d2 = np.zeros(deg + 1)
+        ok = LIB.sw2dt_slope_vjp_reg(arr([z[0], 0.0, 0.0]).ctypes.data_as(dp), arr(n2).ctypes.data_as(dp),
+                                     C.c_int(deg), arr(w).ctypes.data_as(dp), d2.ctypes.data_as(dp))
+        assert ok == 1
+        np.testing.assert_allclose(d2, d1, rtol=1e-13, atol=1e-13 * np.max(np.abs(d1)))
+    nb, z = _slope_cell(rng, 7)
+    d = np.zeros(8)
+    assert LIB.sw2dt_slope_vjp_reg(arr([z[0], 0.0, 0.0]).ctypes.data_as(dp), arr(nb).ctypes.data_as(dp),
+                                   C.c_int(7), arr([1.0, 1.0]).ctypes.data_as(dp), d.ctypes.data_as(dp)) == 0
