@@ -189,3 +189,188 @@ def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5
                 Y.pop(0)
         x, J, g = xn, Jn, gn
     return x, J, trace
+
+
+def minimize_gd(fg, f, x0, project=True, step=1e-6, max_iter=100, c1=1e-4, shrink=0.5, grow=4.0,
+                max_bt=40):
+    """Projected gradient descent with Armijo backtracking: run_minimization's
+    "gd" (assim/swSpE_minimize.cpp:747-831).  Each iterate takes one gradient
+    fg(x) -> (J, g); the trial step starts at the last accepted step times
+    `grow` (the first at `step`), is multiplied by `shrink` up to `max_bt`
+    times until f(x_trial) is finite and <= J - c1 g.(x - x_trial) with
+    g.(x - x_trial) > 0, x_trial = x - t g clipped to [0, 1] when `project`.
+    Stops at max_iter, on a zero / NaN gradient or when no step is accepted.
+    Returns the best iterate by gradient-evaluated J: (x, J, trace)."""
+    x = np.asarray(x0, dtype=np.float64).copy()
+    t = step if step > 0.0 else 1e-6
+    best_x, best_J = x.copy(), np.inf
+    t0 = time.perf_counter()
+    trace = []
+    for _ in range(max_iter):
+        J, g = fg(x)
+        trace.append((time.perf_counter() - t0, J, "grad"))
+        if J < best_J:
+            best_J, best_x = J, x.copy()
+        if not np.dot(g, g) > 0.0:
+            break
+        t_try = t * grow
+        accepted = False
+        for _ in range(max_bt):
+            v = x - t_try * g
+            if project:
+                v = np.clip(v, 0.0, 1.0)
+            pg = float(np.dot(g, x - v))
+            Jn = f(v)
+            trace.append((time.perf_counter() - t0, Jn, "ls"))
+            if np.isfinite(Jn) and Jn <= J - c1 * pg and pg > 0.0:
+                accepted = True
+                break
+            t_try *= shrink
+        if not accepted:
+            break
+        x, t = v, t_try
+    return best_x, best_J, trace
+
+
+def minimize_adam(fg, x0, step=1e-2, max_iter=100, project=True, safe=False, b1=0.9, b2=0.999,
+                  eps=1e-8, explosion=50.0):
+    """Projected Adam: run_minimization's "adam" (assim/swSpE_minimize.cpp:
+    932-1017).  One gradient per iterate; m, v moments with bias correction,
+    x -= step mhat / (sqrt(vhat) + eps), clipped to [0, 1] when `project`.
+    A jump of ||g||_inf above `explosion` x the previous one stops the loop
+    when `safe` (otherwise it is only reported in the trace).  Returns the
+    best iterate by J: (x, J, trace)."""
+    x = np.asarray(x0, dtype=np.float64).copy()
+    m = np.zeros_like(x)
+    v = np.zeros_like(x)
+    best_x, best_J = x.copy(), np.inf
+    g_prev = -1.0
+    t0 = time.perf_counter()
+    trace = []
+    for it in range(1, max_iter + 1):
+        J, g = fg(x)
+        trace.append((time.perf_counter() - t0, J, "grad"))
+        if J < best_J:
+            best_J, best_x = J, x.copy()
+        g_inf = float(np.max(np.abs(g))) if len(g) else 0.0
+        if g_prev > 0.0 and g_inf > explosion * g_prev:
+            trace.append((time.perf_counter() - t0, J, "explosion"))
+            if safe:
+                break
+        g_prev = g_inf
+        bc1 = 1.0 - b1 ** it
+        bc2 = 1.0 - b2 ** it
+        m = b1 * m + (1.0 - b1) * g
+        v = b2 * v + (1.0 - b2) * g * g
+        x = x - step * (m / bc1) / (np.sqrt(v / bc2) + eps)
+        if project:
+            x = np.clip(x, 0.0, 1.0)
+    return best_x, best_J, trace
+
+
+def _wolfe(fp, x, d, J0, g0, c1, c2, t=1.0, max_eval=40):
+    """Step along d satisfying the strong Wolfe conditions: bracketing then
+    zoom by bisection (Nocedal & Wright, Numerical Optimization, algorithms
+    3.5 and 3.6).  fp(x) -> (J, g).  Returns (t, J, g), or None after
+    max_eval evaluations."""
+    d0 = float(np.dot(g0, d))
+    n = [0]
+
+    def ev(s):
+        n[0] += 1
+        J, g = fp(x + s * d)
+        return J, g, float(np.dot(g, d))
+
+    def zoom(lo, J_lo, hi):
+        while n[0] < max_eval:
+            s = 0.5 * (lo + hi)
+            J, g, ds = ev(s)
+            if not np.isfinite(J) or J > J0 + c1 * s * d0 or J >= J_lo:
+                hi = s
+            else:
+                if abs(ds) <= -c2 * d0:
+                    return s, J, g
+                if ds * (hi - lo) >= 0.0:
+                    hi = lo
+                lo, J_lo = s, J
+        return None
+
+    t_prev, J_prev = 0.0, J0
+    while n[0] < max_eval:
+        J, g, ds = ev(t)
+        if not np.isfinite(J) or J > J0 + c1 * t * d0 or (t_prev > 0.0 and J >= J_prev):
+            return zoom(t_prev, J_prev, t)
+        if abs(ds) <= -c2 * d0:
+            return t, J, g
+        if ds >= 0.0:
+            return zoom(t, J, t_prev)
+        t_prev, J_prev, t = t, J, 2.0 * t
+    return None
+
+
+def minimize_bfgs(fg, x0, max_iter=100, rel_eps=1e-8, c1=1e-4, c2=0.9):
+    """Dense BFGS on the inverse Hessian with a strong-Wolfe line search, the
+    method of run_minimization's default branch (PSQN::bfgs with rel_eps 1e-8,
+    c1 1e-4, c2 0.9, assim/swSpE_minimize.cpp:913-930; the PSQN library is
+    not part of the reference tree, so this follows the textbook method:
+    Nocedal & Wright, algorithm 6.1, H0 scaled by s.y / y.y after the first
+    step).  Stops when |J_new - J| <= rel_eps (|J| + rel_eps), at max_iter, or
+    when the line search fails.  Returns (x, J, trace)."""
+    x = np.asarray(x0, dtype=np.float64).copy()
+    n = len(x)
+    t0 = time.perf_counter()
+    J, g = fg(x)
+    trace = [(time.perf_counter() - t0, J, "grad")]
+    H = np.eye(n)
+    first = True
+    for _ in range(max_iter):
+        d = -H @ g
+        if not np.dot(d, g) < 0.0:
+            H = np.eye(n)
+            d = -g
+        res = _wolfe(fg, x, d, J, g, c1, c2)
+        if res is None:
+            break
+        t, Jn, gn = res
+        trace.append((time.perf_counter() - t0, Jn, "grad"))
+        s, y = t * d, gn - g
+        sy = float(np.dot(s, y))
+        if sy > 0.0:
+            if first:
+                H = np.eye(n) * (sy / float(np.dot(y, y)))
+                first = False
+            rho = 1.0 / sy
+            Hy = H @ y
+            H = H - rho * (np.outer(s, Hy) + np.outer(Hy, s)) + (rho * rho * float(np.dot(y, Hy)) + rho) * np.outer(s, s)
+        done = abs(Jn - J) <= rel_eps * (abs(J) + rel_eps)
+        x, J, g = x + s, Jn, gn
+        if done:
+            break
+    return x, J, trace
+
+
+METHODS = ("lbfgsb", "gd", "adam", "bfgs")
+
+
+def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=None, **kw):
+    """The reference's run_minimization dispatch over PorosityProblem
+    (assim/swSpE_minimize.cpp:725-1017): "lbfgsb" (projected L-BFGS on [0, 1]
+    with memory 6 and <= 30 line-search probes), "gd" (projected GD +
+    Armijo), "adam" (projected Adam), "bfgs" (dense BFGS, unconstrained).
+    Projection onto [0, 1] applies, as in the reference, when the change of
+    variable leaves m = phi (none / clamp).  Returns (theta, J, trace)."""
+    if method not in METHODS:
+        raise ValueError(f"method must be one of {METHODS}")
+    if project is None:
+        project = prob.mode in (None, "none", "clamp")
+    theta0 = prob.theta0()
+    if method == "lbfgsb":
+        lo, hi = (0.0, 1.0) if project else (None, None)
+        return minimize(prob.grad, prob.func, theta0, lo=lo, hi=hi, max_iter=max_iter, **kw)
+    if method == "gd":
+        return minimize_gd(prob.grad, prob.func, theta0, project=project, max_iter=max_iter,
+                           step=1e-6 if step is None else step, **kw)
+    if method == "adam":
+        return minimize_adam(prob.grad, theta0, step=1e-2 if step is None else step,
+                             max_iter=max_iter, project=project, **kw)
+    return minimize_bfgs(prob.grad, theta0, max_iter=max_iter, **kw)

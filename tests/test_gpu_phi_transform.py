@@ -64,3 +64,26 @@ def test_gpu_porosity_problem_descends(sw, tmp_path):
     x, J, tr = minimize(P.grad, P.func, th0, lo=-5.0, hi=5.0, max_iter=5)
     assert P.func(x) < P.func(th0)
     s.close()
+
+
+@pytest.mark.parametrize("method", ["gd", "adam", "bfgs", "lbfgsb"])
+def test_gpu_run_minimization_methods_descend(sw, tmp_path, method):
+    """Each of the reference's minimiser methods (run_minimization) lowers
+    the porosity problem's f on the GPU gradient in a few iterations; under
+    the clamp change of variable the projected ones keep m in [0, 1]."""
+    from sw2d_amd import assim
+    from sw2d_amd.porosity import PorosityProblem, run_minimization
+    g, case, s = _setup(sw, str(tmp_path))
+    cells, obs = _gauges(case)
+    print(f"[gpu] run_minimization {method}", flush=True)
+    A = assim.Assim(s, OBS_T, gauge_cells=cells, gauge_h=obs, phi_transform=("clamp", 1.0))
+    phi_b = np.clip(case.a["phi"], 0.05, 0.95)
+    P = PorosityProblem(A, phi_b, alpha=1e-3, phi_transform=("clamp", 1.0), zonation=True)
+    th0 = P.theta0()
+    f0 = P.grad(th0)[0]  # a dry run at th0 (func replays the last dry run's schedule)
+    kw = {"step": 1e-3} if method in ("gd", "adam") else {}
+    th, J, tr = run_minimization(P, method, max_iter=4, **kw)
+    assert P.grad(th)[0] < f0
+    if method != "bfgs":
+        assert np.all(th >= 0.0) and np.all(th <= 1.0)
+    s.close()
