@@ -185,8 +185,18 @@ int sw2dg_phi_transform_eval(int32_t mode, double lambda, double lo, double hi, 
  * reference's); ratio <= 0 turns the guard off. */
 int sw2dg_assim_set_explosion_guard(sw2dg_ctx *ctx, double ratio);
 /* Truncation of the last gradient: first kept segment and its first step
- * (both 0 when the whole window was kept). */
+ * (both 0 when the gradient covers the whole trajectory; the window's first
+ * segment when an assimilation window or a frozen segment starts later, as
+ * the reference's adjoint_truncation_seg_ + 1). */
 int sw2dg_assim_truncation(sw2dg_ctx *ctx, int64_t *first_kept_step, int64_t *first_kept_segment);
+/* Frozen truncation window (the reference's forced_first_seg_,
+ * swSpE_checkpoint.cpp:871-881, set by its L-BFGS-B driver,
+ * swSpE_minimize.cpp:881-903): segment >= 0 makes every later gradient and
+ * cost start at min(segment, K - 1) -- the reverse sweep stops there and the
+ * cost keeps the observations from there on -- with the explosion guard's
+ * dynamic truncation off, so that a line search compares costs over one
+ * window; segment < 0 restores the dynamic truncation (default). */
+int sw2dg_assim_set_first_segment(sw2dg_ctx *ctx, int64_t segment);
 /* Thin-film cut of the reverse sweep: the adjoint of a cell's state after a
  * step is dropped where that state's depth is below h_cut (m), so no
  * sensitivity is carried back through films that are all but dry.  0 (the

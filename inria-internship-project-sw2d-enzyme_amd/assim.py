@@ -45,6 +45,7 @@ _DECL = {
     "sw2dg_assim_set_phi_transform": ([C.c_void_p, C.c_int32, C.c_double, N._i64, C.c_int64,
                                        C.c_int32], C.c_int),
     "sw2dg_assim_truncation": ([C.c_void_p, N._i64, N._i64], C.c_int),
+    "sw2dg_assim_set_first_segment": ([C.c_void_p, C.c_int64], C.c_int),
     "sw2dg_assim_profile": ([C.c_void_p, N._d, N._d, N._d, C.c_int32, N._d, N._i64,
                              C.POINTER(C.c_char_p), C.POINTER(C.c_int32)], C.c_int),
 }
@@ -192,6 +193,14 @@ class Assim:
         st, sg = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
         self.s._check(self.lib.sw2dg_assim_truncation(self.s.ctx, N.i64ptr(st), N.i64ptr(sg)))
         return int(st[0]), int(sg[0])
+
+    def set_first_segment(self, segment):
+        """Freeze the truncation window at `segment` (the reference's
+        forced_first_seg_): later gradients and costs start there, with the
+        explosion guard's dynamic truncation off; None or < 0 restores the
+        dynamic truncation."""
+        seg = -1 if segment is None else int(segment)
+        self.s._check(self.lib.sw2dg_assim_set_first_segment(self.s.ctx, seg))
 
     def profile(self, phi=None, K1=None, K2=None, k_max=32):
         """One gradient with per-launch HIP events: {kernel: (avg_ms, launches)}."""

@@ -104,6 +104,7 @@ struct AssimState {
     // gradient snapshot at the last milestone, truncation of the last gradient
     double expl_ratio = 1e8;
     int64_t trunc_seg = -1;
+    int64_t forced_seg = -1;    // frozen truncation window (forced_first_seg_), -1 = dynamic
     double *d_wm = nullptr;     // 3 nc: gphi | gK1 | gK2 at the watermark
     double *d_nrm = nullptr;    // block partials of ||lam_h||^2, then the sum
     // controls (device order)
@@ -1131,6 +1132,8 @@ Plan make_plan(const AssimState *a, const std::vector<double> &T) {
                 break;
         }
     }
+    // a frozen truncation window overrides the window's start (swSpE_checkpoint.cpp:878-881)
+    if (a->forced_seg >= 0) p.first = std::min<int64_t>(a->forced_seg, std::max<int64_t>(p.K - 1, 0));
     p.tile_top.assign(p.K, 0);
     const int64_t M = p.K - p.first;
     int64_t n = std::max<int64_t>(1, std::min<int64_t>(a->n_tiles, M));
@@ -1236,6 +1239,12 @@ int sw2dg_assim_truncation(sw2dg_ctx *c, int64_t *first_kept_step, int64_t *firs
     const AssimState *a = c->assim;
     if (first_kept_segment) *first_kept_segment = a->trunc_seg + 1;
     if (first_kept_step) *first_kept_step = (a->trunc_seg + 1) * a->stride;
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_set_first_segment(sw2dg_ctx *c, int64_t segment) {
+    if (!c || !c->assim) return SW2DG_ERR_ARG;
+    c->assim->forced_seg = segment >= 0 ? segment : -1;
     return SW2DG_OK;
 }
 
@@ -1512,6 +1521,9 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     if ((st = fetch_record(c, a, T, trig, dt))) return st;
     a->trunc_seg = -1;  // reset from any previous gradient (swSpE_checkpoint.cpp:731)
     const Plan p = make_plan(a, T);
+    // a window or a frozen segment that starts later is a truncation too: the
+    // cost restarts there (swSpE_checkpoint.cpp:850-852, 878-881)
+    if (p.first > 0) a->trunc_seg = p.first - 1;
     const int64_t nc = c->nc;
     const int ns = std::max(c->n_series, 1);
     zero(c, a->d_J, 1 + (size_t)std::max(1, a->n_obs));
@@ -1567,7 +1579,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     // (watermark) at milestones (every 50th, first and last segment, segments
     // with observations) or, tiled, at each tile top; on explosion the
     // gradient returns to the watermark and the reverse sweep stops
-    const bool guard = a->expl_ratio > 0.0;
+    const bool guard = a->expl_ratio > 0.0 && a->forced_seg < 0;  // frozen window: no dynamic truncation
     const bool tiled = p.tile_start.size() > 1;
     const int nbk = nblocks(nc);
     double ref = -1.0;

@@ -366,7 +366,18 @@ def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=Non
     theta0 = prob.theta0()
     if method == "lbfgsb":
         lo, hi = (0.0, 1.0) if project else (None, None)
-        return minimize(prob.grad, prob.func, theta0, lo=lo, hi=hi, max_iter=max_iter, **kw)
+        # a probe gradient at the start fixes where the adjoint truncates; the
+        # window stays frozen there for the whole solve so that every
+        # line-search cost covers the same segments (swSpE_minimize.cpp:881-903)
+        A = getattr(prob, "A", None)
+        if A is None or not hasattr(A, "set_first_segment"):
+            return minimize(prob.grad, prob.func, theta0, lo=lo, hi=hi, max_iter=max_iter, **kw)
+        prob.grad(theta0)
+        A.set_first_segment(max(0, A.truncation()[1]))
+        try:
+            return minimize(prob.grad, prob.func, theta0, lo=lo, hi=hi, max_iter=max_iter, **kw)
+        finally:
+            A.set_first_segment(None)
     if method == "gd":
         return minimize_gd(prob.grad, prob.func, theta0, project=project, max_iter=max_iter,
                            step=1e-6 if step is None else step, **kw)

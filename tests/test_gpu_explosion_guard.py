@@ -113,3 +113,54 @@ def test_gpu_default_guard_truncates_an_exploding_adjoint(sw):
     assert np.all(np.isfinite(gn)) and np.max(np.abs(gn)) < 1e-6 * np.max(np.abs(g_exact))
     assert A.cost(K1=K, K2=K) == G["J"]
     s.close()
+
+
+def test_gpu_frozen_first_segment_is_the_subproblem(sw, oracle):
+    """A frozen truncation window (sw2dg_assim_set_first_segment, the
+    reference's forced_first_seg_): the gradient and the cost start at the
+    segment -- equal to the oracle's exact gradient of the sub-problem from
+    the frozen state there -- the guard no longer truncates (ratio 1e-9
+    would otherwise), and unfreezing restores the full sweep bit for bit."""
+    import copy
+    from sw2d_amd import assim
+    cal, s, obs = _problem(sw)
+    print("[gpu] frozen first segment", flush=True)
+    K = cal.K(cal.n_default)
+    A = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=obs, max_steps=200,
+                    stride=4, explosion_ratio=1e-9)
+    A.set_first_segment(5)
+    G = A.gradient(K1=K, K2=K)
+    n0, seg = A.truncation()
+    assert (n0, seg) == (20, 5)
+    assert A.cost(K1=K, K2=K) == G["J"]
+    rec = A.get_record()
+    E = oracle.OracleE(cal.case)
+    head = {"dt": rec["dt"][:n0], "bc": rec["bc"][:n0], "trig": rec["trig"][:n0]}
+    _, (H, QX, QY) = E.replay(head, K1=K, K2=K)
+    sub = copy.copy(cal.case)
+    sub.a = dict(cal.case.a)
+    sub.a["h0"], sub.a["qx0"], sub.a["qy0"] = H, QX, QY
+    tail = {"dt": rec["dt"][n0:], "bc": rec["bc"][n0:], "trig": rec["trig"][n0:]}
+    O = oracle.OracleE(sub).gradient(tail, K1=K, K2=K, gauges=cal.gauge_cells, obs=obs,
+                                     n_obs=len(cal.obs_t))
+    assert abs(G["J"] - O["J"]) <= 1e-12 * abs(O["J"])
+    for key in ("phi", "K1", "K2"):
+        scale = np.max(np.abs(O[key]))
+        assert scale > 0.0
+        assert np.max(np.abs(G[key] - O[key])) <= 1e-10 * scale, key
+    assert np.max(np.abs(G["bc"][n0:] - O["bc"])) <= 1e-10 * np.max(np.abs(O["bc"]))
+    assert not np.any(G["bc"][:n0])
+    # unfrozen, guard off: the exact sweep, as a fresh Assim gives it
+    A.set_first_segment(None)
+    B = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=obs, max_steps=200,
+                    stride=4, explosion_ratio=0.0)
+    Gb = B.gradient(K1=K, K2=K)
+    C_ = assim.Assim(s, cal.obs_t, gauge_cells=cal.gauge_cells, gauge_h=obs, max_steps=200,
+                     stride=4, explosion_ratio=0.0)
+    C_.set_first_segment(3)
+    C_.set_first_segment(-1)
+    Gc = C_.gradient(K1=K, K2=K)
+    for key in ("phi", "K1", "K2", "bc"):
+        np.testing.assert_array_equal(Gb[key], Gc[key])
+    assert C_.truncation() == (0, 0)
+    s.close()
