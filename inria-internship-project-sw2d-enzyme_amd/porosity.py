@@ -132,15 +132,27 @@ class PorosityProblem:
         return J_data + p["ridge"] + p["l1"] + p["phi_ref"], g
 
 
-def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5, time_limit=None):
+def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5, time_limit=None,
+             first_step="capped", expand=False, fresh_probes=False):
     """Projected L-BFGS on box bounds: fg(x) -> (value, gradient) anchors the
-    trajectory, f(x) evaluates line-search probes.  Stops when the projected
-    gradient's max norm falls below eps, or at max_iter / time_limit.
+    trajectory, f(x) evaluates line-search probes (Armijo backtracking from
+    step 1).  The first iterate's step: "capped" moves no control by more
+    than 0.1; "inv_norm" is 1 / ||d||_2, LBFGSpp's first step (the
+    reference's L-BFGS-B library).  expand: when the first trial is accepted,
+    the step keeps doubling while the cost still falls (within max_ls
+    probes).  With finite bounds no trial moves a control farther than the
+    box's width.  fresh_probes: every probe is a gradient evaluation fg (a new
+    dry run at the probe, as LBFGSpp evaluates the reference's objective),
+    the accepted one's gradient reused; otherwise probes replay the last dry
+    run's schedule through f.  Stops when the projected gradient's max norm
+    falls below eps, or at max_iter / time_limit.
     Returns (x, value, trace [(wall s, value, 'grad' | 'ls')])."""
     x = np.asarray(x0, dtype=np.float64).copy()
     lo = np.full_like(x, -np.inf) if lo is None else np.broadcast_to(np.asarray(lo, float), x.shape)
     hi = np.full_like(x, np.inf) if hi is None else np.broadcast_to(np.asarray(hi, float), x.shape)
     x = np.clip(x, lo, hi)
+    span = hi - lo
+    width = float(np.max(span)) if np.all(np.isfinite(span)) and span.size else 0.0
     t0 = time.perf_counter()
     trace = []
     J, g = fg(x)
@@ -166,20 +178,50 @@ def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5
         if np.dot(d, g) >= 0.0:
             S, Y = [], []
             d = -np.where(free, g, 0.0)
-        step = 1.0 if S else min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
+        if S:
+            step = 1.0
+        elif first_step == "inv_norm":
+            step = 1.0 / max(float(np.linalg.norm(d)), 1e-300)
+        else:
+            step = min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
+        if width > 0.0:  # no control moves farther than the box is wide
+            step = min(step, width / max(float(np.max(np.abs(d))), 1e-300))
         accepted = False
+        probes = 0
+        gn = None
+
+        def probe(xp):
+            if fresh_probes:
+                Jp, gp = fg(xp)
+                trace.append((time.perf_counter() - t0, Jp, "grad"))
+                return Jp, gp
+            Jp = f(xp)
+            trace.append((time.perf_counter() - t0, Jp, "ls"))
+            return Jp, None
+
         for _ in range(max_ls):
             xn = np.clip(x + step * d, lo, hi)
-            Jn = f(xn)
-            trace.append((time.perf_counter() - t0, Jn, "ls"))
-            if Jn <= J + 1e-4 * np.dot(g, xn - x):
+            Jn, gn = probe(xn)
+            probes += 1
+            if np.isfinite(Jn) and Jn <= J + 1e-4 * np.dot(g, xn - x):
                 accepted = True
                 break
             step *= 0.5
         if not accepted:
             break
-        Jn, gn = fg(xn)
-        trace.append((time.perf_counter() - t0, Jn, "grad"))
+        if expand and probes == 1:  # accepted at once: try longer steps while the cost falls
+            while probes < max_ls:
+                xt = np.clip(x + 2.0 * step * d, lo, hi)
+                if np.array_equal(xt, xn):
+                    break
+                Jt, gt = probe(xt)
+                probes += 1
+                if not (Jt < Jn and Jt <= J + 1e-4 * np.dot(g, xt - x)):
+                    break
+                step, xn, Jn, gn = 2.0 * step, xt, Jt, gt
+        if gn is None:
+            Jn, gn = fg(xn)
+            trace.append((time.perf_counter() - t0, Jn, "grad"))
         sk, yk = xn - x, gn - g
         if np.dot(sk, yk) > 1e-12 * np.dot(sk, sk):
             S.append(sk)
@@ -369,6 +411,11 @@ def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=Non
         # a probe gradient at the start fixes where the adjoint truncates; the
         # window stays frozen there for the whole solve so that every
         # line-search cost covers the same segments (swSpE_minimize.cpp:881-903)
+        # LBFGSpp's conventions (the reference's L-BFGS-B library): first step
+        # 1 / ||d||, every evaluation a gradient at a new dry run, steps that
+        # may grow past the first trial
+        for k, v in (("first_step", "inv_norm"), ("expand", True), ("fresh_probes", True)):
+            kw.setdefault(k, v)
         A = getattr(prob, "A", None)
         if A is None or not hasattr(A, "set_first_segment"):
             return minimize(prob.grad, prob.func, theta0, lo=lo, hi=hi, max_iter=max_iter, **kw)

@@ -102,3 +102,26 @@ def test_gpu_sp_toce_cost_and_tiled_gradient(sw):
     gp = G["phi"]
     assert np.all(np.isfinite(gp)) and np.count_nonzero(gp) > case.n_cell // 2
     s.close()
+
+
+def test_gpu_sp_toce_lbfgsb_descent(sw):
+    """The reference's functional case end to end: from phi = 1, data-only
+    L-BFGS-B on [0, 1] (memory 6, <= 30 line-search evaluations, 48 tiles,
+    window from t = 10 s, truncation window frozen at the start) over the
+    GPU adjoint.  The reference's descent log goes 3.104 -> 0.751 (-75.8 %)
+    in 9 evaluations; the path is not reproducible digit for digit (the
+    case's chaotic sensitivity, a different L-BFGS-B code), the descent is:
+    below 0.9 (-71 %) within 16 iterations."""
+    from sw2d_amd.porosity import PorosityProblem, run_minimization
+    case, g = load_sp_toce(sw)
+    print("[gpu] sp_toce L-BFGS-B descent", flush=True)
+    s = sw.SpSolver(case)
+    A = _assim(sw, s, g)
+    P = PorosityProblem(A, np.ones(case.n_cell))
+    th, J, tr = run_minimization(P, "lbfgsb", max_iter=16)
+    grads = [v for (_, v, k) in tr if k == "grad"]
+    print(f"[gpu] J {grads[0]:.6f} -> {J:.6f} in {len(grads)} evaluations", flush=True)
+    assert abs(grads[0] - float(g["expect_J"])) <= 1e-5 * float(g["expect_J"])
+    assert J < 0.9
+    assert np.all((th >= 0.0) & (th <= 1.0))
+    s.close()
