@@ -133,15 +133,15 @@ class PorosityProblem:
 
 
 def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5, time_limit=None,
-             first_step="capped", expand=False, fresh_probes=False):
+             first_step="capped", expand=False, fresh_probes=False, cap_width=False):
     """Projected L-BFGS on box bounds: fg(x) -> (value, gradient) anchors the
     trajectory, f(x) evaluates line-search probes (Armijo backtracking from
     step 1).  The first iterate's step: "capped" moves no control by more
     than 0.1; "inv_norm" is 1 / ||d||_2, LBFGSpp's first step (the
     reference's L-BFGS-B library).  expand: when the first trial is accepted,
     the step keeps doubling while the cost still falls (within max_ls
-    probes).  With finite bounds no trial moves a control farther than the
-    box's width.  fresh_probes: every probe is a gradient evaluation fg (a new
+    probes).  cap_width (finite bounds): no trial moves a control farther
+    than the box's width.  fresh_probes: every probe is a gradient evaluation fg (a new
     dry run at the probe, as LBFGSpp evaluates the reference's objective),
     the accepted one's gradient reused; otherwise probes replay the last dry
     run's schedule through f.  Stops when the projected gradient's max norm
@@ -152,7 +152,7 @@ def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5
     hi = np.full_like(x, np.inf) if hi is None else np.broadcast_to(np.asarray(hi, float), x.shape)
     x = np.clip(x, lo, hi)
     span = hi - lo
-    width = float(np.max(span)) if np.all(np.isfinite(span)) and span.size else 0.0
+    width = float(np.max(span)) if cap_width and np.all(np.isfinite(span)) and span.size else 0.0
     t0 = time.perf_counter()
     trace = []
     J, g = fg(x)
@@ -414,7 +414,8 @@ def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=Non
         # LBFGSpp's conventions (the reference's L-BFGS-B library): first step
         # 1 / ||d||, every evaluation a gradient at a new dry run, steps that
         # may grow past the first trial
-        for k, v in (("first_step", "inv_norm"), ("expand", True), ("fresh_probes", True)):
+        for k, v in (("first_step", "inv_norm"), ("expand", True), ("fresh_probes", True),
+                     ("cap_width", True)):
             kw.setdefault(k, v)
         A = getattr(prob, "A", None)
         if A is None or not hasattr(A, "set_first_segment"):
