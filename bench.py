@@ -82,23 +82,29 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
     the MUSCL-EVR ones."""
     muscl = scheme == "muscl_evr"
     table = {
-        # cells U, zb, beta, phi; internal edges L, R, nx, ny, F out (+ length and
-        # swept out in the dry run); boundary edges cell, nx, ny, series, F out
-        "k_e_edges<false>": 48 * nc + 64 * ni + 48 * nb,
-        # + cell slopes and centroids, edge midpoints
-        "k_e_edges<true>": 80 * nc + 80 * ni + 48 * nb,
+        # cells U, zb, beta, area, CSR ptrs (+ slopes, centroids); incidences,
+        # edge cells and geometry (+ midpoints); boundary incidences and geometry
+        "k_e_cfl<false>": 56 * nc + 40 * ni + 28 * nb,
+        "k_e_cfl<true>": 88 * nc + 56 * ni + 28 * nb,
+        "k_e_dt": 8 * ((nc + 255) // 256),
+        # fused flux + update: cells U in/out, zb, beta, phi, area, K1, K2, cos,
+        # sin, ptrs (+ slopes, centroids); incidences, edge cells and geometry
+        # (+ midpoints); boundary incidences, geometry, series; the replay
+        # (<true, .>) also writes the balanced state + increments and the
+        # boundary fluxes.  Each internal edge's flux is computed by both of its
+        # cells; the bytes count its inputs once.
+        "k_e_update<false, false>": 120 * nc + 40 * ni + 32 * nb,
+        "k_e_update<true, false>": 168 * nc + 40 * ni + 56 * nb,
+        "k_e_update<false, true>": 152 * nc + 56 * ni + 32 * nb,
+        "k_e_update<true, true>": 200 * nc + 56 * ni + 56 * nb,
+        # MUSCL-EVR dry run: edges once (states, fluxes and swept terms out),
+        # the CFL from the swept terms, the update from the stored fluxes
+        "k_e_edges_muscl": 80 * nc + 80 * ni + 48 * nb,
+        "k_e_cfl_sw": 56 * nc + 24 * ni + 28 * nb,
+        "k_e_update<false, true, true>": 104 * nc + 72 * ni + 52 * nb,
         # cells U(h), zb, centroid, CSR ptr, slopes out; incidences, edge cells
         # and midpoints
         "k_e_slopes": 52 * nc + 32 * ni,
-        # cells U, zb, beta, area, CSR ptrs; incidences + swept; boundary edges
-        "k_e_cfl<false>": 56 * nc + 24 * ni + 28 * nb,
-        "k_e_cfl<true>": 56 * nc + 24 * ni + 28 * nb,
-        "k_e_dt": 8 * ((nc + 255) // 256),
-        # cells U in/out, phi, area, K1, K2, cos, sin, ptrs; incidences, F, edge
-        # geometry; boundary incidences, geometry and F
-        "k_e_update<false>": 104 * nc + 72 * ni + 52 * nb,
-        # + balanced state and internal increments out
-        "k_e_update<true>": 152 * nc + 72 * ni + 52 * nb,
         # cells part in/out, ptrs, P in, phi, K1, K2, cos, sin, lams out,
         # gphi/gK1/gK2 read-modify-write; incidences + edge sides (+ the
         # neighbours' slope terms, MUSCL)
@@ -190,6 +196,8 @@ def roofline(res, args):
                          "gbs": (b / (avg * 1e-3) / 1e9) if (b and avg > 0) else None}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     d = kernels[dom]
+    if d["bytes_per_launch"] is None:
+        raise RuntimeError(f"no compulsory byte count for the dominant kernel {dom}")
     achieved = d["gbs"]
     Bf, Ba = pass_bytes(nc, ni, nb, scheme)
     step_ms = total_ms / n_steps  # device time per time step (fwd + recompute + adj)

@@ -75,10 +75,10 @@ struct AssimState {
     int64_t n_ckpt = 0;
     double *d_seg = nullptr;   // (stride + 1) x 3nc
     // work
-    double *d_F = nullptr;     // 5 n_int + 3 n_bnd: edge fluxes of the dry-run step
-    double *d_Fseg = nullptr;  // stride x the same: fluxes of every step of the replayed segment
+    double *d_Fseg = nullptr;  // stride x 3 n_bnd: boundary fluxes of every replayed step
+    double *d_F = nullptr;     // MUSCL-EVR dry run: 5 n_int + 3 n_bnd edge fluxes of the step
+    double *d_sw = nullptr;    // MUSCL-EVR dry run: 2 n_int swept terms
     double *d_Pseg = nullptr;  // stride x 6 nc: balanced state + internal increments per step
-    double *d_sw = nullptr;    // 2 n_int
     double *d_sxy = nullptr;   // 2 nc: limited free-surface slopes of the step (MUSCL-EVR)
     double *d_Sseg = nullptr;  // stride x 2 nc: the slopes of every step of the replayed segment
     double *d_partial = nullptr;
@@ -159,6 +159,13 @@ inline void prof_end(AssimState *a, long i, const char *name) {
         hipLaunchKernelGGL(KERN, __VA_ARGS__);      \
         prof_end(A, pk_, #KERN);                    \
     } while (0)
+// the same with the name given (kernels whose template arguments need parentheses)
+#define KLAUNCH_NAMED(A, NAME, KERN, ...)           \
+    do {                                            \
+        const long pk_ = prof_begin(A);             \
+        hipLaunchKernelGGL(KERN, __VA_ARGS__);      \
+        prof_end(A, pk_, NAME);                     \
+    } while (0)
 
 }  // namespace sw2dg_detail
 
@@ -224,75 +231,37 @@ __global__ __launch_bounds__(kBlock) void k_e_slopes(MeshDev m, const double *__
     sxy[nc + i] = sy;
 }
 
-// Edge fluxes of one _E step.  Threads [0, n_int): internal edges, Godunov _E
-// states + HLLC_E -> F[5e..5e+4] (+ CFL swept terms in the dry run).
-// Threads [n_int, n_int + n_bnd): boundary edges, hypBoundaryFlux_E ->
-// F[5 n_int + 3j ..] with the step's boundary value (bcv, or interpolated at
-// T[n] in the dry run, as build_checkpoint_records does).
-// MUSCL (template): MUSCL-EVR _E states from the step's slopes sxy, the
-// reconstruction reading dt_recon (*dtp when given: the dry run's previous
-// step, 0 before the first; else dtv: the replay's current step).
+// _E Riemann states of the two sides of internal edge e: Godunov
+// (hypGodunovComputeLRStates_E, swSpE_hyp.cpp:507-573) or, MUSCL, the
+// MUSCL-EVR reconstruction from the step's slopes sxy with dt_recon
+// (swSpE_muscl.cpp:66-180)
 template <bool MUSCL>
-__global__ __launch_bounds__(kBlock) void k_e_edges(MeshDev m, Opts o, Controls cs, BcDev bc,
-                                                    const double *__restrict__ U,
-                                                    double *__restrict__ F, double *__restrict__ sw,
-                                                    const double *T, int64_t n, double t_max,
-                                                    const double *bcv, const double *__restrict__ sxy,
-                                                    const double *dtp, double dtv) {
-    if (T && !(T[n] < t_max)) return;
-    const int e = xcd_block() * kBlock + (int)threadIdx.x;
+__device__ __forceinline__ void e_int_states(const MeshDev &m, const Opts &o, const double *__restrict__ U,
+                                             const double *__restrict__ sxy, double dt_recon, int e,
+                                             spe::Side &sL, spe::Side &sR) {
     const int nc = m.n_cell;
-    if (e >= m.n_int) {
-        const int j = e - m.n_int;
-        if (j >= m.n_bnd) return;
-        const int i = m.bcell[j];
-        const double h = U[i];
-        const spe::Side sd =
-            MUSCL ? spe::lr_bnd_muscl(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j])
-                  : spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j],
-                                o.h_min);
-        const int ser = m.bser[j];
-        const double v = bcv ? bcv[ser]
-                             : interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n]);
-        double f[3];
-        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], cs.phi[i], m.zb[i], sd.z, sd.q, sd.r, sd.lp,
-                     bc.type[ser], v, f);
-        double *Fb = F + 5 * (size_t)m.n_int + 3 * (size_t)j;
-        Fb[0] = f[0];
-        Fb[1] = f[1];
-        Fb[2] = f[2];
-        return;
-    }
     const int L = m.eL[e], R = m.eR[e];
     const double nx = m.enx[e], ny = m.eny[e];
-    const double hL = U[L], hR = U[R];
-    spe::Side sL, sR;
     if (MUSCL) {
-        const double dt = dtp ? *dtp : dtv;
         const double xm = m.exm[e], ym = m.eym[e];
-        sL = spe::recon_muscl(0, hL, U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], sxy[L], sxy[nc + L],
-                              nx, ny, xm - m.cx[L], ym - m.cy[L], dt);
-        sR = spe::recon_muscl(1, hR, U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], sxy[R], sxy[nc + R],
-                              nx, ny, xm - m.cx[R], ym - m.cy[R], dt);
+        sL = spe::recon_muscl(0, U[L], U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], sxy[L], sxy[nc + L],
+                              nx, ny, xm - m.cx[L], ym - m.cy[L], dt_recon);
+        sR = spe::recon_muscl(1, U[R], U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], sxy[R], sxy[nc + R],
+                              nx, ny, xm - m.cx[R], ym - m.cy[R], dt_recon);
     } else {
-        sL = spe::lr_fwd(hL, U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], nx, ny, o.h_min);
-        sR = spe::lr_fwd(hR, U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], nx, ny, o.h_min);
+        sL = spe::lr_fwd(U[L], U[nc + L], U[2 * nc + L], m.zb[L], m.beta[L], nx, ny, o.h_min);
+        sR = spe::lr_fwd(U[R], U[nc + R], U[2 * nc + R], m.zb[R], m.beta[R], nx, ny, o.h_min);
     }
-    double f[5];
-    spe::hllc_fwd(o.h_min, o.phi_min, m.beta[L], m.beta[R], cs.phi[L], cs.phi[R], m.zb[L], m.zb[R],
-                  sL.z, sL.q, sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
-    double *Fe = F + 5 * (size_t)e;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Fe[k] = f[k];
-    if (sw) {
-        double m1 = fabs(sL.lm), v = fabs(sL.lp);
-        if (m1 < v) m1 = v;
-        if (m1 < (v = fabs(sR.lm))) m1 = v;
-        if (m1 < (v = fabs(sR.lp))) m1 = v;
-        const double len = m.elen[e];
-        sw[2 * e] = (hL > o.h_min) ? m1 * len : 0.0;
-        sw[2 * e + 1] = (hR > o.h_min) ? m1 * len : 0.0;
-    }
+}
+
+// boundary state of boundary edge j of cell i
+template <bool MUSCL>
+__device__ __forceinline__ spe::Side e_bnd_state(const MeshDev &m, const Opts &o, const double *__restrict__ U,
+                                                 int i, int j) {
+    const int nc = m.n_cell;
+    return MUSCL ? spe::lr_bnd_muscl(U[i], U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j])
+                 : spe::lr_fwd(U[i], U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j],
+                               o.h_min);
 }
 
 __device__ __forceinline__ double wave_min(double v) {
@@ -304,12 +273,105 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
+// MUSCL-EVR dry run: the reconstruction is costly enough that the dry run
+// computes each edge once and keeps its fluxes and swept terms for the CFL
+// and the update (the replay and every Godunov pass compute fluxes in the
+// update itself).  Threads [0, n_int): internal edges -> F[5e..5e+4] and the
+// swept terms sw[2e + side]; threads [n_int, n_int + n_bnd): boundary edges
+// -> F[5 n_int + 3j ..], with the boundary value interpolated at T[n], as
+// build_checkpoint_records does.  dtp: the previous step's dt (nullptr: 0).
+__global__ __launch_bounds__(kBlock) void k_e_edges_muscl(MeshDev m, Opts o, Controls cs, BcDev bc,
+                                                          const double *__restrict__ U,
+                                                          double *__restrict__ F, double *__restrict__ sw,
+                                                          const double *T, int64_t n, double t_max,
+                                                          const double *__restrict__ sxy,
+                                                          const double *dtp) {
+    if (!(T[n] < t_max)) return;
+    const int e = xcd_block() * kBlock + (int)threadIdx.x;
+    if (e >= m.n_int) {
+        const int j = e - m.n_int;
+        if (j >= m.n_bnd) return;
+        const int i = m.bcell[j];
+        const spe::Side sd = e_bnd_state<true>(m, o, U, i, j);
+        const int ser = m.bser[j];
+        const double v = interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n]);
+        double f[3];
+        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], cs.phi[i], m.zb[i], sd.z, sd.q, sd.r, sd.lp,
+                     bc.type[ser], v, f);
+        double *Fb = F + 5 * (size_t)m.n_int + 3 * (size_t)j;
+        Fb[0] = f[0];
+        Fb[1] = f[1];
+        Fb[2] = f[2];
+        return;
+    }
+    const int L = m.eL[e], R = m.eR[e];
+    spe::Side sL, sR;
+    e_int_states<true>(m, o, U, sxy, dtp ? *dtp : 0.0, e, sL, sR);
+    double f[5];
+    spe::hllc_fwd(o.h_min, o.phi_min, m.beta[L], m.beta[R], cs.phi[L], cs.phi[R], m.zb[L], m.zb[R],
+                  sL.z, sL.q, sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+    double *Fe = F + 5 * (size_t)e;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Fe[k] = f[k];
+    double m1 = fabs(sL.lm), v = fabs(sL.lp);
+    if (m1 < v) m1 = v;
+    if (m1 < (v = fabs(sR.lm))) m1 = v;
+    if (m1 < (v = fabs(sR.lp))) m1 = v;
+    const double len = m.elen[e];
+    sw[2 * e] = (U[L] > o.h_min) ? m1 * len : 0.0;
+    sw[2 * e + 1] = (U[R] > o.h_min) ? m1 * len : 0.0;
+}
+
+// the MUSCL-EVR dry run's CFL from the swept terms of k_e_edges_muscl (the
+// same sums as k_e_cfl<true>)
+__global__ __launch_bounds__(kBlock) void k_e_cfl_sw(MeshDev m, Opts o, const double *__restrict__ U,
+                                                     const double *__restrict__ sw, const EClock *ck,
+                                                     double *partial, const double *T, int64_t n) {
+    if (!(T[n] < ck->t_max)) return;
+    __shared__ double red[kBlock / 64];
+    const int lb = xcd_block();
+    const int i = lb * kBlock + (int)threadIdx.x;
+    double cand = ck->dt_max;
+    if (i < m.n_cell) {
+        const double h = U[i];
+        double swept = 0.0;
+        if (h > o.h_min) {
+            for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
+                const int j = m.bnd_inc[k];
+                const spe::Side s = e_bnd_state<true>(m, o, U, i, j);
+                const double lm = (fabs(s.lp) < fabs(s.lm)) ? fabs(s.lm) : fabs(s.lp);
+                swept += lm * m.blen[j];
+            }
+        }
+        for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+            const int inc = m.int_inc[k];
+            swept += sw[2 * (size_t)(inc >> 1) + (inc & 1)];
+        }
+        const double den = smax_(swept, kSmall);
+        const double x = ck->cfl * (m.area[i] / den);
+        if (x < cand) cand = x;
+    }
+    cand = wave_min(cand);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = cand;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = red[0];
+        for (int w = 1; w < kBlock / 64; ++w) v = (red[w] < v) ? red[w] : v;
+        partial[lb] = v;
+    }
+}
+
 // dry-run CFL (hypComputeTimeStep_E, swSpE_hyp.cpp:578-635) and the step
-// schedule of build_checkpoint_records (swSpE_checkpoint.cpp:224-366)
+// schedule of build_checkpoint_records (swSpE_checkpoint.cpp:224-366): per
+// cell, the swept term of each boundary edge (wet cell) and of each internal
+// edge (max |lambda| of its two sides times its length, when this side is
+// wet), summed in the cell's reference incidence order; per-block minima
 template <bool MUSCL>
 __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const double *__restrict__ U,
-                                                  const double *__restrict__ sw, const EClock *ck,
-                                                  double *partial, const double *T, int64_t n) {
+                                                  const double *__restrict__ sxy, const double *dtp,
+                                                  const EClock *ck, double *partial, const double *T,
+                                                  int64_t n) {
     if (!(T[n] < ck->t_max)) return;
     __shared__ double red[kBlock / 64];
     const int lb = xcd_block();
@@ -323,17 +385,22 @@ __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const doubl
         if (h > o.h_min) {
             for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
                 const int j = m.bnd_inc[k];
-                const spe::Side s =
-                    MUSCL ? spe::lr_bnd_muscl(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j])
-                          : spe::lr_fwd(h, U[nc + i], U[2 * nc + i], m.zb[i], m.beta[i], m.bnx[j], m.bny[j],
-                                        o.h_min);
+                const spe::Side s = e_bnd_state<MUSCL>(m, o, U, i, j);
                 const double lm = (fabs(s.lp) < fabs(s.lm)) ? fabs(s.lm) : fabs(s.lp);
                 swept += lm * m.blen[j];
             }
-        }
-        for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
-            const int inc = m.int_inc[k];
-            swept += sw[2 * (size_t)(inc >> 1) + (inc & 1)];
+            // the dry run reconstructs with the previous step's dt (0 before the first)
+            const double dt_recon = (MUSCL && dtp) ? *dtp : 0.0;
+            for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+                const int e = m.int_inc[k] >> 1;
+                spe::Side sL, sR;
+                e_int_states<MUSCL>(m, o, U, sxy, dt_recon, e, sL, sR);
+                double m1 = fabs(sL.lm), v = fabs(sL.lp);
+                if (m1 < v) m1 = v;
+                if (m1 < (v = fabs(sR.lm))) m1 = v;
+                if (m1 < (v = fabs(sR.lp))) m1 = v;
+                swept += m1 * m.elen[e];
+            }
         }
         const double den = smax_(swept, kSmall);
         const double x = ck->cfl * (m.area[i] / den);
@@ -402,12 +469,69 @@ struct CellSt {
 };
 
 // balance of one cell (hypComputeBalance_E, swSpE_hyp.cpp:336-394) before the
-// round-off clean-up, from the step's stored edge fluxes: boundary edges
-// first, then internal edges, each in reference order.
-// dint (optional): sum of the internal-edge contributions alone, in the same order
+// round-off clean-up, with the edge fluxes computed here (hypBoundaryFlux_E,
+// hypHllc_E): boundary edges first, then internal edges, each in reference
+// order.  Every internal edge's flux is computed by both of its cells, from
+// the same inputs by the same operations, so both see the same values.
+// bcv: the step's boundary values per series; Fb (optional): the boundary
+// fluxes out (each boundary edge belongs to one cell), for the reverse sweep;
+// dint (optional): sum of the internal-edge contributions alone, in order.
+template <bool MUSCL>
 __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, const Controls &cs,
-                                            const double *U, const double *F, int i, double dt,
-                                            CellSt *dint = nullptr) {
+                                            const BcDev &bc, const double *__restrict__ U,
+                                            const double *__restrict__ sxy, double dt_recon,
+                                            const double *__restrict__ bcv, int i, double dt,
+                                            double *__restrict__ Fb, CellSt *dint = nullptr) {
+    const int nc = m.n_cell;
+    CellSt s{U[i], U[nc + i], U[2 * nc + i]};
+    const double phi = cs.phi[i], area = m.area[i];
+    for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
+        const int j = m.bnd_inc[k];
+        const double nx = m.bnx[j], ny = m.bny[j];
+        const spe::Side sd = e_bnd_state<MUSCL>(m, o, U, i, j);
+        const int ser = m.bser[j];
+        double f[3];
+        spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], phi, m.zb[i], sd.z, sd.q, sd.r, sd.lp, bc.type[ser],
+                     bcv[ser], f);
+        if (Fb) {
+            Fb[3 * (size_t)j] = f[0];
+            Fb[3 * (size_t)j + 1] = f[1];
+            Fb[3 * (size_t)j + 2] = f[2];
+        }
+        const double co = spe::edge_coeff(m.blen[j], area, phi, o.phi_min);
+        s.h += (+co) * f[0] * dt;
+        s.qx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
+        s.qy += ((+co * ny) * f[1] + (+co * ny) * 0.0 + (+co * nx) * f[2]) * dt;
+    }
+    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+        const int inc = m.int_inc[k];
+        const int e = inc >> 1, side = inc & 1;
+        const int L = m.eL[e], R = m.eR[e];
+        spe::Side sL, sR;
+        e_int_states<MUSCL>(m, o, U, sxy, dt_recon, e, sL, sR);
+        double f[5];
+        spe::hllc_fwd(o.h_min, o.phi_min, m.beta[L], m.beta[R], cs.phi[L], cs.phi[R], m.zb[L], m.zb[R],
+                      sL.z, sL.q, sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+        const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
+        double a, b, c;
+        spe::side_contrib(side, co, m.enx[e], m.eny[e], f, dt, a, b, c);
+        s.h += a;
+        s.qx += b;
+        s.qy += c;
+        if (dint) {
+            dint->h += a;
+            dint->qx += b;
+            dint->qy += c;
+        }
+    }
+    return s;
+}
+
+// the same balance from stored fluxes F (k_e_edges_muscl's layout: 5 per
+// internal edge, then 3 per boundary edge)
+__device__ __forceinline__ CellSt e_balance_stored(const MeshDev &m, const Opts &o, const Controls &cs,
+                                                   const double *__restrict__ U,
+                                                   const double *__restrict__ F, int i, double dt) {
     const int nc = m.n_cell;
     CellSt s{U[i], U[nc + i], U[2 * nc + i]};
     const double phi = cs.phi[i], area = m.area[i];
@@ -430,25 +554,28 @@ __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, con
         s.h += a;
         s.qx += b;
         s.qy += c;
-        if (dint) {
-            dint->h += a;
-            dint->qx += b;
-            dint->qy += c;
-        }
     }
     return s;
 }
 
-// per cell: balance, clean-up, clamp, friction _E, head loss _E.  STORE (the
-// replay before a reverse sweep): also keep the balanced state before the
-// clean-up and the sum of its internal-edge increments, P = [pre | dint]
-template <bool STORE>
-__global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls cs,
+// per cell: edge fluxes and balance, clean-up, clamp, friction _E, head loss
+// _E.  STORED (the MUSCL-EVR dry run): the fluxes come from k_e_edges_muscl
+// in Fst instead.  The dry run (T given) takes the boundary values the schedule kernel
+// recorded for the step and reconstructs (MUSCL) with the previous step's
+// dt (*dtp, 0 before the first); the replay takes dtr, the step's own.
+// STORE (the replay before a reverse sweep): also keep the balanced state
+// before the clean-up and the sum of its internal-edge increments,
+// P = [pre | dint], and the boundary fluxes Fb.
+template <bool STORE, bool MUSCL, bool STORED = false>
+__global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls cs, BcDev bc,
                                                      const double *__restrict__ Uin,
                                                      double *__restrict__ Uout,
-                                                     const double *__restrict__ F,
+                                                     const double *__restrict__ sxy, const double *dtp,
+                                                     double dtr, const double *__restrict__ bcv,
                                                      const double *dtv, double *T, int64_t n,
-                                                     double t_max, double *__restrict__ P) {
+                                                     double t_max, double *__restrict__ P,
+                                                     double *__restrict__ Fb,
+                                                     const double *__restrict__ Fst = nullptr) {
     const int nc = m.n_cell;
     if (T && !(T[n] < t_max)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) T[n + 1] = T[n];
@@ -463,8 +590,11 @@ __global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= nc) return;
     const double dt = dtv[n];
+    const double dt_recon = MUSCL ? (T ? (dtp ? *dtp : 0.0) : dtr) : 0.0;
     CellSt dint{0.0, 0.0, 0.0};
-    CellSt s = e_balance(m, o, cs, Uin, F, i, dt, STORE ? &dint : nullptr);
+    CellSt s = STORED ? e_balance_stored(m, o, cs, Uin, Fst, i, dt)
+                      : e_balance<MUSCL>(m, o, cs, bc, Uin, sxy, dt_recon, bcv, i, dt, Fb,
+                                         STORE ? &dint : nullptr);
     if (STORE) {
         P[i] = s.h;
         P[nc + i] = s.qx;
@@ -774,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs
     double ph = part[i], px = part[nc + i], py = part[2 * nc + i], gp = 0.0;
     const double phi = cs.phi[i], area = m.area[i];
     const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i];
-    const double *Fb = F + 5 * (size_t)m.n_int;
+    const double *Fb = F;  // the step's boundary fluxes, 3 per boundary edge
     for (int k = m.bnd_ptr[i]; k < m.bnd_ptr[i + 1]; ++k) {
         const int j = m.bnd_inc[k];
         const double nx = m.bnx[j], ny = m.bny[j];
@@ -1020,54 +1150,61 @@ double *ckpt_state(AssimState *a, sw2dg_ctx *c, int64_t k) {
     return a->d_ckpt + (size_t)k * 3 * c->nc;
 }
 
-// fluxes of one step: 5 per internal edge, then 3 per boundary edge
-size_t flux_len(const sw2dg_ctx *c) {
-    return 5 * (size_t)std::max<int64_t>(c->ni, 1) + 3 * (size_t)std::max<int64_t>(c->nb, 1);
-}
+// boundary fluxes of one step (3 per boundary edge), kept per replayed step
+// for the boundary reverse
+size_t flux_len(const sw2dg_ctx *c) { return 3 * (size_t)std::max<int64_t>(c->nb, 1); }
 
 double *seg_flux(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Fseg + (size_t)j * flux_len(c); }
 double *seg_pre(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Pseg + (size_t)j * 6 * c->nc; }
 double *seg_slopes(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Sseg + (size_t)j * 2 * c->nc; }
 
-// one _E step U_in -> U_out; the edge fluxes go to F (kept for the reverse
-// sweep when F is a slot of the segment flux buffer)
+// one _E step U_in -> U_out.  Dry run: (MUSCL slopes,) CFL partials, the
+// schedule (dt, boundary values, observation trigger), the update.  Replay:
+// (MUSCL slopes,) the update on the recorded schedule; Fb (a slot of the
+// segment's boundary-flux buffer), P and S keep what the reverse sweep reads.
+// The dry run reconstructs (MUSCL) with the previous step's dt (0 before the
+// first step), the replay with the step's own.
 void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout, int64_t n,
-                   bool dry, double *F, double *P = nullptr, double *S = nullptr) {
-    const dim3 blk(kBlock);
+                   bool dry, double *Fb, double *P = nullptr, double *S = nullptr) {
+    const dim3 blk(kBlock), gc(nblocks(c->nc));
     const Controls cs = ctl(a);
     const int ns = std::max(c->n_series, 1);
     const bool muscl = c->o.scheme == 1;
-    const dim3 ge(nblocks(c->ni + c->nb));
-    const double *bcv = dry ? nullptr : a->d_bc + (size_t)n * ns;
-    if (muscl) {
-        // the dry run reconstructs with the previous step's dt (0 before the
-        // first step), the replay with the step's own dt (see k_e_edges)
-        double *sxy = S ? S : a->d_sxy;
-        KLAUNCH(a, k_e_slopes, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, Uin, sxy,
-                dry ? a->d_T : nullptr, n, c->prm.t_max);
-        KLAUNCH(a, k_e_edges<true>, ge, blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin, F,
-                dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max, bcv, sxy,
-                (dry && n > 0) ? a->d_dt + (n - 1) : nullptr, dry ? 0.0 : a->h_dt[n]);
-    } else {
-        KLAUNCH(a, k_e_edges<false>, ge, blk, 0, c->stream, c->m, c->o, cs, c->bc, Uin, F,
-                dry ? a->d_sw : nullptr, a->d_T, n, c->prm.t_max, bcv, nullptr, nullptr, 0.0);
-    }
+    const double *bcv = a->d_bc + (size_t)n * ns;
+    double *sxy = S ? S : a->d_sxy;
+    const double *dtp = (dry && n > 0) ? a->d_dt + (n - 1) : nullptr;
+    const double dtr = dry ? 0.0 : a->h_dt[n];
+    double *T = dry ? a->d_T : nullptr;
+    if (muscl)
+        KLAUNCH(a, k_e_slopes, gc, blk, 0, c->stream, c->m, Uin, sxy, T, n, c->prm.t_max);
     if (dry) {
-        if (muscl)
-            KLAUNCH(a, k_e_cfl<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
-                    a->d_ck, a->d_partial, a->d_T, n);
-        else
-            KLAUNCH(a, k_e_cfl<false>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, Uin, a->d_sw,
-                    a->d_ck, a->d_partial, a->d_T, n);
+        if (muscl) {  // edges once, their fluxes and swept terms kept
+            KLAUNCH(a, k_e_edges_muscl, dim3(nblocks(c->ni + c->nb)), blk, 0, c->stream, c->m, c->o, cs,
+                    c->bc, Uin, a->d_F, a->d_sw, a->d_T, n, c->prm.t_max, sxy, dtp);
+            KLAUNCH(a, k_e_cfl_sw, gc, blk, 0, c->stream, c->m, c->o, Uin, a->d_sw, a->d_ck, a->d_partial,
+                    a->d_T, n);
+        } else {
+            KLAUNCH(a, k_e_cfl<false>, gc, blk, 0, c->stream, c->m, c->o, Uin, sxy, dtp, a->d_ck,
+                    a->d_partial, a->d_T, n);
+        }
         KLAUNCH(a, k_e_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
                 a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
     }
-    if (P)
-        KLAUNCH(a, k_e_update<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, Uin,
-                Uout, F, a->d_dt, dry ? a->d_T : nullptr, n, c->prm.t_max, P);
-    else
-        KLAUNCH(a, k_e_update<false>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, cs, Uin,
-                Uout, F, a->d_dt, dry ? a->d_T : nullptr, n, c->prm.t_max, P);
+#define E_UPDATE(STORE, MUS)                                                                       \
+    KLAUNCH_NAMED(a, "k_e_update<" #STORE ", " #MUS ">", (k_e_update<STORE, MUS>), gc, blk, 0, c->stream, \
+                  c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n, c->prm.t_max, P, Fb)
+    if (P) {
+        if (muscl) E_UPDATE(true, true);
+        else E_UPDATE(true, false);
+    } else if (dry && muscl) {
+        KLAUNCH_NAMED(a, "k_e_update<false, true, true>", (k_e_update<false, true, true>), gc, blk, 0,
+                      c->stream, c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n,
+                      c->prm.t_max, P, Fb, a->d_F);
+    } else {
+        if (muscl) E_UPDATE(false, true);
+        else E_UPDATE(false, false);
+    }
+#undef E_UPDATE
     if (dry && a->n_gauges > 0)
         KLAUNCH(a, k_gauge_sample, dim3(1), dim3(64), 0, c->stream, Uout, a->d_gcell,
                            a->n_gauges, a->d_trig, n, a->d_gser);
@@ -1338,12 +1475,12 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->n_ckpt = (max_steps + a->stride - 1) / a->stride + 1;
     a->d_ckpt = a->alloc<double>((size_t)a->n_ckpt * 3 * nc);
     a->d_seg = a->alloc<double>((size_t)(a->stride + 1) * 3 * nc);
-    a->d_F = a->alloc<double>(flux_len(c));
     a->d_Fseg = a->alloc<double>((size_t)a->stride * flux_len(c));
     a->d_Pseg = a->alloc<double>((size_t)a->stride * 6 * (size_t)nc);
-    a->d_sw = a->alloc<double>(2 * (size_t)std::max<int64_t>(ni, 1));
     const bool muscl = c->o.scheme == 1;
     a->d_sxy = a->alloc<double>(muscl ? 2 * (size_t)nc : 1);
+    a->d_F = a->alloc<double>(muscl ? 5 * (size_t)std::max<int64_t>(ni, 1) + 3 * (size_t)std::max<int64_t>(nb, 1) : 1);
+    a->d_sw = a->alloc<double>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
     a->d_Sseg = a->alloc<double>(muscl ? (size_t)a->stride * 2 * (size_t)nc : 1);
     a->d_sd = a->alloc<double>(muscl ? 4 * (size_t)std::max<int64_t>(ni, 1) : 1);
     a->d_zn = a->alloc<double>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
@@ -1425,7 +1562,7 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
         const int64_t n1 = std::min(a->max_steps, n + S);
         for (int64_t m2 = n; m2 < n1; ++m2) {
             const int j = (int)(m2 - n);
-            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true, a->d_F);
+            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true, nullptr);
         }
         HIPCHK(c, hipMemcpyAsync(ckpt_state(a, c, k + 1), seg_state(a, c, (int)(n1 - n)),
                                  3 * nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
