@@ -90,13 +90,13 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
         # fused flux + update: cells U in/out, zb, beta, phi, area, K1, K2, cos,
         # sin, ptrs (+ slopes, centroids); incidences, edge cells and geometry
         # (+ midpoints); boundary incidences, geometry, series; the replay
-        # (<true, .>) also writes the balanced state + increments and the
+        # (<true, ., .>) also writes the balanced state + increments and the
         # boundary fluxes.  Each internal edge's flux is computed by both of its
         # cells; the bytes count its inputs once.
-        "k_e_update<false, false>": 120 * nc + 40 * ni + 32 * nb,
-        "k_e_update<true, false>": 168 * nc + 40 * ni + 56 * nb,
-        "k_e_update<false, true>": 152 * nc + 56 * ni + 32 * nb,
-        "k_e_update<true, true>": 200 * nc + 56 * ni + 56 * nb,
+        "k_e_update<false, false, false>": 120 * nc + 40 * ni + 32 * nb,
+        "k_e_update<true, false, false>": 168 * nc + 40 * ni + 56 * nb,
+        "k_e_update<false, true, false>": 152 * nc + 56 * ni + 32 * nb,
+        "k_e_update<true, true, false>": 200 * nc + 56 * ni + 56 * nb,
         # MUSCL-EVR dry run: edges once (states, fluxes and swept terms out),
         # the CFL from the swept terms, the update from the stored fluxes
         "k_e_edges_muscl": 80 * nc + 80 * ni + 48 * nb,

@@ -1191,7 +1191,7 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
                 a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
     }
 #define E_UPDATE(STORE, MUS)                                                                       \
-    KLAUNCH_NAMED(a, "k_e_update<" #STORE ", " #MUS ">", (k_e_update<STORE, MUS>), gc, blk, 0, c->stream, \
+    KLAUNCH_NAMED(a, "k_e_update<" #STORE ", " #MUS ", false>", (k_e_update<STORE, MUS>), gc, blk, 0, c->stream, \
                   c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n, c->prm.t_max, P, Fb)
     if (P) {
         if (muscl) E_UPDATE(true, true);
