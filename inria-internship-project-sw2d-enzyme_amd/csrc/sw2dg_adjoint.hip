@@ -6,7 +6,7 @@
 //       the per-series BC values and the observation triggers of every step,
 //       landing on observation times, and keeps a state checkpoint every
 //       `stride` steps (sqrt(N) checkpointing)
-//   ckpt_segment_E (584-715)           -> k_e_edges + k_e_update replayed with
+//   ckpt_segment_E (584-715)           -> k_e_update (fluxes + update) replayed with
 //       the frozen schedule; cost at observation steps
 //   get_spatial_gradient_checkpointed (722-1281) -> assim_gradient: reverse
 //       loop over segments (K-1 .. window start), each segment recomputed

@@ -5,7 +5,7 @@
 // by a cluster of c >= 1 workgroups: edge fluxes -> CFL minimum with the
 // observation landing of build_checkpoint_records
 // (assim/swSpE_checkpoint.cpp:224-366) -> cell update -> gauge sampling.
-// The per-step operations are those of the single-run dry run (k_e_edges /
+// The per-step operations are those of the single-run dry run (edge fluxes /
 // k_e_cfl / k_e_dt / k_e_update in sw2dg_adjoint.hip), element by element
 // and in the same order, so each run reproduces sw2dg_assim_record on its
 // mesh bit for bit whatever its cluster size (the CFL minimum is exact in
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double
     while (t < t_max && step < max_steps) {
         if (tid < R.ns) s_bcv[tid] = interp_e_(st, sv + (size_t)tid * R.nt, R.nt, t);
         __syncthreads();
-        // ---- edge fluxes (k_e_edges)
+        // ---- edge fluxes (as the single-run dry run's)
         for (int e = e_lo + tid; e < e_hi; e += B) {
             if (e >= ni) {
                 const int j = e - ni;

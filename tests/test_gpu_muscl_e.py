@@ -1,4 +1,4 @@
-"""MUSCL-EVR _E on the GPU: forward (k_e_slopes + k_e_edges<true>) and its
+"""MUSCL-EVR _E on the GPU: forward (k_e_slopes, k_e_edges_muscl, k_e_update) and its
 adjoint (k_r_edge<true> + k_r_slopes + k_r_bnd<true>).
 
 The dry run against the reference's own _E physics is in test_gpu_adjoint.py
