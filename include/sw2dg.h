@@ -82,6 +82,18 @@ int sw2dg_set_cell_params(sw2dg_ctx *ctx, const double *phi, const double *beta,
  * column-major ([series][time]); type[] uses SW2DG_BC_*. */
 int sw2dg_set_bc_series(sw2dg_ctx *ctx, int32_t n_series, int32_t n_times, const int32_t *type,
                         const double *t, const double *v);
+/* Replaces swParent::setWindStationCodes and setWindVelocityTimeSeriesTables
+ * (swParent.cpp:571-619) and switches on swSp::sourceWind for model_wind
+ * smithbanke (swSpSourceWind.cpp:11-85, called from swSp.cpp:582-587): each
+ * step interpolates the station velocities at t + dt/2 and adds the Smith &
+ * Banke stress after friction and head loss.  code[n_cell] (reference order)
+ * is each cell's 1-based station; t[n_times] non-decreasing record times;
+ * vx, vy[n_station * n_times] the Cartesian wind velocity per station
+ * ([station][record]; the polar-to-Cartesian step of swSourceWind.cpp:9-38
+ * stays with the caller).  n_station = 0 turns the source off.  The 4D-Var
+ * layer (sw2dg_assim_*) has no wind source and refuses a ctx that has one. */
+int sw2dg_set_wind(sw2dg_ctx *ctx, int32_t n_station, int32_t n_times, const int32_t *code,
+                   const double *t, const double *vx, const double *vy);
 /* Replaces swSp::setInitialConditions (swSp.cpp:1119-1224): sets the state
  * and the clock (t, previous dt = 0, step count 0, BC cursors rewound). */
 int sw2dg_set_state(sw2dg_ctx *ctx, const double *h, const double *qx, const double *qy,

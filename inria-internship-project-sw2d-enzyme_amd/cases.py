@@ -126,16 +126,27 @@ def write_case(folder, geom, solv, h0, qx0, qy0, phi=None, beta=None, friction="
 
 def dambreak_case(folder, nx=100, ny=100, Lx=200.0, Ly=100.0, x_dam=100.0, h_left=2.0,
                   h_right=0.5, scheme="muscl_evr", topo="discontinuous", friction="manning",
-                  n_manning=0.02, t_max=10.0, cfl=0.9, dt_map=1e6, dt_ts=1e6, store_flag="default"):
-    """Plain SP forward dam-break (20,000 triangles at the default size)."""
+                  n_manning=0.02, t_max=10.0, cfl=0.9, dt_map=1e6, dt_ts=1e6, store_flag="default",
+                  wind=False):
+    """Plain SP forward dam-break (20,000 triangles at the default size).
+    With wind=True it adds the Smith & Banke wind source: one station on each
+    side of the dam, records at 1/6, 1/2 and 5/6 of t_max."""
     g = Geometry.rect(nx, ny, Lx, Ly)
     x = g.array("cell_x")
     h0 = np.where(x < x_dam, h_left, h_right)
     solv = solv_text(scheme=scheme, topo=topo, friction=friction, cfl=cfl, t_max=t_max,
-                     dt_max=10.0, dt_map=dt_map, dt_ts=dt_ts, store_flag=store_flag)
+                     dt_max=10.0, dt_map=dt_map, dt_ts=dt_ts, store_flag=store_flag,
+                     extra={"model_wind": "smithbanke"} if wind else None)
     write_case(folder, g, solv, h0, 0.0, 0.0, friction=friction,
                fric=None if friction == "none" else n_manning,
                bc_codes=(1,), bc_types=("q",), bc_t=(0.0, 1e9), bc_v=((0.0, 0.0),))
+    if wind:
+        write_distrib_map(os.path.join(folder, "input", "wind_station_codes_map.txt"),
+                          ["wind_station"], [np.where(x < x_dam, 1, 2)])
+        tw = t_max * np.array([1.0, 3.0, 5.0]) / 6.0
+        write_wind_series(os.path.join(folder, "input", "wind_time_series.txt"), tw,
+                          np.array([[30.0, 40.0, 25.0], [35.0, 20.0, 45.0]]),
+                          np.array([[30.0, 120.0, 250.0], [300.0, 10.0, 170.0]]))
     return folder
 
 
@@ -260,13 +271,17 @@ def write_distrib_map(path, names, cols):
 
 def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
                topo="discontinuous", friction="strickler", head_loss=True, t_max=20.0,
-               dt_map=1e6, dt_ts=1e6, cfl=0.9, store_flag="default", closed_fraction=0.04):
+               dt_map=1e6, dt_ts=1e6, cfl=0.9, store_flag="default", closed_fraction=0.04,
+               wind=False):
     """Small case that exercises every branch of the SP step: Delaunay mesh,
     bumpy bed with wet/dry fronts, porosity field including closed cells
     (phi = 0), all four boundary types (west q hydrograph, east fr, north h,
     south z) and singular head loss.  (With MUSCL-EVR the reference stops on
     an impossible HLLC regime when a dry closed cell borders a dry cell, so
-    MUSCL cases use closed_fraction = 0.)"""
+    MUSCL cases use closed_fraction = 0.)  With wind=True it adds the Smith &
+    Banke wind source: two stations (west and east halves) whose records start
+    after t = 0 and end before t_max, so both clamps and the interpolation
+    run."""
     rng = np.random.default_rng(seed)
     Lx, Ly = L
     g = Geometry.delaunay(px, py, Lx, Ly, jitter=0.25, seed=seed)
@@ -300,7 +315,7 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
     ])
     solv = solv_text(scheme=scheme, topo=topo, friction=friction, cfl=cfl, t_max=t_max,
                      dt_max=0.5, dt_map=dt_map, dt_ts=dt_ts, head_loss=head_loss,
-                     store_flag=store_flag)
+                     store_flag=store_flag, extra={"model_wind": "smithbanke"} if wind else None)
     write_case(folder, g, solv, h0, qx0, qy0, phi=phi, friction=friction, fric=fric,
                bc_codes=(1, 2, 3, 4, 5), bc_types=("q", "q", "fr", "h", "z"), bc_t=tq, bc_v=bc_v)
     if head_loss:
@@ -310,7 +325,28 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
         al = rng.uniform(-30, 30, nc)
         write_distrib_map(os.path.join(folder, "input", "singular_head_loss_map.txt"),
                           ["h_sing", "k1", "k2", "alpha"], [hs, k1, k2, al])
+    if wind:
+        station = np.where(cx < 0.5 * Lx, 1, 2)
+        write_distrib_map(os.path.join(folder, "input", "wind_station_codes_map.txt"),
+                          ["wind_station"], [station])
+        tw = np.array([2.0, 7.0, 11.0, 0.75 * t_max])
+        norm = rng.uniform(15.0, 40.0, (2, len(tw)))
+        angle = rng.uniform(0.0, 360.0, (2, len(tw)))
+        write_wind_series(os.path.join(folder, "input", "wind_time_series.txt"), tw, norm, angle)
     return folder
+
+
+def write_wind_series(path, t, norm, angle_deg):
+    """wind_time_series.txt (layout read by swIO::readTimeSeries2dVectorFile,
+    src/common/swIO.cpp:906-1000): per record the time, then (norm, angle
+    from North in degrees) for each station."""
+    norm, angle_deg = np.atleast_2d(norm), np.atleast_2d(angle_deg)
+    with open(path, "w") as f:
+        f.write(f"# wind time series\n# number of stations\n{norm.shape[0]}\n")
+        f.write("# records\n# time" + "".join(f"\tnorm_{k + 1}\tangle_{k + 1}" for k in range(norm.shape[0])) + "\n")
+        for r in range(len(t)):
+            f.write(f"{t[r]:.17e}" + "".join(f"\t{norm[k, r]:.17e}\t{angle_deg[k, r]:.17e}"
+                                          for k in range(norm.shape[0])) + "\n")
 
 
 class Case:
@@ -319,7 +355,8 @@ class Case:
     ARRAYS = ("cell_area", "cell_x", "cell_y", "cell_zb", "bnd_cell", "bnd_len", "bnd_nx",
               "bnd_ny", "bnd_series", "bnd_code", "int_L", "int_R", "int_len", "int_nx", "int_ny",
               "int_xm", "int_ym", "h0", "qx0", "qy0", "phi", "beta", "fc1", "fc2", "falpha",
-              "hsing", "hk1", "hk2", "halpha", "series_type", "series_t", "series_v")
+              "hsing", "hk1", "hk2", "halpha", "series_type", "series_t", "series_v",
+              "wind_code", "wind_t", "wind_x", "wind_y")
 
     def __init__(self, folder):
         self.folder = str(folder)

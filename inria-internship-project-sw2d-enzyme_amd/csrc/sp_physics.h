@@ -18,6 +18,8 @@ namespace sp {
 
 constexpr double kSmall = 1e-16;  // src/common/param.h:2 (small_real)
 constexpr double kG = 9.81;       // src/common/param.h:3 (gravity)
+constexpr double kRhoAir = 1.2;     // src/common/param.h:4 (rho_air)
+constexpr double kRhoWater = 1.0e3; // src/common/param.h:5 (rho_water)
 
 enum BcType { BC_FR = 0, BC_H = 1, BC_Z = 2, BC_Q = 3 };
 enum Friction { FRIC_NONE = 0, FRIC_MANNING = 1, FRIC_STRICKLER = 2, FRIC_CHEZY = 3 };

@@ -333,6 +333,27 @@ __global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, Clock *ck, in
     }
 }
 
+// Record bracket of swParent::interpolateTimeSeries (swParent.cpp:331-387)
+// at time tw on non-decreasing record times: -1 at or before the first record
+// less small_real (value of record 0), -2 at or after the last record (value
+// of the last), else the first k >= 1 with tw <= t[k] (its sweep stops there;
+// the value interpolates records k-1 and k).  A one-record table gives -1 in
+// the sliver (t[0] - small_real, t[0]), where the reference reads past it.
+__device__ int wind_bracket(const double *tv, int n, double tw) {
+    if (tw <= tv[0] - kSmall) return -1;
+    if (tw >= tv[n - 1]) return -2;
+    if (n < 2) return -1;
+    int lo = 1, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (tw <= tv[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck, int p, int n_part) {
     if (!step_active(ck, p)) return;
     __shared__ double red[kDtBlock / 64];
@@ -357,6 +378,11 @@ __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck,
     }
     if ((t + dt) > ck->t_max) dt = ck->t_max - t;
     for (int k = 0; k < ck->n_series; ++k) bc_advance(bc, k, t);
+    if (m.wind_nt > 0) {
+        const double tw = t + 0.5 * dt;
+        ck->wind_tw = tw;
+        ck->wind_k = wind_bracket(m.wind_t, m.wind_nt, tw);
+    }
     const double tn = t + dt;
     int sf = 0;
     if (ck->dt_map <= -kSmall) {
@@ -476,6 +502,34 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
         }
         qx = ax * ca - ay * sa;
         qy = ax * sa + ay * ca;
+    }
+    if (o.wind) {
+        // swSp::sourceWind + sourceWindSmithBanke (swSpSourceWind.cpp:11-85):
+        // station velocity interpolated at t + dt/2, then the Smith & Banke
+        // stress (its drag coefficient grows with the water speed |q|/h)
+        const int k = ck->wind_k, nt = m.wind_nt;
+        const double *X = m.wind_x + (size_t)m.wcode[i] * nt;
+        const double *Y = m.wind_y + (size_t)m.wcode[i] * nt;
+        double wx, wy;
+        if (k < 1) {
+            const int j = (k == -2) ? nt - 1 : 0;
+            wx = X[j];
+            wy = Y[j];
+        } else {
+            const double t1 = m.wind_t[k - 1], t2 = m.wind_t[k], tw = ck->wind_tw;
+            wx = X[k - 1] + (X[k] - X[k - 1]) * (tw - t1) / (t2 - t1);
+            wy = Y[k - 1] + (Y[k] - Y[k - 1]) * (tw - t1) / (t2 - t1);
+        }
+        if (h > kSmall) {
+            const double u = qx / h, v = qy / h;
+            const double rx = wx - u, ry = wy - v;
+            const double un = sqrt(u * u + v * v);
+            const double rn = sqrt(rx * rx + ry * ry);
+            const double csd = 1e-3 * (0.63 + 0.066 * un);
+            const double mc = smin(h, o.h_min) / smax(o.h_min, kSmall);
+            qx = qx + sp::kRhoAir * csd * rn * rx / sp::kRhoWater * dt * mc;
+            qy = qy + sp::kRhoAir * csd * rn * ry / sp::kRhoWater * dt * mc;
+        }
     }
     if (h > o.h_min) {
         const double qn = sqrt(qx * qx + qy * qy);
@@ -929,6 +983,54 @@ int sw2dg_set_bc_series(sw2dg_ctx *c, int32_t n_series, int32_t n_times, const i
     c->bc.n_times = nt;
     c->hck.n_series = n_series;
     if (c->gexec) {  // arguments changed: rebuild the graph
+        hipGraphExecDestroy(c->gexec);
+        hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+    }
+    return SW2DG_OK;
+}
+
+int sw2dg_set_wind(sw2dg_ctx *c, int32_t n_station, int32_t n_times, const int32_t *code,
+                   const double *t, const double *vx, const double *vy) {
+    if (!c || n_station < 0) return SW2DG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n_station > 0 && c->assim)
+        return fail(c, SW2DG_ERR_ARG, "the 4D-Var layer has no wind source");
+    if (n_station == 0) {
+        c->o.wind = 0;
+    } else {
+        if (n_times < 1 || !code || !t || !vx || !vy) return SW2DG_ERR_ARG;
+        for (int k = 1; k < n_times; ++k)
+            if (t[k - 1] > t[k])
+                return fail(c, SW2DG_ERR_ARG, "wind record times must be non-decreasing");
+        std::vector<int> wc(c->nc);
+        for (int64_t k = 0; k < c->nc; ++k) {
+            const int32_t s = code[c->perm[k]];
+            if (s < 1 || s > n_station)
+                return fail(c, SW2DG_ERR_ARG, "a wind station code lies outside [1, n_station]");
+            wc[k] = s - 1;
+        }
+        const size_t nv = (size_t)n_station * n_times;
+        if (!c->m.wcode) c->m.wcode = c->upload(wc);
+        else HIPCHK(c, hipMemcpy((void *)c->m.wcode, wc.data(), wc.size() * sizeof(int),
+                                 hipMemcpyHostToDevice));
+        if (c->m.wind_nt != n_times || c->wind_nv != nv) {
+            // a new table shape: fresh buffers (the old ones stay owned by the ctx)
+            c->m.wind_t = c->dalloc<double>(n_times);
+            c->m.wind_x = c->dalloc<double>(nv);
+            c->m.wind_y = c->dalloc<double>(nv);
+            c->wind_nv = nv;
+        }
+        if (!c->m.wcode || !c->m.wind_t || !c->m.wind_x || !c->m.wind_y)
+            return fail(c, SW2DG_ERR_HIP, "wind tables: device allocation failed");
+        HIPCHK(c, hipMemcpy((void *)c->m.wind_t, t, n_times * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->m.wind_x, vx, nv * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->m.wind_y, vy, nv * sizeof(double), hipMemcpyHostToDevice));
+        c->o.wind = 1;
+    }
+    c->m.wind_nt = c->o.wind ? n_times : 0;
+    if (c->gexec) {  // kernel arguments changed: rebuild the graph
         hipGraphExecDestroy(c->gexec);
         hipGraphDestroy(c->graph);
         c->gexec = nullptr;

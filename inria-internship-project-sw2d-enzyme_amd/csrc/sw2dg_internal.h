@@ -31,6 +31,8 @@ struct Clock {
     long long step_stop;
     double dt_last;      // dt of the last active step
     int n_series;
+    double wind_tw;      // wind interpolation time of the step (t + dt/2)
+    int wind_k;          // record bracket at wind_tw: -1 first record, -2 last, k >= 1 (k-1, k)
 };
 
 struct BcDev {
@@ -65,10 +67,15 @@ struct MeshDev {
     int n_bnd_cells;
     double *bF1;
     double *partial;               // [n_blocks of k_cfl]: per-block dt candidates
+    // wind (model_wind smithbanke): 0-based station of each cell, record
+    // times, Cartesian velocity per station ([station][record])
+    const int *wcode;
+    const double *wind_t, *wind_x, *wind_y;
+    int wind_nt;
 };
 
 struct Opts {
-    int scheme, topo, friction, head_loss, neg_det;
+    int scheme, topo, friction, head_loss, neg_det, wind;
     double h_min, phi_min, v_max, fr_max;
     double adj_cut;  // reverse sweep: adjoint zeroed at cells shallower than this (0: exact)
 };
@@ -120,6 +127,7 @@ struct sw2dg_ctx {
     std::vector<void *> allocs;
     int64_t bytes = 0;
     int n_series = 0, n_times = 0;
+    size_t wind_nv = 0;  // station x record size of the device wind tables
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool state_set = false;

@@ -265,6 +265,10 @@ const void *sw2dh_case_array(const sw2dh_case *h, const char *name, int64_t *n, 
     if (k == "series_type") return view(h->stype, n, dt, 2);
     if (k == "series_t") return view(c.bc.t, n, dt, 0);
     if (k == "series_v") return view(h->sval, n, dt, 0);
+    if (k == "wind_code") return view(c.wind_code, n, dt, 2);
+    if (k == "wind_t") return view(c.wind_t, n, dt, 0);
+    if (k == "wind_x") return view(c.wind_x, n, dt, 0);
+    if (k == "wind_y") return view(c.wind_y, n, dt, 0);
     return geom_array(c.g, c.bcell64, c.L64, c.R64, k, n, dt);
 }
 

@@ -68,6 +68,10 @@ int main(int argc, char **argv) {
     check(sw2dg_set_bc_series(ctx, (int32_t)types.size(), (int32_t)cs.bc.t.size(), types.data(),
                               cs.bc.t.data(), vals.data()),
           "set_bc_series");
+    if (cs.wind_stations > 0)
+        check(sw2dg_set_wind(ctx, cs.wind_stations, (int32_t)cs.wind_t.size(), cs.wind_code.data(),
+                             cs.wind_t.data(), cs.wind_x.data(), cs.wind_y.data()),
+              "set_wind");
     check(sw2dg_set_state(ctx, cs.h.data(), cs.qx.data(), cs.qy.data(), prm.t0), "set_state");
 
     const std::string out = (fs::path(folder) / "output").string() + "/";

@@ -671,6 +671,58 @@ bool read_bc_series(const std::string &path, BcSeries &bc, std::string &err) {
     return true;
 }
 
+bool read_wind_series(const std::string &path, std::vector<double> &t,
+                      std::vector<std::vector<double>> &norm,
+                      std::vector<std::vector<double>> &angle_deg, std::string &err) {
+    std::ifstream f(path);
+    if (!f) {
+        err = path + " cannot be opened";
+        return false;
+    }
+    std::string line;
+    get_line(f, line);
+    get_line(f, line);
+    get_line(f, line);
+    const long ns = std::atol(trim(line).c_str());
+    if (ns < 0) {
+        err = path + ": bad station count";
+        return false;
+    }
+    get_line(f, line);
+    get_line(f, line);
+    t.clear();
+    norm.assign(ns, {});
+    angle_deg.assign(ns, {});
+    while (get_line(f, line)) {
+        auto tok = split_ws(line);
+        if (tok.empty()) continue;
+        if ((long)tok.size() < 2 * ns + 1) {
+            err = path + ": line is misformatted";
+            return false;
+        }
+        double v;
+        if (!to_double(tok[0], v)) {
+            err = path + ": format conversion";
+            return false;
+        }
+        t.push_back(v);
+        for (long j = 0; j < ns; ++j) {
+            double a, b;
+            if (!to_double(tok[2 * j + 1], a) || !to_double(tok[2 * j + 2], b)) {
+                err = path + ": format conversion";
+                return false;
+            }
+            norm[j].push_back(a);
+            angle_deg[j].push_back(b);
+        }
+    }
+    if (t.empty()) {
+        err = "The file " + path + " does not contain any wind values";
+        return false;
+    }
+    return true;
+}
+
 bool write_bc_series(const std::string &path, const BcSeries &bc, std::string &err) {
     std::ofstream f(path);
     if (!f) {
@@ -861,11 +913,15 @@ bool load_case(const std::string &folder, Case &c, std::string &err) {
         return false;
     }
     for (const auto *opt : {&c.cfg.div_corr, &c.cfg.model_runoff, &c.cfg.model_infiltration,
-                            &c.cfg.model_wind, &c.cfg.model_momentum_diffusion})
+                            &c.cfg.model_momentum_diffusion})
         if (*opt != "none") {
             err = "option not handled by the GPU solver: " + *opt;
             return false;
         }
+    if (c.cfg.model_wind != "none" && c.cfg.model_wind != "smithbanke") {
+        err = "Wind model unknown : " + c.cfg.model_wind;  // swSpSourceWind.cpp:37-41
+        return false;
+    }
     if (!read_geometry(in + "geometry.sw2d", c.g, err)) return false;
     const size_t nc = c.g.n_cell();
     std::vector<std::string> names;
@@ -968,6 +1024,51 @@ bool load_case(const std::string &folder, Case &c, std::string &err) {
         c.hk1 = cols[1];
         c.hk2 = cols[2];
         c.halpha = cols[3];
+    }
+    if (c.cfg.model_wind != "none") {
+        // station codes: swParent::setWindStationCodes (swParent.cpp:571-595)
+        // and the checks of swSp::init (swSp.cpp:292-333)
+        if (!read_map(in + "wind_station_codes_map.txt", 1, nc, names, cols, err)) return false;
+        c.wind_code.assign(nc, 0);
+        for (size_t i = 0; i < nc; ++i) {
+            double ip;
+            const double x = cols[0][i];
+            if (std::abs(std::modf(x, &ip)) > 1e-16 || x < 0) {
+                err = "Non-integer/negative wind code read for cell " + std::to_string(i);
+                return false;
+            }
+            if (x < 1) {
+                err = "Wind station code is zero or smaller in cell " + std::to_string(i);
+                return false;
+            }
+            c.wind_code[i] = (int32_t)x;
+        }
+        std::vector<std::vector<double>> norm, angle;
+        if (!read_wind_series(in + "wind_time_series.txt", c.wind_t, norm, angle, err))
+            return false;
+        for (size_t k = 1; k < c.wind_t.size(); ++k)
+            if (c.wind_t[k - 1] > c.wind_t[k]) {
+                err = "Wind time series - non-increasing times for records " + std::to_string(k - 1) +
+                      " " + std::to_string(k);
+                return false;
+            }
+        c.wind_stations = (int32_t)norm.size();
+        for (size_t i = 0; i < nc; ++i)
+            if (c.wind_code[i] > c.wind_stations) {
+                err = "Wind station code " + std::to_string(c.wind_code[i]) + " of cell " +
+                      std::to_string(i) + " exceeds the number of stations";
+                return false;
+            }
+        // (norm, angle clockwise from North) -> (x, y): swSourceWind.cpp:9-38
+        const size_t nt = c.wind_t.size();
+        c.wind_x.assign(norm.size() * nt, 0.0);
+        c.wind_y.assign(norm.size() * nt, 0.0);
+        for (size_t s = 0; s < norm.size(); ++s)
+            for (size_t r = 0; r < nt; ++r) {
+                const double a = atan(1.0) / 45. * (90. - angle[s][r]);
+                c.wind_x[s * nt + r] = -norm[s][r] * cos(a);
+                c.wind_y[s * nt + r] = -norm[s][r] * sin(a);
+            }
     }
     c.bcell64.assign(c.g.bnd_cell.begin(), c.g.bnd_cell.end());
     c.L64.assign(c.g.int_L.begin(), c.g.int_L.end());

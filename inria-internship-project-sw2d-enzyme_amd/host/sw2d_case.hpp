@@ -78,6 +78,13 @@ struct BcSeries {
 bool read_bc_series(const std::string &path, BcSeries &bc, std::string &err);
 bool write_bc_series(const std::string &path, const BcSeries &bc, std::string &err);
 
+// wind time series (swIO::readTimeSeries2dVectorFile, src/common/swIO.cpp:906-1010):
+// two comment lines, the station count, two comment lines, then one row per
+// record: time, then (norm, angle from North in degrees) per station.
+bool read_wind_series(const std::string &path, std::vector<double> &t,
+                      std::vector<std::vector<double>> &norm,
+                      std::vector<std::vector<double>> &angle_deg, std::string &err);
+
 // input.solv: key/value, '#' or '=' comment lines (swSolverParam.cpp:664-880).
 // Keys of the 4D-Var layer (assim/assimConfig.cpp:27-39) and "gpu_*" keys are
 // kept aside, with '#' tail comments removed as the assimilation layer does.
@@ -106,6 +113,11 @@ struct Case {
     std::vector<double> phi, beta;
     std::vector<double> fc1, fc2, falpha;        // friction (degrees)
     std::vector<double> hsing, hk1, hk2, halpha; // singular head loss (degrees)
+    // wind (model_wind smithbanke): station code per cell (1-based), record
+    // times, Cartesian components per station ([station][record] row-major)
+    std::vector<int32_t> wind_code;
+    std::vector<double> wind_t, wind_x, wind_y;
+    int32_t wind_stations = 0;
     sw2dg_params params() const;
     sw2dg_mesh mesh() const;                     // views into g (int64 copies kept here)
     std::vector<int64_t> bcell64, L64, R64;

@@ -54,6 +54,10 @@ class SpSolver:
         stt = keep("series_t", a["series_t"], np.float64)
         self._check(self.lib.sw2dg_set_bc_series(ctx, case.n_series, case.n_times, N.i32ptr(st),
                                                  N.dptr(stt), N.dptr(sv)))
+        if "wind_code" in a and len(a["wind_code"]) > 0:
+            nt = len(a["wind_t"])
+            self.set_wind(a["wind_code"], a["wind_t"], np.reshape(a["wind_x"], (-1, nt)),
+                          np.reshape(a["wind_y"], (-1, nt)))
         self.set_state(a["h0"], a["qx0"], a["qy0"], self.params.t0)
 
     # -- plumbing -----------------------------------------------------------
@@ -86,6 +90,24 @@ class SpSolver:
         vv = self._hold("bc_v", v.reshape(-1))
         self._check(self.lib.sw2dg_set_bc_series(self.ctx, ns, len(t), N.i32ptr(ty), N.dptr(t),
                                                  N.dptr(vv)))
+
+    def set_wind(self, code, t, vx, vy):
+        """Wind stations (swParent::setWindStationCodes / setWindVelocityTimeSeriesTables,
+        swParent.cpp:571-619): code[n_cell] 1-based station per cell, t[n_times]
+        record times, vx, vy[n_station, n_times] Cartesian velocities.  An
+        empty code array turns the source off."""
+        code = np.ascontiguousarray(code, dtype=np.int32)
+        self._keep["wind_code"] = code
+        if code.size == 0:
+            self._check(self.lib.sw2dg_set_wind(self.ctx, 0, 0, None, None, None, None))
+            return
+        t = self._hold("wind_t", np.ascontiguousarray(t, dtype=np.float64))
+        vx = self._hold("wind_x", np.ascontiguousarray(vx, dtype=np.float64).reshape(-1, len(t)))
+        vy = self._hold("wind_y", np.ascontiguousarray(vy, dtype=np.float64).reshape(-1, len(t)))
+        if code.shape != (self.case.n_cell,) or vx.shape != vy.shape:
+            raise ValueError("set_wind: code[n_cell] and vx, vy[n_station, n_times] expected")
+        self._check(self.lib.sw2dg_set_wind(self.ctx, vx.shape[0], len(t), N.i32ptr(code),
+                                            N.dptr(t), N.dptr(vx), N.dptr(vy)))
 
     def _hold(self, name, arr):
         arr = np.ascontiguousarray(arr, dtype=np.float64)

@@ -35,6 +35,8 @@ class OrcCase(C.Structure):
         ("h_min", C.c_double), ("phi_min", C.c_double), ("cfl_max", C.c_double),
         ("dt_max", C.c_double), ("dt_map", C.c_double), ("t0", C.c_double), ("t_max", C.c_double),
         ("v_max", C.c_double), ("fr_max", C.c_double),
+        ("n_wind_station", C.c_int32), ("n_wind_times", C.c_int32),
+        ("wind_code", _i32), ("wind_t", _d), ("wind_x", _d), ("wind_y", _d),
     ]
 
 
@@ -63,6 +65,8 @@ def lib():
         L.orc_get_bnd_f1.argtypes = [C.c_void_p, _d]
         L.orc_hllc.argtypes = [C.c_int, C.c_double, C.c_double] + [C.c_double] * 16 + [_d, C.POINTER(C.c_int)]
         L.orc_bnd_flux.argtypes = [C.c_double] * 8 + [C.c_int, C.c_double, _d, C.POINTER(C.c_int)]
+        L.orc_interp_series.argtypes = [_d, _d, C.c_int64, C.c_double]
+        L.orc_interp_series.restype = C.c_double
         L.orc_friction_factor.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_friction_factor.restype = C.c_double
         _lib = L
@@ -368,6 +372,12 @@ class OracleSolver:
         for f in ("scheme", "topo", "friction", "head_loss", "neg_depth_detection", "h_min",
                   "phi_min", "cfl_max", "dt_max", "dt_map", "t0", "t_max", "v_max", "fr_max"):
             setattr(c, f, getattr(p, f))
+        wc = a["wind_code"] if "wind_code" in a else np.zeros(0, np.int32)
+        if len(wc) > 0:
+            c.n_wind_times = len(a["wind_t"])
+            c.n_wind_station = len(a["wind_x"]) // c.n_wind_times
+            c.wind_code = k(wc, np.int32).ctypes.data_as(_i32)
+            c.wind_t, c.wind_x, c.wind_y = (_dp(k(a[f])) for f in ("wind_t", "wind_x", "wind_y"))
         h0, qx0, qy0 = k(a["h0"]), k(a["qx0"]), k(a["qy0"])
         self._s = lib().orc_create(C.byref(c), _dp(h0), _dp(qx0), _dp(qy0))
         if not self._s:

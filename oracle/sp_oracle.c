@@ -15,6 +15,8 @@
 
 #define SMALL_REAL 1e-16 /* src/common/param.h:2 */
 #define GRAVITY 9.81     /* src/common/param.h:3 */
+#define RHO_AIR 1.2      /* src/common/param.h:4 */
+#define RHO_WATER 1.0e3  /* src/common/param.h:5 */
 
 /* std::max / std::min semantics: max returns a unless a < b, min returns a unless b < a */
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
@@ -44,6 +46,7 @@ struct orc_solver {
     double *bc_vinf, *bc_vsup, *bc_tinf, *bc_tsup, *bc_val;
     int64_t *bc_rank;
     int *bc_atend;
+    double *wvx, *wvy;  /* wind velocity per station at t + dt/2 */
     double t, dt;
 };
 
@@ -631,6 +634,50 @@ static void source_head_loss(orc_solver *s, double dt) {
     }
 }
 
+/* swParent::interpolateTimeSeries -- swParent.cpp:331-387: clamped at both
+ * ends, else a sweep from the first record that stops at the first record
+ * k >= 1 with at <= t[k] (it visits record 1 twice, without effect), then
+ * linear interpolation between the records around it.  A one-record table
+ * returns v[0] in the sliver (t[0] - small_real, t[0]) where the reference
+ * reads past the table. */
+static double interp_series(const double *t, const double *v, int64_t n, double at) {
+    if (at <= t[0] - SMALL_REAL) return v[0];
+    if (at >= t[n - 1]) return v[n - 1];
+    if (n < 2) return v[0];
+    int64_t k = 1;
+    while (at > t[k] && k < n - 1) k++;
+    return v[k - 1] + (v[k] - v[k - 1]) * (at - t[k - 1]) / (t[k] - t[k - 1]);
+}
+
+double orc_interp_series(const double *t, const double *v, int64_t n, double at) {
+    return interp_series(t, v, n, at);
+}
+
+/* swSp::sourceWind + sourceWindSmithBanke -- swSpSourceWind.cpp:11-85 */
+static void source_wind(orc_solver *s, double dt) {
+    const orc_case *c = &s->c;
+    const int64_t nt = c->n_wind_times;
+    const double at = s->t + 0.5 * dt;
+    for (int32_t k = 0; k < c->n_wind_station; k++) {
+        s->wvx[k] = interp_series(c->wind_t, c->wind_x + k * nt, nt, at);
+        s->wvy[k] = interp_series(c->wind_t, c->wind_y + k * nt, nt, at);
+    }
+    for (int64_t i = 0; i < c->n_cell; i++) {
+        const int32_t st = c->wind_code[i] - 1;
+        const double h = s->h[i];
+        if (!(h > SMALL_REAL)) continue;
+        const double u = s->qx[i] / h, v = s->qy[i] / h;
+        const double rx = s->wvx[st] - u, ry = s->wvy[st] - v;
+        const double water_speed = sqrt(u * u + v * v);
+        const double rel = sqrt(rx * rx + ry * ry);
+        const double drag = 1e-3 * (0.63 + 0.066 * water_speed);
+        const double m = smin(h, c->h_min) / smax(c->h_min, SMALL_REAL);
+        const double tx = RHO_AIR * drag * rel * rx, ty = RHO_AIR * drag * rel * ry;
+        s->qx[i] = s->qx[i] + tx / RHO_WATER * dt * m;
+        s->qy[i] = s->qy[i] + ty / RHO_WATER * dt * m;
+    }
+}
+
 /* swSp::correctVelocity -- swSp.cpp:447-479 */
 static void correct_velocity(orc_solver *s) {
     const orc_case *c = &s->c;
@@ -673,6 +720,7 @@ int orc_step(orc_solver *s) {
     }
     if (c->friction != ORC_FRIC_NONE) source_friction(s, dt);
     if (c->head_loss) source_head_loss(s, dt);
+    if (c->n_wind_station > 0) source_wind(s, dt);
     correct_velocity(s);
     s->t += dt;
     return ORC_OK;
@@ -733,6 +781,16 @@ orc_solver *orc_create(const orc_case *c, const double *h0, const double *qx0, c
     s->c.series_type = xcopy(c->series_type, sizeof(int32_t) * (size_t)c->n_series);
     s->c.series_t = xcopy(c->series_t, sizeof(double) * (size_t)c->n_times);
     s->c.series_v = xcopy(c->series_v, sizeof(double) * (size_t)c->n_times * (size_t)c->n_series);
+    {
+        const size_t nw = (size_t)(c->n_wind_station > 0 ? c->n_wind_station : 0);
+        const size_t wt = (size_t)(c->n_wind_station > 0 ? c->n_wind_times : 0);
+        s->c.wind_code = nw ? xcopy(c->wind_code, sizeof(int32_t) * (size_t)nc) : NULL;
+        s->c.wind_t = nw ? xcopy(c->wind_t, sizeof(double) * wt) : NULL;
+        s->c.wind_x = nw ? xcopy(c->wind_x, sizeof(double) * nw * wt) : NULL;
+        s->c.wind_y = nw ? xcopy(c->wind_y, sizeof(double) * nw * wt) : NULL;
+        s->wvx = (double *)calloc(nw ? nw : 1, sizeof(double));
+        s->wvy = (double *)calloc(nw ? nw : 1, sizeof(double));
+    }
 
     s->h = xcopy(h0, dc);
     s->qx = xcopy(qx0, dc);
@@ -810,7 +868,8 @@ void orc_destroy(orc_solver *s) {
         s->lmaxb, s->zri, s->qri, s->rri, s->lmi, s->lpi, s->lmaxi, s->F1b, s->F2b, s->F3b, s->S2Rb,
         s->F1i, s->F2i, s->F3i, s->S2Li, s->S2Ri, s->zc, s->dzdx, s->dzdy, s->nb_ptr, s->nb_idx,
         s->swept, s->fcos, s->fsin, s->hl_alpha, s->bc_vinf, s->bc_vsup, s->bc_tinf, s->bc_tsup,
-        s->bc_val, s->bc_rank, s->bc_atend};
+        s->bc_val, s->bc_rank, s->bc_atend, (void *)s->c.wind_code, (void *)s->c.wind_t,
+        (void *)s->c.wind_x, (void *)s->c.wind_y, s->wvx, s->wvy};
     for (size_t i = 0; i < sizeof(owned) / sizeof(owned[0]); i++) free(owned[i]);
     free(s);
 }

@@ -14,6 +14,7 @@
  *   src/sw2dSolver/models/sp/swSpSourceFriction.cpp,
  *   src/sw2dSolver/models/common/swSourceFriction.cpp
  *   src/sw2dSolver/models/common/swBoundaryConditionTimeSeries.cpp
+ *   src/sw2dSolver/models/sp/swSpSourceWind.cpp, swParent.cpp (wind series)
  * All loops run in the reference's edge order with its scatter updates, so
  * the summation order is the reference's by construction.
  */
@@ -63,6 +64,11 @@ typedef struct orc_case {
     /* options */
     int32_t scheme, topo, friction, head_loss, neg_depth_detection;
     double h_min, phi_min, cfl_max, dt_max, dt_map, t0, t_max, v_max, fr_max;
+    /* wind, model_wind smithbanke (n_wind_station = 0: off) */
+    int32_t n_wind_station, n_wind_times;
+    const int32_t *wind_code;        /* [n_cell] 1-based station of each cell */
+    const double *wind_t;            /* [n_wind_times] */
+    const double *wind_x, *wind_y;   /* [n_wind_station * n_wind_times], per station */
 } orc_case;
 
 typedef struct orc_solver orc_solver;
@@ -89,6 +95,7 @@ void orc_hllc(int blending, double h_min, double phi_min,
 void orc_bnd_flux(double h_min, double bR, double phiR, double zbR, double zR, double qR, double rR,
                   double lpR, int bc_type, double bc_val, double out[4], int *status);
 double orc_friction_factor(int model, double coeff, double h, double u_norm, double dt);
+double orc_interp_series(const double *t, const double *v, int64_t n, double at);
 
 #ifdef __cplusplus
 }

@@ -3,7 +3,7 @@
 Run in the development container (needs oracle/_ref/sw2dSolver, built from
 /root/reference by `make -C oracle ref`):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--only NAME ...]
 
 For every case below the script writes the case folder with the package's
 case tooling (deterministic: fixed seeds), runs the reference's own
@@ -63,6 +63,21 @@ def golden_cases():
                                                      closed_fraction=0.0 if s == "muscl_evr" else 0.04))
            for s in ("godunov", "muscl_evr") for t in ("discontinuous", "blending")
            for f in ("strickler", "manning")},
+        # wind source, model_wind smithbanke (swSpSourceWind.cpp); the dam_small
+        # ones have walls only and no friction: no exp/log on their path
+        "wind_dam_small_godunov": ("dambreak_case", dict(nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0,
+                                                         scheme="godunov", friction="none",
+                                                         t_max=3.0, dt_map=1.0, wind=True)),
+        "wind_dam_small_muscl_evr": ("dambreak_case", dict(nx=20, ny=10, Lx=40.0, Ly=20.0,
+                                                           x_dam=20.0, scheme="muscl_evr",
+                                                           friction="none", t_max=3.0, dt_map=1.0,
+                                                           wind=True)),
+        "wind_mixed_godunov_strickler": ("mixed_case", dict(scheme="godunov", friction="strickler",
+                                                            t_max=20.0, dt_map=5.0, wind=True)),
+        "wind_mixed_muscl_evr_manning": ("mixed_case", dict(scheme="muscl_evr",
+                                                            friction="manning", t_max=20.0,
+                                                            dt_map=5.0, closed_fraction=0.0,
+                                                            wind=True)),
         # the reference stops on an impossible HLLC regime (swSpHyp.cpp:729-739)
         "mixed_muscl_closed_cells_error": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
                                                              t_max=20.0, closed_fraction=0.04)),
@@ -152,11 +167,13 @@ def input_digests(folder):
     return d
 
 
-def main():
+def main(only=None):
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: run `make -C {os.path.join(REPO, 'oracle')} ref` first")
     sw = load_pkg()
     for name, (gen, kw) in golden_cases().items():
+        if only and name not in only:
+            continue
         tmp = tempfile.mkdtemp(prefix="sw2d_golden_")
         try:
             make_case(sw, tmp, gen, kw)
@@ -190,7 +207,8 @@ def main():
             print(name, "maps:", [times[k] for k in keep])
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
-    main_e(sw)
+    if not only:
+        main_e(sw)
 
 
 if __name__ == "__main__":
@@ -198,5 +216,7 @@ if __name__ == "__main__":
         sys.path.insert(0, REPO)
         import bench
         main_e(bench.load_pkg(), only=set(sys.argv[2:]))
+    elif len(sys.argv) > 2 and sys.argv[1] == "--only":  # regenerate named fixtures only
+        main(only=set(sys.argv[2:]))
     else:
         main()
