@@ -39,6 +39,14 @@ extern "C" {
 #define SW2DG_FRIC_MANNING 1
 #define SW2DG_FRIC_STRICKLER 2
 #define SW2DG_FRIC_CHEZY 3
+/* model_runoff and model_infiltration values (scs and h_proportional are
+ * industrial-only in the reference and are refused there too) */
+#define SW2DG_RUNOFF_NONE 0
+#define SW2DG_RUNOFF_FULL 1          /* full_rainfall            */
+#define SW2DG_RUNOFF_INFIL_COEFF 2   /* infiltration_coefficient */
+#define SW2DG_RUNOFF_RUNOFF_COEFF 3  /* runoff_coefficient       */
+#define SW2DG_INFIL_NONE 0
+#define SW2DG_INFIL_HORTON 1
 /* boundary series types ("fr", "h", "z", "q") */
 #define SW2DG_BC_FR 0
 #define SW2DG_BC_H 1
@@ -91,9 +99,29 @@ int sw2dg_set_bc_series(sw2dg_ctx *ctx, int32_t n_series, int32_t n_times, const
  * vx, vy[n_station * n_times] the Cartesian wind velocity per station
  * ([station][record]; the polar-to-Cartesian step of swSourceWind.cpp:9-38
  * stays with the caller).  n_station = 0 turns the source off.  The 4D-Var
- * layer (sw2dg_assim_*) has no wind source and refuses a ctx that has one. */
+ * layer (sw2dg_assim_*) has no wind source and refuses a ctx that has one
+ * (likewise for runoff and infiltration below). */
 int sw2dg_set_wind(sw2dg_ctx *ctx, int32_t n_station, int32_t n_times, const int32_t *code,
                    const double *t, const double *vx, const double *vy);
+/* Replaces swSp::setRunoffParameters (swSp.cpp:1276-1320) and
+ * swParent::setPrecipitationStationCodes / setPrecipitationTimeSeriesTable
+ * (swParent.cpp:516-569) and switches on the runoff step of swSp::execute
+ * (swSp.cpp:561-573): sourceRunoffEverywhere when water_only = 0
+ * (model_porosity_rainfall everywhere), sourceRunoffWateronly when 1
+ * (swSpSourceRunoffInfiltration.cpp:43-263).  model is SW2DG_RUNOFF_*;
+ * code[n_cell] (reference order) each cell's 1-based rain station; t[n_times]
+ * non-decreasing; rate[n_station * n_times] the rain rate (m/s, negative =
+ * evaporation) per station ([station][record]), interpolated at t + dt/2;
+ * coeff[n_cell] the infiltration or runoff coefficient in [0, 1] (unused,
+ * may be NULL, for SW2DG_RUNOFF_FULL).  SW2DG_RUNOFF_NONE turns it off. */
+int sw2dg_set_runoff(sw2dg_ctx *ctx, int32_t model, int32_t water_only, int32_t n_station,
+                     int32_t n_times, const int32_t *code, const double *t, const double *rate,
+                     const double *coeff);
+/* Replaces swSp::setInfiltrationParameters (swSp.cpp:1075-1115) and switches
+ * on swSp::sourceInfiltration (swSpSourceRunoffInfiltration.cpp:12-40, after
+ * the runoff step): model SW2DG_INFIL_HORTON with rate[n_cell] >= 0 (m/s,
+ * reference order); SW2DG_INFIL_NONE turns it off. */
+int sw2dg_set_infiltration(sw2dg_ctx *ctx, int32_t model, const double *rate);
 /* Replaces swSp::setInitialConditions (swSp.cpp:1119-1224): sets the state
  * and the clock (t, previous dt = 0, step count 0, BC cursors rewound). */
 int sw2dg_set_state(sw2dg_ctx *ctx, const double *h, const double *qx, const double *qy,

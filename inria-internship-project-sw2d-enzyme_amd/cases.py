@@ -127,16 +127,17 @@ def write_case(folder, geom, solv, h0, qx0, qy0, phi=None, beta=None, friction="
 def dambreak_case(folder, nx=100, ny=100, Lx=200.0, Ly=100.0, x_dam=100.0, h_left=2.0,
                   h_right=0.5, scheme="muscl_evr", topo="discontinuous", friction="manning",
                   n_manning=0.02, t_max=10.0, cfl=0.9, dt_map=1e6, dt_ts=1e6, store_flag="default",
-                  wind=False):
+                  wind=False, rain=None, rain_where="water_only", infil=False):
     """Plain SP forward dam-break (20,000 triangles at the default size).
     With wind=True it adds the Smith & Banke wind source: one station on each
-    side of the dam, records at 1/6, 1/2 and 5/6 of t_max."""
+    side of the dam, records at 1/6, 1/2 and 5/6 of t_max; rain / infil add
+    rain, evaporation and infiltration as in mixed_case."""
     g = Geometry.rect(nx, ny, Lx, Ly)
     x = g.array("cell_x")
     h0 = np.where(x < x_dam, h_left, h_right)
     solv = solv_text(scheme=scheme, topo=topo, friction=friction, cfl=cfl, t_max=t_max,
                      dt_max=10.0, dt_map=dt_map, dt_ts=dt_ts, store_flag=store_flag,
-                     extra={"model_wind": "smithbanke"} if wind else None)
+                     extra=_source_keys(wind, rain, rain_where, infil))
     write_case(folder, g, solv, h0, 0.0, 0.0, friction=friction,
                fric=None if friction == "none" else n_manning,
                bc_codes=(1,), bc_types=("q",), bc_t=(0.0, 1e9), bc_v=((0.0, 0.0),))
@@ -147,6 +148,8 @@ def dambreak_case(folder, nx=100, ny=100, Lx=200.0, Ly=100.0, x_dam=100.0, h_lef
         write_wind_series(os.path.join(folder, "input", "wind_time_series.txt"), tw,
                           np.array([[30.0, 40.0, 25.0], [35.0, 20.0, 45.0]]),
                           np.array([[30.0, 120.0, 250.0], [300.0, 10.0, 170.0]]))
+    if rain or infil:
+        _write_rain(folder, np.where(x < x_dam, 1, 2), t_max, np.random.default_rng(17), rain, infil)
     return folder
 
 
@@ -272,7 +275,7 @@ def write_distrib_map(path, names, cols):
 def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
                topo="discontinuous", friction="strickler", head_loss=True, t_max=20.0,
                dt_map=1e6, dt_ts=1e6, cfl=0.9, store_flag="default", closed_fraction=0.04,
-               wind=False):
+               wind=False, rain=None, rain_where="water_only", infil=False):
     """Small case that exercises every branch of the SP step: Delaunay mesh,
     bumpy bed with wet/dry fronts, porosity field including closed cells
     (phi = 0), all four boundary types (west q hydrograph, east fr, north h,
@@ -281,7 +284,9 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
     MUSCL cases use closed_fraction = 0.)  With wind=True it adds the Smith &
     Banke wind source: two stations (west and east halves) whose records start
     after t = 0 and end before t_max, so both clamps and the interpolation
-    run."""
+    run.  rain (a model_runoff value) adds rain and evaporation from two
+    stations, rain_where is model_porosity_rainfall, infil adds Horton
+    infiltration."""
     rng = np.random.default_rng(seed)
     Lx, Ly = L
     g = Geometry.delaunay(px, py, Lx, Ly, jitter=0.25, seed=seed)
@@ -315,7 +320,7 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
     ])
     solv = solv_text(scheme=scheme, topo=topo, friction=friction, cfl=cfl, t_max=t_max,
                      dt_max=0.5, dt_map=dt_map, dt_ts=dt_ts, head_loss=head_loss,
-                     store_flag=store_flag, extra={"model_wind": "smithbanke"} if wind else None)
+                     store_flag=store_flag, extra=_source_keys(wind, rain, rain_where, infil))
     write_case(folder, g, solv, h0, qx0, qy0, phi=phi, friction=friction, fric=fric,
                bc_codes=(1, 2, 3, 4, 5), bc_types=("q", "q", "fr", "h", "z"), bc_t=tq, bc_v=bc_v)
     if head_loss:
@@ -333,7 +338,46 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
         norm = rng.uniform(15.0, 40.0, (2, len(tw)))
         angle = rng.uniform(0.0, 360.0, (2, len(tw)))
         write_wind_series(os.path.join(folder, "input", "wind_time_series.txt"), tw, norm, angle)
+    if rain or infil:
+        _write_rain(folder, np.where(cy < 0.5 * Ly, 1, 2), t_max, rng, rain, infil)
     return folder
+
+
+def _source_keys(wind, rain, rain_where, infil):
+    keys = {}
+    if wind:
+        keys["model_wind"] = "smithbanke"
+    if rain:
+        keys["model_runoff"] = rain
+        keys["model_porosity_rainfall"] = rain_where
+    if infil:
+        keys["model_infiltration"] = "horton"
+    return keys or None
+
+
+def _write_rain(folder, station, t_max, rng, rain, infil):
+    """Rain and evaporation from two stations (records at 1/10, 7/20, 11/20
+    and 3/4 of t_max, so the runs start before the first and end after the
+    last), the runoff-model coefficient map and the Horton rate map."""
+    inp = os.path.join(folder, "input")
+    nc = len(station)
+    if rain:
+        write_distrib_map(os.path.join(inp, "precipitation_station_codes_map.txt"),
+                          ["precipitation_station"], [station])
+        tr = t_max * np.array([0.1, 0.35, 0.55, 0.75])
+        rate = np.array([[2e-3, 5e-4, -1e-3, 3e-3], [-5e-4, 1e-3, 2e-3, -2e-3]])
+        with open(os.path.join(inp, "precipitation_time_series.txt"), "w") as f:
+            f.write("# precipitation time series\n# number of stations\n2\n# rate (m/s)\n# time\t1\t2\n")
+            for r in range(len(tr)):
+                f.write(f"{tr[r]:.17e}\t{rate[0, r]:.17e}\t{rate[1, r]:.17e}\n")
+        coeff = rng.uniform(0.0, 1.0, nc)
+        if rain == "infiltration_coefficient":
+            write_distrib_map(os.path.join(inp, "infiltration_coefficient_map.txt"), ["coeff"], [coeff])
+        elif rain == "runoff_coefficient":
+            write_distrib_map(os.path.join(inp, "runoff_coefficient_map.txt"), ["coeff"], [coeff])
+    if infil:
+        write_distrib_map(os.path.join(inp, "infiltration_horton_parameter_map.txt"), ["rate"],
+                          [rng.uniform(0.0, 5e-4, nc)])
 
 
 def write_wind_series(path, t, norm, angle_deg):
@@ -356,7 +400,8 @@ class Case:
               "bnd_ny", "bnd_series", "bnd_code", "int_L", "int_R", "int_len", "int_nx", "int_ny",
               "int_xm", "int_ym", "h0", "qx0", "qy0", "phi", "beta", "fc1", "fc2", "falpha",
               "hsing", "hk1", "hk2", "halpha", "series_type", "series_t", "series_v",
-              "wind_code", "wind_t", "wind_x", "wind_y")
+              "wind_code", "wind_t", "wind_x", "wind_y", "rain_code", "rain_t", "rain_v",
+              "runoff_coeff", "infil_rate", "source_models")
 
     def __init__(self, folder):
         self.folder = str(folder)

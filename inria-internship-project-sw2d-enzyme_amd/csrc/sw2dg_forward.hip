@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, Clock *ck, in
 // of the last), else the first k >= 1 with tw <= t[k] (its sweep stops there;
 // the value interpolates records k-1 and k).  A one-record table gives -1 in
 // the sliver (t[0] - small_real, t[0]), where the reference reads past it.
-__device__ int wind_bracket(const double *tv, int n, double tw) {
+__device__ int record_bracket(const double *tv, int n, double tw) {
     if (tw <= tv[0] - kSmall) return -1;
     if (tw >= tv[n - 1]) return -2;
     if (n < 2) return -1;
@@ -352,6 +352,15 @@ __device__ int wind_bracket(const double *tv, int n, double tw) {
             lo = mid + 1;
     }
     return lo;
+}
+
+// Value of one station's records at time tw given its bracket k
+// (record_bracket): the interpolation of swParent.cpp:383-384.
+__device__ __forceinline__ double record_value(const double *tv, const double *v, int n, int k,
+                                              double tw) {
+    if (k < 1) return v[(k == -2) ? n - 1 : 0];
+    const double t1 = tv[k - 1], t2 = tv[k];
+    return v[k - 1] + (v[k] - v[k - 1]) * (tw - t1) / (t2 - t1);
 }
 
 __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck, int p, int n_part) {
@@ -378,10 +387,11 @@ __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck,
     }
     if ((t + dt) > ck->t_max) dt = ck->t_max - t;
     for (int k = 0; k < ck->n_series; ++k) bc_advance(bc, k, t);
-    if (m.wind_nt > 0) {
+    if (m.wind_nt > 0 || m.rain_nt > 0) {
         const double tw = t + 0.5 * dt;
-        ck->wind_tw = tw;
-        ck->wind_k = wind_bracket(m.wind_t, m.wind_nt, tw);
+        ck->t_half = tw;
+        if (m.wind_nt > 0) ck->wind_k = record_bracket(m.wind_t, m.wind_nt, tw);
+        if (m.rain_nt > 0) ck->rain_k = record_bracket(m.rain_t, m.rain_nt, tw);
     }
     const double tn = t + dt;
     int sf = 0;
@@ -503,23 +513,54 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
         qx = ax * ca - ay * sa;
         qy = ax * sa + ay * ca;
     }
+    if (o.runoff) {
+        // swSp::sourceRunoffEverywhere / sourceRunoffWateronly
+        // (swSpSourceRunoffInfiltration.cpp:43-263): rain rate of the cell's
+        // station at t + dt/2; rain adds (a share of) the depth, evaporation
+        // removes it and scales the discharge with the depth
+        const double r = record_value(m.rain_t, m.rain_v + (size_t)m.rcode[i] * m.rain_nt,
+                                      m.rain_nt, ck->rain_k, ck->t_half);
+        const double h0 = h;
+        double dp;
+        if (o.rain_water_only) {
+            dp = dt * r;
+        } else {
+            const double ph = m.phi[i];
+            dp = 0.0;
+            if (ph > kSmall) {
+                dp = dt * r;
+                if (o.runoff != SW2DG_RUNOFF_RUNOFF_COEFF || dp >= 0.0) dp /= ph;
+            }
+        }
+        if (dp > 0.) {
+            if (o.runoff == SW2DG_RUNOFF_FULL)
+                h += dp;
+            else if (o.runoff == SW2DG_RUNOFF_INFIL_COEFF)
+                h += (1.0 - m.rcoef[i]) * dp;
+            else
+                h += m.rcoef[i] * dp;
+        } else {
+            h = smax(0.0, h0 + dp);
+            const double ratio = h / smax(h0, kSmall);
+            qx *= ratio;
+            qy *= ratio;
+        }
+    }
+    if (o.infil) {
+        // swSp::sourceInfiltration, Horton (swSpSourceRunoffInfiltration.cpp:12-40)
+        const double h0 = h;
+        h = smax(0.0, h0 + -m.infil[i] * dt);
+        const double ratio = h / smax(h0, kSmall);
+        qx *= ratio;
+        qy *= ratio;
+    }
     if (o.wind) {
         // swSp::sourceWind + sourceWindSmithBanke (swSpSourceWind.cpp:11-85):
         // station velocity interpolated at t + dt/2, then the Smith & Banke
         // stress (its drag coefficient grows with the water speed |q|/h)
         const int k = ck->wind_k, nt = m.wind_nt;
-        const double *X = m.wind_x + (size_t)m.wcode[i] * nt;
-        const double *Y = m.wind_y + (size_t)m.wcode[i] * nt;
-        double wx, wy;
-        if (k < 1) {
-            const int j = (k == -2) ? nt - 1 : 0;
-            wx = X[j];
-            wy = Y[j];
-        } else {
-            const double t1 = m.wind_t[k - 1], t2 = m.wind_t[k], tw = ck->wind_tw;
-            wx = X[k - 1] + (X[k] - X[k - 1]) * (tw - t1) / (t2 - t1);
-            wy = Y[k - 1] + (Y[k] - Y[k - 1]) * (tw - t1) / (t2 - t1);
-        }
+        const double wx = record_value(m.wind_t, m.wind_x + (size_t)m.wcode[i] * nt, nt, k, ck->t_half);
+        const double wy = record_value(m.wind_t, m.wind_y + (size_t)m.wcode[i] * nt, nt, k, ck->t_half);
         if (h > kSmall) {
             const double u = qx / h, v = qy / h;
             const double rx = wx - u, ry = wy - v;
@@ -1031,6 +1072,87 @@ int sw2dg_set_wind(sw2dg_ctx *c, int32_t n_station, int32_t n_times, const int32
     }
     c->m.wind_nt = c->o.wind ? n_times : 0;
     if (c->gexec) {  // kernel arguments changed: rebuild the graph
+        hipGraphExecDestroy(c->gexec);
+        hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+    }
+    return SW2DG_OK;
+}
+
+int sw2dg_set_runoff(sw2dg_ctx *c, int32_t model, int32_t water_only, int32_t n_station,
+                     int32_t n_times, const int32_t *code, const double *t, const double *rate,
+                     const double *coeff) {
+    if (!c || model < SW2DG_RUNOFF_NONE || model > SW2DG_RUNOFF_RUNOFF_COEFF) return SW2DG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (model != SW2DG_RUNOFF_NONE && c->assim)
+        return fail(c, SW2DG_ERR_ARG, "the 4D-Var layer has no runoff source");
+    if (model != SW2DG_RUNOFF_NONE) {
+        if (n_station < 1 || n_times < 1 || !code || !t || !rate ||
+            (model != SW2DG_RUNOFF_FULL && !coeff))
+            return SW2DG_ERR_ARG;
+        for (int k = 1; k < n_times; ++k)
+            if (t[k - 1] > t[k])
+                return fail(c, SW2DG_ERR_ARG, "rain record times must be non-decreasing");
+        std::vector<int> rc(c->nc);
+        std::vector<double> cf(c->nc, 0.0);
+        for (int64_t k = 0; k < c->nc; ++k) {
+            const int32_t s = code[c->perm[k]];
+            if (s < 1 || s > n_station)
+                return fail(c, SW2DG_ERR_ARG, "a rain station code lies outside [1, n_station]");
+            rc[k] = s - 1;
+            if (model != SW2DG_RUNOFF_FULL) {
+                cf[k] = coeff[c->perm[k]];
+                if (!(cf[k] >= 0.0 && cf[k] <= 1.0))
+                    return fail(c, SW2DG_ERR_ARG, "a runoff coefficient lies outside [0, 1]");
+            }
+        }
+        const size_t nv = (size_t)n_station * n_times;
+        if (!c->m.rcode) {
+            c->m.rcode = c->dalloc<int>(c->nc);
+            c->m.rcoef = c->dalloc<double>(c->nc);
+        }
+        if (c->m.rain_nt != n_times || c->rain_nv != nv) {
+            c->m.rain_t = c->dalloc<double>(n_times);
+            c->m.rain_v = c->dalloc<double>(nv);
+            c->rain_nv = nv;
+        }
+        if (!c->m.rcode || !c->m.rcoef || !c->m.rain_t || !c->m.rain_v)
+            return fail(c, SW2DG_ERR_HIP, "rain tables: device allocation failed");
+        HIPCHK(c, hipMemcpy((void *)c->m.rcode, rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->m.rcoef, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->m.rain_t, t, n_times * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)c->m.rain_v, rate, nv * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->o.runoff = model;
+    c->o.rain_water_only = water_only ? 1 : 0;
+    c->m.rain_nt = model != SW2DG_RUNOFF_NONE ? n_times : 0;
+    if (c->gexec) {  // kernel arguments changed: rebuild the graph
+        hipGraphExecDestroy(c->gexec);
+        hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+    }
+    return SW2DG_OK;
+}
+
+int sw2dg_set_infiltration(sw2dg_ctx *c, int32_t model, const double *rate) {
+    if (!c || (model != SW2DG_INFIL_NONE && model != SW2DG_INFIL_HORTON)) return SW2DG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (model != SW2DG_INFIL_NONE) {
+        if (!rate) return SW2DG_ERR_ARG;
+        if (c->assim) return fail(c, SW2DG_ERR_ARG, "the 4D-Var layer has no infiltration source");
+        std::vector<double> v(c->nc);
+        for (int64_t k = 0; k < c->nc; ++k) {
+            v[k] = rate[c->perm[k]];
+            if (!(v[k] >= 0.0)) return fail(c, SW2DG_ERR_ARG, "an infiltration rate is negative");
+        }
+        if (!c->m.infil) c->m.infil = c->dalloc<double>(c->nc);
+        if (!c->m.infil) return fail(c, SW2DG_ERR_HIP, "infiltration map: device allocation failed");
+        HIPCHK(c, hipMemcpy((void *)c->m.infil, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->o.infil = model;
+    if (c->gexec) {
         hipGraphExecDestroy(c->gexec);
         hipGraphDestroy(c->graph);
         c->gexec = nullptr;

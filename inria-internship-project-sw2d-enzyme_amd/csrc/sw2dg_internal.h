@@ -31,8 +31,9 @@ struct Clock {
     long long step_stop;
     double dt_last;      // dt of the last active step
     int n_series;
-    double wind_tw;      // wind interpolation time of the step (t + dt/2)
-    int wind_k;          // record bracket at wind_tw: -1 first record, -2 last, k >= 1 (k-1, k)
+    double t_half;       // source interpolation time of the step (t + dt/2)
+    int wind_k;          // record bracket at t_half: -1 first record, -2 last, k >= 1 (k-1, k)
+    int rain_k;          // the same on the rain records
 };
 
 struct BcDev {
@@ -72,10 +73,16 @@ struct MeshDev {
     const int *wcode;
     const double *wind_t, *wind_x, *wind_y;
     int wind_nt;
+    // runoff / infiltration: 0-based rain station of each cell, record times,
+    // rate per station ([station][record]), coefficient and Horton rate per cell
+    const int *rcode;
+    const double *rain_t, *rain_v, *rcoef, *infil;
+    int rain_nt;
 };
 
 struct Opts {
     int scheme, topo, friction, head_loss, neg_det, wind;
+    int runoff, rain_water_only, infil;  // SW2DG_RUNOFF_*, model_porosity_rainfall, SW2DG_INFIL_*
     double h_min, phi_min, v_max, fr_max;
     double adj_cut;  // reverse sweep: adjoint zeroed at cells shallower than this (0: exact)
 };
@@ -128,6 +135,7 @@ struct sw2dg_ctx {
     int64_t bytes = 0;
     int n_series = 0, n_times = 0;
     size_t wind_nv = 0;  // station x record size of the device wind tables
+    size_t rain_nv = 0;  // station x record size of the device rain table
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool state_set = false;

@@ -29,6 +29,7 @@ struct sw2dh_geom {
 struct sw2dh_case {
     sw2d::Case c;
     std::vector<int32_t> stype;
+    std::vector<int32_t> models;  // runoff model, rain on water only, infiltration model
     std::vector<double> sval;
 };
 
@@ -224,6 +225,7 @@ int sw2dh_case_load(const char *folder, sw2dh_case **out) {
     }
     h->stype = h->c.series_types();
     h->sval = h->c.series_values();
+    h->models = {h->c.runoff_model, h->c.rain_water_only, h->c.infil_model};
     *out = h;
     return 0;
 }
@@ -269,6 +271,12 @@ const void *sw2dh_case_array(const sw2dh_case *h, const char *name, int64_t *n, 
     if (k == "wind_t") return view(c.wind_t, n, dt, 0);
     if (k == "wind_x") return view(c.wind_x, n, dt, 0);
     if (k == "wind_y") return view(c.wind_y, n, dt, 0);
+    if (k == "rain_code") return view(c.rain_code, n, dt, 2);
+    if (k == "rain_t") return view(c.rain_t, n, dt, 0);
+    if (k == "rain_v") return view(c.rain_v, n, dt, 0);
+    if (k == "runoff_coeff") return view(c.runoff_coeff, n, dt, 0);
+    if (k == "infil_rate") return view(c.infil_rate, n, dt, 0);
+    if (k == "source_models") return view(h->models, n, dt, 2);
     return geom_array(c.g, c.bcell64, c.L64, c.R64, k, n, dt);
 }
 

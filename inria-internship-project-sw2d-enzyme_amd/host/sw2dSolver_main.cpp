@@ -68,6 +68,13 @@ int main(int argc, char **argv) {
     check(sw2dg_set_bc_series(ctx, (int32_t)types.size(), (int32_t)cs.bc.t.size(), types.data(),
                               cs.bc.t.data(), vals.data()),
           "set_bc_series");
+    if (cs.runoff_model != SW2DG_RUNOFF_NONE)
+        check(sw2dg_set_runoff(ctx, cs.runoff_model, cs.rain_water_only, cs.rain_stations,
+                               (int32_t)cs.rain_t.size(), cs.rain_code.data(), cs.rain_t.data(),
+                               cs.rain_v.data(), cs.runoff_coeff.empty() ? nullptr : cs.runoff_coeff.data()),
+              "set_runoff");
+    if (cs.infil_model != SW2DG_INFIL_NONE)
+        check(sw2dg_set_infiltration(ctx, cs.infil_model, cs.infil_rate.data()), "set_infiltration");
     if (cs.wind_stations > 0)
         check(sw2dg_set_wind(ctx, cs.wind_stations, (int32_t)cs.wind_t.size(), cs.wind_code.data(),
                              cs.wind_t.data(), cs.wind_x.data(), cs.wind_y.data()),

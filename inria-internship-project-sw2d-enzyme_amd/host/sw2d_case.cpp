@@ -671,6 +671,54 @@ bool read_bc_series(const std::string &path, BcSeries &bc, std::string &err) {
     return true;
 }
 
+bool read_scalar_series(const std::string &path, std::vector<double> &t,
+                        std::vector<std::vector<double>> &v, std::string &err) {
+    std::ifstream f(path);
+    if (!f) {
+        err = path + " cannot be opened";
+        return false;
+    }
+    std::string line;
+    get_line(f, line);
+    get_line(f, line);
+    get_line(f, line);
+    const long ns = std::atol(trim(line).c_str());
+    if (ns < 0) {
+        err = path + ": bad series count";
+        return false;
+    }
+    get_line(f, line);
+    get_line(f, line);
+    t.clear();
+    v.assign(ns, {});
+    while (get_line(f, line)) {
+        auto tok = split_ws(line);
+        if (tok.empty()) continue;
+        if ((long)tok.size() < ns + 1) {
+            err = path + ": line is misformatted";
+            return false;
+        }
+        double x;
+        if (!to_double(tok[0], x)) {
+            err = path + ": format conversion";
+            return false;
+        }
+        t.push_back(x);
+        for (long j = 0; j < ns; ++j) {
+            if (!to_double(tok[j + 1], x)) {
+                err = path + ": format conversion";
+                return false;
+            }
+            v[j].push_back(x);
+        }
+    }
+    if (t.empty()) {
+        err = "The file " + path + " does not contain any values";
+        return false;
+    }
+    return true;
+}
+
 bool read_wind_series(const std::string &path, std::vector<double> &t,
                       std::vector<std::vector<double>> &norm,
                       std::vector<std::vector<double>> &angle_deg, std::string &err) {
@@ -833,6 +881,7 @@ bool read_solver_config(const std::string &path, SolverConfig &cfg, std::string 
     str("model_infiltration", cfg.model_infiltration);
     str("model_wind", cfg.model_wind);
     str("model_momentum_diffusion", cfg.model_momentum_diffusion);
+    str("model_porosity_rainfall", cfg.model_porosity_rainfall);
     return true;
 }
 
@@ -912,12 +961,35 @@ bool load_case(const std::string &folder, Case &c, std::string &err) {
         err = "only model_hyperb sp is handled by this solver (got " + c.cfg.model_hyp + ")";
         return false;
     }
-    for (const auto *opt : {&c.cfg.div_corr, &c.cfg.model_runoff, &c.cfg.model_infiltration,
-                            &c.cfg.model_momentum_diffusion})
+    for (const auto *opt : {&c.cfg.div_corr, &c.cfg.model_momentum_diffusion})
         if (*opt != "none") {
             err = "option not handled by the GPU solver: " + *opt;
             return false;
         }
+    // runoff / infiltration models (swSolverParam.cpp:1486-1530, swSpSourceRunoffInfiltration.cpp)
+    const std::string &ro = c.cfg.model_runoff, &inf = c.cfg.model_infiltration;
+    if (ro == "scs" || inf == "h_proportional") {
+        err = "model '" + (ro == "scs" ? ro : inf) + "' available only for industrial use (CEREG)";
+        return false;
+    }
+    c.runoff_model = ro == "none" ? SW2DG_RUNOFF_NONE : ro == "full_rainfall" ? SW2DG_RUNOFF_FULL
+                   : ro == "infiltration_coefficient" ? SW2DG_RUNOFF_INFIL_COEFF
+                   : ro == "runoff_coefficient" ? SW2DG_RUNOFF_RUNOFF_COEFF : -1;
+    if (c.runoff_model < 0) {
+        err = "unallowed value for parameter 'model_runoff' " + ro;
+        return false;
+    }
+    if (inf != "none" && inf != "horton") {
+        err = "Infiltration model not handled = " + inf;
+        return false;
+    }
+    c.infil_model = inf == "horton" ? SW2DG_INFIL_HORTON : SW2DG_INFIL_NONE;
+    const std::string &pr = c.cfg.model_porosity_rainfall;
+    if (pr != "everywhere" && pr != "water_only") {
+        err = "unallowed value for parameter 'model_porosity_rainfall' " + pr;
+        return false;
+    }
+    c.rain_water_only = pr == "water_only" ? 1 : 0;
     if (c.cfg.model_wind != "none" && c.cfg.model_wind != "smithbanke") {
         err = "Wind model unknown : " + c.cfg.model_wind;  // swSpSourceWind.cpp:37-41
         return false;
@@ -1025,24 +1097,75 @@ bool load_case(const std::string &folder, Case &c, std::string &err) {
         c.hk2 = cols[2];
         c.halpha = cols[3];
     }
-    if (c.cfg.model_wind != "none") {
-        // station codes: swParent::setWindStationCodes (swParent.cpp:571-595)
-        // and the checks of swSp::init (swSp.cpp:292-333)
-        if (!read_map(in + "wind_station_codes_map.txt", 1, nc, names, cols, err)) return false;
-        c.wind_code.assign(nc, 0);
+    // 1-based station codes of a Unif/Distrib map (swParent.cpp:516-541, 571-595)
+    auto station_codes = [&](const std::string &file, const char *what,
+                             std::vector<int32_t> &code) -> bool {
+        if (!read_map(in + file, 1, nc, names, cols, err)) return false;
+        code.assign(nc, 0);
         for (size_t i = 0; i < nc; ++i) {
             double ip;
             const double x = cols[0][i];
             if (std::abs(std::modf(x, &ip)) > 1e-16 || x < 0) {
-                err = "Non-integer/negative wind code read for cell " + std::to_string(i);
+                err = std::string("Non-integer/negative ") + what + " code read for cell " +
+                      std::to_string(i);
                 return false;
             }
             if (x < 1) {
-                err = "Wind station code is zero or smaller in cell " + std::to_string(i);
+                err = std::string(what) + " station code is zero or smaller in cell " +
+                      std::to_string(i);
                 return false;
             }
-            c.wind_code[i] = (int32_t)x;
+            code[i] = (int32_t)x;
         }
+        return true;
+    };
+    if (c.runoff_model != SW2DG_RUNOFF_NONE) {
+        // swSp::setRunoffParameters (swSp.cpp:1276-1320), setPrecipitationStationCodes /
+        // setPrecipitationTimeSeriesTable (swParent.cpp:516-569), checks of swSp.cpp:150-215
+        if (c.runoff_model != SW2DG_RUNOFF_FULL) {
+            const char *f = c.runoff_model == SW2DG_RUNOFF_INFIL_COEFF ? "infiltration_coefficient_map.txt"
+                                                                        : "runoff_coefficient_map.txt";
+            if (!read_map(in + f, 1, nc, names, cols, err)) return false;
+            c.runoff_coeff = cols[0];
+            for (size_t i = 0; i < nc; ++i)
+                if (c.runoff_coeff[i] < 0.0 || c.runoff_coeff[i] > 1.0) {
+                    err = "Infiltrataion / runoff coefficient < 0 or > 1 in cell " + std::to_string(i);
+                    return false;
+                }
+        }
+        if (!station_codes("precipitation_station_codes_map.txt", "Precipitation", c.rain_code))
+            return false;
+        std::vector<std::vector<double>> rate;
+        if (!read_scalar_series(in + "precipitation_time_series.txt", c.rain_t, rate, err)) return false;
+        for (size_t k = 1; k < c.rain_t.size(); ++k)
+            if (c.rain_t[k - 1] > c.rain_t[k]) {
+                err = "Precipitation time series - non-increasing times for records " +
+                      std::to_string(k - 1) + " " + std::to_string(k);
+                return false;
+            }
+        c.rain_stations = (int32_t)rate.size();
+        for (size_t i = 0; i < nc; ++i)
+            if (c.rain_code[i] > c.rain_stations) {
+                err = "Precipitation station code " + std::to_string(c.rain_code[i]) + " of cell " +
+                      std::to_string(i) + " exceeds the number of stations";
+                return false;
+            }
+        for (const auto &r : rate) c.rain_v.insert(c.rain_v.end(), r.begin(), r.end());
+    }
+    if (c.infil_model != SW2DG_INFIL_NONE) {
+        // swSp::setInfiltrationParameters (swSp.cpp:1075-1115), checks of swSp.cpp:99-124
+        if (!read_map(in + "infiltration_horton_parameter_map.txt", 1, nc, names, cols, err)) return false;
+        c.infil_rate = cols[0];
+        for (size_t i = 0; i < nc; ++i)
+            if (c.infil_rate[i] < 0.) {
+                err = "Infiltration model = horton parameter is negative in cell " + std::to_string(i);
+                return false;
+            }
+    }
+    if (c.cfg.model_wind != "none") {
+        // station codes: swParent::setWindStationCodes (swParent.cpp:571-595)
+        // and the checks of swSp::init (swSp.cpp:292-333)
+        if (!station_codes("wind_station_codes_map.txt", "Wind", c.wind_code)) return false;
         std::vector<std::vector<double>> norm, angle;
         if (!read_wind_series(in + "wind_time_series.txt", c.wind_t, norm, angle, err))
             return false;

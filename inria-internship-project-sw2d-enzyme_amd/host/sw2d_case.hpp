@@ -81,6 +81,11 @@ bool write_bc_series(const std::string &path, const BcSeries &bc, std::string &e
 // wind time series (swIO::readTimeSeries2dVectorFile, src/common/swIO.cpp:906-1010):
 // two comment lines, the station count, two comment lines, then one row per
 // record: time, then (norm, angle from North in degrees) per station.
+// scalar time series (swIO::readTimeSeriesScalarFile, src/common/swIO.cpp:820-904):
+// two comment lines, the series count, two comment lines, then one row per
+// record: time, then one value per series.
+bool read_scalar_series(const std::string &path, std::vector<double> &t,
+                        std::vector<std::vector<double>> &v, std::string &err);
 bool read_wind_series(const std::string &path, std::vector<double> &t,
                       std::vector<std::vector<double>> &norm,
                       std::vector<std::vector<double>> &angle_deg, std::string &err);
@@ -95,7 +100,7 @@ struct SolverConfig {
     std::string num_scheme = "godunov", topo = "discontinuous", store_flag = "default";
     std::string negative_depth_detection = "none", div_corr = "none";
     std::string model_runoff = "none", model_infiltration = "none", model_wind = "none";
-    std::string model_momentum_diffusion = "none";
+    std::string model_momentum_diffusion = "none", model_porosity_rainfall = "water_only";
     double cfl_max = 1, dt_map = 100, dt_max = 10, dt_time_series = 100, fr_max = 100;
     double h_min = 1e-5, phi_min = 1e-16, t0 = 0, t_max = 100, v_max = 100;
     int n_decimal = 10;
@@ -118,6 +123,13 @@ struct Case {
     std::vector<int32_t> wind_code;
     std::vector<double> wind_t, wind_x, wind_y;
     int32_t wind_stations = 0;
+    // runoff (model_runoff; SW2DG_RUNOFF_*) and infiltration (model_infiltration;
+    // SW2DG_INFIL_*): rain station code per cell (1-based), record times, rate
+    // per station ([station][record], m/s), runoff / infiltration coefficient
+    // and Horton rate per cell
+    int32_t runoff_model = 0, rain_water_only = 1, infil_model = 0, rain_stations = 0;
+    std::vector<int32_t> rain_code;
+    std::vector<double> rain_t, rain_v, runoff_coeff, infil_rate;
     sw2dg_params params() const;
     sw2dg_mesh mesh() const;                     // views into g (int64 copies kept here)
     std::vector<int64_t> bcell64, L64, R64;

@@ -54,6 +54,14 @@ class SpSolver:
         stt = keep("series_t", a["series_t"], np.float64)
         self._check(self.lib.sw2dg_set_bc_series(ctx, case.n_series, case.n_times, N.i32ptr(st),
                                                  N.dptr(stt), N.dptr(sv)))
+        models = a["source_models"] if "source_models" in a else (0, 1, 0)
+        if models[0]:
+            nt = len(a["rain_t"])
+            self.set_runoff(int(models[0]), bool(models[1]), a["rain_code"], a["rain_t"],
+                            np.reshape(a["rain_v"], (-1, nt)),
+                            a["runoff_coeff"] if models[0] != 1 else None)
+        if models[2]:
+            self.set_infiltration(int(models[2]), a["infil_rate"])
         if "wind_code" in a and len(a["wind_code"]) > 0:
             nt = len(a["wind_t"])
             self.set_wind(a["wind_code"], a["wind_t"], np.reshape(a["wind_x"], (-1, nt)),
@@ -90,6 +98,34 @@ class SpSolver:
         vv = self._hold("bc_v", v.reshape(-1))
         self._check(self.lib.sw2dg_set_bc_series(self.ctx, ns, len(t), N.i32ptr(ty), N.dptr(t),
                                                  N.dptr(vv)))
+
+    def set_runoff(self, model, water_only, code=None, t=None, rate=None, coeff=None):
+        """Runoff (swSp::setRunoffParameters + the precipitation tables,
+        swSp.cpp:1276-1320, swParent.cpp:516-569): model 0 none, 1
+        full_rainfall, 2 infiltration_coefficient, 3 runoff_coefficient;
+        water_only is model_porosity_rainfall water_only; code[n_cell]
+        1-based rain station per cell, t[n_times], rate[n_station, n_times]
+        (m/s), coeff[n_cell] for models 2 and 3."""
+        if not model:
+            self._check(self.lib.sw2dg_set_runoff(self.ctx, 0, int(bool(water_only)), 0, 0, None,
+                                                  None, None, None))
+            return
+        code = np.ascontiguousarray(code, dtype=np.int32)
+        self._keep["rain_code"] = code
+        t = self._hold("rain_t", t)
+        rate = self._hold("rain_v", np.reshape(rate, (-1, len(t))))
+        cf = None if coeff is None else N.dptr(self._hold("runoff_coeff", coeff))
+        if code.shape != (self.case.n_cell,):
+            raise ValueError("set_runoff: code[n_cell] expected")
+        self._check(self.lib.sw2dg_set_runoff(self.ctx, int(model), int(bool(water_only)),
+                                              rate.shape[0], len(t), N.i32ptr(code), N.dptr(t),
+                                              N.dptr(rate), cf))
+
+    def set_infiltration(self, model, rate=None):
+        """Horton infiltration (swSp::setInfiltrationParameters, swSp.cpp:1075-1115):
+        model 1 with rate[n_cell] (m/s), 0 off."""
+        r = None if not model else N.dptr(self._hold("infil_rate", rate))
+        self._check(self.lib.sw2dg_set_infiltration(self.ctx, int(model), r))
 
     def set_wind(self, code, t, vx, vy):
         """Wind stations (swParent::setWindStationCodes / setWindVelocityTimeSeriesTables,

@@ -37,6 +37,10 @@ class OrcCase(C.Structure):
         ("v_max", C.c_double), ("fr_max", C.c_double),
         ("n_wind_station", C.c_int32), ("n_wind_times", C.c_int32),
         ("wind_code", _i32), ("wind_t", _d), ("wind_x", _d), ("wind_y", _d),
+        ("runoff_model", C.c_int32), ("rain_water_only", C.c_int32),
+        ("n_rain_station", C.c_int32), ("n_rain_times", C.c_int32),
+        ("rain_code", _i32), ("rain_t", _d), ("rain_v", _d), ("runoff_coeff", _d),
+        ("infil_model", C.c_int32), ("infil_rate", _d),
     ]
 
 
@@ -378,6 +382,17 @@ class OracleSolver:
             c.n_wind_station = len(a["wind_x"]) // c.n_wind_times
             c.wind_code = k(wc, np.int32).ctypes.data_as(_i32)
             c.wind_t, c.wind_x, c.wind_y = (_dp(k(a[f])) for f in ("wind_t", "wind_x", "wind_y"))
+        models = a["source_models"] if "source_models" in a else np.zeros(3, np.int32)
+        c.runoff_model, c.rain_water_only, c.infil_model = (int(x) for x in models)
+        if c.runoff_model:
+            c.n_rain_times = len(a["rain_t"])
+            c.n_rain_station = len(a["rain_v"]) // c.n_rain_times
+            c.rain_code = k(a["rain_code"], np.int32).ctypes.data_as(_i32)
+            c.rain_t, c.rain_v = _dp(k(a["rain_t"])), _dp(k(a["rain_v"]))
+            if c.runoff_model != 1:
+                c.runoff_coeff = _dp(k(a["runoff_coeff"]))
+        if c.infil_model:
+            c.infil_rate = _dp(k(a["infil_rate"]))
         h0, qx0, qy0 = k(a["h0"]), k(a["qx0"]), k(a["qy0"])
         self._s = lib().orc_create(C.byref(c), _dp(h0), _dp(qx0), _dp(qy0))
         if not self._s:

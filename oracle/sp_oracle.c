@@ -47,6 +47,7 @@ struct orc_solver {
     int64_t *bc_rank;
     int *bc_atend;
     double *wvx, *wvy;  /* wind velocity per station at t + dt/2 */
+    double *rain_now;   /* rain rate per station at t + dt/2 */
     double t, dt;
 };
 
@@ -653,6 +654,54 @@ double orc_interp_series(const double *t, const double *v, int64_t n, double at)
     return interp_series(t, v, n, at);
 }
 
+/* swSp::sourceRunoffEverywhere -- swSpSourceRunoffInfiltration.cpp:43-162,
+ * swSp::sourceRunoffWateronly -- :164-263 */
+static void source_runoff(orc_solver *s, double dt) {
+    const orc_case *c = &s->c;
+    const int64_t nt = c->n_rain_times;
+    const double at = s->t + 0.5 * dt;
+    for (int32_t k = 0; k < c->n_rain_station; k++)
+        s->rain_now[k] = interp_series(c->rain_t, c->rain_v + k * nt, nt, at);
+    for (int64_t i = 0; i < c->n_cell; i++) {
+        const double rate = s->rain_now[c->rain_code[i] - 1];
+        const double h0 = s->h[i];
+        double dp;
+        if (c->rain_water_only) {
+            dp = dt * rate;
+        } else {
+            dp = 0.0;
+            if (c->phi[i] > SMALL_REAL) {
+                dp = dt * rate;
+                /* the runoff-coefficient model divides rain only, not evaporation */
+                if (c->runoff_model != 3 || dp >= 0.0) dp /= c->phi[i];
+            }
+        }
+        if (dp > 0.) {
+            if (c->runoff_model == 1) s->h[i] += dp;
+            else if (c->runoff_model == 2) s->h[i] += (1.0 - c->runoff_coeff[i]) * dp;
+            else s->h[i] += c->runoff_coeff[i] * dp;
+        } else {
+            s->h[i] = smax(0.0, h0 + dp);
+            const double ratio = s->h[i] / smax(h0, SMALL_REAL);
+            s->qx[i] *= ratio;
+            s->qy[i] *= ratio;
+        }
+    }
+}
+
+/* swSp::sourceInfiltration, Horton -- swSpSourceRunoffInfiltration.cpp:12-40 */
+static void source_infiltration(orc_solver *s, double dt) {
+    const orc_case *c = &s->c;
+    for (int64_t i = 0; i < c->n_cell; i++) {
+        const double h0 = s->h[i];
+        const double dh = -c->infil_rate[i] * dt;
+        s->h[i] = smax(0.0, h0 + dh);
+        const double ratio = s->h[i] / smax(h0, SMALL_REAL);
+        s->qx[i] *= ratio;
+        s->qy[i] *= ratio;
+    }
+}
+
 /* swSp::sourceWind + sourceWindSmithBanke -- swSpSourceWind.cpp:11-85 */
 static void source_wind(orc_solver *s, double dt) {
     const orc_case *c = &s->c;
@@ -720,6 +769,8 @@ int orc_step(orc_solver *s) {
     }
     if (c->friction != ORC_FRIC_NONE) source_friction(s, dt);
     if (c->head_loss) source_head_loss(s, dt);
+    if (c->runoff_model) source_runoff(s, dt);
+    if (c->infil_model) source_infiltration(s, dt);
     if (c->n_wind_station > 0) source_wind(s, dt);
     correct_velocity(s);
     s->t += dt;
@@ -790,6 +841,14 @@ orc_solver *orc_create(const orc_case *c, const double *h0, const double *qx0, c
         s->c.wind_y = nw ? xcopy(c->wind_y, sizeof(double) * nw * wt) : NULL;
         s->wvx = (double *)calloc(nw ? nw : 1, sizeof(double));
         s->wvy = (double *)calloc(nw ? nw : 1, sizeof(double));
+        const size_t nr = (size_t)(c->runoff_model ? c->n_rain_station : 0);
+        const size_t rt = (size_t)(c->runoff_model ? c->n_rain_times : 0);
+        s->c.rain_code = nr ? xcopy(c->rain_code, sizeof(int32_t) * (size_t)nc) : NULL;
+        s->c.rain_t = nr ? xcopy(c->rain_t, sizeof(double) * rt) : NULL;
+        s->c.rain_v = nr ? xcopy(c->rain_v, sizeof(double) * nr * rt) : NULL;
+        s->c.runoff_coeff = (nr && c->runoff_model != 1) ? xcopy(c->runoff_coeff, dc) : NULL;
+        s->c.infil_rate = c->infil_model ? xcopy(c->infil_rate, dc) : NULL;
+        s->rain_now = (double *)calloc(nr ? nr : 1, sizeof(double));
     }
 
     s->h = xcopy(h0, dc);
@@ -869,7 +928,9 @@ void orc_destroy(orc_solver *s) {
         s->F1i, s->F2i, s->F3i, s->S2Li, s->S2Ri, s->zc, s->dzdx, s->dzdy, s->nb_ptr, s->nb_idx,
         s->swept, s->fcos, s->fsin, s->hl_alpha, s->bc_vinf, s->bc_vsup, s->bc_tinf, s->bc_tsup,
         s->bc_val, s->bc_rank, s->bc_atend, (void *)s->c.wind_code, (void *)s->c.wind_t,
-        (void *)s->c.wind_x, (void *)s->c.wind_y, s->wvx, s->wvy};
+        (void *)s->c.wind_x, (void *)s->c.wind_y, s->wvx, s->wvy, (void *)s->c.rain_code,
+        (void *)s->c.rain_t, (void *)s->c.rain_v, (void *)s->c.runoff_coeff,
+        (void *)s->c.infil_rate, s->rain_now};
     for (size_t i = 0; i < sizeof(owned) / sizeof(owned[0]); i++) free(owned[i]);
     free(s);
 }

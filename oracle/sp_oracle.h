@@ -15,6 +15,7 @@
  *   src/sw2dSolver/models/common/swSourceFriction.cpp
  *   src/sw2dSolver/models/common/swBoundaryConditionTimeSeries.cpp
  *   src/sw2dSolver/models/sp/swSpSourceWind.cpp, swParent.cpp (wind series)
+ *   src/sw2dSolver/models/sp/swSpSourceRunoffInfiltration.cpp
  * All loops run in the reference's edge order with its scatter updates, so
  * the summation order is the reference's by construction.
  */
@@ -69,6 +70,16 @@ typedef struct orc_case {
     const int32_t *wind_code;        /* [n_cell] 1-based station of each cell */
     const double *wind_t;            /* [n_wind_times] */
     const double *wind_x, *wind_y;   /* [n_wind_station * n_wind_times], per station */
+    /* runoff (model_runoff: 0 none, 1 full_rainfall, 2 infiltration_coefficient,
+     * 3 runoff_coefficient; rain_water_only = model_porosity_rainfall water_only)
+     * and Horton infiltration (infil_model 1) */
+    int32_t runoff_model, rain_water_only, n_rain_station, n_rain_times;
+    const int32_t *rain_code;        /* [n_cell] 1-based station of each cell */
+    const double *rain_t;            /* [n_rain_times] */
+    const double *rain_v;            /* [n_rain_station * n_rain_times] rate (m/s) */
+    const double *runoff_coeff;      /* [n_cell], models 2 and 3 */
+    int32_t infil_model;
+    const double *infil_rate;        /* [n_cell] */
 } orc_case;
 
 typedef struct orc_solver orc_solver;

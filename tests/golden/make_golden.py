@@ -78,6 +78,23 @@ def golden_cases():
                                                             friction="manning", t_max=20.0,
                                                             dt_map=5.0, closed_fraction=0.0,
                                                             wind=True)),
+        # runoff (rain and evaporation) and Horton infiltration
+        # (swSpSourceRunoffInfiltration.cpp), both porosity-rainfall modes
+        "rain_dam_small_godunov_full_infil": ("dambreak_case", dict(
+            nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0, scheme="godunov", friction="none",
+            t_max=3.0, dt_map=1.0, rain="full_rainfall", rain_where="everywhere", infil=True)),
+        "rain_mixed_godunov_infil_coeff_everywhere": ("mixed_case", dict(
+            scheme="godunov", friction="strickler", t_max=20.0, dt_map=5.0,
+            rain="infiltration_coefficient", rain_where="everywhere", infil=True)),
+        "rain_mixed_godunov_runoff_coeff_everywhere": ("mixed_case", dict(
+            scheme="godunov", friction="manning", t_max=20.0, dt_map=5.0,
+            rain="runoff_coefficient", rain_where="everywhere")),
+        "rain_mixed_godunov_runoff_coeff_water_only": ("mixed_case", dict(
+            scheme="godunov", friction="strickler", t_max=20.0, dt_map=5.0, closed_fraction=0.0,
+            rain="runoff_coefficient", rain_where="water_only", infil=True)),
+        "rain_mixed_muscl_evr_full_water_only": ("mixed_case", dict(
+            scheme="muscl_evr", friction="manning", t_max=20.0, dt_map=5.0, closed_fraction=0.0,
+            rain="full_rainfall", rain_where="water_only", infil=True, wind=True)),
         # the reference stops on an impossible HLLC regime (swSpHyp.cpp:729-739)
         "mixed_muscl_closed_cells_error": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
                                                              t_max=20.0, closed_fraction=0.04)),

@@ -91,7 +91,7 @@ def test_gpu_assim_refuses_wind(sw, tmp_path):
     case = sw.Case(folder)
     _with_wind(case, *_long_table(case, n_st=2, n_rec=10))
     s = sw.SpSolver(case)
-    with pytest.raises(sw._native.SW2DError, match="no wind source"):
+    with pytest.raises(sw._native.SW2DError, match="has no wind"):
         assim.Assim(s, np.array([10.0, 20.0]), gauge_cells=np.array([0]), gauge_h=np.zeros((2, 1)))
     s.close()
 
