@@ -122,6 +122,17 @@ int sw2dg_set_runoff(sw2dg_ctx *ctx, int32_t model, int32_t water_only, int32_t 
  * the runoff step): model SW2DG_INFIL_HORTON with rate[n_cell] >= 0 (m/s,
  * reference order); SW2DG_INFIL_NONE turns it off. */
 int sw2dg_set_infiltration(sw2dg_ctx *ctx, int32_t model, const double *rate);
+/* Switches on swSp::parabComputeExplicit (swSpParab.cpp:10-96) for
+ * model_momentum_diffusion velocity_independent (model 1; 0 off): after the
+ * hyperbolic balance and clamp and before friction (swSp.cpp:535-545), qx
+ * and qy diffuse with the edge coefficient of the harmonic mean of
+ * 1e-6 h phi of its cells (the reference's fixed coefficient, swSp.h:244)
+ * and the factor length / |centre distance along the normal|
+ * (swSp.cpp:1236-1246).  The reference divides q by h of both cells, so a
+ * dry cell makes NaN there as in the reference.  One explicit substep per
+ * step is built: a step whose bound asks for more stops with SW2DG_ERR_ARG
+ * (it needs a cell area below 2e-6 dt h). */
+int sw2dg_set_momentum_diffusion(sw2dg_ctx *ctx, int32_t model);
 /* Replaces swSp::setInitialConditions (swSp.cpp:1119-1224): sets the state
  * and the clock (t, previous dt = 0, step count 0, BC cursors rewound). */
 int sw2dg_set_state(sw2dg_ctx *ctx, const double *h, const double *qx, const double *qy,

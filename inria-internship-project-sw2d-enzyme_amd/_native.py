@@ -123,6 +123,7 @@ _GPU_DECL = {
     "sw2dg_set_runoff": ([C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _i32, _d, _d, _d],
                          C.c_int),
     "sw2dg_set_infiltration": ([C.c_void_p, C.c_int32, _d], C.c_int),
+    "sw2dg_set_momentum_diffusion": ([C.c_void_p, C.c_int32], C.c_int),
     "sw2dg_set_state": ([C.c_void_p, _d, _d, _d, C.c_double], C.c_int),
     "sw2dg_get_state": ([C.c_void_p, _d, _d, _d], C.c_int),
     "sw2dg_run": ([C.c_void_p, C.c_double, C.c_int64, C.c_int32, _i64], C.c_int),

@@ -127,17 +127,18 @@ def write_case(folder, geom, solv, h0, qx0, qy0, phi=None, beta=None, friction="
 def dambreak_case(folder, nx=100, ny=100, Lx=200.0, Ly=100.0, x_dam=100.0, h_left=2.0,
                   h_right=0.5, scheme="muscl_evr", topo="discontinuous", friction="manning",
                   n_manning=0.02, t_max=10.0, cfl=0.9, dt_map=1e6, dt_ts=1e6, store_flag="default",
-                  wind=False, rain=None, rain_where="water_only", infil=False):
+                  wind=False, rain=None, rain_where="water_only", infil=False, mdiff=False):
     """Plain SP forward dam-break (20,000 triangles at the default size).
     With wind=True it adds the Smith & Banke wind source: one station on each
     side of the dam, records at 1/6, 1/2 and 5/6 of t_max; rain / infil add
-    rain, evaporation and infiltration as in mixed_case."""
+    rain, evaporation and infiltration as in mixed_case; mdiff turns on
+    momentum diffusion (all cells wet: h_right > 0)."""
     g = Geometry.rect(nx, ny, Lx, Ly)
     x = g.array("cell_x")
     h0 = np.where(x < x_dam, h_left, h_right)
     solv = solv_text(scheme=scheme, topo=topo, friction=friction, cfl=cfl, t_max=t_max,
                      dt_max=10.0, dt_map=dt_map, dt_ts=dt_ts, store_flag=store_flag,
-                     extra=_source_keys(wind, rain, rain_where, infil))
+                     extra=_source_keys(wind, rain, rain_where, infil, mdiff))
     write_case(folder, g, solv, h0, 0.0, 0.0, friction=friction,
                fric=None if friction == "none" else n_manning,
                bc_codes=(1,), bc_types=("q",), bc_t=(0.0, 1e9), bc_v=((0.0, 0.0),))
@@ -343,8 +344,10 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
     return folder
 
 
-def _source_keys(wind, rain, rain_where, infil):
+def _source_keys(wind, rain, rain_where, infil, mdiff=False):
     keys = {}
+    if mdiff:
+        keys["model_momentum_diffusion"] = "velocity_independent"
     if wind:
         keys["model_wind"] = "smithbanke"
     if rain:

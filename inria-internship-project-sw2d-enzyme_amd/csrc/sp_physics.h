@@ -20,10 +20,12 @@ constexpr double kSmall = 1e-16;  // src/common/param.h:2 (small_real)
 constexpr double kG = 9.81;       // src/common/param.h:3 (gravity)
 constexpr double kRhoAir = 1.2;     // src/common/param.h:4 (rho_air)
 constexpr double kRhoWater = 1.0e3; // src/common/param.h:5 (rho_water)
+constexpr double kMomDiff = 1e-6;   // swSp.h:244 (momentum_diffusion_coefficient, never read from input)
 
 enum BcType { BC_FR = 0, BC_H = 1, BC_Z = 2, BC_Q = 3 };
 enum Friction { FRIC_NONE = 0, FRIC_MANNING = 1, FRIC_STRICKLER = 2, FRIC_CHEZY = 3 };
-enum Status { ST_OK = 0, ST_NAN = 2, ST_NEG_DEPTH = 3, ST_FLOW_CONFIG = 4, ST_BC = 5 };
+enum Status { ST_OK = 0, ST_NAN = 2, ST_NEG_DEPTH = 3, ST_FLOW_CONFIG = 4, ST_BC = 5,
+              ST_PARAB_SUBSTEPS = 6 };
 
 // std::max / std::min semantics (first argument wins unless strictly beaten)
 SPD double smax(double a, double b) { return (a < b) ? b : a; }

@@ -1395,8 +1395,9 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
                       double t_window_start, int64_t max_steps) {
     if (!c || !obs || max_steps <= 0 || n_tiles < 1 || stride < 0) return SW2DG_ERR_ARG;
     if (!c->state_set) return fail(c, SW2DG_ERR_ARG, "sw2dg_set_state must be called first");
-    if (c->o.wind || c->o.runoff || c->o.infil)
-        return fail(c, SW2DG_ERR_ARG, "the 4D-Var layer has no wind, runoff or infiltration source");
+    if (c->o.wind || c->o.runoff || c->o.infil || c->o.mdiff)
+        return fail(c, SW2DG_ERR_ARG,
+                    "the 4D-Var layer has no wind, runoff, infiltration or momentum diffusion");
     if (obs->n_obs < 0 || obs->n_gauges < 0 || (obs->n_obs > 0 && !obs->t)) return SW2DG_ERR_ARG;
     if (obs->n_gauges > 0 && (!obs->gauge_cell || !obs->gauge_h)) return SW2DG_ERR_ARG;
     for (int g = 0; g < obs->n_gauges; ++g)

@@ -354,6 +354,109 @@ __device__ int record_bracket(const double *tv, int n, double tw) {
     return lo;
 }
 
+// ---------------------------------------------------------------------------
+// momentum diffusion, velocity_independent (swSpParab.cpp:10-96): edge
+// coefficient = harmonic mean of nu h phi of its cells, explicit substeps of
+// dt / n with n = int(dt / min(t_max - t_0, dt, min over edge sides of
+// area phi / (2 D))), on qx and qy.  One substep (n = 1) is built here; a
+// step that needs more stops with an error.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double parab_coef(const MeshDev &m, int L, int R) {
+    const double dL = sp::kMomDiff * m.h[L] * m.phi[L];
+    const double dR = sp::kMomDiff * m.h[R] * m.phi[R];
+    return (smax(dL, dR) > kSmall) ? 2.0 * dL * dR / (dL + dR) : 0.0;
+}
+
+// edge factor length / max(|d . n|, small_real) (swSp.cpp:1236-1246), once per mesh
+__global__ __launch_bounds__(kBlock) void k_parab_fac(MeshDev m, double *fac) {
+    const int e = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    if (e >= m.n_int) return;
+    const int L = m.eL[e], R = m.eR[e];
+    const double dx = m.cx[R] - m.cx[L];
+    const double dy = m.cy[R] - m.cy[L];
+    const double d = fabs(dx * m.enx[e] + dy * m.eny[e]);
+    fac[e] = m.elen[e] / smax(d, kSmall);
+}
+
+// per-block minimum of the substep bounds of the internal edges
+__global__ __launch_bounds__(kBlock) void k_parab_dt(MeshDev m, Clock *ck, int p) {
+    if (!step_active(ck, p)) return;
+    __shared__ double red[kBlock / 64];
+    const int e = xcd_block() * kBlock + (int)threadIdx.x;
+    double v = __builtin_inf();
+    if (e < m.n_int) {
+        const int L = m.eL[e], R = m.eR[e];
+        const double D = parab_coef(m, L, R);
+        const double a = 0.5 * m.area[L] * m.phi[L] / D;
+        const double b = 0.5 * m.area[R] * m.phi[R] / D;
+        v = (a < v) ? a : v;  // a NaN bound (0 / 0) is never taken, as with std::min
+        v = (b < v) ? b : v;
+    }
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    v = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) v = (red[w] < v) ? red[w] : v;
+    m.ppart[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kDtBlock) void k_parab_n(MeshDev m, Opts o, Clock *ck, int p, int n_part) {
+    if (!step_active(ck, p)) return;
+    __shared__ double red[kDtBlock / 64];
+    double v = o.parab_span;
+    for (int k = threadIdx.x; k < n_part; k += kDtBlock) {
+        const double x = m.ppart[k];
+        v = (x < v) ? x : v;
+    }
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double dtp = red[0];
+    for (int w = 1; w < kDtBlock / 64; ++w) dtp = (red[w] < dtp) ? red[w] : dtp;
+    const double dt = ck->DT[1 - p];
+    dtp = (dt < dtp) ? dt : dtp;
+    const int n = (int)(dt / dtp);
+    if (n != 1) {
+        raise_err(ck, sp::ST_PARAB_SUBSTEPS);
+        return;
+    }
+    ck->parab_dt = dt / n;
+}
+
+// one explicit substep of qx and qy; each internal edge flux is formed by
+// both of its cells from the same values, in the reference's edge order
+__global__ __launch_bounds__(kBlock) void k_parab_sub(MeshDev m, Clock *ck, int p) {
+    if (!step_active(ck, p)) return;
+    const int i = xcd_block() * kBlock + (int)threadIdx.x;
+    if (i >= m.n_cell) return;
+    const double dtp = ck->parab_dt;
+    const double den = smax(kSmall, m.phi[i]);
+    const double area = m.area[i];
+    double vx = m.qx[i], vy = m.qy[i];
+    const int ib = m.int_ptr[i], ie = m.int_ptr[i + 1];
+    for (int k = ib; k < ie; ++k) {
+        const int e = m.int_inc[k] >> 1;
+        const int L = m.eL[e], R = m.eR[e];
+        const double w = parab_coef(m, L, R) * m.pfac[e];
+        const double hL = m.h[L], hR = m.h[R];
+        const double fx = w * (m.qx[L] / hL - m.qx[R] / hR);
+        const double fy = w * (m.qy[L] / hL - m.qy[R] / hR);
+        if (i == L) {
+            vx = vx - dtp * fx / area / den;
+            vy = vy - dtp * fy / area / den;
+        } else {
+            vx = vx + dtp * fx / area / den;
+            vy = vy + dtp * fy / area / den;
+        }
+    }
+    m.pqx[i] = vx;
+    m.pqy[i] = vy;
+}
+
 // Value of one station's records at time tw given its bracket k
 // (record_bracket): the interpolation of swParent.cpp:383-384.
 __device__ __forceinline__ double record_value(const double *tv, const double *v, int n, int k,
@@ -423,7 +526,11 @@ __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck,
 // (swSpSourceFriction.cpp:39-171) + singular head loss (:174-218) +
 // correctVelocity (swSp.cpp:447-479), all local to one cell.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, Clock *ck, int p) {
+// part 0: the whole update; with momentum diffusion the parabolic step sits
+// between the clamp and friction (swSp.cpp:535-545): part 1 stops after the
+// clamp, part 2 starts from the diffused discharges.
+__global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, Clock *ck, int p,
+                                                   int part) {
     if (!step_active(ck, p)) {
         carry_clock(ck, p);
         return;
@@ -431,8 +538,11 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= m.n_cell) return;
     const double dt = ck->DT[1 - p];
-    const double h0 = m.h[i], qx0 = m.qx[i], qy0 = m.qy[i];
+    const double h0 = m.h[i];
+    const double qx0 = (part == 2) ? m.pqx[i] : m.qx[i];
+    const double qy0 = (part == 2) ? m.pqy[i] : m.qy[i];
     double h = h0, qx = qx0, qy = qy0;
+    if (part != 2) {
     const int bb = m.bnd_ptr[i], be = m.bnd_ptr[i + 1];
     if (be > bb) {
         const double zb = m.zb[i], phi = m.phi[i], b = m.beta[i];
@@ -474,6 +584,13 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
         h = 0.;
         qx = 0.;
         qy = 0.;
+    }
+    }
+    if (part == 1) {
+        m.h[i] = h;
+        m.qx[i] = qx;
+        m.qy[i] = qy;
+        return;
     }
     if (o.friction != sp::FRIC_NONE) {
         if (h > smax(o.h_min, kSmall)) {
@@ -623,6 +740,20 @@ int fail(sw2dg_ctx *c, int code, const std::string &msg) {
 
 namespace {
 
+void launch_update(sw2dg_ctx *c, int p) {
+    const dim3 blk(kBlock);
+    if (!c->o.mdiff) {
+        hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 0);
+        return;
+    }
+    hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 1);
+    hipLaunchKernelGGL(k_parab_dt, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->ck, p);
+    hipLaunchKernelGGL(k_parab_n, dim3(1), dim3(kDtBlock), 0, c->stream, c->m, c->o, c->ck, p,
+                       nblocks(c->ni));
+    hipLaunchKernelGGL(k_parab_sub, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->ck, p);
+    hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 2);
+}
+
 void launch_step(sw2dg_ctx *c, int p) {
     const dim3 blk(kBlock);
     if (c->o.scheme == 1) {
@@ -633,7 +764,7 @@ void launch_step(sw2dg_ctx *c, int p) {
     }
     hipLaunchKernelGGL(k_cfl, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->ck, p);
     hipLaunchKernelGGL(k_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->m, c->bc, c->ck, p, nblocks(c->nc));
-    hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p);
+    launch_update(c, p);
 }
 
 int build_graph(sw2dg_ctx *c) {
@@ -663,6 +794,7 @@ int map_err(int st) {
         case sp::ST_NEG_DEPTH: return SW2DG_ERR_NEG_DEPTH;
         case sp::ST_FLOW_CONFIG: return SW2DG_ERR_FLOW_CONFIG;
         case sp::ST_BC: return SW2DG_ERR_BC;
+        case sp::ST_PARAB_SUBSTEPS: return SW2DG_ERR_ARG;
         default: return SW2DG_ERR_HIP;
     }
 }
@@ -673,6 +805,9 @@ const char *err_text(int st) {
         case sp::ST_NEG_DEPTH: return "SP - Negative water depth";
         case sp::ST_FLOW_CONFIG: return "Impossible flow configuration in HLLC";
         case sp::ST_BC: return "invalid boundary condition value or type";
+        case sp::ST_PARAB_SUBSTEPS:
+            return "momentum diffusion needs more than one explicit substep in this step "
+                   "(only one is built)";
         default: return "device error";
     }
 }
@@ -1161,6 +1296,34 @@ int sw2dg_set_infiltration(sw2dg_ctx *c, int32_t model, const double *rate) {
     return SW2DG_OK;
 }
 
+int sw2dg_set_momentum_diffusion(sw2dg_ctx *c, int32_t model) {
+    if (!c || model < 0 || model > 1) return SW2DG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (model && c->assim) return fail(c, SW2DG_ERR_ARG, "the 4D-Var layer has no momentum diffusion");
+    if (model && !c->m.pfac) {
+        double *fac = c->dalloc<double>(c->ni);
+        c->m.pqx = c->dalloc<double>(c->nc);
+        c->m.pqy = c->dalloc<double>(c->nc);
+        c->m.ppart = c->dalloc<double>(nblocks(c->ni));
+        if (!fac || !c->m.pqx || !c->m.pqy || !c->m.ppart)
+            return fail(c, SW2DG_ERR_HIP, "momentum diffusion: device allocation failed");
+        if (c->ni > 0)
+            hipLaunchKernelGGL(k_parab_fac, dim3(nblocks(c->ni)), dim3(kBlock), 0, c->stream, c->m, fac);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->m.pfac = fac;
+    }
+    c->o.mdiff = model;
+    c->o.parab_span = c->prm.t_max - c->prm.t0;
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+    }
+    return SW2DG_OK;
+}
+
 int sw2dg_set_state(sw2dg_ctx *c, const double *h, const double *qx, const double *qy, double t) {
     if (!c || !h || !qx || !qy) return SW2DG_ERR_ARG;
     if (!c->bc.t) return fail(c, SW2DG_ERR_ARG, "sw2dg_set_bc_series must be called first");
@@ -1339,7 +1502,7 @@ int sw2dg_profile_kernels(sw2dg_ctx *c, int64_t n_steps, int32_t k_max, double *
         hipEventRecord(ev[2 * k + 1], c->stream);
         ++k;
         hipEventRecord(ev[2 * k], c->stream);
-        hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p);
+        launch_update(c, p);
         hipEventRecord(ev[2 * k + 1], c->stream);
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int j = 0; j < nk; ++j) {

@@ -34,6 +34,7 @@ struct Clock {
     double t_half;       // source interpolation time of the step (t + dt/2)
     int wind_k;          // record bracket at t_half: -1 first record, -2 last, k >= 1 (k-1, k)
     int rain_k;          // the same on the rain records
+    double parab_dt;     // explicit momentum-diffusion substep of the step
 };
 
 struct BcDev {
@@ -78,11 +79,17 @@ struct MeshDev {
     const int *rcode;
     const double *rain_t, *rain_v, *rcoef, *infil;
     int rain_nt;
+    // momentum diffusion: per internal edge length / |centre distance along
+    // the normal|, diffused discharges, per-block parabolic dt candidates
+    const double *pfac;
+    double *pqx, *pqy, *ppart;
 };
 
 struct Opts {
     int scheme, topo, friction, head_loss, neg_det, wind;
     int runoff, rain_water_only, infil;  // SW2DG_RUNOFF_*, model_porosity_rainfall, SW2DG_INFIL_*
+    int mdiff;           // momentum diffusion (velocity_independent)
+    double parab_span;   // t_max - t_0: the first bound of the parabolic substep
     double h_min, phi_min, v_max, fr_max;
     double adj_cut;  // reverse sweep: adjoint zeroed at cells shallower than this (0: exact)
 };

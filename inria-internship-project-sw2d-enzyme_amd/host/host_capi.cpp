@@ -29,7 +29,7 @@ struct sw2dh_geom {
 struct sw2dh_case {
     sw2d::Case c;
     std::vector<int32_t> stype;
-    std::vector<int32_t> models;  // runoff model, rain on water only, infiltration model
+    std::vector<int32_t> models;  // runoff, rain on water only, infiltration, momentum diffusion
     std::vector<double> sval;
 };
 
@@ -225,7 +225,7 @@ int sw2dh_case_load(const char *folder, sw2dh_case **out) {
     }
     h->stype = h->c.series_types();
     h->sval = h->c.series_values();
-    h->models = {h->c.runoff_model, h->c.rain_water_only, h->c.infil_model};
+    h->models = {h->c.runoff_model, h->c.rain_water_only, h->c.infil_model, h->c.momentum_diffusion};
     *out = h;
     return 0;
 }

@@ -75,6 +75,8 @@ int main(int argc, char **argv) {
               "set_runoff");
     if (cs.infil_model != SW2DG_INFIL_NONE)
         check(sw2dg_set_infiltration(ctx, cs.infil_model, cs.infil_rate.data()), "set_infiltration");
+    if (cs.momentum_diffusion)
+        check(sw2dg_set_momentum_diffusion(ctx, 1), "set_momentum_diffusion");
     if (cs.wind_stations > 0)
         check(sw2dg_set_wind(ctx, cs.wind_stations, (int32_t)cs.wind_t.size(), cs.wind_code.data(),
                              cs.wind_t.data(), cs.wind_x.data(), cs.wind_y.data()),

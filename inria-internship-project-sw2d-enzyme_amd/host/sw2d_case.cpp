@@ -961,11 +961,13 @@ bool load_case(const std::string &folder, Case &c, std::string &err) {
         err = "only model_hyperb sp is handled by this solver (got " + c.cfg.model_hyp + ")";
         return false;
     }
-    for (const auto *opt : {&c.cfg.div_corr, &c.cfg.model_momentum_diffusion})
-        if (*opt != "none") {
-            err = "option not handled by the GPU solver: " + *opt;
-            return false;
-        }
+    if (c.cfg.div_corr != "none") {
+        err = "option not handled by the GPU solver: " + c.cfg.div_corr;
+        return false;
+    }
+    // swSolverParam::setModelMomentumDiffusion (swSolverParam.cpp:1615-1635) accepts
+    // any value and swSp::execute diffuses for anything but "none" (swSp.cpp:537)
+    c.momentum_diffusion = c.cfg.model_momentum_diffusion != "none" ? 1 : 0;
     // runoff / infiltration models (swSolverParam.cpp:1486-1530, swSpSourceRunoffInfiltration.cpp)
     const std::string &ro = c.cfg.model_runoff, &inf = c.cfg.model_infiltration;
     if (ro == "scs" || inf == "h_proportional") {

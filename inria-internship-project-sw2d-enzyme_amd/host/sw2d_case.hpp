@@ -128,6 +128,7 @@ struct Case {
     // per station ([station][record], m/s), runoff / infiltration coefficient
     // and Horton rate per cell
     int32_t runoff_model = 0, rain_water_only = 1, infil_model = 0, rain_stations = 0;
+    int32_t momentum_diffusion = 0;  // model_momentum_diffusion other than none
     std::vector<int32_t> rain_code;
     std::vector<double> rain_t, rain_v, runoff_coeff, infil_rate;
     sw2dg_params params() const;

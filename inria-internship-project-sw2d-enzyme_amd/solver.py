@@ -54,7 +54,9 @@ class SpSolver:
         stt = keep("series_t", a["series_t"], np.float64)
         self._check(self.lib.sw2dg_set_bc_series(ctx, case.n_series, case.n_times, N.i32ptr(st),
                                                  N.dptr(stt), N.dptr(sv)))
-        models = a["source_models"] if "source_models" in a else (0, 1, 0)
+        models = a["source_models"] if "source_models" in a else (0, 1, 0, 0)
+        if len(models) > 3 and models[3]:
+            self.set_momentum_diffusion(1)
         if models[0]:
             nt = len(a["rain_t"])
             self.set_runoff(int(models[0]), bool(models[1]), a["rain_code"], a["rain_t"],
@@ -126,6 +128,11 @@ class SpSolver:
         model 1 with rate[n_cell] (m/s), 0 off."""
         r = None if not model else N.dptr(self._hold("infil_rate", rate))
         self._check(self.lib.sw2dg_set_infiltration(self.ctx, int(model), r))
+
+    def set_momentum_diffusion(self, model):
+        """Momentum diffusion (swSp::parabComputeExplicit, swSpParab.cpp:10-96):
+        1 on (velocity_independent, coefficient 1e-6 as the reference's), 0 off."""
+        self._check(self.lib.sw2dg_set_momentum_diffusion(self.ctx, int(model)))
 
     def set_wind(self, code, t, vx, vy):
         """Wind stations (swParent::setWindStationCodes / setWindVelocityTimeSeriesTables,

@@ -40,7 +40,7 @@ class OrcCase(C.Structure):
         ("runoff_model", C.c_int32), ("rain_water_only", C.c_int32),
         ("n_rain_station", C.c_int32), ("n_rain_times", C.c_int32),
         ("rain_code", _i32), ("rain_t", _d), ("rain_v", _d), ("runoff_coeff", _d),
-        ("infil_model", C.c_int32), ("infil_rate", _d),
+        ("infil_model", C.c_int32), ("infil_rate", _d), ("momentum_diffusion", C.c_int32),
     ]
 
 
@@ -383,7 +383,8 @@ class OracleSolver:
             c.wind_code = k(wc, np.int32).ctypes.data_as(_i32)
             c.wind_t, c.wind_x, c.wind_y = (_dp(k(a[f])) for f in ("wind_t", "wind_x", "wind_y"))
         models = a["source_models"] if "source_models" in a else np.zeros(3, np.int32)
-        c.runoff_model, c.rain_water_only, c.infil_model = (int(x) for x in models)
+        c.runoff_model, c.rain_water_only, c.infil_model = (int(x) for x in models[:3])
+        c.momentum_diffusion = int(models[3]) if len(models) > 3 else 0
         if c.runoff_model:
             c.n_rain_times = len(a["rain_t"])
             c.n_rain_station = len(a["rain_v"]) // c.n_rain_times

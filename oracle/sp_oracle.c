@@ -17,6 +17,7 @@
 #define GRAVITY 9.81     /* src/common/param.h:3 */
 #define RHO_AIR 1.2      /* src/common/param.h:4 */
 #define RHO_WATER 1.0e3  /* src/common/param.h:5 */
+#define MOM_DIFF 1e-6    /* swSp.h:244, not read from the input */
 
 /* std::max / std::min semantics: max returns a unless a < b, min returns a unless b < a */
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
@@ -48,6 +49,7 @@ struct orc_solver {
     int *bc_atend;
     double *wvx, *wvy;  /* wind velocity per station at t + dt/2 */
     double *rain_now;   /* rain rate per station at t + dt/2 */
+    double *pfac, *pcoef, *pflux;  /* momentum diffusion per internal edge */
     double t, dt;
 };
 
@@ -654,6 +656,46 @@ double orc_interp_series(const double *t, const double *v, int64_t n, double at)
     return interp_series(t, v, n, at);
 }
 
+/* swSp::parabComputeScalarExplicit -- swSpParab.cpp:48-96: substeps of
+ * dt / n on one discharge component, fluxes from the values at the start of
+ * the substep, scattered L then R in edge order */
+static void parab_scalar(orc_solver *s, const double *D, double *v, double dt) {
+    const orc_case *c = &s->c;
+    double dtp = c->t_max - c->t0;
+    for (int64_t e = 0; e < c->n_int; e++) {
+        const int64_t L = c->int_L[e], R = c->int_R[e];
+        dtp = smin(dtp, 0.5 * c->cell_area[L] * c->phi[L] / D[e]);
+        dtp = smin(dtp, 0.5 * c->cell_area[R] * c->phi[R] / D[e]);
+    }
+    dtp = smin(dtp, dt);
+    const int64_t n = (int64_t)(int)(dt / dtp);
+    dtp = dt / (double)n;
+    for (int64_t k = 0; k < n; k++) {
+        for (int64_t e = 0; e < c->n_int; e++) {
+            const int64_t L = c->int_L[e], R = c->int_R[e];
+            s->pflux[e] = D[e] * s->pfac[e] * (v[L] / s->h[L] - v[R] / s->h[R]);
+        }
+        for (int64_t e = 0; e < c->n_int; e++) {
+            const int64_t L = c->int_L[e], R = c->int_R[e];
+            v[L] -= dtp * s->pflux[e] / c->cell_area[L] / smax(SMALL_REAL, c->phi[L]);
+            v[R] += dtp * s->pflux[e] / c->cell_area[R] / smax(SMALL_REAL, c->phi[R]);
+        }
+    }
+}
+
+/* swSp::parabComputeExplicit -- swSpParab.cpp:10-46, coefficient 1e-6 (swSp.h:244) */
+static void source_parab(orc_solver *s, double dt) {
+    const orc_case *c = &s->c;
+    for (int64_t e = 0; e < c->n_int; e++) {
+        const int64_t L = c->int_L[e], R = c->int_R[e];
+        const double dL = MOM_DIFF * s->h[L] * c->phi[L];
+        const double dR = MOM_DIFF * s->h[R] * c->phi[R];
+        s->pcoef[e] = (smax(dL, dR) > SMALL_REAL) ? 2.0 * dL * dR / (dL + dR) : 0.0;
+    }
+    parab_scalar(s, s->pcoef, s->qx, dt);
+    parab_scalar(s, s->pcoef, s->qy, dt);
+}
+
 /* swSp::sourceRunoffEverywhere -- swSpSourceRunoffInfiltration.cpp:43-162,
  * swSp::sourceRunoffWateronly -- :164-263 */
 static void source_runoff(orc_solver *s, double dt) {
@@ -767,6 +809,7 @@ int orc_step(orc_solver *s) {
             s->qy[n] = 0.;
         }
     }
+    if (c->momentum_diffusion) source_parab(s, dt);
     if (c->friction != ORC_FRIC_NONE) source_friction(s, dt);
     if (c->head_loss) source_head_loss(s, dt);
     if (c->runoff_model) source_runoff(s, dt);
@@ -849,6 +892,17 @@ orc_solver *orc_create(const orc_case *c, const double *h0, const double *qx0, c
         s->c.runoff_coeff = (nr && c->runoff_model != 1) ? xcopy(c->runoff_coeff, dc) : NULL;
         s->c.infil_rate = c->infil_model ? xcopy(c->infil_rate, dc) : NULL;
         s->rain_now = (double *)calloc(nr ? nr : 1, sizeof(double));
+        s->pfac = (double *)calloc((size_t)(ni ? ni : 1), sizeof(double));
+        s->pcoef = (double *)calloc((size_t)(ni ? ni : 1), sizeof(double));
+        s->pflux = (double *)calloc((size_t)(ni ? ni : 1), sizeof(double));
+        /* swSp::setParabParameters -- swSp.cpp:1236-1246 */
+        for (int64_t e = 0; e < ni; e++) {
+            const int64_t L = c->int_L[e], R = c->int_R[e];
+            const double dx = c->cell_x[R] - c->cell_x[L];
+            const double dy = c->cell_y[R] - c->cell_y[L];
+            const double d = fabs(dx * c->int_nx[e] + dy * c->int_ny[e]);
+            s->pfac[e] = c->int_len[e] / smax(d, SMALL_REAL);
+        }
     }
 
     s->h = xcopy(h0, dc);
@@ -930,7 +984,7 @@ void orc_destroy(orc_solver *s) {
         s->bc_val, s->bc_rank, s->bc_atend, (void *)s->c.wind_code, (void *)s->c.wind_t,
         (void *)s->c.wind_x, (void *)s->c.wind_y, s->wvx, s->wvy, (void *)s->c.rain_code,
         (void *)s->c.rain_t, (void *)s->c.rain_v, (void *)s->c.runoff_coeff,
-        (void *)s->c.infil_rate, s->rain_now};
+        (void *)s->c.infil_rate, s->rain_now, s->pfac, s->pcoef, s->pflux};
     for (size_t i = 0; i < sizeof(owned) / sizeof(owned[0]); i++) free(owned[i]);
     free(s);
 }

@@ -15,7 +15,7 @@
  *   src/sw2dSolver/models/common/swSourceFriction.cpp
  *   src/sw2dSolver/models/common/swBoundaryConditionTimeSeries.cpp
  *   src/sw2dSolver/models/sp/swSpSourceWind.cpp, swParent.cpp (wind series)
- *   src/sw2dSolver/models/sp/swSpSourceRunoffInfiltration.cpp
+ *   src/sw2dSolver/models/sp/swSpSourceRunoffInfiltration.cpp, swSpParab.cpp
  * All loops run in the reference's edge order with its scatter updates, so
  * the summation order is the reference's by construction.
  */
@@ -80,6 +80,7 @@ typedef struct orc_case {
     const double *runoff_coeff;      /* [n_cell], models 2 and 3 */
     int32_t infil_model;
     const double *infil_rate;        /* [n_cell] */
+    int32_t momentum_diffusion;      /* model_momentum_diffusion other than none */
 } orc_case;
 
 typedef struct orc_solver orc_solver;

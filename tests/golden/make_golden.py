@@ -95,6 +95,14 @@ def golden_cases():
         "rain_mixed_muscl_evr_full_water_only": ("mixed_case", dict(
             scheme="muscl_evr", friction="manning", t_max=20.0, dt_map=5.0, closed_fraction=0.0,
             rain="full_rainfall", rain_where="water_only", infil=True, wind=True)),
+        # momentum diffusion (swSpParab.cpp); every cell wet, since the
+        # reference divides q by h of both cells of each edge
+        "mdiff_dam_small_godunov": ("dambreak_case", dict(
+            nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0, scheme="godunov", friction="none",
+            t_max=3.0, dt_map=1.0, mdiff=True)),
+        "mdiff_dam_small_muscl_manning_wind": ("dambreak_case", dict(
+            nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0, scheme="muscl_evr", friction="manning",
+            t_max=3.0, dt_map=1.0, mdiff=True, wind=True)),
         # the reference stops on an impossible HLLC regime (swSpHyp.cpp:729-739)
         "mixed_muscl_closed_cells_error": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
                                                              t_max=20.0, closed_fraction=0.04)),

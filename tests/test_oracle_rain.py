@@ -74,7 +74,7 @@ def test_case_reader_rejects_what_the_reference_rejects(sw, tmp_path, edit, msg)
 
 def test_case_reader_tables(sw, tmp_path):
     c = sw.Case(_case(sw, str(tmp_path)))
-    np.testing.assert_array_equal(c["source_models"], [3, 0, 1])   # runoff_coefficient, everywhere, horton
+    np.testing.assert_array_equal(c["source_models"], [3, 0, 1, 0])   # runoff_coefficient, everywhere, horton, no diffusion
     assert c["rain_t"].shape == (4,) and c["rain_v"].shape == (8,)
     assert set(np.unique(c["rain_code"])) == {1, 2}
     assert np.all((c["runoff_coeff"] >= 0) & (c["runoff_coeff"] <= 1))
