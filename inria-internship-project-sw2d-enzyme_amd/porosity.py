@@ -433,3 +433,38 @@ def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=Non
         return minimize_adam(prob.grad, theta0, step=1e-2 if step is None else step,
                              max_iter=max_iter, project=project, **kw)
     return minimize_bfgs(prob.grad, theta0, max_iter=max_iter, **kw)
+
+
+def qsva_window_ends(t0, t_max, delta_tau):
+    """Window ends t0 + k delta_tau (k = 1, 2, ...) below t_max, then t_max."""
+    if not delta_tau > 0.0 or not t_max > t0:
+        raise ValueError("qsva: need delta_tau > 0 and t_max > t0")
+    ends, k = [], 1
+    while t0 + k * delta_tau < t_max:
+        ends.append(t0 + k * delta_tau)
+        k += 1
+    return ends + [t_max]
+
+
+def run_qsva(make_problem, t0, t_max, delta_tau, phi_start, method="lbfgsb", max_iter=100,
+             on_step=None, **kw):
+    """Quasi-static variational assimilation: the reference documents it
+    (Doc_assim_4Dvar/txt/Documentation_détaillé_assim.txt:1101-1160, swSp::run_qsva,
+    keys qsva_enabled / qsva_delta_tau) but does not implement it.  The window
+    start t0 stays fixed and its end grows by delta_tau up to t_max; each
+    window's minimisation (run_minimization, the same methods) starts from the
+    previous window's optimum.  make_problem(t_end, phi) returns the
+    PorosityProblem of the window [t0, t_end] (its observations in that window)
+    whose starting point is phi; phi_start starts the first window.  Returns
+    (phi of the full window, per-window records {t_end, J, n_eval, phi})."""
+    phi = np.asarray(phi_start, dtype=np.float64).copy()
+    steps = []
+    for t_end in qsva_window_ends(float(t0), float(t_max), float(delta_tau)):
+        prob = make_problem(t_end, phi)
+        theta, J, trace = run_minimization(prob, method, max_iter=max_iter, **kw)
+        phi = prob.m_of(theta)
+        rec = dict(t_end=t_end, J=float(J), n_eval=len(trace), phi=phi.copy())
+        steps.append(rec)
+        if on_step is not None:
+            on_step(rec)
+    return phi, steps

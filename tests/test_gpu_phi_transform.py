@@ -87,3 +87,34 @@ def test_gpu_run_minimization_methods_descend(sw, tmp_path, method):
     if method != "bfgs":
         assert np.all(th >= 0.0) and np.all(th <= 1.0)
     s.close()
+
+
+def test_gpu_qsva_continuation_descends_every_window(sw, tmp_path):
+    """QSVA (run_qsva) on the GPU porosity problem: windows [0, 10], [0, 20]
+    (delta_tau 10 s over the observations at 5, 10, 15, 20 s), each window's
+    minimisation starts from the previous optimum and lowers its own f, and
+    the final porosity stays in [0, 1] under the clamp change of variable."""
+    from sw2d_amd import assim
+    from sw2d_amd.porosity import PorosityProblem, run_qsva
+    g, case, s = _setup(sw, str(tmp_path))
+    cells, obs = _gauges(case)
+    t_obs = np.asarray(OBS_T)
+    print("[gpu] qsva", case.n_cell, "cells", flush=True)
+    starts = []
+
+    def make(t_end, phi):
+        k = int(np.sum(t_obs <= t_end + 1e-9))
+        A = assim.Assim(s, t_obs[:k], gauge_cells=cells, gauge_h=obs[:k],
+                        phi_transform=("clamp", 1.0))
+        P = PorosityProblem(A, phi, alpha=1e-3, phi_transform=("clamp", 1.0), zonation=True)
+        starts.append((P, P.grad(P.theta0())[0]))
+        return P
+
+    phi0 = np.clip(case.a["phi"], 0.05, 0.95)
+    phi, steps = run_qsva(make, 0.0, 20.0, 10.0, phi0, method="lbfgsb", max_iter=3)
+    assert [st["t_end"] for st in steps] == [10.0, 20.0]
+    for (P, f0), st in zip(starts, steps):
+        assert st["J"] < f0
+    np.testing.assert_array_equal(starts[1][0].phi_b, steps[0]["phi"])   # warm start
+    assert np.all((phi >= 0.0) & (phi <= 1.0))
+    s.close()

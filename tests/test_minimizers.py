@@ -118,3 +118,64 @@ def test_run_minimization_dispatch(sw):
         np.testing.assert_allclose(th, want, atol=3e-2 if m == "adam" else 1e-5)
     with pytest.raises(ValueError):
         run_minimization(Quad(), "nelder-mead")
+
+
+def test_qsva_window_ends():
+    from sw2d_amd.porosity import qsva_window_ends
+    assert qsva_window_ends(0.0, 10.0, 3.0) == [3.0, 6.0, 9.0, 10.0]
+    assert qsva_window_ends(5.0, 11.0, 2.0) == [7.0, 9.0, 11.0]
+    assert qsva_window_ends(0.0, 1.0, 5.0) == [1.0]
+    with pytest.raises(ValueError):
+        qsva_window_ends(0.0, 1.0, 0.0)
+
+
+def test_qsva_follows_the_moving_minimum_with_warm_starts():
+    """A cost whose minimum moves with the window end (phi* = t_end / 10 on
+    every cell) and whose far-away basin traps a cold start on the full
+    window: the continuation reaches the full window's absolute minimum from
+    warm starts, each window starting at the previous optimum."""
+    from sw2d_amd.porosity import run_qsva
+
+    class Prob:
+        mode = None
+
+        def __init__(self, t_end, phi):
+            self.target = t_end / 10.0
+            self.start = np.asarray(phi, dtype=np.float64).copy()
+            self.starts = []
+
+        def theta0(self):
+            self.starts.append(self.start.copy())
+            return self.start.copy()
+
+        def m_of(self, theta):
+            return np.asarray(theta, dtype=np.float64)
+
+        def func(self, x):
+            d = x - self.target
+            # a double well: a secondary minimum near 1 - target beyond a ridge
+            return float(np.sum(d ** 2 * (x - (1.0 - self.target)) ** 2 + 0.01 * d ** 2))
+
+        def grad(self, x):
+            d = x - self.target
+            e = x - (1.0 - self.target)
+            g = 2 * d * e ** 2 + 2 * d ** 2 * e + 0.02 * d
+            return self.func(x), g
+
+    made = []
+
+    def make(t_end, phi):
+        p = Prob(t_end, phi)
+        made.append(p)
+        return p
+
+    phi, steps = run_qsva(make, 0.0, 8.0, 1.0, np.full(3, 0.0), method="bfgs", max_iter=200)
+    assert [s["t_end"] for s in steps] == [1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0]
+    np.testing.assert_allclose(phi, 0.8, atol=1e-6)
+    for k in range(1, len(made)):
+        np.testing.assert_allclose(made[k].starts[0], steps[k - 1]["phi"])
+    # a cold start from 0 on the full window [0, 8] falls into the other well
+    cold = Prob(8.0, np.full(3, 0.0))
+    from sw2d_amd.porosity import minimize_bfgs
+    x, _, _ = minimize_bfgs(cold.grad, cold.theta0(), max_iter=200)
+    assert np.all(np.abs(x - 0.2) < 0.05) and np.all(np.abs(x - 0.8) > 0.5)
