@@ -11,7 +11,8 @@ test_gpu_forward.py (1e-12 against the reference's own maps).  Here:
   with every runoff model and both porosity-rainfall modes on the reduced
   scale workload: 40 steps within 1e-12 of the oracle at every step;
 - switching the sources off on a live context gives the plain solver;
-- the 4D-Var layer refuses a context with runoff or infiltration.
+- the 4D-Var layer refuses a context with runoff or infiltration;
+- bad source tables are refused by the C-ABI.
 """
 import glob
 import os
@@ -111,4 +112,34 @@ def test_gpu_assim_refuses_runoff(sw, tmp_path):
     s = sw.SpSolver(case)
     with pytest.raises(sw._native.SW2DError, match="runoff"):
         assim.Assim(s, np.array([10.0, 20.0]), gauge_cells=np.array([0]), gauge_h=np.zeros((2, 1)))
+    s.close()
+
+
+def test_gpu_source_tables_are_validated(sw, tmp_path):
+    """Station codes outside [1, n_station], decreasing record times,
+    coefficients outside [0, 1] and negative infiltration rates are refused
+    before anything reaches the device."""
+    folder, _ = sw.scale_case(str(tmp_path), px=10, py=10, L=100.0, t_max=60.0)
+    case = sw.Case(folder)
+    print("[gpu] source validation", case.n_cell, "cells", flush=True)
+    s = sw.SpSolver(case)
+    nc = case.n_cell
+    code = np.ones(nc, np.int32)
+    t = np.array([0.0, 10.0])
+    v = np.zeros((2, 2))
+    E = sw._native.SW2DError
+    bad_code = code.copy()
+    bad_code[3] = 3
+    with pytest.raises(E, match="station code"):
+        s.set_runoff(1, True, bad_code, t, v)
+    with pytest.raises(E, match="non-decreasing"):
+        s.set_runoff(1, True, code, t[::-1].copy(), v)
+    with pytest.raises(E, match="outside \\[0, 1\\]"):
+        s.set_runoff(2, False, code, t, v, np.full(nc, 1.5))
+    with pytest.raises(E, match="negative"):
+        s.set_infiltration(1, np.full(nc, -1e-4))
+    with pytest.raises(E, match="station code"):
+        s.set_wind(bad_code, t, v, v)
+    s.set_runoff(1, True, code, t, v)   # a valid table is taken
+    s.step(2)
     s.close()
