@@ -310,7 +310,8 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
     h0[phi <= 0.0] = 0.0  # closed cells hold no water
     qx0 = 0.05 * h0 * rng.standard_normal(nc)
     qy0 = 0.05 * h0 * rng.standard_normal(nc)
-    fric = 20.0 + 30.0 * rng.random(nc) if friction == "strickler" else 0.01 + 0.04 * rng.random(nc)
+    fric = (20.0 + 30.0 * rng.random(nc) if friction in ("strickler", "chezy")
+            else 0.01 + 0.04 * rng.random(nc))
     tq = np.array([0.0, 5.0, 12.0, 100.0])
     bc_v = np.array([
         [0.0, 0.0, 0.0, 0.0],                 # code 1 unused

@@ -63,6 +63,10 @@ def golden_cases():
                                                      closed_fraction=0.0 if s == "muscl_evr" else 0.04))
            for s in ("godunov", "muscl_evr") for t in ("discontinuous", "blending")
            for f in ("strickler", "manning")},
+        # Chezy friction (swSourceFriction.cpp, exp(-g |u| / C^2 / h dt))
+        **{f"mixed_{s}_discontinuous_chezy": ("mixed_case", dict(
+            scheme=s, friction="chezy", t_max=20.0, dt_map=5.0,
+            closed_fraction=0.0 if s == "muscl_evr" else 0.04)) for s in ("godunov", "muscl_evr")},
         # wind source, model_wind smithbanke (swSpSourceWind.cpp); the dam_small
         # ones have walls only and no friction: no exp/log on their path
         "wind_dam_small_godunov": ("dambreak_case", dict(nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0,
