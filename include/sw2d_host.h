@@ -75,6 +75,48 @@ int sw2dh_write_result(const sw2dh_case *c, const char *path, const double *h, c
 int sw2dh_result_name(double t, char *buf, int32_t buflen);
 void sw2dh_case_free(sw2dh_case *c);
 
+/* ---- box-constrained minimiser ----
+ * Replaces LBFGSpp::LBFGSBSolver<double>::minimize as the reference's
+ * run_minimization drives it (assim/swSpE_minimize.cpp:832-912, parameters
+ * :857-862, call :873-898): L-BFGS-B (generalized Cauchy point, subspace
+ * minimisation, More-Thuente line search).  The objective returns the value
+ * in *f and the gradient in grad[n] at x; its return code is 0 to go on, > 0
+ * to stop after this evaluation (the point is kept when it is the best), < 0
+ * for an error.  A non-finite *f marks x as unusable (the line search steps
+ * back).  lb / ub may be NULL (unbounded).  On return x holds the last
+ * accepted iterate (the reference keeps the last iterate when its library
+ * throws, :895-902); the status is 0 unless the arguments are invalid (1) or
+ * the objective fails or is not finite at the start (2). */
+typedef int (*sw2dh_objective)(const double *x, double *f, double *grad, void *user);
+typedef struct sw2dh_lbfgsb_param {
+    int32_t m;              /* correction pairs kept (lbfgsb_m)                      */
+    double epsilon;         /* projected-gradient max-norm tolerance (lbfgsb_epsilon) */
+    double epsilon_rel;     /* ... relative to ||x||_2                                */
+    int32_t past;           /* objective-change test over `past` iterations (0 off)   */
+    double delta;           /* ... relative tolerance                                 */
+    int32_t max_iterations; /* 0 = no limit (max_it)                                  */
+    int32_t max_linesearch; /* evaluations per line search (lbfgsb_max_linesearch)    */
+    double ftol, wolfe;     /* sufficient decrease and curvature constants            */
+    double max_step;        /* largest step along a direction                         */
+    int32_t first_step;     /* first direction (no correction pairs yet): 0 = unit
+                               length, LBFGSpp's rule; 1 = the full step to the
+                               Cauchy point, the 1997 Fortran code's rule for boxed
+                               problems                                            */
+} sw2dh_lbfgsb_param;
+#define SW2DH_LBFGSB_CONVERGED_GRAD 0
+#define SW2DH_LBFGSB_CONVERGED_F 1
+#define SW2DH_LBFGSB_MAX_ITER 2
+#define SW2DH_LBFGSB_LINESEARCH_FAILED 3
+#define SW2DH_LBFGSB_STOPPED 4
+/* Library defaults: m 6, epsilon 1e-5, epsilon_rel 1e-5, past 1, delta
+ * 1e-10, no iteration limit, 20 line-search evaluations, ftol 1e-4, wolfe
+ * 0.9, max_step 1e20, first_step 0. */
+void sw2dh_lbfgsb_default(sw2dh_lbfgsb_param *p);
+int sw2dh_lbfgsb(int64_t n, double *x, const double *lb, const double *ub, sw2dh_objective fg,
+                 void *user, const sw2dh_lbfgsb_param *param, double *fx, int32_t *n_iter,
+                 int32_t *n_eval, double *proj_grad_norm, int32_t *stop_reason);
+const char *sw2dh_lbfgsb_error(void);
+
 #ifdef __cplusplus
 }
 #endif

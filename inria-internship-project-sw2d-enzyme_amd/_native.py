@@ -89,8 +89,26 @@ def i32ptr(a):
 # ---------------------------------------------------------------------------
 # declarations (argtypes/restype) shared by both libraries
 # ---------------------------------------------------------------------------
+class LbfgsbParam(C.Structure):
+    """sw2dh_lbfgsb_param (include/sw2d_host.h)."""
+
+    _fields_ = [
+        ("m", C.c_int32), ("epsilon", C.c_double), ("epsilon_rel", C.c_double),
+        ("past", C.c_int32), ("delta", C.c_double), ("max_iterations", C.c_int32),
+        ("max_linesearch", C.c_int32), ("ftol", C.c_double), ("wolfe", C.c_double),
+        ("max_step", C.c_double), ("first_step", C.c_int32),
+    ]
+
+
+# int (*)(const double *x, double *f, double *grad, void *user)
+OBJECTIVE = C.CFUNCTYPE(C.c_int, _d, _d, _d, C.c_void_p)
+
 _HOST_DECL = {
     "sw2dh_error": ([], C.c_char_p),
+    "sw2dh_lbfgsb_default": ([C.POINTER(LbfgsbParam)], None),
+    "sw2dh_lbfgsb": ([C.c_int64, _d, _d, _d, OBJECTIVE, C.c_void_p, C.POINTER(LbfgsbParam), _d, _i32,
+                      _i32, _d, _i32], C.c_int),
+    "sw2dh_lbfgsb_error": ([], C.c_char_p),
     "sw2dh_geom_rect": ([C.c_int32, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
     "sw2dh_geom_delaunay": ([C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                              C.POINTER(C.c_void_p), _i64], C.c_int),

@@ -11,8 +11,9 @@ tanh / normalize).  As in the reference, the J that grad() returns leaves
 the spatial term out (swSpE_minimize.cpp:487) while its gradient includes
 it; func() includes it.
 
-``minimize`` is a projected L-BFGS (memory 6, at most 30 line-search
-probes, the reference's settings for its functional case) over box bounds.
+``minimize`` is L-BFGS-B (memory 6, at most 30 line-search evaluations, the
+reference's settings for its functional case) over box bounds, run by the
+native minimiser of the host library.
 """
 import time
 
@@ -132,105 +133,90 @@ class PorosityProblem:
         return J_data + p["ridge"] + p["l1"] + p["phi_ref"], g
 
 
-def minimize(fg, f, x0, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5, time_limit=None,
-             first_step="capped", expand=False, fresh_probes=False, cap_width=False):
-    """Projected L-BFGS on box bounds: fg(x) -> (value, gradient) anchors the
-    trajectory, f(x) evaluates line-search probes (Armijo backtracking from
-    step 1).  The first iterate's step: "capped" moves no control by more
-    than 0.1; "inv_norm" is 1 / ||d||_2, LBFGSpp's first step (the
-    reference's L-BFGS-B library).  expand: when the first trial is accepted,
-    the step keeps doubling while the cost still falls (within max_ls
-    probes).  cap_width (finite bounds): no trial moves a control farther
-    than the box's width.  fresh_probes: every probe is a gradient evaluation fg (a new
-    dry run at the probe, as LBFGSpp evaluates the reference's objective),
-    the accepted one's gradient reused; otherwise probes replay the last dry
-    run's schedule through f.  Stops when the projected gradient's max norm
-    falls below eps, or at max_iter / time_limit.
-    Returns (x, value, trace [(wall s, value, 'grad' | 'ls')])."""
-    x = np.asarray(x0, dtype=np.float64).copy()
-    lo = np.full_like(x, -np.inf) if lo is None else np.broadcast_to(np.asarray(lo, float), x.shape)
-    hi = np.full_like(x, np.inf) if hi is None else np.broadcast_to(np.asarray(hi, float), x.shape)
-    x = np.clip(x, lo, hi)
-    span = hi - lo
-    width = float(np.max(span)) if cap_width and np.all(np.isfinite(span)) and span.size else 0.0
+LBFGSB_STOP = ("converged (projected gradient)", "converged (objective change)",
+               "max iterations", "line search failed", "stopped by the caller")
+
+
+def lbfgsb(fg, x0, lo=None, hi=None, m=6, eps=1e-5, eps_rel=None, max_iter=0, max_ls=20, past=1,
+           delta=1e-10, ftol=1e-4, wolfe=0.9, first_step="unit", stop=None):
+    """L-BFGS-B over fg(x) -> (value, gradient): the native minimiser
+    (sw2dh_lbfgsb, include/sw2d_host.h), which stands in for the reference's
+    LBFGSpp::LBFGSBSolver (assim/swSpE_minimize.cpp:857-898; parameters m =
+    lbfgsb_m, epsilon = epsilon_rel = lbfgsb_epsilon, max_linesearch =
+    lbfgsb_max_linesearch, max_iterations = max_it).  Every evaluation is a
+    value and a gradient.  first_step "unit" (LBFGSpp) tries the first
+    direction at unit length, "cauchy" (the 1997 Fortran code on boxed
+    problems) goes to the first Cauchy point.  stop(x, value) -> True ends the run after that
+    evaluation (its point is kept when it is the best).  Returns (x, value,
+    info) with info = {n_iter, n_eval, pgnorm, reason, trace [(wall s, value,
+    'grad')]}."""
+    import ctypes as C
+
+    from . import _native as N
+    lib = N.host_lib()
+    x = np.ascontiguousarray(np.asarray(x0, dtype=np.float64).copy())
+    n = x.size
+    lo_a = None if lo is None else np.ascontiguousarray(np.broadcast_to(np.asarray(lo, float), x.shape))
+    hi_a = None if hi is None else np.ascontiguousarray(np.broadcast_to(np.asarray(hi, float), x.shape))
+    P = N.LbfgsbParam()
+    lib.sw2dh_lbfgsb_default(C.byref(P))
+    P.m, P.epsilon = int(m), float(eps)
+    P.epsilon_rel = float(eps if eps_rel is None else eps_rel)
+    P.past, P.delta = int(past), float(delta)
+    P.max_iterations, P.max_linesearch = int(max_iter), int(max_ls)
+    P.ftol, P.wolfe = float(ftol), float(wolfe)
+    if first_step not in ("unit", "cauchy"):
+        raise ValueError("first_step must be 'unit' or 'cauchy'")
+    P.first_step = 0 if first_step == "unit" else 1
     t0 = time.perf_counter()
-    trace = []
-    J, g = fg(x)
-    trace.append((time.perf_counter() - t0, J, "grad"))
-    S, Y = [], []
-    for _ in range(max_iter):
-        free = ~(((x <= lo) & (g > 0)) | ((x >= hi) & (g < 0)))
-        if np.max(np.abs(np.where(free, g, 0.0)), initial=0.0) <= eps:
-            break
+    trace, err = [], []
+
+    def cb(xp, fp, gp, _user):
+        try:
+            xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+            J, g = fg(xv)
+            J = float(J)
+            g = np.asarray(g, dtype=np.float64).reshape(n)
+            fp[0] = J
+            np.ctypeslib.as_array(gp, shape=(n,))[:] = g
+            trace.append((time.perf_counter() - t0, J, "grad"))
+            return 1 if (stop is not None and stop(xv, J)) else 0
+        except BaseException as e:  # noqa: BLE001 -- re-raised below
+            err.append(e)
+            return -1
+
+    cfun = N.OBJECTIVE(cb)
+    fx, pg = C.c_double(0.0), C.c_double(0.0)
+    it, ne, why = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    rc = lib.sw2dh_lbfgsb(n, N.dptr(x), N.dptr(lo_a), N.dptr(hi_a), cfun, None, C.byref(P),
+                          C.byref(fx), C.byref(it), C.byref(ne), C.byref(pg), C.byref(why))
+    if err:
+        raise err[0]
+    if rc != 0:
+        raise RuntimeError(lib.sw2dh_lbfgsb_error().decode())
+    return x, fx.value, dict(n_iter=it.value, n_eval=ne.value, pgnorm=pg.value,
+                             reason=LBFGSB_STOP[why.value], trace=trace)
+
+
+def minimize(fg, f=None, x0=None, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5,
+             time_limit=None, stop=None):
+    """run_minimization's "lbfgsb" branch over box bounds: L-BFGS-B (memory m,
+    at most max_ls line-search evaluations, projected-gradient tolerance eps
+    absolute and relative, the reference's settings for its functional case),
+    every evaluation a gradient fg(x) -> (value, gradient) as the reference's
+    objective adaptor (swSpE_minimize.cpp:850-856).  f is not used (kept for
+    callers that pass a cost-only function).  Stops also after time_limit
+    seconds or when stop(x, value) is true.  Returns (x, value, trace)."""
+    t0 = time.perf_counter()
+
+    def halt(xv, J):
         if time_limit and time.perf_counter() - t0 > time_limit:
-            break
-        q = np.where(free, g, 0.0)
-        alphas = []
-        for s_, y_ in reversed(list(zip(S, Y))):
-            a = np.dot(s_, q) / np.dot(y_, s_)
-            alphas.append(a)
-            q = q - a * y_
-        if S:
-            q = q * (np.dot(S[-1], Y[-1]) / np.dot(Y[-1], Y[-1]))
-        for (s_, y_), a in zip(zip(S, Y), reversed(alphas)):
-            q = q + s_ * (a - np.dot(y_, q) / np.dot(y_, s_))
-        d = -np.where(free, q, 0.0)
-        if np.dot(d, g) >= 0.0:
-            S, Y = [], []
-            d = -np.where(free, g, 0.0)
-        if S:
-            step = 1.0
-        elif first_step == "inv_norm":
-            step = 1.0 / max(float(np.linalg.norm(d)), 1e-300)
-        else:
-            step = min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
-        if width > 0.0:  # no control moves farther than the box is wide
-            step = min(step, width / max(float(np.max(np.abs(d))), 1e-300))
-        accepted = False
-        probes = 0
-        gn = None
+            return True
+        return bool(stop(xv, J)) if stop is not None else False
 
-        def probe(xp):
-            if fresh_probes:
-                Jp, gp = fg(xp)
-                trace.append((time.perf_counter() - t0, Jp, "grad"))
-                return Jp, gp
-            Jp = f(xp)
-            trace.append((time.perf_counter() - t0, Jp, "ls"))
-            return Jp, None
-
-        for _ in range(max_ls):
-            xn = np.clip(x + step * d, lo, hi)
-            Jn, gn = probe(xn)
-            probes += 1
-            if np.isfinite(Jn) and Jn <= J + 1e-4 * np.dot(g, xn - x):
-                accepted = True
-                break
-            step *= 0.5
-        if not accepted:
-            break
-        if expand and probes == 1:  # accepted at once: try longer steps while the cost falls
-            while probes < max_ls:
-                xt = np.clip(x + 2.0 * step * d, lo, hi)
-                if np.array_equal(xt, xn):
-                    break
-                Jt, gt = probe(xt)
-                probes += 1
-                if not (Jt < Jn and Jt <= J + 1e-4 * np.dot(g, xt - x)):
-                    break
-                step, xn, Jn, gn = 2.0 * step, xt, Jt, gt
-        if gn is None:
-            Jn, gn = fg(xn)
-            trace.append((time.perf_counter() - t0, Jn, "grad"))
-        sk, yk = xn - x, gn - g
-        if np.dot(sk, yk) > 1e-12 * np.dot(sk, sk):
-            S.append(sk)
-            Y.append(yk)
-            if len(S) > m:
-                S.pop(0)
-                Y.pop(0)
-        x, J, g = xn, Jn, gn
-    return x, J, trace
+    x, J, info = lbfgsb(fg, x0, lo=lo, hi=hi, m=m, eps=eps, max_iter=max_iter, max_ls=max_ls,
+                        stop=halt)
+    return x, J, info["trace"]
 
 
 def minimize_gd(fg, f, x0, project=True, step=1e-6, max_iter=100, c1=1e-4, shrink=0.5, grow=4.0,
@@ -396,8 +382,8 @@ METHODS = ("lbfgsb", "gd", "adam", "bfgs")
 
 def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=None, **kw):
     """The reference's run_minimization dispatch over PorosityProblem
-    (assim/swSpE_minimize.cpp:725-1017): "lbfgsb" (projected L-BFGS on [0, 1]
-    with memory 6 and <= 30 line-search probes), "gd" (projected GD +
+    (assim/swSpE_minimize.cpp:725-1017): "lbfgsb" (L-BFGS-B on [0, 1]
+    with memory 6 and <= 30 line-search evaluations), "gd" (projected GD +
     Armijo), "adam" (projected Adam), "bfgs" (dense BFGS, unconstrained).
     Projection onto [0, 1] applies, as in the reference, when the change of
     variable leaves m = phi (none / clamp).  Returns (theta, J, trace)."""
@@ -410,13 +396,8 @@ def run_minimization(prob, method="lbfgsb", max_iter=100, step=None, project=Non
         lo, hi = (0.0, 1.0) if project else (None, None)
         # a probe gradient at the start fixes where the adjoint truncates; the
         # window stays frozen there for the whole solve so that every
-        # line-search cost covers the same segments (swSpE_minimize.cpp:881-903)
-        # LBFGSpp's conventions (the reference's L-BFGS-B library): first step
-        # 1 / ||d||, every evaluation a gradient at a new dry run, steps that
-        # may grow past the first trial
-        for k, v in (("first_step", "inv_norm"), ("expand", True), ("fresh_probes", True),
-                     ("cap_width", True)):
-            kw.setdefault(k, v)
+        # line-search evaluation covers the same segments
+        # (swSpE_minimize.cpp:881-903)
         A = getattr(prob, "A", None)
         if A is None or not hasattr(A, "set_first_segment"):
             return minimize(prob.grad, prob.func, theta0, lo=lo, hi=hi, max_iter=max_iter, **kw)
