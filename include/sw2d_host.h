@@ -63,7 +63,10 @@ int sw2dh_case_sizes(const sw2dh_case *c, int64_t *n_cell, int64_t *n_bnd, int64
                      int32_t *n_series, int32_t *n_times);
 int sw2dh_case_params(const sw2dh_case *c, sw2dg_params *out);
 /* Geometry names as above plus: h0 qx0 qy0 phi beta fc1 fc2 falpha hsing hk1
- * hk2 halpha (empty when unused) bnd_series series_type series_t series_v */
+ * hk2 halpha (empty when unused) bnd_series series_type series_t series_v
+ * probe_cell probe_x probe_y (input/probe_locations.txt, read as
+ * swSolverParam::readProbes, swSolverParam.cpp:958-1022; each probe's cell
+ * located as swSp::init does, swSp.cpp:754-771) */
 const void *sw2dh_case_array(const sw2dh_case *c, const char *name, int64_t *n, int32_t *dtype);
 /* value of an input.solv key as text (native, assimilation or gpu_* keys), NULL if absent */
 const char *sw2dh_case_key(const sw2dh_case *c, const char *key);
@@ -74,6 +77,17 @@ int sw2dh_write_result(const sw2dh_case *c, const char *path, const double *h, c
 /* ResultHydro_p....txt file name for time t (swIO::computeFileNameFromTime) */
 int sw2dh_result_name(double t, char *buf, int32_t buflen);
 void sw2dh_case_free(sw2dh_case *c);
+/* name of probe i of the case, NULL out of range */
+const char *sw2dh_case_probe_name(const sw2dh_case *c, int32_t i);
+/* Cell containing each point (x[i], y[i]), -1 outside the mesh:
+ * swMesh::getCellFromCoordinates (src/common/geometry/swMesh.cpp:695-759),
+ * the first cell in file order whose vertex angle sum exceeds 6 in
+ * magnitude; any polygon (triangles, quads, mixed meshes).  Used for probes
+ * and for locate_gauges (assim/swSpE_gauges.cpp:41-77). */
+int sw2dh_geom_locate(const sw2dh_geom *g, int64_t n, const double *x, const double *y,
+                      int64_t *cell);
+int sw2dh_case_locate(const sw2dh_case *c, int64_t n, const double *x, const double *y,
+                      int64_t *cell);
 
 /* ---- box-constrained minimiser ----
  * Replaces LBFGSpp::LBFGSBSolver<double>::minimize as the reference's

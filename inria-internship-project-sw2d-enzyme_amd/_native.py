@@ -130,6 +130,9 @@ _HOST_DECL = {
     "sw2dh_write_result": ([C.c_void_p, C.c_char_p, _d, _d, _d, C.c_int32, C.c_char_p], C.c_int),
     "sw2dh_result_name": ([C.c_double, C.c_char_p, C.c_int32], C.c_int),
     "sw2dh_case_free": ([C.c_void_p], None),
+    "sw2dh_case_probe_name": ([C.c_void_p, C.c_int32], C.c_char_p),
+    "sw2dh_geom_locate": ([C.c_void_p, C.c_int64, _d, _d, _i64], C.c_int),
+    "sw2dh_case_locate": ([C.c_void_p, C.c_int64, _d, _d, _i64], C.c_int),
 }
 
 _GPU_DECL = {

@@ -52,27 +52,10 @@ def kmeans_zones(x, y, k, seed, L=L_DOMAIN):
 
 
 def locate_cells(geom, px, py):
-    """Cell containing each point (-1 if outside): sign test against the
-    three edges of the nearest triangles."""
-    nx_, ny_ = geom.array("node_x"), geom.array("node_y")
-    tri = geom.array("cell_tri").reshape(-1, 3)
-    cx, cy = nx_[tri].mean(axis=1), ny_[tri].mean(axis=1)
-    out = []
-    for x, y in zip(np.atleast_1d(px), np.atleast_1d(py)):
-        d2 = (cx - x) ** 2 + (cy - y) ** 2
-        cand = np.argpartition(d2, 16)[:16]
-        cand = cand[np.argsort(d2[cand])]
-        found = -1
-        for c in cand:
-            a, b, d = tri[c]
-            s = []
-            for (i, j) in ((a, b), (b, d), (d, a)):
-                s.append((nx_[j] - nx_[i]) * (y - ny_[i]) - (ny_[j] - ny_[i]) * (x - nx_[i]))
-            if (min(s) >= 0.0) or (max(s) <= 0.0):
-                found = int(c)
-                break
-        out.append(found)
-    return np.array(out, dtype=np.int64)
+    """Cell containing each point (-1 if outside): locate_gauges
+    (assim/swSpE_gauges.cpp:41-77) through the reference's winding-angle test
+    (swMesh::getCellFromCoordinates, swMesh.cpp:695-759), any polygon mesh."""
+    return geom.locate(px, py)
 
 
 def bed(bumps, x, y, L=L_DOMAIN):

@@ -79,6 +79,15 @@ class Geometry:
         codes = np.ascontiguousarray(codes, dtype=np.int32)
         N.host_check(self.lib.sw2dh_geom_set_bnd_codes(self._h, N.i32ptr(codes)))
 
+    def locate(self, x, y):
+        """Cell of each point (-1 outside): the reference's winding-angle test
+        (swMesh::getCellFromCoordinates, swMesh.cpp:695-759) on any polygon."""
+        x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+        y = np.ascontiguousarray(np.atleast_1d(y), dtype=np.float64)
+        out = np.empty(len(x), dtype=np.int64)
+        N.host_check(self.lib.sw2dh_geom_locate(self._h, len(x), N.dptr(x), N.dptr(y), N.i64ptr(out)))
+        return out
+
 
 def _copy(p, n, dt):
     ctype, np_t = {0: (C.c_double, np.float64), 1: (C.c_int64, np.int64),
@@ -405,7 +414,7 @@ class Case:
               "int_xm", "int_ym", "h0", "qx0", "qy0", "phi", "beta", "fc1", "fc2", "falpha",
               "hsing", "hk1", "hk2", "halpha", "series_type", "series_t", "series_v",
               "wind_code", "wind_t", "wind_x", "wind_y", "rain_code", "rain_t", "rain_v",
-              "runoff_coeff", "infil_rate", "source_models")
+              "runoff_coeff", "infil_rate", "source_models", "probe_cell", "probe_x", "probe_y")
 
     def __init__(self, folder):
         self.folder = str(folder)
@@ -429,6 +438,8 @@ class Case:
                 ptr = lib.sw2dh_case_array(h, name.encode(), C.byref(n), C.byref(dt))
                 self.a[name] = _copy(ptr, n.value, dt.value)
             self.n_decimal = lib.sw2dh_case_n_decimal(h)
+            self.probe_names = [lib.sw2dh_case_probe_name(h, i).decode()
+                                for i in range(len(self.a["probe_cell"]))]
             self.keys = {}
             for k in ("store_flag", "checkpoint_stride", "cost_normalize"):
                 v = lib.sw2dh_case_key(h, k.encode())
@@ -439,6 +450,20 @@ class Case:
 
     def __getitem__(self, k):
         return self.a[k]
+
+    def write_result(self, path, h, qx, qy, n_decimal=None, mode="default"):
+        """ResultHydro map of a state (reference order), as the solver writes
+        it (swIO::createTxtFileMapDecimal, swIO.cpp:182-223)."""
+        lib = N.host_lib()
+        hd = C.c_void_p()
+        N.host_check(lib.sw2dh_case_load(self.folder.encode(), C.byref(hd)))
+        try:
+            arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (h, qx, qy)]
+            nd = self.n_decimal if n_decimal is None else int(n_decimal)
+            N.host_check(lib.sw2dh_write_result(hd, str(path).encode(), *[N.dptr(a) for a in arr], nd,
+                                                mode.encode()))
+        finally:
+            lib.sw2dh_case_free(hd)
 
 
 def result_name(t):

@@ -277,6 +277,9 @@ const void *sw2dh_case_array(const sw2dh_case *h, const char *name, int64_t *n, 
     if (k == "runoff_coeff") return view(c.runoff_coeff, n, dt, 0);
     if (k == "infil_rate") return view(c.infil_rate, n, dt, 0);
     if (k == "source_models") return view(h->models, n, dt, 2);
+    if (k == "probe_cell") return view(c.probe_cell, n, dt, 1);
+    if (k == "probe_x") return view(c.probes.x, n, dt, 0);
+    if (k == "probe_y") return view(c.probes.y, n, dt, 0);
     return geom_array(c.g, c.bcell64, c.L64, c.R64, k, n, dt);
 }
 
@@ -311,5 +314,24 @@ int sw2dh_result_name(double t, char *buf, int32_t buflen) {
 }
 
 void sw2dh_case_free(sw2dh_case *c) { delete c; }
+
+const char *sw2dh_case_probe_name(const sw2dh_case *h, int32_t i) {
+    if (!h || i < 0 || (size_t)i >= h->c.probes.names.size()) return nullptr;
+    return h->c.probes.names[i].c_str();
+}
+
+int sw2dh_geom_locate(const sw2dh_geom *g, int64_t n, const double *x, const double *y,
+                      int64_t *cell) {
+    if (!g || n < 0 || (n > 0 && (!x || !y || !cell))) return fail("bad arguments");
+    for (int64_t i = 0; i < n; ++i) cell[i] = sw2d::locate_cell(g->g, x[i], y[i]);
+    return 0;
+}
+
+int sw2dh_case_locate(const sw2dh_case *h, int64_t n, const double *x, const double *y,
+                      int64_t *cell) {
+    if (!h || n < 0 || (n > 0 && (!x || !y || !cell))) return fail("bad arguments");
+    for (int64_t i = 0; i < n; ++i) cell[i] = sw2d::locate_cell(h->c.g, x[i], y[i]);
+    return 0;
+}
 
 }  // extern "C"

@@ -2,7 +2,8 @@
 // (src/sw2dSolver/main.cpp:18-188) for model_hyperb sp: reads the same case
 // folder, runs swSp::execute's time loop (swSp.cpp:482-670) on the GPU through
 // the C-ABI, and writes the same output/ResultHydro_p*.txt maps and
-// output/ResultBoundaryFluxes.txt at the same storage instants.
+// output/ResultBoundaryFluxes.txt and output/ProbesTimeSeries.txt at the same
+// storage instants.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -88,6 +89,17 @@ int main(int argc, char **argv) {
     fs::create_directories(fs::path(folder) / "log");
     const size_t nc = cs.g.n_cell(), nb = cs.g.n_bnd();
     std::vector<double> h(nc), qx(nc), qy(nc), f1(nb, 0.0);
+    std::ofstream prb;
+    const bool probes = !cs.probe_cell.empty();
+    if (probes) {
+        prb.open(out + "ProbesTimeSeries.txt");
+        prb << sw2d::probe_header(cs) << std::flush;
+    }
+    auto store_probes = [&](double t) {
+        if (!probes) return;
+        check(sw2dg_get_state(ctx, h.data(), qx.data(), qy.data()), "get_state");
+        prb << sw2d::probe_line(cs, t, h, qx, qy) << std::flush;
+    };
     std::ofstream bfl(out + "ResultBoundaryFluxes.txt");
     bfl << "time";
     for (int32_t code : cs.g.boundary_codes) bfl << "\t" << code;
@@ -114,6 +126,7 @@ int main(int argc, char **argv) {
     double t = prm.t0;
     store_maps(t);
     store_fluxes(t);
+    store_probes(t);
     int64_t total = 0;
     while (t < prm.t_max) {
         int64_t n = 0;
@@ -128,11 +141,13 @@ int main(int argc, char **argv) {
         if (flags & 2) {
             check(sw2dg_get_bnd_f1(ctx, f1.data()), "get_bnd_f1");
             store_fluxes(t);
+            store_probes(t);
         }
     }
     check(sw2dg_get_bnd_f1(ctx, f1.data()), "get_bnd_f1");
     store_maps(t);
     store_fluxes(t);
+    store_probes(t);
     const double wall =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0w).count();
     std::ofstream log((fs::path(folder) / "log" / "console_Solver.log").string());

@@ -951,6 +951,110 @@ std::vector<double> Case::series_values() const {
     return v;
 }
 
+bool read_probes(const std::string &path, Probes &p, std::string &err) {
+    p = Probes();
+    std::ifstream f(path);
+    if (!f) return true;  // no probe file: no probes
+    std::string line;
+    std::getline(f, line);  // header
+    while (std::getline(f, line)) {
+        line.erase(std::remove(line.begin(), line.end(), '\r'), line.end());
+        if (line.empty()) break;
+        // tokens between runs of white space; a leading run gives an empty first
+        // token, a trailing run none (std::sregex_token_iterator with "\\s+")
+        std::vector<std::string> tok;
+        size_t i = 0;
+        const auto ws = [](char ch) { return std::isspace((unsigned char)ch) != 0; };
+        if (ws(line[0])) tok.emplace_back();
+        while (i < line.size()) {
+            while (i < line.size() && ws(line[i])) ++i;
+            size_t j = i;
+            while (j < line.size() && !ws(line[j])) ++j;
+            if (j > i) tok.emplace_back(line.substr(i, j - i));
+            i = j;
+        }
+        if (tok.size() != 3) {
+            err = path + ": line is misformatted: " + line;
+            return false;
+        }
+        char *end = nullptr;
+        const double x = std::strtod(tok[0].c_str(), &end);
+        const bool okx = end != tok[0].c_str();
+        const double y = std::strtod(tok[1].c_str(), &end);
+        if (!okx || end == tok[1].c_str()) {
+            err = path + ": line is misformatted: format conversion: " + line;
+            return false;
+        }
+        p.names.push_back(tok[2]);
+        p.x.push_back(x);
+        p.y.push_back(y);
+    }
+    return true;
+}
+
+int64_t locate_cell(const Geometry &g, double x, double y) {
+    const double pi = 4.0 * std::atan(1.0);
+    for (size_t c = 0; c < g.n_cell(); ++c) {
+        const auto &nd = g.cell_nodes[c];
+        const size_t nn = nd.size();
+        if (nn == 0) continue;
+        // outside the vertex bounding box the angle sum is 0 up to rounding,
+        // so skipping such cells cannot change which cell is found first
+        double x0 = g.node_x[nd[0]], x1 = x0, y0 = g.node_y[nd[0]], y1 = y0;
+        for (size_t p = 1; p < nn; ++p) {
+            x0 = std::min(x0, g.node_x[nd[p]]), x1 = std::max(x1, g.node_x[nd[p]]);
+            y0 = std::min(y0, g.node_y[nd[p]]), y1 = std::max(y1, g.node_y[nd[p]]);
+        }
+        if (x < x0 || x > x1 || y < y0 || y > y1) continue;
+        double theta = 0.0;
+        for (size_t p = 0; p < nn; ++p) {
+            const double xa = g.node_x[nd[p]], ya = g.node_y[nd[p]];
+            const double xb = g.node_x[nd[(p + 1) % nn]], yb = g.node_y[nd[(p + 1) % nn]];
+            const double cr = (yb - y) * (xa - x) - (xb - x) * (ya - y);
+            const double dp = (xa - x) * (xb - x) + (ya - y) * (yb - y);
+            const double na = std::max(std::sqrt((xa - x) * (xa - x) + (ya - y) * (ya - y)), 1e-16);
+            const double nb = std::max(std::sqrt((xb - x) * (xb - x) + (yb - y) * (yb - y)), 1e-16);
+            const double s = cr / na / nb, cs = dp / na / nb;
+            double dth;
+            if (cs > 0.0)
+                dth = std::asin(std::min(s, 1.0));
+            else if (s < 0.0)
+                dth = -pi - std::asin(std::min(s, 1.0));
+            else
+                dth = pi - std::asin(std::min(s, 1.0));
+            theta = theta + dth;
+        }
+        if (std::fabs(theta) > 6.0) return (int64_t)c;
+    }
+    return -1;
+}
+
+namespace {
+std::string sci10(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%.10e", v);
+    return b;
+}
+}  // namespace
+
+std::string probe_header(const Case &c) {
+    std::string s;
+    for (size_t i = 0; i < c.probes.names.size(); ++i)
+        s += c.probes.names[i] + "\t" + sci10(c.probes.x[i]) + "\t" + sci10(c.probes.y[i]) + "\n";
+    s += "time\t";
+    for (const auto &n : c.probes.names) s += n + "_zb\t" + n + "_h\t" + n + "_q\t" + n + "_r\t";
+    return s + "\n";
+}
+
+std::string probe_line(const Case &c, double t, const std::vector<double> &h,
+                       const std::vector<double> &qx, const std::vector<double> &qy) {
+    std::string s = sci10(t) + "\t";
+    for (int64_t k : c.probe_cell)
+        s += sci10(c.g.cell_zb[k]) + "\t" + sci10(h[k]) + "\t" + sci10(qx[k]) + "\t" + sci10(qy[k]) +
+             "\t";
+    return s + "\n";
+}
+
 bool load_case(const std::string &folder, Case &c, std::string &err) {
     namespace fs = std::filesystem;
     c = Case();
@@ -1194,6 +1298,17 @@ bool load_case(const std::string &folder, Case &c, std::string &err) {
                 c.wind_x[s * nt + r] = -norm[s][r] * cos(a);
                 c.wind_y[s * nt + r] = -norm[s][r] * sin(a);
             }
+    }
+    // probes (swSolverParam::readProbes, then their cells in swSp::init, swSp.cpp:754-771)
+    if (!read_probes(in + "probe_locations.txt", c.probes, err)) return false;
+    for (size_t i = 0; i < c.probes.names.size(); ++i) {
+        const int64_t cell = locate_cell(c.g, c.probes.x[i], c.probes.y[i]);
+        if (cell < 0) {
+            err = "Error while looking for probe coordinates. Probe could be out of the mesh (" +
+                  c.probes.names[i] + ")";
+            return false;
+        }
+        c.probe_cell.push_back(cell);
     }
     c.bcell64.assign(c.g.bnd_cell.begin(), c.g.bnd_cell.end());
     c.L64.assign(c.g.int_L.begin(), c.g.int_L.end());

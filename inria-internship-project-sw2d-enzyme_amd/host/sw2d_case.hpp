@@ -107,6 +107,19 @@ struct SolverConfig {
 };
 bool read_solver_config(const std::string &path, SolverConfig &cfg, std::string &err);
 
+// input/probe_locations.txt (swSolverParam::readProbes, swSolverParam.cpp:
+// 958-1022; file name swFile.h:45): a header line, then "x y name" lines up to
+// the first empty line.  An absent file means no probes.
+struct Probes {
+    std::vector<std::string> names;
+    std::vector<double> x, y;
+};
+bool read_probes(const std::string &path, Probes &p, std::string &err);
+// Index of the first cell (file order) whose vertex polygon winds around
+// (x, y), -1 if none: swMesh::getCellFromCoordinates (swMesh.cpp:695-759),
+// the angle sum over the cell's vertices with |sum| > 6 meaning inside.
+int64_t locate_cell(const Geometry &g, double x, double y);
+
 // A loaded case folder (what swSp::init assembles, swSp.cpp:720-969).
 struct Case {
     std::string folder;
@@ -131,6 +144,10 @@ struct Case {
     int32_t momentum_diffusion = 0;  // model_momentum_diffusion other than none
     std::vector<int32_t> rain_code;
     std::vector<double> rain_t, rain_v, runoff_coeff, infil_rate;
+    // probes (swSp::init, swSp.cpp:754-783): names, coordinates and the cell
+    // of each (a probe outside the mesh stops the load, as the reference)
+    Probes probes;
+    std::vector<int64_t> probe_cell;
     sw2dg_params params() const;
     sw2dg_mesh mesh() const;                     // views into g (int64 copies kept here)
     std::vector<int64_t> bcell64, L64, R64;
@@ -138,6 +155,14 @@ struct Case {
     std::vector<double> series_values() const;   // column-major
 };
 bool load_case(const std::string &folder, Case &c, std::string &err);
+
+// output/ProbesTimeSeries.txt (swFile.h:100): header of swSp::init
+// (swSp.cpp:754-783) and one line per storage instant of
+// swSp::storeResultProbes (swSp.cpp:1611-1639): time, then zb, h, q, r of
+// each probe's cell, %.10e, tab separated (a tab after every value).
+std::string probe_header(const Case &c);
+std::string probe_line(const Case &c, double t, const std::vector<double> &h,
+                       const std::vector<double> &qx, const std::vector<double> &qy);
 
 // ResultHydro_pDDDDdHHhMMmSSsMMM.txt naming (swIO.cpp:51-86) and writer
 // (swIO::createTxtFileMapDecimal, swIO.cpp:182-223; columns of

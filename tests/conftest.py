@@ -98,3 +98,32 @@ def load_sp_toce(sw):
             setattr(p, k[2:], g[k].item())
     case = MemCase(arrays, p, int(g["n_series"]), int(g["n_times"]))
     return case, g
+
+
+def sp_toce_folder(sw, folder, probe_variant=False, frames=True):
+    """Rebuild the reference's functional case sp_toce in `folder`: input/
+    from tests/golden/sp_toce_input.npz (the case's files byte for byte) and
+    reference/ResultHydro_p*.txt observation frames from sp_toce.npz (the
+    reference's frames in the assimilation window, re-printed at their 11
+    significant digits, which reproduces the values exactly).  probe_variant:
+    the native 2 s probe run of make_sp_toce_case.py instead (no frames)."""
+    g = np.load(os.path.join(GOLDEN, "sp_toce_input.npz"), allow_pickle=False)
+    inp = os.path.join(folder, "input")
+    os.makedirs(inp, exist_ok=True)
+    for k in g.files:
+        if k.startswith("in_"):
+            with open(os.path.join(inp, k[3:]), "wb") as f:
+                f.write(g[k].tobytes())
+    if probe_variant:
+        from make_sp_toce_case import probe_variant as pv
+        pv(folder)
+        return folder
+    if frames:
+        t = dict(np.load(os.path.join(GOLDEN, "sp_toce.npz"), allow_pickle=False))
+        case = sw.Case(folder)
+        ref = os.path.join(folder, "reference")
+        os.makedirs(ref, exist_ok=True)
+        for tk, fr in zip(t["obs_t"], t["frames"]):
+            case.write_result(os.path.join(ref, sw.result_name(float(tk))), fr[0], fr[1], fr[2],
+                              n_decimal=10)
+    return folder
