@@ -264,6 +264,22 @@ int sw2dg_assim_set_adjoint_cut(sw2dg_ctx *ctx, double h_cut);
 int sw2dg_assim_profile(sw2dg_ctx *ctx, const double *phi, const double *K1, const double *K2,
                         int32_t k_max, double *avg_ms, int64_t *launches, const char **names,
                         int32_t *n_kernels);
+/* Largest L2 norm of the adjoint of h at a segment start over the reverse
+ * sweep of the last sw2dg_assim_gradient (0 before any): the dh_peak
+ * diagnostic of get_spatial_gradient_checkpointed (swSpE_checkpoint.cpp:
+ * 1044-1049, 1242), a column of the descent log. */
+int sw2dg_assim_adjoint_peak(sw2dg_ctx *ctx, double *dh_peak);
+/* Keep the full state (h, qx, qy) that each later dry run reaches at every
+ * observation time (on != 0 holds n_obs x 3 x n_cell doubles on the device):
+ * the maps the reference's _E forward dumps at each observation trigger
+ * (storeReferenceMap, assim/swSpE_core.cpp:300-341, called from
+ * build_checkpoint_records, swSpE_checkpoint.cpp:366-375) for gen-ref-e and
+ * the forward mode.  The next sw2dg_assim_record fills them. */
+int sw2dg_assim_keep_obs_states(sw2dg_ctx *ctx, int32_t on);
+/* The kept states of the last dry run, [n_obs * n_cell] each (reference
+ * order; any may be NULL), reached[n_obs] (may be NULL) 1 for the
+ * observations the run reached (others are 0-filled). */
+int sw2dg_assim_obs_states(sw2dg_ctx *ctx, double *h, double *qx, double *qy, int32_t *reached);
 /* State at the end of the recorded run (reference order). */
 int sw2dg_assim_final_state(sw2dg_ctx *ctx, double *h, double *qx, double *qy);
 

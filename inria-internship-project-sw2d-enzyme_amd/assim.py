@@ -40,6 +40,9 @@ _DECL = {
                              C.c_int),
     "sw2dg_assim_final_state": ([C.c_void_p, N._d, N._d, N._d], C.c_int),
     "sw2dg_assim_gauge_series": ([C.c_void_p, N._d], C.c_int),
+    "sw2dg_assim_keep_obs_states": ([C.c_void_p, C.c_int32], C.c_int),
+    "sw2dg_assim_adjoint_peak": ([C.c_void_p, N._d], C.c_int),
+    "sw2dg_assim_obs_states": ([C.c_void_p, N._d, N._d, N._d, N._i32], C.c_int),
     "sw2dg_assim_set_adjoint_cut": ([C.c_void_p, C.c_double], C.c_int),
     "sw2dg_assim_set_explosion_guard": ([C.c_void_p, C.c_double], C.c_int),
     "sw2dg_assim_set_phi_transform": ([C.c_void_p, C.c_int32, C.c_double, N._i64, C.c_int64,
@@ -220,6 +223,27 @@ class Assim:
         out = np.zeros(max(self.n_obs * ng, 1))
         self.s._check(self.lib.sw2dg_assim_gauge_series(self.s.ctx, N.dptr(out)))
         return out[: self.n_obs * ng].reshape(self.n_obs, ng)
+
+    def adjoint_peak(self):
+        """Largest ||adjoint of h|| at a segment start of the last gradient."""
+        v = C.c_double()
+        self.s._check(self.lib.sw2dg_assim_adjoint_peak(self.s.ctx, C.byref(v)))
+        return v.value
+
+    def keep_obs_states(self, on=True):
+        """Keep the full state of every observation step of later dry runs."""
+        self.s._check(self.lib.sw2dg_assim_keep_obs_states(self.s.ctx, 1 if on else 0))
+
+    def obs_states(self):
+        """(h, qx, qy) [n_obs, n_cell] at the observation steps of the last dry
+        run, and reached[n_obs] (False where the run did not get there)."""
+        nc = self.s.case.n_cell
+        h, qx, qy = (np.empty(self.n_obs * nc) for _ in range(3))
+        reached = np.zeros(self.n_obs, dtype=np.int32)
+        self.s._check(self.lib.sw2dg_assim_obs_states(self.s.ctx, N.dptr(h), N.dptr(qx), N.dptr(qy),
+                                                      N.i32ptr(reached)))
+        sh = (self.n_obs, nc)
+        return h.reshape(sh), qx.reshape(sh), qy.reshape(sh), reached.astype(bool)
 
     def final_state(self):
         nc = self.s.case.n_cell

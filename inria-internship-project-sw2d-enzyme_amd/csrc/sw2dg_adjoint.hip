@@ -59,6 +59,8 @@ struct AssimState {
     int *d_gcell = nullptr;
     double *d_gw = nullptr, *d_gobs = nullptr;
     double *d_gser = nullptr;  // [n_obs][n_gauges] depths sampled by the last dry run
+    double *d_oframes = nullptr;  // [n_obs][3 nc] states at the observation steps (opt-in)
+    bool keep_frames = false;
     int *d_gfirst = nullptr, *d_gnext = nullptr;  // per-cell gauge chains (gauge order)
     double *d_fh = nullptr, *d_fqx = nullptr, *d_fqy = nullptr, *d_wh = nullptr, *d_wq = nullptr;
     // schedule
@@ -107,6 +109,7 @@ struct AssimState {
     int64_t forced_seg = -1;    // frozen truncation window (forced_first_seg_), -1 = dynamic
     double *d_wm = nullptr;     // 3 nc: gphi | gK1 | gK2 at the watermark
     double *d_nrm = nullptr;    // block partials of ||lam_h||^2, then the sum
+    double dh_peak = 0.0;       // largest ||lam_h|| at a segment start in the last gradient
     // controls (device order)
     double *d_phi = nullptr, *d_K1 = nullptr, *d_K2 = nullptr;
     // series -> boundary edges (renumbered) in reference order
@@ -712,6 +715,18 @@ __global__ void k_gauge_sample(const double *__restrict__ U, const int *gcell, i
     for (int g = threadIdx.x; g < n_g; g += blockDim.x) gser[(size_t)oi * n_g + g] = U[gcell[g]];
 }
 
+// full state after a step that reaches an observation time (the frame the
+// reference's storeReferenceMap writes at that trigger)
+__global__ void k_frame_sample(const double *__restrict__ U, int64_t nc, const int *trig, int64_t n,
+                               double *frames) {
+    const int oi = trig[n];
+    if (oi < 0) return;
+    double *dst = frames + (size_t)oi * 3 * nc;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 3 * nc;
+         q += (int64_t)gridDim.x * blockDim.x)
+        dst[q] = U[q];
+}
+
 // ---------------------------------------------------------------------------
 // reverse kernels
 // ---------------------------------------------------------------------------
@@ -1208,6 +1223,9 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
     if (dry && a->n_gauges > 0)
         KLAUNCH(a, k_gauge_sample, dim3(1), dim3(64), 0, c->stream, Uout, a->d_gcell,
                            a->n_gauges, a->d_trig, n, a->d_gser);
+    if (dry && a->d_oframes)
+        KLAUNCH(a, k_frame_sample, dim3(nblocks(3 * c->nc)), blk, 0, c->stream, Uout, c->nc, a->d_trig,
+                n, a->d_oframes);
 }
 
 void launch_cost(sw2dg_ctx *c, AssimState *a, const double *U, int oi) {
@@ -1732,6 +1750,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         return SW2DG_OK;
     };
     bool started = false;
+    a->dh_peak = 0.0;
     for (int64_t k = p.K - 1; k >= p.first; --k) {
         if (guard && tiled && p.tile_top[k]) {  // clean state before a new tile
             if ((st = snapshot((k + 1) * a->stride))) return st;
@@ -1763,13 +1782,16 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         // observation of that state belongs to segment k - 1), the porosity
         // terms of its first step and the boundary-value sums of step n0
         gc(false, 1, n0, seed_args(c, a, -1, nullptr), nullptr, 0.0);
-        if (!guard) continue;
+        // ||adjoint of h|| at the segment start: its peak over the sweep is a
+        // diagnostic of the reference's descent log (dh_peak, :1044-1049)
         KLAUNCH(a, k_sq_part, dim3(nbk), blk, 0, c->stream, a->d_lam, (int)nc, a->d_nrm);
         KLAUNCH(a, k_sq_sum, dim3(1), blk, 0, c->stream, a->d_nrm, nbk, a->d_nrm + nbk);
         double nrm2 = 0.0;
         HIPCHK(c, hipMemcpyAsync(&nrm2, a->d_nrm + nbk, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const double nrm = std::sqrt(nrm2);
+        if (nrm > a->dh_peak) a->dh_peak = nrm;
+        if (!guard) continue;
         if (ref < 0.0 && nrm > 0.0) ref = nrm;
         if (!tiled) {
             bool has_obs = false;
@@ -1882,6 +1904,54 @@ int sw2dg_assim_gauge_series(sw2dg_ctx *c, double *out) {
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpy(out, a->d_gser, (size_t)a->n_obs * a->n_gauges * sizeof(double),
                         hipMemcpyDeviceToHost));
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_adjoint_peak(sw2dg_ctx *c, double *dh_peak) {
+    if (!c || !c->assim || !dh_peak) return SW2DG_ERR_ARG;
+    *dh_peak = c->assim->dh_peak;
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_keep_obs_states(sw2dg_ctx *c, int32_t on) {
+    if (!c || !c->assim) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (on && !a->d_oframes && a->n_obs > 0) {
+        a->d_oframes = a->alloc<double>((size_t)a->n_obs * 3 * c->nc);
+        if (!a->d_oframes) return fail(c, SW2DG_ERR_HIP, "keep_obs_states: out of device memory");
+    }
+    if (!on) a->d_oframes = nullptr;  // the buffer stays with the ctx until it is destroyed
+    a->recorded = a->recorded && !on;  // the next dry run fills the frames
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_obs_states(sw2dg_ctx *c, double *h, double *qx, double *qy, int32_t *reached) {
+    if (!c || !c->assim || !c->assim->recorded || !c->assim->d_oframes) return SW2DG_ERR_ARG;
+    AssimState *a = c->assim;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nc = c->nc;
+    std::vector<int> trig(std::max<int64_t>(a->n_steps, 1));
+    if (a->n_steps > 0)
+        HIPCHK(c, hipMemcpy(trig.data(), a->d_trig, a->n_steps * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int32_t> hit(a->n_obs, 0);
+    for (int64_t n = 0; n < a->n_steps; ++n)
+        if (trig[n] >= 0 && trig[n] < a->n_obs) hit[trig[n]] = 1;
+    std::vector<double> U(3 * nc);
+    double *dst[3] = {h, qx, qy};
+    for (int k = 0; k < a->n_obs; ++k) {
+        if (reached) reached[k] = hit[k];
+        if (!hit[k]) {
+            for (int f = 0; f < 3; ++f)
+                if (dst[f]) std::fill(dst[f] + (size_t)k * nc, dst[f] + (size_t)(k + 1) * nc, 0.0);
+            continue;
+        }
+        HIPCHK(c, hipMemcpy(U.data(), a->d_oframes + (size_t)k * 3 * nc, 3 * nc * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        for (int f = 0; f < 3; ++f)
+            if (dst[f])
+                for (int64_t q = 0; q < nc; ++q) dst[f][(size_t)k * nc + c->perm[q]] = U[f * nc + q];
+    }
     return SW2DG_OK;
 }
 
