@@ -343,6 +343,12 @@ int sw2dg_mlp_train_step(sw2dg_mlp *mlp, int32_t batch, const float *x, const fl
 const char *sw2dg_mlp_last_error(sw2dg_mlp *mlp);
 void sw2dg_mlp_destroy(sw2dg_mlp *mlp);
 
+/* The device's exp (fn 0), log (1) or cbrt (2) of n values: the functions
+ * every kernel uses (exp and cbrt equal to glibc's, which the reference runs
+ * on; log correctly rounded).  No reference counterpart: it lets a host build
+ * of the same code check the device bit for bit.  device < 0 = current. */
+int sw2dg_math_eval(int32_t fn, int64_t n, const double *x, double *y, int32_t device);
+
 /* Bytes of device memory held by the ctx. */
 int64_t sw2dg_device_bytes(sw2dg_ctx *ctx);
 const char *sw2dg_last_error(sw2dg_ctx *ctx);

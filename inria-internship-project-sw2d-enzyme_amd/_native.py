@@ -137,6 +137,7 @@ _HOST_DECL = {
 
 _GPU_DECL = {
     "sw2dg_version": ([], C.c_char_p),
+    "sw2dg_math_eval": ([C.c_int32, C.c_int64, _d, _d, C.c_int32], C.c_int),
     "sw2dg_ctx_create": ([C.POINTER(Mesh), C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)], C.c_int),
     "sw2dg_set_cell_params": ([C.c_void_p, _d, _d, _d, _d, _d, _d, _d, _d, _d], C.c_int),
     "sw2dg_set_bc_series": ([C.c_void_p, C.c_int32, C.c_int32, _i32, _d, _d], C.c_int),

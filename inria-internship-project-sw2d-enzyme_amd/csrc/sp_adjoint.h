@@ -690,7 +690,7 @@ SPD void bnd_fwd(double h_min, double phi_min, double bR, double phiR, double zb
         f1 = bc;
         if (bc > kSmall) {
             const double ps = (phiR > phi_min) ? phiR : phi_min;
-            const double hc = exp((1.0 / 3.0) * log((bc / ps) / kG));
+            const double hc = sw_exp((1.0 / 3.0) * sw_log((bc / ps) / kG));
             if (hc >= hRe) {
                 const double hcs = (hc > kSmall) ? hc : kSmall;
                 f1 = phiR * 1.5 * sqrt(kG * hcs) * hc;
@@ -766,7 +766,7 @@ SPD void bnd_vjp(double h_min, double phi_min, double bR, double phiR, double zb
         if (bc > kSmall) {
             const double ps = (phiR > phi_min) ? phiR : phi_min;
             const double x = (bc / ps) / kG;
-            const double hc = exp((1.0 / 3.0) * log(x));
+            const double hc = sw_exp((1.0 / 3.0) * sw_log(x));
             if (hc >= hRe) {
                 const double hcs = (hc > kSmall) ? hc : kSmall;
                 const double sg = sqrt(kG * hcs);
@@ -855,7 +855,7 @@ SPD void fric_fwd(double h_min, double h, double &qx, double &qy, double ca, dou
         const double un = sqrt(u * u + v * v + 1e-16);
         const double K1 = (K1in > 1e-10) ? K1in : 20.0;
         const double K2 = (K2in > 1e-10) ? K2in : 20.0;
-        const double h43 = h * cbrt(h + 1e-16);
+        const double h43 = h * sw_cbrt(h + 1e-16);
         const double d1 = 1.0 + (dt * kG * un) / (K1 * K1 * h43 + 1e-10);
         const double d2 = 1.0 + (dt * kG * un) / (K2 * K2 * h43 + 1e-10);
         ul /= d1;
@@ -887,7 +887,7 @@ SPD void fric_vjp(double h_min, double h, double qx, double qy, double ca, doubl
     const double un = sqrt(u * u + v * v + 1e-16);
     const double K1 = (K1in > 1e-10) ? K1in : 20.0;
     const double K2 = (K2in > 1e-10) ? K2in : 20.0;
-    const double cb = cbrt(h + 1e-16);
+    const double cb = sw_cbrt(h + 1e-16);
     const double h43 = h * cb;
     const double D1 = K1 * K1 * h43 + 1e-10, D2 = K2 * K2 * h43 + 1e-10;
     const double iD1 = 1.0 / D1, iD2 = 1.0 / D2;
@@ -933,8 +933,8 @@ SPD void hl_fwd(double h_min, double h, double &qx, double &qy, double ca, doubl
             Ax = qn * k1 / hs;
             Ay = qn * k2 / hs;
         }
-        ax = ax * exp(-Ax * dt);
-        ay = ay * exp(-Ay * dt);
+        ax = ax * sw_exp(-Ax * dt);
+        ay = ay * sw_exp(-Ay * dt);
     } else {
         ax = 0.0;
         ay = 0.0;
@@ -959,7 +959,7 @@ SPD void hl_vjp(double h_min, double h, double qx, double qy, double ca, double 
             Ax = qn * k1 / hs;
             Ay = qn * k2 / hs;
         }
-        const double ex = exp(-Ax * dt), ey = exp(-Ay * dt);
+        const double ex = sw_exp(-Ax * dt), ey = sw_exp(-Ay * dt);
         dax = dbx * ex;
         day = dby * ey;
         const double dAx = -dbx * ax * ex * dt, dAy = -dby * ay * ey * dt;

@@ -3,7 +3,9 @@
 // expression keeps the reference's left-to-right evaluation order and the
 // library is compiled with -ffp-contract=off, so results are bit-identical
 // to the reference CPU solver wherever only +,-,*,/ and sqrt are involved
-// (sqrt and division are correctly rounded in IEEE FP64 on both sides).
+// (sqrt and division are correctly rounded in IEEE FP64 on both sides); exp,
+// log and cbrt are the project's own (sw_math.h), equal to glibc's for exp and
+// cbrt and correctly rounded for log.
 #pragma once
 
 #ifdef __HIPCC__
@@ -13,6 +15,8 @@
 #endif
 
 #include <math.h>
+
+#include "sw_math.h"
 
 namespace sp {
 
@@ -79,7 +83,7 @@ SPD int bnd_flux(double h_min, double bR, double phiR, double zbR, double zR, do
         }
     } else if (bc_type == BC_Q) {
         flux1 = bc_val;
-        double hc = exp((1. / 3.) * log((bc_val / smax(phiR, kSmall)) / kG));
+        double hc = sw_exp((1. / 3.) * sw_log((bc_val / smax(phiR, kSmall)) / kG));
         if (hc >= hR) flux1 = phiR * 1.5 * sqrt(kG * hc) * hc;
         flux2 = flux2R + (flux1 - q1 * phiR) * L1;
     } else {
@@ -183,22 +187,22 @@ SPD double friction_factor(int model, double coeff, double h, double u_norm, dou
     double factor = 1.0;
     if (model == FRIC_MANNING) {
         if (h > kSmall) {
-            double arg = -kG * u_norm * coeff * coeff * exp(-4.0 / 3.0 * log(h));
-            factor = exp(arg * dt);
+            double arg = -kG * u_norm * coeff * coeff * sw_exp(-4.0 / 3.0 * sw_log(h));
+            factor = sw_exp(arg * dt);
         } else {
             factor = 0.0;
         }
     } else if (model == FRIC_STRICKLER) {
         if ((h > kSmall) && (coeff > kSmall)) {
-            double arg = -kG * u_norm / coeff / coeff * exp(-4.0 / 3.0 * log(h));
-            factor = exp(arg * dt);
+            double arg = -kG * u_norm / coeff / coeff * sw_exp(-4.0 / 3.0 * sw_log(h));
+            factor = sw_exp(arg * dt);
         } else {
             factor = 0.0;
         }
     } else if (model == FRIC_CHEZY) {
         if ((h > kSmall) && (coeff > kSmall)) {
             double arg = -kG * u_norm / coeff / coeff / h;
-            factor = exp(arg * dt);
+            factor = sw_exp(arg * dt);
         } else {
             factor = 0.0;
         }

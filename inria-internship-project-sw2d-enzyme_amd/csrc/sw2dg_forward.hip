@@ -621,8 +621,8 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
                 aax = sqrt(ax * ax + ay * ay) * m.hk1[i] / h;
                 aay = sqrt(ax * ax + ay * ay) * m.hk2[i] / h;
             }
-            ax = ax * exp(-aax * dt);
-            ay = ay * exp(-aay * dt);
+            ax = ax * sw_exp(-aax * dt);
+            ay = ay * sw_exp(-aay * dt);
         } else {
             ax = 0.;
             ay = 0.;

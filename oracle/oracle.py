@@ -11,6 +11,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
+# the same restatement built with the kernels' exp / log / cbrt (orc_math.h)
+LIB_DM = os.path.join(HERE, "_build", "liboracle_dm.so")
+MATHS = ("glibc", "device")
 REF_SOLVER = os.path.join(HERE, "_ref", "sw2dSolver")
 
 _d = C.POINTER(C.c_double)
@@ -44,15 +47,19 @@ class OrcCase(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(math="glibc"):
+    """The oracle library: math "glibc" (the reference's exp / log / cbrt) or
+    "device" (the kernels' own, every call audited against glibc)."""
+    if math not in MATHS:
+        raise ValueError(f"math must be one of {MATHS}")
+    if math not in _libs:
+        path = LIB if math == "glibc" else LIB_DM
+        if not os.path.exists(path):
             subprocess.check_call(["make", "-C", HERE], stdout=subprocess.DEVNULL)
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         L.orc_create.argtypes = [C.POINTER(OrcCase), _d, _d, _d]
         L.orc_create.restype = C.c_void_p
         L.orc_destroy.argtypes = [C.c_void_p]
@@ -73,8 +80,38 @@ def lib():
         L.orc_interp_series.restype = C.c_double
         L.orc_friction_factor.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_friction_factor.restype = C.c_double
-        _lib = L
-    return _lib
+        L.orc_math_audit_reset.argtypes = []
+        L.orc_math_audit_reset.restype = None
+        L.orc_math_audit.argtypes = [_i64, _i32, _d, _d, _d, _i64]
+        L.orc_math_audit.restype = C.c_int
+        L.orc_math_eval.argtypes = [C.c_int32, C.c_int64, _d, _d]
+        L.orc_math_eval.restype = None
+        _libs[math] = L
+    return _libs[math]
+
+
+def math_eval(fn, x, math="glibc"):
+    """The oracle's exp / log / cbrt (fn "exp" | "log" | "cbrt") of an array."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib(math).orc_math_eval({"exp": 0, "log": 1, "cbrt": 2}[fn], len(x), _dp(x), _dp(y))
+    return y
+
+
+def math_audit_reset():
+    lib("device").orc_math_audit_reset()
+
+
+def math_audit():
+    """First transcendental call of the device-math oracle whose result
+    differs from glibc's since the last reset: dict(step, fn, x, glibc, device,
+    count), step -1 / count 0 when none did."""
+    st, fn, cnt = C.c_int64(), C.c_int32(), C.c_int64()
+    x, g, d = C.c_double(), C.c_double(), C.c_double()
+    lib("device").orc_math_audit(C.byref(st), C.byref(fn), C.byref(x), C.byref(g), C.byref(d),
+                                 C.byref(cnt))
+    return dict(step=st.value, fn=("exp", "log", "cbrt")[fn.value] if fn.value >= 0 else None,
+                x=x.value, glibc=g.value, device=d.value, count=cnt.value)
 
 
 def _dp(a):
@@ -115,8 +152,8 @@ def build_case(case, params=None, keep=None):
     return c, keep
 
 
-def _e_decl():
-    L = lib()
+def _e_decl(math="glibc"):
+    L = lib(math)
     if getattr(L, "_e_ready", False):
         return L
     L.orc_e_record.argtypes = [C.POINTER(OrcCase), _d, _d, _d, _d, _d, _d, C.c_int32, _d, C.c_int64,
@@ -157,8 +194,9 @@ def _e_decl():
 class OracleE:
     """CPU restatement of the reference's _E forward: dry run and frozen replay."""
 
-    def __init__(self, case):
+    def __init__(self, case, math="glibc"):
         self.case = case
+        self.math = math
         self.c, self._keep = build_case(case)
         self.nc = case.n_cell
         self.ns = max(case.n_series, 1)
@@ -180,7 +218,7 @@ class OracleE:
         trig = np.empty(max_steps, dtype=np.int32)
         h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
         S = np.empty((max_steps + 1) * 3 * self.nc) if states else None
-        n = _e_decl().orc_e_record_states(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
+        n = _e_decl(self.math).orc_e_record_states(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
                                           _dp(qx0), _dp(qy0), len(obs_t), _dp(obs_t), max_steps,
                                           _dp(dt), _dp(bc), trig.ctypes.data_as(_i32),
                                           _dp(S) if states else None)
@@ -205,7 +243,7 @@ class OracleE:
         ob = np.ascontiguousarray(obs if obs is not None else np.zeros(1), dtype=np.float64).reshape(-1)
         nob = int(n_obs if n_obs is not None else (trig.max() + 1 if len(trig) else 0))
         H, QX, QY = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
-        J = _e_decl().orc_e_replay(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+        J = _e_decl(self.math).orc_e_replay(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
                                    _dp(qy0), len(dt), _dp(dt), _dp(bcs), trig.ctypes.data_as(_i32),
                                    nob, len(gc), gc.ctypes.data_as(_i64), _dp(gw), _dp(ob),
                                    1 if normalize else 0, _dp(H), _dp(QX), _dp(QY))
@@ -237,7 +275,7 @@ def _oracle_e_gradient(self, rec, phi=None, K1=None, K2=None, gauges=None, obs=N
     nob = int(n_obs if n_obs is not None else (trig.max() + 1 if len(trig) else 0))
     gphi, gK1, gK2 = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
     gbc = np.empty(max(n, 1) * self.ns)
-    L = _e_decl()
+    L = _e_decl(self.math)
     L.orc_e_set_adjoint_cut(float(adj_cut))
     if explosion is not None:
         L.orc_e_set_explosion_guard(float(explosion[0]), int(explosion[1]))
@@ -274,7 +312,7 @@ def _oracle_e_dry_states(self, rec, phi=None, K1=None, K2=None, bc=None):
     bcs = np.ascontiguousarray(rec["bc"] if bc is None else bc, dtype=np.float64).reshape(-1)
     h0, qx0, qy0 = (np.ascontiguousarray(a[k]) for k in ("h0", "qx0", "qy0"))
     S = np.empty((len(dt) + 1) * 3 * self.nc)
-    _e_decl().orc_e_dry_states(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
+    _e_decl(self.math).orc_e_dry_states(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0), _dp(qx0),
                                _dp(qy0), len(dt), _dp(dt), _dp(bcs), _dp(S))
     return S.reshape(len(dt) + 1, 3, self.nc)
 
@@ -302,7 +340,7 @@ def _oracle_e_frame_cost(self, rec, frames, phi=None, K1=None, K2=None, w_h=None
     wh = None if w_h is None else _dp(self._ctl(w_h, None))
     wq = None if w_q is None else _dp(self._ctl(w_q, None))
     nob = int(n_obs if n_obs is not None else fr.shape[0])
-    return _e_decl().orc_e_replay_frames(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
+    return _e_decl(self.math).orc_e_replay_frames(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
                                          _dp(qx0), _dp(qy0), len(dt), _dp(dt), _dp(bcs),
                                          trig.ctypes.data_as(_i32), nob, _dp(fr), wh, wq,
                                          1 if normalize else 0)
@@ -331,7 +369,7 @@ def _oracle_e_frame_gradient(self, rec, frames, phi=None, K1=None, K2=None, norm
     gphi, gK1, gK2 = np.empty(self.nc), np.empty(self.nc), np.empty(self.nc)
     gbc = np.empty(max(n, 1) * self.ns)
     nob = int(n_obs if n_obs is not None else fr.shape[0])
-    J = _e_decl().orc_e_gradient_frames(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
+    J = _e_decl(self.math).orc_e_gradient_frames(C.byref(self.c), _dp(phi), _dp(K1), _dp(K2), _dp(h0),
                                         _dp(qx0), _dp(qy0), n, _dp(dt), _dp(bcs),
                                         trig.ctypes.data_as(_i32), nob, _dp(fr), None, None,
                                         1 if normalize else 0, _dp(gphi), _dp(gK1), _dp(gK2),
@@ -345,9 +383,10 @@ OracleE.frame_gradient = _oracle_e_frame_gradient
 class OracleSolver:
     """CPU restatement of swSp's time loop on one case (arrays in reference order)."""
 
-    def __init__(self, case, params=None):
+    def __init__(self, case, params=None, math="glibc"):
         a = case.a
         p = params if params is not None else case.params
+        self.math = math
         self.n_cell = case.n_cell
         self._keep = []
 
@@ -395,25 +434,25 @@ class OracleSolver:
         if c.infil_model:
             c.infil_rate = _dp(k(a["infil_rate"]))
         h0, qx0, qy0 = k(a["h0"]), k(a["qx0"]), k(a["qy0"])
-        self._s = lib().orc_create(C.byref(c), _dp(h0), _dp(qx0), _dp(qy0))
+        self._s = lib(self.math).orc_create(C.byref(c), _dp(h0), _dp(qx0), _dp(qy0))
         if not self._s:
             raise RuntimeError("orc_create failed")
 
     def __del__(self):
         if getattr(self, "_s", None):
-            lib().orc_destroy(self._s)
+            lib(self.math).orc_destroy(self._s)
             self._s = None
 
     def step(self, n=1):
         done = C.c_int64()
-        st = lib().orc_run(self._s, 1e300, int(n), C.byref(done))
+        st = lib(self.math).orc_run(self._s, 1e300, int(n), C.byref(done))
         if st:
             raise RuntimeError(f"oracle error {st}")
         return done.value
 
     def run(self, t_end=1e300, max_steps=2 ** 62):
         done = C.c_int64()
-        st = lib().orc_run(self._s, float(t_end), int(max_steps), C.byref(done))
+        st = lib(self.math).orc_run(self._s, float(t_end), int(max_steps), C.byref(done))
         if st:
             raise RuntimeError(f"oracle error {st}")
         return done.value
@@ -421,16 +460,16 @@ class OracleSolver:
     def state(self):
         n = self.n_cell
         h, qx, qy = np.empty(n), np.empty(n), np.empty(n)
-        lib().orc_get_state(self._s, _dp(h), _dp(qx), _dp(qy))
+        lib(self.math).orc_get_state(self._s, _dp(h), _dp(qx), _dp(qy))
         return h, qx, qy
 
     @property
     def t(self):
-        return lib().orc_time(self._s)
+        return lib(self.math).orc_time(self._s)
 
     @property
     def dt(self):
-        return lib().orc_last_dt(self._s)
+        return lib(self.math).orc_last_dt(self._s)
 
 
 def run_reference(folder, timeout=600):

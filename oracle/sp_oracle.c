@@ -8,6 +8,7 @@
  * -ffp-contract=off so no multiply-add is fused.
  */
 #include "sp_oracle.h"
+#include "orc_math.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -25,6 +26,7 @@ static inline double smin(double a, double b) { return (b < a) ? b : a; }
 
 struct orc_solver {
     orc_case c;
+    int64_t n_audit; /* steps taken, for the math audit */
     /* owned copies of the per-cell parameters the step needs */
     double *h, *qx, *qy;
     /* edge factors (swSp.cpp:338-444) */
@@ -792,6 +794,7 @@ static void correct_velocity(orc_solver *s) {
  * swSp::hypCompute (swSpHyp.cpp:137-211) inlined */
 int orc_step(orc_solver *s) {
     const orc_case *c = &s->c;
+    ORC_AUDIT_STEP(s->n_audit++);
     if (c->scheme == ORC_MUSCL_EVR)
         lr_muscl(s);
     else
@@ -837,6 +840,54 @@ static void *xcopy(const void *p, size_t bytes) {
     void *q = malloc(bytes ? bytes : 1);
     memcpy(q, p, bytes);
     return q;
+}
+
+/* ---- math audit (see orc_math.h) ---- */
+#ifdef ORC_SW_MATH
+int64_t orc_audit_step = -1;
+static int64_t g_audit_count = 0, g_audit_step = -1;
+static int g_audit_fn = -1;
+static double g_audit_x = 0.0, g_audit_g = 0.0, g_audit_s = 0.0;
+void orc_audit_note(int fn, double x, double glibc, double sw) {
+    if (g_audit_count++ == 0) {
+        g_audit_step = orc_audit_step;
+        g_audit_fn = fn;
+        g_audit_x = x;
+        g_audit_g = glibc;
+        g_audit_s = sw;
+    }
+}
+#endif
+
+void orc_math_audit_reset(void) {
+#ifdef ORC_SW_MATH
+    g_audit_count = 0;
+    g_audit_step = -1;
+    g_audit_fn = -1;
+#endif
+}
+
+/* 1 in the ORC_SW_MATH build: the first call (step, function 0 exp / 1 log
+ * / 2 cbrt, argument, glibc's and the kernels' result) whose results differ
+ * since the last reset, and how many did; 0 (nothing set) otherwise. */
+int orc_math_audit(int64_t *step, int32_t *fn, double *x, double *glibc, double *sw, int64_t *count) {
+#ifdef ORC_SW_MATH
+    if (step) *step = g_audit_step;
+    if (fn) *fn = g_audit_fn;
+    if (x) *x = g_audit_x;
+    if (glibc) *glibc = g_audit_g;
+    if (sw) *sw = g_audit_s;
+    if (count) *count = g_audit_count;
+    return 1;
+#else
+    (void)step, (void)fn, (void)x, (void)glibc, (void)sw, (void)count;
+    return 0;
+#endif
+}
+
+/* The oracle's exp (0), log (1) or cbrt (2) of n values */
+void orc_math_eval(int32_t fn, int64_t n, const double *x, double *y) {
+    for (int64_t i = 0; i < n; ++i) y[i] = fn == 0 ? exp(x[i]) : fn == 1 ? log(x[i]) : cbrt(x[i]);
 }
 
 orc_solver *orc_create(const orc_case *c, const double *h0, const double *qx0, const double *qy0) {

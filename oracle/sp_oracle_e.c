@@ -26,6 +26,7 @@
 #include <string.h>
 
 #include "sp_oracle.h"
+#include "orc_math.h"
 
 #define SMALL 1e-16
 #define G 9.81
@@ -606,6 +607,7 @@ int64_t orc_e_record_states(const orc_case *c, const double *phi, const double *
     int64_t n = 0;
     double *bc = (double *)calloc((size_t)(c->n_series ? c->n_series : 1), sizeof(double));
     while (t < c->t_max && n < max_steps) {
+        ORC_AUDIT_STEP(n);
         for (int k = 0; k < c->n_series; k++)
             bc[k] = orc_e_interp(c->series_t, c->series_v + (int64_t)k * c->n_times, c->n_times, t);
         double target = c->dt_max;
@@ -663,6 +665,7 @@ double orc_e_replay(const orc_case *c, const double *phi, const double *K1, cons
     e_init(&s, c, phi, K1, K2, h0, qx0, qy0);
     double J = 0.0;
     for (int64_t n = 0; n < n_steps; n++) {
+        ORC_AUDIT_STEP(n);
         e_step(&s, dt_seq[n], bc_seq + n * c->n_series);
         int32_t oi = obs_trig[n];
         if (oi >= 0 && oi < n_obs && n_gauges > 0) {
@@ -700,6 +703,7 @@ void orc_e_dry_states(const orc_case *c, const double *phi, const double *K1, co
     memcpy(states + 2 * nc, qy0, b);
     double prev = 0.0;
     for (int64_t n = 0; n < n_steps; n++) {
+        ORC_AUDIT_STEP(n);
         const double dt = dt_seq[n];
         s.dt_recon = prev;
         lr_any(&s);
@@ -1652,6 +1656,7 @@ double orc_e_replay_frames(const orc_case *c, const double *phi, const double *K
     const int64_t nc = c->n_cell;
     double J = 0.0;
     for (int64_t n = 0; n < n_steps; n++) {
+        ORC_AUDIT_STEP(n);
         e_step(&s, dt_seq[n], bc_seq + n * c->n_series);
         int32_t oi = obs_trig[n];
         if (oi >= 0 && oi < n_obs) {

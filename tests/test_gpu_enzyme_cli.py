@@ -52,7 +52,7 @@ def set_keys(folder, **kv):
 def write_sp_map(path, phi):
     with open(path, "w") as f:
         f.write("Unif\t0\n==== Default Param\nPhi\n1\n==== Distrib\n")
-        f.write("".join(f"{v!r}\n" for v in phi))
+        f.write("".join(f"{float(v)!r}\n" for v in phi))
 
 
 @pytest.fixture(scope="module")
@@ -81,7 +81,7 @@ def test_gpu_cli_forward_matches_reference_e(sw, tmp_path):
     r = run(sw, d, "forward")
     assert r.returncode == 0, r.stderr[-2000:]
     maps = sorted(glob.glob(os.path.join(d, "output", "ResultHydro_p*.txt")))
-    assert [os.path.basename(m)[-14:] for m in maps] == [
+    assert [os.path.basename(m)[-13:] for m in maps] == [
         "00m05s000.txt", "00m10s000.txt", "00m15s000.txt", "00m20s000.txt"]
     m = sw.read_result_map(maps[-1])
     for k, col in (("h", "water_depth"), ("qx", "unit_discharge_x"), ("qy", "unit_discharge_y")):

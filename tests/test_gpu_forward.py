@@ -2,10 +2,11 @@
 
 Tolerance (north_star): h, qx, qy within 1e-12 relative, measured as
 max|gpu - ref| / max|ref| per field.  Division and sqrt are correctly
-rounded FP64 on both sides and the kernels sum in the reference's order with
-contraction disabled, so the only source of difference is exp/log (Manning,
-Strickler, head loss, critical depth), whose device and glibc results may
-differ by 1 ulp.
+rounded FP64 on both sides, the kernels sum in the reference's order with
+contraction disabled, and exp and cbrt are glibc's own (csrc/sw_math.h), so
+the only possible difference is a log result where glibc's log is not
+correctly rounded.  Against the oracle built with the kernels' functions
+(math "device") every map of every golden case is bit-identical.
 """
 import numpy as np
 import pytest
@@ -34,6 +35,31 @@ def test_gpu_matches_reference_maps(sw, tmp_path, name):
         for got, ref, lab in ((h, g["h"][k], "h"), (qx, g["qx"][k], "qx"), (qy, g["qy"][k], "qy")):
             e = rel_err(got, ref)
             assert e <= TOL, f"{name} {lab} at {f}: rel err {e:.3e}"
+    s.close()
+
+
+@pytest.mark.parametrize("name", [n for n in golden_names() if not n.endswith("_error")])
+def test_gpu_maps_are_bitwise_the_device_math_oracle(sw, oracle, tmp_path, name):
+    """The kernels against the oracle's restatement of the reference's loops
+    run with the same exp / log / cbrt: every stored map bit-identical; and
+    bit-identical to the reference itself where no log result of glibc's
+    differs in a way the scheme keeps (all but one golden case)."""
+    g = regen_case(sw, name, str(tmp_path))
+    case = sw.Case(str(tmp_path))
+    s = _gpu_solver(sw, str(tmp_path))
+    o = oracle.OracleSolver(case, math="device")
+    same_as_ref = True
+    for k, f in enumerate(g["files"]):
+        t = time_of(str(f))
+        if t > 0:
+            s.run(t_stop=t)
+            o.run(t_end=t)
+        for got, ref, gold in zip(s.state(), o.state(), (g["h"][k], g["qx"][k], g["qy"][k])):
+            np.testing.assert_array_equal(got, ref)
+            same_as_ref &= np.array_equal(got, gold)
+    print(f"[gpu] {name}: bit-identical to the reference too: {same_as_ref}", flush=True)
+    if name != "rain_mixed_godunov_infil_coeff_everywhere":
+        assert same_as_ref
     s.close()
 
 

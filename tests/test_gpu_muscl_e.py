@@ -6,14 +6,12 @@ The dry run against the reference's own _E physics is in test_gpu_adjoint.py
 - the frozen replay -- which reconstructs with the step's own dt, as the
   reference's taped segment does (swSpE_checkpoint.cpp:658-659), while the
   dry run used the previous step's -- against the CPU oracle's replay on the
-  recorded schedule: cost within 1e-12 where no transcendental function is on
-  the path (closed boxes without friction), within 1e-4 where the inflow's
-  exp/log and the friction's cbrt differ from glibc's by an ulp and the
-  second-order scheme amplifies it (CHAOTIC in test_gpu_adjoint.py);
+  recorded schedule, the oracle built with the kernels' exp / log / cbrt:
+  cost and final state bit-identical over the full run, and within 1e-12 of
+  the glibc oracle (the reference's arithmetic);
 - the gradient against the oracle's reverse sweep (itself checked against
   central differences on the CPU, slope limiter included) on the same
-  recorded schedule, every control kind; for the CHAOTIC cases the gauges
-  observe the first 8 s, before the 1-ulp differences have grown;
+  recorded schedule over the full run, every control kind;
 - the gradient against central differences of the GPU's own replay cost
   (one segment covers the window: the gradient is then that of the
   continuous replay the cost runs);
@@ -24,7 +22,6 @@ import numpy as np
 import pytest
 
 from conftest import e_golden_names, regen_case
-from test_gpu_adjoint import CHAOTIC
 
 pytestmark = pytest.mark.gpu
 MUSCL = [n for n in e_golden_names() if "muscl" in n]
@@ -54,28 +51,27 @@ def test_gpu_muscl_replay_cost_matches_oracle(sw, oracle, tmp_path, name):
     A.record()
     rec = A.get_record()
     J = A.cost()
-    E = oracle.OracleE(case)
-    Jo, (h, qx, qy) = E.replay({"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}, gauges=cells,
-                               obs=obs, n_obs=len(obs_t))
-    tol = 1e-4 if name in CHAOTIC else 1e-12
-    assert abs(J - Jo) <= tol * abs(Jo), (J, Jo)
+    rr = {"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}
+    Jd, _ = oracle.OracleE(case, math="device").replay(rr, gauges=cells, obs=obs, n_obs=len(obs_t))
+    assert J == Jd, (J, Jd)
+    Jo, _ = oracle.OracleE(case).replay(rr, gauges=cells, obs=obs, n_obs=len(obs_t))
+    assert abs(J - Jo) <= 1e-12 * abs(Jo), (J, Jo)
     s.close()
 
 
 @pytest.mark.parametrize("name", MUSCL)
 def test_gpu_muscl_gradient_matches_oracle_reverse_sweep(sw, oracle, tmp_path, name):
-    """max |gpu - oracle| / max |oracle| <= 1e-10 per control kind (the two
-    differ in summation order and, on the CHAOTIC cases, by 1-ulp
-    transcendentals)."""
+    """Over the whole run: max |gpu - oracle| / max |oracle| <= 1e-10 per
+    control kind (the two reverse sweeps sum in different orders)."""
     from sw2d_amd import assim
     g, case, s = _open(sw, tmp_path, name)
     print(f"[gpu] muscl gradient vs oracle {name}", flush=True)
-    obs_t = [2.0, 4.0, 6.0, 8.0] if name in CHAOTIC else list(g["obs_t"])
+    obs_t = list(g["obs_t"])
     cells, obs = _gauges(case, len(obs_t))
     A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs, normalize=True)
     G = A.gradient()
     rec = A.get_record()
-    E = oracle.OracleE(case)
+    E = oracle.OracleE(case, math="device")
     O = E.gradient({"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}, gauges=cells, obs=obs,
                    normalize=True, n_obs=len(obs_t))
     print(f"[gpu]   J {G['J']!r} oracle {O['J']!r}", flush=True)
@@ -150,7 +146,7 @@ def test_gpu_muscl_gradient_segments_restart_from_dry_run_states(sw, oracle, tmp
         np.testing.assert_array_equal(G[key], G2[key])
     rec = A.get_record()
     rec = {"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}
-    E = oracle.OracleE(case)
+    E = oracle.OracleE(case, math="device")
     S = E.dry_states(rec)
     O = E.gradient(rec, gauges=cells, obs=obs, normalize=True, n_obs=len(obs_t), checkpoints=(5, S))
     O1 = E.gradient(rec, gauges=cells, obs=obs, normalize=True, n_obs=len(obs_t))

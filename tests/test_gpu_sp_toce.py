@@ -2,26 +2,16 @@
 cells, quads and triangles, head loss, Strickler friction, q inflow with
 critical depth, fr outlet), through the C-ABI.
 
-What can be matched, and how tightly:
-- the _E forward is bit-identical to the oracle (itself bit-identical to the
-  reference) for the first 100 steps;
-- on that horizon the gradient of a frame misfit matches the oracle's reverse
-  sweep to 1e-10 for every control kind;
-- over the full case (3157 steps) the GPU and glibc exp/log round differently
-  in rare cases (the inflow's critical depth exp(log(q/phi/g)/3) first), and
-  the wet/dry fronts amplify such 1-ulp differences: changing ONE input of the
-  oracle by 1 ulp (a boundary value at one step, or phi of one cell) moves J
-  by 0.7e-6 .. 4.2e-6 relative.  The cost at phi = 1 must therefore match the
-  reference's descent log (J = 3.1042911536) to 1e-5 relative, not to its 11
-  printed digits.  The porosity gradient of this case is ill-conditioned
-  (1-ulp changes of one porosity move its L2 norm between 0.09 and 3), so the
-  log's gradient figures are not a parity target.  Neither is a finite
-  difference over the full window: the frozen, tiled cost is only piecewise
-  smooth there (cells cross the wet/dry threshold), and its central
-  difference along a random direction does not converge as eps shrinks
-  (measured for eps = 1e-3 .. 1e-9).  The adjoint's exactness is shown where
-  it is well posed: against the oracle on the bit-identical horizon (and on
-  the e_* cases), and against finite differences on the CPU.
+With the kernels' exp / cbrt equal to glibc's and a correctly rounded log
+(csrc/sw_math.h), the GPU's _E forward is bit for bit the reference's over
+the whole case (3157 steps; the 36 log calls where glibc's log misrounds are
+absorbed by the following exp, tests/test_oracle_math_audit.py), so the
+reference's own descent log is a parity target to its printed digits: J at
+phi = 1 (11 digits), the L2 norm, largest entry and its cell of the porosity
+gradient computed by the reference's Enzyme adjoint, and the adjoint peak
+(7 digits).  The gradient is ill-conditioned on this case (1-ulp changes of
+one porosity move its L2 norm between 0.09 and 3), so matching those digits
+requires the forward to be exact, which it now is.
 """
 import sys
 import os
@@ -45,22 +35,22 @@ def _assim(sw, s, g, **kw):
     return assim.Assim(s, g["obs_t"], **args)
 
 
-def test_gpu_sp_toce_first_100_steps_bit_identical(sw, oracle):
+def test_gpu_sp_toce_dry_run_bit_identical(sw, oracle):
     from sw2d_amd import assim
     case, g = load_sp_toce(sw)
-    print("[gpu] sp_toce first 100 steps", flush=True)
+    print("[gpu] sp_toce dry run, 3157 steps", flush=True)
     ones = np.ones(case.n_cell)
     s = sw.SpSolver(case)
-    A = assim.Assim(s, g["obs_t"], max_steps=100)
-    assert A.record(phi=ones) == 100
+    A = assim.Assim(s, g["obs_t"], max_steps=20000)
+    assert A.record(phi=ones) == 3157
     h, qx, qy = A.final_state()
-    E = oracle.OracleE(case)
-    rec = E.record(g["obs_t"], phi=ones, max_steps=100)
+    E = oracle.OracleE(case)  # glibc: the reference's arithmetic
+    rec = E.record(g["obs_t"], phi=ones, max_steps=20000, states=True)
     np.testing.assert_array_equal(A.get_record()["dt"], rec["dt"])
-    _, (ho, qxo, qyo) = E.replay(rec, phi=ones)
-    np.testing.assert_array_equal(h, ho)
-    np.testing.assert_array_equal(qx, qxo)
-    np.testing.assert_array_equal(qy, qyo)
+    last = rec["states"][-1]
+    np.testing.assert_array_equal(h, last[0])
+    np.testing.assert_array_equal(qx, last[1])
+    np.testing.assert_array_equal(qy, last[2])
     s.close()
 
 
@@ -86,21 +76,28 @@ def test_gpu_sp_toce_short_horizon_gradient_matches_oracle(sw, oracle):
     s.close()
 
 
-def test_gpu_sp_toce_cost_and_tiled_gradient(sw):
+def test_gpu_sp_toce_cost_and_gradient_equal_the_reference_log(sw):
+    """The first row of descent_log_reference_-75.8pct.csv, the reference's
+    Enzyme gradient at phi = 1 (48 tiles, window [10, 30] s): every printed
+    digit."""
     case, g = load_sp_toce(sw)
     print("[gpu] sp_toce full window", flush=True)
     s = sw.SpSolver(case)
     A = _assim(sw, s, g)
     ones = np.ones(case.n_cell)
     G = A.gradient(phi=ones)
-    Jref = float(g["expect_J"])
-    print(f"[gpu] J {G['J']:.10e} (log {Jref:.10e})", flush=True)
-    assert abs(G["J"] - Jref) <= 1e-5 * Jref, (G["J"], Jref)
+    gp = G["phi"]
+    i = int(np.argmax(np.abs(gp)))
+    print(f"[gpu] J {G['J']:.10e} |g| {np.linalg.norm(gp):.6e} max {gp[i]:.6e} at {i} "
+          f"dh_peak {A.adjoint_peak():.6e}", flush=True)
     assert G["n_steps"] == 3157
+    assert f"{G['J']:.10e}" == "3.1042911536e+00"
+    assert f"{np.linalg.norm(gp):.6e}" == f"{float(g['expect_grad_l2']):.6e}" == "8.189797e-02"
+    assert f"{gp[i]:.6e}" == f"{float(g['expect_grad_max']):.6e}" == "-4.373691e-02"
+    assert i == int(g["expect_grad_argmax"]) == 3931
+    assert f"{A.adjoint_peak():.6e}" == "2.377364e+04"
     # the tiled cost on the frozen schedule is the cost the gradient call reports
     assert A.cost(phi=ones) == G["J"]
-    gp = G["phi"]
-    assert np.all(np.isfinite(gp)) and np.count_nonzero(gp) > case.n_cell // 2
     s.close()
 
 
