@@ -116,6 +116,12 @@ typedef struct sw2dh_lbfgsb_param {
                                length, LBFGSpp's rule; 1 = the full step to the
                                Cauchy point, the 1997 Fortran code's rule for boxed
                                problems                                            */
+    int32_t max_submin;     /* passes of the box-constrained subspace minimisation  */
+    int32_t restart;        /* after a failed line search: 0 = stop (LBFGSpp throws,
+                               the reference keeps the last iterate); 1 = drop the
+                               correction pairs and search again from the last
+                               iterate along the Cauchy direction, the Fortran
+                               L-BFGS-B 3.0 rule (stop if that search fails too) */
 } sw2dh_lbfgsb_param;
 #define SW2DH_LBFGSB_CONVERGED_GRAD 0
 #define SW2DH_LBFGSB_CONVERGED_F 1
@@ -124,7 +130,7 @@ typedef struct sw2dh_lbfgsb_param {
 #define SW2DH_LBFGSB_STOPPED 4
 /* Library defaults: m 6, epsilon 1e-5, epsilon_rel 1e-5, past 1, delta
  * 1e-10, no iteration limit, 20 line-search evaluations, ftol 1e-4, wolfe
- * 0.9, max_step 1e20, first_step 0. */
+ * 0.9, max_step 1e20, first_step 0, max_submin 10, restart 0. */
 void sw2dh_lbfgsb_default(sw2dh_lbfgsb_param *p);
 int sw2dh_lbfgsb(int64_t n, double *x, const double *lb, const double *ub, sw2dh_objective fg,
                  void *user, const sw2dh_lbfgsb_param *param, double *fx, int32_t *n_iter,

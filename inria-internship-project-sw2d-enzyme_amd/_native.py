@@ -96,7 +96,8 @@ class LbfgsbParam(C.Structure):
         ("m", C.c_int32), ("epsilon", C.c_double), ("epsilon_rel", C.c_double),
         ("past", C.c_int32), ("delta", C.c_double), ("max_iterations", C.c_int32),
         ("max_linesearch", C.c_int32), ("ftol", C.c_double), ("wolfe", C.c_double),
-        ("max_step", C.c_double), ("first_step", C.c_int32),
+        ("max_step", C.c_double), ("first_step", C.c_int32), ("max_submin", C.c_int32),
+        ("restart", C.c_int32),
     ]
 
 

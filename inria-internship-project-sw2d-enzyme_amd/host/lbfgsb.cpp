@@ -1,27 +1,32 @@
 // lbfgsb.cpp -- box-constrained limited-memory BFGS (L-BFGS-B), the method
 // the reference's minimiser drives through LBFGSpp's LBFGSBSolver
-// (assim/swSpE_minimize.cpp:832-912, call at :873-898).  LBFGSpp is not in
-// the reference tree; this is written from the published method:
+// (assim/swSpE_minimize.cpp:832-912, call at :873-898).  LBFGSpp (a
+// header-only library found through LBFGSPP_DIR, assim/CMakeLists.txt:24-27,
+// no version pinned) is not in the reference tree; this is written from the
+// published methods that library implements:
 //   * R. H. Byrd, P. Lu, J. Nocedal, C. Zhu, "A limited memory algorithm for
 //     bound constrained optimization", SIAM J. Sci. Comput. 16 (1995):
 //     compact representation B = theta I - W M W^T, W = [Y, theta S]
 //     (section 3), generalized Cauchy point along the projected gradient
-//     path (algorithm CP, section 4), subspace minimisation over the free
-//     variables by the direct primal method (section 5.1);
-//   * J. L. Morales, J. Nocedal, "Remark on Algorithm 778", ACM TOMS 38
-//     (2011): the subspace step is projected onto the box and kept when it
-//     is a descent direction, else the step is cut back to the box;
+//     path (algorithm CP, section 4);
+//   * C. Voglis, I. E. Lagaris, "BOXCQP: an algorithm for bound constrained
+//     convex quadratic problems" (2004): the subspace step minimises the
+//     model over the variables free at the Cauchy point exactly inside the
+//     box (primal-dual active sets), LBFGSpp's replacement for the direct
+//     primal method of section 5.1;
 //   * J. J. More, D. J. Thuente, "Line search algorithms with guaranteed
 //     sufficient decrease", ACM TOMS 20 (1994): the line search (sufficient
 //     decrease ftol, curvature wolfe, safeguarded cubic / quadratic steps).
 // Conventions of the reference's library kept so that a user switching over
-// gets the same kind of iteration: the first direction (no correction pairs)
-// is normalised to unit length and tried at step min(1, step to the box)
+// gets the same iteration: the first direction (no correction pairs) is
+// normalised to unit length and tried at step min(1, step to the box)
 // (first_step 1 takes the Fortran code's full step to the Cauchy point);
 // later directions at step 1; convergence when the projected gradient's max
 // norm is <= max(epsilon, epsilon_rel ||x||_2), or when the objective
-// changes by <= delta max(|f|, |f_prev|, 1) over `past` iterations; every
-// evaluation is a value and a gradient.
+// changes by <= delta max(|f|, |f_prev|, 1) over `past` iterations; a line
+// search without sufficient decrease ends the run (restart 1: the Fortran
+// code's reset of the memory instead); every evaluation is a value and a
+// gradient.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -41,6 +46,30 @@ double dot(const double *a, const double *b, size_t n) {
     double s = 0.0;
     for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
+}
+
+// a . b summed in the order of a 2-wide vector reduction with two packet
+// accumulators (the reduction order of the reference library's vector
+// arithmetic on x86-64): lanes (0,1) and (2,3) accumulate every fourth pair,
+// are added, a leftover pair joins, the two lanes are added, then the last
+// element.  Used for the search direction's norm and its slopes, whose last
+// bits decide the trial points.
+double dot2(const double *a, const double *b, size_t n) {
+    if (n < 2) return n ? a[0] * b[0] : 0.0;
+    const size_t end2 = (n / 4) * 4, end = (n / 2) * 2;
+    double p0 = a[0] * b[0], p1 = a[1] * b[1];
+    if (end > 2) {
+        double q0 = a[2] * b[2], q1 = a[3] * b[3];
+        for (size_t i = 4; i < end2; i += 4) {
+            p0 += a[i] * b[i], p1 += a[i + 1] * b[i + 1];
+            q0 += a[i + 2] * b[i + 2], q1 += a[i + 3] * b[i + 3];
+        }
+        p0 += q0, p1 += q1;
+        if (end > end2) p0 += a[end2] * b[end2], p1 += a[end2 + 1] * b[end2 + 1];
+    }
+    double r = p0 + p1;
+    for (size_t i = end; i < n; ++i) r += a[i] * b[i];
+    return r;
 }
 
 // LU factorisation with partial pivoting of a small dense row-major matrix.
@@ -168,29 +197,34 @@ double proj_grad_norm(const Vec &x, const Vec &g, const double *lb, const double
     return r;
 }
 
-// Generalized Cauchy point (algorithm CP).  Outputs xcp, c = W^T (xcp - x)
-// and the free set at xcp.
+// Generalized Cauchy point (algorithm CP).  Outputs xcp, c = W^T (xcp - x),
+// the variables free at xcp and those that reached a bound on the way
+// (`passed`).  A variable is held from the start only when its breakpoint is
+// 0 (at a bound with the gradient pointing out of the box, or lb == ub); a
+// zero gradient component leaves it free whatever its position.
 void cauchy(const Bfgs &B, const Vec &x, const Vec &g, const double *lb, const double *ub,
-            Vec &xcp, Vec &c, std::vector<char> &free_at) {
+            Vec &xcp, Vec &c, std::vector<char> &free_at, std::vector<char> &passed) {
     const size_t n = x.size();
     const int t2 = 2 * B.k();
     xcp = x;
     c.assign(t2, 0.0);
     free_at.assign(n, 1);
+    passed.assign(n, 0);
     Vec d(n, 0.0), tb(n, kInf);
     std::vector<std::pair<double, size_t>> brk;
     brk.reserve(n);
     for (size_t i = 0; i < n; ++i) {
         const double gi = g[i];
-        if ((x[i] - lb[i] <= 0.0 && gi >= 0.0) || (ub[i] - x[i] <= 0.0 && gi <= 0.0)) {
-            tb[i] = 0.0;  // at a bound, the gradient pushes outwards: fixed
-            free_at[i] = 0;
-            continue;
-        }
-        if (gi < 0.0)
+        if (lb[i] == ub[i])
+            tb[i] = 0.0;
+        else if (gi < 0.0)
             tb[i] = (x[i] - ub[i]) / gi;
         else if (gi > 0.0)
             tb[i] = (x[i] - lb[i]) / gi;
+        if (tb[i] == 0.0) {
+            free_at[i] = 0;
+            continue;
+        }
         d[i] = -gi;
         if (tb[i] < kInf) brk.emplace_back(tb[i], i);
     }
@@ -208,7 +242,6 @@ void cauchy(const Bfgs &B, const Vec &x, const Vec &g, const double *lb, const d
     size_t ib = 0;
     double tcur = brk.empty() ? kInf : brk[0].first;
     double dt = tcur;
-    std::vector<char> passed(n, 0);
     while (ib < brk.size() && dt_min >= dt) {
         const size_t b = brk[ib].second;
         const double gb = g[b];
@@ -240,148 +273,250 @@ void cauchy(const Bfgs &B, const Vec &x, const Vec &g, const double *lb, const d
     for (int j = 0; j < t2; ++j) c[j] += dt_min * p[j];
 }
 
-// Subspace minimisation from the Cauchy point (direct primal method with the
-// Morales-Nocedal projection).  Returns the point xbar.
-void subspace(const Bfgs &B, const Vec &x, const Vec &g, const double *lb, const double *ub,
-              const Vec &xcp, const Vec &c, const std::vector<char> &free_at, Vec &xbar) {
-    const size_t n = x.size();
-    const int kk = B.k(), t2 = 2 * kk;
-    xbar = xcp;
-    std::vector<size_t> Z;
-    for (size_t i = 0; i < n; ++i)
-        if (free_at[i]) Z.push_back(i);
-    if (Z.empty() || kk == 0) return;
-    // reduced gradient r = Z^T (g + theta (xcp - x) - W M c)
-    Vec Mc(t2), r(Z.size());
-    B.Mv(c.data(), Mc.data());
-    Vec WMc(n, 0.0);
-    B.add_Wv(Mc.data(), 1.0, WMc.data());
-    for (size_t a = 0; a < Z.size(); ++a) {
-        const size_t i = Z[a];
-        r[a] = g[i] + B.theta * (xcp[i] - x[i]) - WMc[i];
+// The quadratic model restricted to the variables free at the Cauchy point,
+// H = B_FF = theta I - W_F M W_F^T: products and solves on subsets of F
+// through the rows of W (Sherman-Morrison-Woodbury, a 2k x 2k system).
+struct FreeModel {
+    const Bfgs &B;
+    int t2;
+    std::vector<double> WF;  // |F| x 2k, row a = row F[a] of W
+    FreeModel(const Bfgs &b, const std::vector<size_t> &F) : B(b), t2(2 * b.k()) {
+        WF.resize(F.size() * (size_t)t2);
+        for (size_t a = 0; a < F.size(); ++a) B.Wrow(F[a], &WF[a * t2]);
     }
-    // A = W^T Z (2k x |Z|): v = A r, N = I - (1/theta) M A A^T
-    Vec v(t2, 0.0), wr(t2);
-    std::vector<double> AAt((size_t)t2 * t2, 0.0);
-    for (size_t a = 0; a < Z.size(); ++a) {
-        B.Wrow(Z[a], wr.data());
-        for (int j = 0; j < t2; ++j) {
-            v[j] += wr[j] * r[a];
-            for (int l = 0; l < t2; ++l) AAt[j * t2 + l] += wr[j] * wr[l];
+    const double *w(size_t a) const { return &WF[a * t2]; }
+    // out[2k] = sum over a in S of W_F[a] v[a]  (v indexed like S)
+    void Wt(const std::vector<size_t> &S, const double *v, double *out) const {
+        std::fill(out, out + t2, 0.0);
+        for (size_t q = 0; q < S.size(); ++q)
+            for (int j = 0; j < t2; ++j) out[j] += w(S[q])[j] * v[q];
+    }
+    // out[q] = -(W_F[S[q]] . M u)   (the off-diagonal part of H applied to W^T v = u)
+    void minus_WMu(const std::vector<size_t> &S, const double *u, double *out) const {
+        Vec Mu(t2);
+        B.Mv(u, Mu.data());
+        for (size_t q = 0; q < S.size(); ++q) out[q] = -dot(w(S[q]), Mu.data(), t2);
+    }
+    // out = H_SS^{-1} rhs; false if the small system is singular
+    bool solve(const std::vector<size_t> &S, const double *rhs, double *out) const {
+        const double th = B.theta;
+        if (t2 == 0) {
+            for (size_t q = 0; q < S.size(); ++q) out[q] = rhs[q] / th;
+            return true;
         }
+        Vec v(t2), Mv_(t2), col(t2), Mcol(t2);
+        std::vector<double> AtA((size_t)t2 * t2, 0.0), N((size_t)t2 * t2);
+        Wt(S, rhs, v.data());
+        for (size_t q = 0; q < S.size(); ++q) {
+            const double *r = w(S[q]);
+            for (int j = 0; j < t2; ++j)
+                for (int l = 0; l < t2; ++l) AtA[j * t2 + l] += r[j] * r[l];
+        }
+        B.Mv(v.data(), Mv_.data());
+        for (int l = 0; l < t2; ++l) {
+            for (int j = 0; j < t2; ++j) col[j] = AtA[j * t2 + l];
+            B.Mv(col.data(), Mcol.data());
+            for (int j = 0; j < t2; ++j) N[j * t2 + l] = (j == l ? 1.0 : 0.0) - Mcol[j] / th;
+        }
+        std::vector<int> piv;
+        if (!lu_factor(N, t2, piv)) return false;
+        lu_solve(N, t2, piv, Mv_.data());
+        for (size_t q = 0; q < S.size(); ++q)
+            out[q] = rhs[q] / th + dot(w(S[q]), Mv_.data(), t2) / (th * th);
+        return true;
     }
-    Vec Mv_(t2);
-    B.Mv(v.data(), Mv_.data());
-    std::vector<double> N((size_t)t2 * t2, 0.0), col(t2), Mcol(t2);
-    for (int l = 0; l < t2; ++l) {
-        for (int j = 0; j < t2; ++j) col[j] = AAt[j * t2 + l];
-        B.Mv(col.data(), Mcol.data());
-        for (int j = 0; j < t2; ++j) N[j * t2 + l] = (j == l ? 1.0 : 0.0) - Mcol[j] / B.theta;
+};
+
+// Subspace minimisation from the Cauchy point, the reference library's
+// method: with the variables that are not free held at their xcp values,
+// the model m(x0 + y) = c.y + y^T H y / 2 over the free ones (y = x_F - x0_F,
+// c = g_F + B_FA (xcp - x0)_A) is minimised exactly inside the box
+// l = lb - x0 <= y <= u = ub - x0 by the primal-dual active-set iteration
+// BOXCQP of C. Voglis, I. E. Lagaris, "BOXCQP: an algorithm for bound
+// constrained convex quadratic problems" (2004): start from the unconstrained
+// minimiser; each pass puts below-l (or at-l with multiplier >= 0) variables
+// at l, above-u ones at u, solves the rest, and recomputes the multipliers
+// lambda_L = (H y + c)_L, mu_U = -(H y + c)_U; done when they are >= 0 and
+// the solved variables lie in the box.  After maxit passes without that, the
+// projected last iterate, else the projected unconstrained minimiser, when it
+// is a descent direction, else the unconstrained minimiser.  Output: the
+// direction d = xsm - x0 (d = xcp - x0 outside F).
+void subspace(const Bfgs &B, const Vec &x, const Vec &g, const double *lb, const double *ub,
+              const Vec &xcp, const std::vector<char> &free_at, const std::vector<char> &passed,
+              int maxit, Vec &d) {
+    const size_t n = x.size();
+    for (size_t i = 0; i < n; ++i) d[i] = xcp[i] - x[i];
+    std::vector<size_t> F;
+    for (size_t i = 0; i < n; ++i)
+        if (free_at[i]) F.push_back(i);
+    const size_t nf = F.size();
+    if (nf == 0) return;
+    const FreeModel H(B, F);
+    const int t2 = H.t2;
+    const double th = B.theta;
+    // c = g_F + B_FA d_A, B_FA d_A = -W_F M W_A^T d_A over the variables that became active
+    Vec cvec(nf), l(nf), u(nf), y(nf), tmp(nf);
+    std::vector<size_t> all(nf);
+    std::iota(all.begin(), all.end(), 0);
+    {
+        Vec wd(t2, 0.0), wr(t2);
+        for (size_t i = 0; i < n; ++i)
+            if (passed[i]) {
+                B.Wrow(i, wr.data());
+                for (int j = 0; j < t2; ++j) wd[j] += wr[j] * d[i];
+            }
+        H.minus_WMu(all, wd.data(), cvec.data());
     }
-    std::vector<int> piv;
-    if (!lu_factor(N, t2, piv)) return;  // keep the Cauchy point
-    lu_solve(N, t2, piv, Mv_.data());
-    // du = -(1/theta) r - (1/theta^2) A^T v
-    Vec du(Z.size());
-    for (size_t a = 0; a < Z.size(); ++a) {
-        B.Wrow(Z[a], wr.data());
-        du[a] = -r[a] / B.theta - dot(wr.data(), Mv_.data(), t2) / (B.theta * B.theta);
+    for (size_t a = 0; a < nf; ++a) {
+        const size_t i = F[a];
+        cvec[a] += g[i];
+        l[a] = lb[i] - x[i];
+        u[a] = ub[i] - x[i];
+        tmp[a] = -cvec[a];
     }
-    // projected step, kept when it is a descent direction from x
-    Vec xp = xcp;
-    for (size_t a = 0; a < Z.size(); ++a) {
-        const size_t i = Z[a];
-        xp[i] = std::min(ub[i], std::max(lb[i], xcp[i] + du[a]));
-    }
-    double dd = 0.0;
-    for (size_t i = 0; i < n; ++i) dd += (xp[i] - x[i]) * g[i];
-    if (dd < 0.0) {
-        xbar = xp;
+    if (!H.solve(all, tmp.data(), y.data())) return;  // singular model: the Cauchy point
+    bool inside = true;
+    for (size_t a = 0; a < nf && inside; ++a) inside = (y[a] >= l[a] && y[a] <= u[a]);
+    auto assign = [&](const Vec &yy) {
+        for (size_t a = 0; a < nf; ++a) d[F[a]] = yy[a];
+    };
+    if (inside) {
+        assign(y);
         return;
     }
-    // otherwise the longest feasible fraction of the step
-    double alpha = 1.0;
-    for (size_t a = 0; a < Z.size(); ++a) {
-        const size_t i = Z[a];
-        if (du[a] < 0.0)
-            alpha = std::min(alpha, (lb[i] - xcp[i]) / du[a]);
-        else if (du[a] > 0.0)
-            alpha = std::min(alpha, (ub[i] - xcp[i]) / du[a]);
+    const Vec yfb = y;
+    Vec lam(nf, 0.0), mu(nf, 0.0), Fy(t2), sub(nf), vals(nf);
+    std::vector<size_t> Ls, Us, Ps;
+    int it;
+    for (it = 0; it < maxit; ++it) {
+        Ls.clear(), Us.clear(), Ps.clear();
+        for (size_t a = 0; a < nf; ++a) {
+            if (y[a] < l[a] || (y[a] == l[a] && lam[a] >= 0.0)) {
+                Ls.push_back(a), y[a] = l[a], mu[a] = 0.0;
+            } else if (y[a] > u[a] || (y[a] == u[a] && mu[a] >= 0.0)) {
+                Us.push_back(a), y[a] = u[a], lam[a] = 0.0;
+            } else {
+                Ps.push_back(a), lam[a] = 0.0, mu[a] = 0.0;
+            }
+        }
+        if (!Ps.empty()) {
+            // y_P = -H_PP^{-1} (c_P + H_PL l_L + H_PU u_U)
+            Vec wq(t2, 0.0), wtmp(t2);
+            for (size_t q = 0; q < Ls.size(); ++q) vals[q] = l[Ls[q]];
+            H.Wt(Ls, vals.data(), wtmp.data());
+            for (int j = 0; j < t2; ++j) wq[j] += wtmp[j];
+            for (size_t q = 0; q < Us.size(); ++q) vals[q] = u[Us[q]];
+            H.Wt(Us, vals.data(), wtmp.data());
+            for (int j = 0; j < t2; ++j) wq[j] += wtmp[j];
+            H.minus_WMu(Ps, wq.data(), sub.data());
+            for (size_t q = 0; q < Ps.size(); ++q) sub[q] = -(sub[q] + cvec[Ps[q]]);
+            if (!H.solve(Ps, sub.data(), tmp.data())) break;
+            for (size_t q = 0; q < Ps.size(); ++q) y[Ps[q]] = tmp[q];
+        }
+        if (!Ls.empty() || !Us.empty()) {
+            H.Wt(all, y.data(), Fy.data());
+            H.minus_WMu(Ls, Fy.data(), sub.data());
+            for (size_t q = 0; q < Ls.size(); ++q) {
+                const size_t a = Ls[q];
+                lam[a] = sub[q] + cvec[a] + th * y[a];
+            }
+            H.minus_WMu(Us, Fy.data(), sub.data());
+            for (size_t q = 0; q < Us.size(); ++q) {
+                const size_t a = Us[q];
+                mu[a] = -(sub[q] + cvec[a] + th * y[a]);
+            }
+        }
+        bool ok = true;
+        for (size_t a : Ls) ok = ok && lam[a] >= 0.0;
+        for (size_t a : Us) ok = ok && mu[a] >= 0.0;
+        for (size_t a : Ps) ok = ok && y[a] >= l[a] && y[a] <= u[a];
+        if (ok) break;
     }
-    alpha = std::max(alpha, 0.0);
-    for (size_t a = 0; a < Z.size(); ++a) xbar[Z[a]] = xcp[Z[a]] + alpha * du[a];
+    if (it < maxit) {
+        assign(y);
+        return;
+    }
+    const double eps = std::numeric_limits<double>::epsilon();
+    auto descent = [&](const Vec &yy) {
+        assign(yy);
+        return dot2(d.data(), g.data(), n) <= -eps;
+    };
+    Vec yp(nf);
+    for (size_t a = 0; a < nf; ++a) yp[a] = std::min(u[a], std::max(l[a], y[a]));
+    if (descent(yp)) return;
+    for (size_t a = 0; a < nf; ++a) yp[a] = std::min(u[a], std::max(l[a], yfb[a]));
+    if (descent(yp)) return;
+    assign(yfb);
 }
 
-// One step of the More-Thuente interval update (their section 4, cases 1-4).
-void mt_step(double &stx, double &fx, double &dx, double &sty, double &fy, double &dy, double &stp,
-             double fp, double dp, bool &brackt, double stpmin, double stpmax) {
-    const double sgnd = dp * (dx / std::fabs(dx));
-    double stpf;
-    if (fp > fx) {  // case 1: higher value, the minimum is bracketed
-        const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
-        const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
-        double gamma = s * std::sqrt(std::max(0.0, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
-        if (stp < stx) gamma = -gamma;
-        const double p = (gamma - dx) + theta, q = ((gamma - dx) + gamma) + dp;
-        const double stpc = stx + (p / q) * (stp - stx);
-        const double stpq = stx + ((dx / ((fx - fp) / (stp - stx) + dx)) / 2.0) * (stp - stx);
-        stpf = std::fabs(stpc - stx) < std::fabs(stpq - stx) ? stpc : stpc + (stpq - stpc) / 2.0;
-        brackt = true;
-    } else if (sgnd < 0.0) {  // case 2: lower value, derivatives of opposite sign
-        const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
-        const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
-        double gamma = s * std::sqrt(std::max(0.0, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
-        if (stp > stx) gamma = -gamma;
-        const double p = (gamma - dp) + theta, q = ((gamma - dp) + gamma) + dx;
-        const double stpc = stp + (p / q) * (stx - stp);
-        const double stpq = stp + (dp / (dp - dx)) * (stx - stp);
-        stpf = std::fabs(stpc - stp) > std::fabs(stpq - stp) ? stpc : stpq;
-        brackt = true;
-    } else if (std::fabs(dp) < std::fabs(dx)) {  // case 3: derivative magnitude decreases
-        const double theta = 3.0 * (fx - fp) / (stp - stx) + dx + dp;
-        const double s = std::max({std::fabs(theta), std::fabs(dx), std::fabs(dp)});
-        double gamma = s * std::sqrt(std::max(0.0, (theta / s) * (theta / s) - (dx / s) * (dp / s)));
-        if (stp > stx) gamma = -gamma;
-        const double p = (gamma - dp) + theta, q = (gamma + (dx - dp)) + gamma;
-        const double r = p / q;
-        double stpc;
-        if (r < 0.0 && gamma != 0.0)
-            stpc = stp + r * (stx - stp);
-        else if (stp > stx)
-            stpc = stpmax;
-        else
-            stpc = stpmin;
-        const double stpq = stp + (dp / (dp - dx)) * (stx - stp);
-        if (brackt) {
-            stpf = std::fabs(stpc - stp) < std::fabs(stpq - stp) ? stpc : stpq;
-            if (stp > stx)
-                stpf = std::min(stp + 0.66 * (sty - stp), stpf);
-            else
-                stpf = std::max(stp + 0.66 * (sty - stp), stpf);
-        } else {
-            stpf = std::fabs(stpc - stp) > std::fabs(stpq - stp) ? stpc : stpq;
-            stpf = std::max(stpmin, std::min(stpmax, stpf));
-        }
-    } else {  // case 4: lower value, same sign, derivative does not decrease
-        if (brackt) {
-            const double theta = 3.0 * (fp - fy) / (sty - stp) + dy + dp;
-            const double s = std::max({std::fabs(theta), std::fabs(dy), std::fabs(dp)});
-            double gamma =
-                s * std::sqrt(std::max(0.0, (theta / s) * (theta / s) - (dy / s) * (dp / s)));
-            if (stp > sty) gamma = -gamma;
-            const double p = (gamma - dp) + theta, q = ((gamma - dp) + gamma) + dy;
-            stpf = stp + (p / q) * (sty - stp);
-        } else {
-            stpf = stp > stx ? stpmax : stpmin;
-        }
+// Moré-Thuente line search in the form of the reference's library: the
+// auxiliary function psi(a) = f(a) - f(0) - a ftol f'(0) brackets the step;
+// a trial with psi above the best one shrinks the interval (case 1), a lower
+// psi with psi' still of the initial sign extrapolates by 1.1 x the last
+// move (case 2), a sign change interpolates (case 3); interpolation picks
+// among the cubic and quadratic models of J. J. More, D. J. Thuente, ACM
+// TOMS 20 (1994), section 4.  Every trial is a value and a gradient.
+double quad_min_f(double a, double b, double fa, double ga, double fb) {
+    const double ba = b - a;
+    const double w = 0.5 * ba * ga / (fa - fb + ba * ga);
+    return a + w * ba;
+}
+
+double quad_min_g(double a, double b, double ga, double gb) {
+    const double w = ga / (ga - gb);
+    return a + w * (b - a);
+}
+
+double cubic_min(double a, double b, double fa, double fb, double ga, double gb, bool &exists) {
+    const double apb = a + b, ba = b - a, ba2 = ba * ba, fba = fb - fa, gba = gb - ga;
+    const double z3 = (ga + gb) * ba - 2.0 * fba;
+    const double z2 = 0.5 * (gba * ba2 - 3.0 * apb * z3);
+    const double z1 = fba * ba2 - apb * z2 - (a * apb + b * b) * z3;
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (std::fabs(z3) < eps * std::fabs(z2) || std::fabs(z3) < eps * std::fabs(z1)) {
+        exists = z2 * ba > 0.0;
+        return exists ? (-0.5 * z1 / z2) : b;
     }
-    if (fp > fx) {
-        sty = stp, fy = fp, dy = dp;
+    const double u = z2 / (3.0 * z3), v = z1 / z2;
+    const double vu = v / u;
+    exists = vu <= 1.0;
+    if (!exists) return b;
+    double r1, r2;
+    if (std::fabs(u) >= std::fabs(v)) {
+        const double w = 1.0 + std::sqrt(1.0 - vu);
+        r1 = -u * w;
+        r2 = -v / w;
     } else {
-        if (sgnd < 0.0) sty = stx, fy = fx, dy = dx;
-        stx = stp, fx = fp, dx = dp;
+        const double sq = std::sqrt(std::fabs(u)) * std::sqrt(std::fabs(v)) * std::sqrt(1.0 - u / v);
+        r1 = -u - sq;
+        r2 = -u + sq;
     }
-    stp = stpf;
+    return (z3 * ba > 0.0) ? std::max(r1, r2) : std::min(r1, r2);
+}
+
+double step_selection(double al, double au, double at, double fl, double fu, double ft, double gl,
+                      double gu, double gt) {
+    if (al == au) return al;
+    if (!std::isfinite(ft) || !std::isfinite(gt)) return (al + at) / 2.0;
+    bool ac_ok;
+    const double ac = cubic_min(al, at, fl, ft, gl, gt, ac_ok);
+    const double aq = quad_min_f(al, at, fl, gl, ft);
+    if (ft > fl) {
+        if (!ac_ok) return aq;
+        return (std::fabs(ac - al) < std::fabs(aq - al)) ? ac : ((aq + ac) / 2.0);
+    }
+    const double as = quad_min_g(al, at, gl, gt);
+    if (gt * gl < 0.0) return (std::fabs(ac - at) >= std::fabs(as - at)) ? ac : as;
+    const double deltal = 1.1, deltau = 0.66;
+    if (std::fabs(gt) < std::fabs(gl)) {
+        const double res =
+            (ac_ok && (ac - at) * (at - al) > 0.0 && std::fabs(ac - at) < std::fabs(as - at)) ? ac : as;
+        return (at > al) ? std::min(at + deltau * (au - at), res) : std::max(at + deltau * (au - at), res);
+    }
+    if (!std::isfinite(au) || !std::isfinite(fu) || !std::isfinite(gu)) return at + deltal * (at - al);
+    bool ae_ok;
+    const double ae = cubic_min(at, au, ft, fu, gt, gu, ae_ok);
+    return (at > al) ? std::min(at + deltau * (au - at), ae) : std::max(at + deltau * (au - at), ae);
 }
 
 struct Eval {
@@ -398,82 +533,96 @@ struct Eval {
     }
 };
 
-// More-Thuente search along d from x (value f0, directional derivative dg0 <
-// 0).  On return: 0 = strong Wolfe point found, 1 = the best point of the
-// search lowers f (accepted), 2 = no decrease, 3 = stop requested by the
-// objective (the evaluated point is in xn / fn / gn), -1 = objective error.
-int line_search(Eval &ev, const Vec &x, double f0, const Vec &g0, const Vec &d, double dg0,
-                double stp, double stpmax, const sw2dh_lbfgsb_param &P, Vec &xn, double &fn,
-                Vec &gn, const double *lb, const double *ub) {
-    const size_t n = x.size();
-    const double ftol = P.ftol, gtol = P.wolfe, xtol = 0.1, stpmin = 0.0;
-    const double xtrapl = 1.1, xtrapu = 4.0;
-    bool brackt = false;
-    int stage = 1;
-    const double gtest = ftol * dg0;
-    double width = stpmax - stpmin, width1 = 2.0 * width;
-    double stx = 0.0, fx = f0, gx = dg0, sty = 0.0, fy = f0, gy = dg0;
-    double stmin = 0.0, stmax = stp + xtrapu * stp;
-    // best point seen (lowest value)
-    double best_f = f0;
-    Vec best_x = x, best_g = g0;
-    Vec xt(n), gt(n);
-    for (int it = 0; it < P.max_linesearch; ++it) {
-        for (size_t i = 0; i < n; ++i)
-            xt[i] = std::min(ub[i], std::max(lb[i], x[i] + stp * d[i]));
-        double f;
-        const int rc = ev.call(xt, f, gt);
-        if (rc < 0) return -1;
-        if (f < best_f) best_f = f, best_x = xt, best_g = gt;
-        if (ev.stop) {
-            xn = xt, fn = f, gn = gt;
-            return 3;
+// Search along d from xp (value fx0, slope dg0 < 0), first trial at `step`,
+// never beyond stpmax.  Trial points are xp + step d as computed (the caller
+// clamps the accepted one).  On return xn / fn / gn hold the accepted point
+// and `step` its step: 0 = strong Wolfe met or the largest step kept, 1 =
+// the best point of an exhausted search (sufficient decrease), 2 = no step
+// of sufficient decrease, 3 = stop requested by the objective (xn is that
+// evaluation), 4 = the step fell below the minimum, -1 = objective error.
+int line_search(Eval &ev, const Vec &xp, double fx0, const Vec &gp, const Vec &d, double dg0,
+                double &step, double stpmax, const sw2dh_lbfgsb_param &P, Vec &xn, double &fn,
+                Vec &gn) {
+    const size_t n = xp.size();
+    const double min_step = 1e-20;
+    const double test_decr = P.ftol * dg0, test_curv = -P.wolfe * dg0;
+    double I_lo = 0.0, I_hi = kInf, fI_lo = 0.0, fI_hi = kInf;
+    double gI_lo = (1.0 - P.ftol) * dg0, gI_hi = kInf;
+    Vec x_lo = xp, g_lo = gp, x(n), g(n);
+    double fx_lo = fx0;
+    auto eval_at = [&](double stp, double &fx, double &dg) -> int {
+        for (size_t i = 0; i < n; ++i) x[i] = xp[i] + stp * d[i];
+        const int rc = ev.call(x, fx, g);
+        dg = dot2(g.data(), d.data(), n);
+        return rc;
+    };
+    double fx, dg;
+    if (eval_at(step, fx, dg) < 0) return -1;
+    if (ev.stop) {
+        xn = x, fn = fx, gn = g;
+        return 3;
+    }
+    if (fx <= fx0 + step * test_decr && std::fabs(dg) <= test_curv) {
+        xn = x, fn = fx, gn = g;
+        return 0;
+    }
+    const double delta = 1.1;
+    int it;
+    for (it = 0; it < P.max_linesearch; ++it) {
+        const double ft = fx - fx0 - step * test_decr;
+        const double gt = dg - P.ftol * dg0;
+        double new_step;
+        if (ft > fI_lo) {  // case 1: the interval shrinks to [I_lo, step]
+            new_step = step_selection(I_lo, I_hi, step, fI_lo, fI_hi, ft, gI_lo, gI_hi, gt);
+            if (new_step <= min_step) new_step = (I_lo + step) / 2.0;
+            I_hi = step, fI_hi = ft, gI_hi = gt;
+        } else if (gt * (I_lo - step) > 0.0) {  // case 2: extrapolate
+            new_step = std::min(stpmax, step + delta * (step - I_lo));
+            I_lo = step, fI_lo = ft, gI_lo = gt;
+            x_lo.swap(x), g_lo.swap(g), fx_lo = fx;
+        } else {  // case 3: the minimum lies between the old and the new lower end
+            new_step = step_selection(I_lo, I_hi, step, fI_lo, fI_hi, ft, gI_lo, gI_hi, gt);
+            I_hi = I_lo, fI_hi = fI_lo, gI_hi = gI_lo;
+            I_lo = step, fI_lo = ft, gI_lo = gt;
+            x_lo.swap(x), g_lo.swap(g), fx_lo = fx;
         }
-        if (!std::isfinite(f)) {  // step back towards the best bracket end
-            brackt = true;
-            sty = stp, fy = kInf, gy = 0.0;
-            stp = stx + 0.5 * (stp - stx);
-            stmin = std::min(stx, sty), stmax = std::max(stx, sty);
-            continue;
-        }
-        const double dg = dot(gt.data(), d.data(), n);
-        const double ftest = f0 + stp * gtest;
-        if (stage == 1 && f <= ftest && dg >= 0.0) stage = 2;
-        if (f <= ftest && std::fabs(dg) <= gtol * (-dg0)) {  // strong Wolfe
-            xn = xt, fn = f, gn = gt;
+        if (step == stpmax && new_step >= stpmax) {  // the largest step stays: keep the best
+            xn = x_lo, fn = fx_lo, gn = g_lo;
+            step = I_lo;
             return 0;
         }
-        const bool warn = (brackt && (stp <= stmin || stp >= stmax)) ||
-                          (brackt && stmax - stmin <= xtol * stmax) ||
-                          (stp == stpmax && f <= ftest && dg <= gtest) ||
-                          (stp == stpmin && (f > ftest || dg >= gtest));
-        if (warn) break;
-        if (stage == 1 && f <= fx && f > ftest) {  // modified function psi
-            double fm = f - stp * gtest, fxm = fx - stx * gtest, fym = fy - sty * gtest;
-            double gm = dg - gtest, gxm = gx - gtest, gym = gy - gtest;
-            mt_step(stx, fxm, gxm, sty, fym, gym, stp, fm, gm, brackt, stmin, stmax);
-            fx = fxm + stx * gtest, fy = fym + sty * gtest;
-            gx = gxm + gtest, gy = gym + gtest;
-        } else {
-            mt_step(stx, fx, gx, sty, fy, gy, stp, f, dg, brackt, stmin, stmax);
+        // a collapsed bracket proposes the best point again: its value is known,
+        // and the search would only repeat it until the evaluations run out
+        if (new_step == I_lo) {
+            if (!(I_lo > 0.0)) return 2;
+            xn = x_lo, fn = fx_lo, gn = g_lo;
+            step = I_lo;
+            return 1;
         }
-        if (brackt) {
-            if (std::fabs(sty - stx) >= 0.66 * width1) stp = stx + 0.5 * (sty - stx);
-            width1 = width;
-            width = std::fabs(sty - stx);
-            stmin = std::min(stx, sty), stmax = std::max(stx, sty);
-        } else {
-            stmin = stp + xtrapl * (stp - stx);
-            stmax = stp + xtrapu * (stp - stx);
+        step = new_step;
+        if (!(step >= min_step)) return 4;
+        if (eval_at(step, fx, dg) < 0) return -1;
+        if (ev.stop) {
+            xn = x, fn = fx, gn = g;
+            return 3;
         }
-        stp = std::max(stpmin, std::min(stpmax, stp));
-        if ((brackt && (stp <= stmin || stp >= stmax)) ||
-            (brackt && stmax - stmin <= xtol * stmax))
-            stp = stx;
-        if (!(stp > 0.0)) break;
+        if (fx <= fx0 + step * test_decr && std::fabs(dg) <= test_curv) {
+            xn = x, fn = fx, gn = g;
+            return 0;
+        }
+        if (step >= stpmax && fx - fx0 - step * test_decr <= fI_lo) {
+            xn = x, fn = fx, gn = g;
+            return 0;
+        }
     }
-    if (best_f < f0) {
-        xn = best_x, fn = best_f, gn = best_g;
+    // exhausted: the last trial if it is the best, else the best end point
+    if (fx - fx0 - step * test_decr <= fI_lo) {
+        xn = x, fn = fx, gn = g;
+        return 1;
+    }
+    if (I_lo > 0.0) {
+        xn = x_lo, fn = fx_lo, gn = g_lo;
+        step = I_lo;
         return 1;
     }
     return 2;
@@ -496,6 +645,8 @@ extern "C" void sw2dh_lbfgsb_default(sw2dh_lbfgsb_param *p) {
     p->wolfe = 0.9;
     p->max_step = 1e20;
     p->first_step = 0;
+    p->max_submin = 10;
+    p->restart = 0;
 }
 
 extern "C" const char *sw2dh_lbfgsb_error(void) { return g_lb_err.c_str(); }
@@ -512,8 +663,10 @@ extern "C" int sw2dh_lbfgsb(int64_t n, double *x, const double *lb, const double
     sw2dh_lbfgsb_param P;
     sw2dh_lbfgsb_default(&P);
     if (param) P = *param;
-    if (P.m < 1 || P.max_linesearch < 1 || !(P.ftol > 0.0 && P.ftol < P.wolfe && P.wolfe < 1.0)) {
-        g_lb_err = "sw2dh_lbfgsb: need m >= 1, max_linesearch >= 1, 0 < ftol < wolfe < 1";
+    if (P.m < 1 || P.max_linesearch < 1 || P.max_submin < 1 ||
+        !(P.ftol > 0.0 && P.ftol < P.wolfe && P.wolfe < 1.0)) {
+        g_lb_err = "sw2dh_lbfgsb: need m >= 1, max_linesearch >= 1, max_submin >= 1, "
+                   "0 < ftol < wolfe < 1";
         return 1;
     }
     const size_t N = (size_t)n;
@@ -556,63 +709,75 @@ extern "C" int sw2dh_lbfgsb(int64_t n, double *x, const double *lb, const double
     B.m = P.m;
     std::vector<double> f_hist;
     f_hist.push_back(f);
-    Vec xcp, c, xbar, d(N), Xn(N), Gn(N);
-    std::vector<char> free_at;
-    bool retried = false;
+    Vec xcp, c, d(N), Xn(N), Gn(N);
+    std::vector<char> free_at, passed;
+    auto max_step_to_box = [&](const Vec &dir) {
+        double st = kInf;
+        for (size_t i = 0; i < N; ++i) {
+            if (dir[i] > 0.0) st = std::min(st, (U[i] - X[i]) / dir[i]);
+            else if (dir[i] < 0.0) st = std::min(st, (L[i] - X[i]) / dir[i]);
+        }
+        return st;
+    };
+    // first direction: towards the Cauchy point of the identity model
+    cauchy(B, X, G, L.data(), U.data(), xcp, c, free_at, passed);
+    for (size_t i = 0; i < N; ++i) d[i] = xcp[i] - X[i];
+    if (P.first_step == 0) {  // unit length (the reference library's rule)
+        const double dn = std::sqrt(dot2(d.data(), d.data(), N));
+        if (dn > 0.0)
+            for (size_t i = 0; i < N; ++i) d[i] /= dn;
+    }
     for (;;) {
-        cauchy(B, X, G, L.data(), U.data(), xcp, c, free_at);
-        subspace(B, X, G, L.data(), U.data(), xcp, c, free_at, xbar);
-        for (size_t i = 0; i < N; ++i) d[i] = xbar[i] - X[i];
-        double dg = dot(G.data(), d.data(), N);
-        if (!(dg < 0.0) && B.k() > 0) {  // pathological subspace step: restart from the GCP
-            B.reset();
-            cauchy(B, X, G, L.data(), U.data(), xcp, c, free_at);
+        double dg = dot2(G.data(), d.data(), N);
+        double stpmax = max_step_to_box(d);
+        if (stpmax <= 1e-20 || !(dg < 0.0)) {  // pathological step: the Cauchy direction, no memory
             for (size_t i = 0; i < N; ++i) d[i] = xcp[i] - X[i];
-            dg = dot(G.data(), d.data(), N);
+            B.reset();
+            dg = dot2(G.data(), d.data(), N);
+            stpmax = max_step_to_box(d);
         }
         if (!(dg < 0.0)) return finish(SW2DH_LBFGSB_CONVERGED_GRAD);
-        if (B.k() == 0 && P.first_step == 0) {  // LBFGSpp's first step: unit-length direction
-            const double dn = std::sqrt(dot(d.data(), d.data(), N));
-            for (size_t i = 0; i < N; ++i) d[i] /= dn;
-            dg /= dn;
-        }
-        double stpmax = P.max_step;
-        for (size_t i = 0; i < N; ++i) {
-            if (d[i] < 0.0 && L[i] > -kInf) stpmax = std::min(stpmax, (L[i] - X[i]) / d[i]);
-            if (d[i] > 0.0 && U[i] < kInf) stpmax = std::min(stpmax, (U[i] - X[i]) / d[i]);
-        }
-        stpmax = std::max(stpmax, 0.0);
-        const double stp = std::min(1.0, stpmax);
+        stpmax = std::min(P.max_step, stpmax);
+        if (!(stpmax > 0.0)) return finish(SW2DH_LBFGSB_LINESEARCH_FAILED);
+        double step = std::min(1.0, stpmax);
         double fn = kInf;
-        const int ls = stp > 0.0 ? line_search(ev, X, f, G, d, dg, stp, stpmax, P, Xn, fn, Gn,
-                                               L.data(), U.data())
-                                 : 2;
+        const int ls = line_search(ev, X, f, G, d, dg, step, stpmax, P, Xn, fn, Gn);
         if (ls < 0) {
             g_lb_err = "sw2dh_lbfgsb: the objective failed";
             return 2;
         }
         if (ls == 3) {
-            if (fn < f || !std::isfinite(f)) X = Xn, f = fn, G = Gn;
+            if (fn < f) {
+                X = Xn, f = fn, G = Gn;
+                for (size_t i = 0; i < N; ++i) X[i] = std::min(U[i], std::max(L[i], X[i]));
+            }
             ++k;
             return finish(SW2DH_LBFGSB_STOPPED);
         }
-        if (ls == 2) {  // no decrease: drop the memory once and retry from the GCP
-            if (B.k() > 0 && !retried) {
-                B.reset();
-                retried = true;
-                continue;
+        // no step of sufficient decrease: the reference's library gives up here
+        // and the reference keeps the last iterate (swSpE_minimize.cpp:895-902);
+        // with `restart` the correction pairs go and the search starts again
+        // along the Cauchy direction of the identity model, once per failure
+        if (ls == 2 || ls == 4) {
+            if (!P.restart || B.k() == 0) return finish(SW2DH_LBFGSB_LINESEARCH_FAILED);
+            B.reset();
+            cauchy(B, X, G, L.data(), U.data(), xcp, c, free_at, passed);
+            for (size_t i = 0; i < N; ++i) d[i] = xcp[i] - X[i];
+            if (P.first_step == 0) {
+                const double dn = std::sqrt(dot2(d.data(), d.data(), N));
+                if (dn > 0.0)
+                    for (size_t i = 0; i < N; ++i) d[i] /= dn;
             }
-            return finish(SW2DH_LBFGSB_LINESEARCH_FAILED);
+            continue;
         }
-        retried = false;
         ++k;
-        Vec s(N), y(N);
-        for (size_t i = 0; i < N; ++i) s[i] = Xn[i] - X[i], y[i] = Gn[i] - G[i];
+        Vec sv(N), yv(N);
+        for (size_t i = 0; i < N; ++i) sv[i] = Xn[i] - X[i], yv[i] = Gn[i] - G[i];
         X = Xn, G = Gn;
         f = fn;
         f_hist.push_back(f);
         const double pg = proj_grad_norm(X, G, L.data(), U.data());
-        if (pg <= std::max(P.epsilon, P.epsilon_rel * std::sqrt(dot(X.data(), X.data(), N))))
+        if (pg <= std::max(P.epsilon, P.epsilon_rel * std::sqrt(dot2(X.data(), X.data(), N))))
             return finish(SW2DH_LBFGSB_CONVERGED_GRAD);
         if (P.past > 0 && (int)f_hist.size() > P.past) {
             const double fp = f_hist[f_hist.size() - 1 - P.past];
@@ -620,7 +785,11 @@ extern "C" int sw2dh_lbfgsb(int64_t n, double *x, const double *lb, const double
                 return finish(SW2DH_LBFGSB_CONVERGED_F);
         }
         if (P.max_iterations > 0 && k >= P.max_iterations) return finish(SW2DH_LBFGSB_MAX_ITER);
-        const double sy = dot(s.data(), y.data(), N);
-        if (sy > std::numeric_limits<double>::epsilon() * dot(y.data(), y.data(), N)) B.add(s, y);
+        if (dot2(sv.data(), yv.data(), N) >
+            std::numeric_limits<double>::epsilon() * dot2(yv.data(), yv.data(), N))
+            B.add(sv, yv);
+        for (size_t i = 0; i < N; ++i) X[i] = std::min(U[i], std::max(L[i], X[i]));
+        cauchy(B, X, G, L.data(), U.data(), xcp, c, free_at, passed);
+        subspace(B, X, G, L.data(), U.data(), xcp, free_at, passed, P.max_submin, d);
     }
 }

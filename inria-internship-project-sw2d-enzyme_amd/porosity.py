@@ -138,7 +138,8 @@ LBFGSB_STOP = ("converged (projected gradient)", "converged (objective change)",
 
 
 def lbfgsb(fg, x0, lo=None, hi=None, m=6, eps=1e-5, eps_rel=None, max_iter=0, max_ls=20, past=1,
-           delta=1e-10, ftol=1e-4, wolfe=0.9, first_step="unit", stop=None):
+           delta=1e-10, ftol=1e-4, wolfe=0.9, first_step="unit", max_submin=10, restart=False,
+           stop=None):
     """L-BFGS-B over fg(x) -> (value, gradient): the native minimiser
     (sw2dh_lbfgsb, include/sw2d_host.h), which stands in for the reference's
     LBFGSpp::LBFGSBSolver (assim/swSpE_minimize.cpp:857-898; parameters m =
@@ -146,7 +147,12 @@ def lbfgsb(fg, x0, lo=None, hi=None, m=6, eps=1e-5, eps_rel=None, max_iter=0, ma
     lbfgsb_max_linesearch, max_iterations = max_it).  Every evaluation is a
     value and a gradient.  first_step "unit" (LBFGSpp) tries the first
     direction at unit length, "cauchy" (the 1997 Fortran code on boxed
-    problems) goes to the first Cauchy point.  stop(x, value) -> True ends the run after that
+    problems) goes to the first Cauchy point.  max_submin bounds the passes of
+    the box-constrained subspace minimisation.  restart=True drops the
+    correction pairs after a failed line search and searches again from the
+    last iterate (the Fortran L-BFGS-B 3.0 rule); False stops there (LBFGSpp,
+    whose exception the reference catches, keeping the last iterate).
+    stop(x, value) -> True ends the run after that
     evaluation (its point is kept when it is the best).  Returns (x, value,
     info) with info = {n_iter, n_eval, pgnorm, reason, trace [(wall s, value,
     'grad')]}."""
@@ -168,6 +174,7 @@ def lbfgsb(fg, x0, lo=None, hi=None, m=6, eps=1e-5, eps_rel=None, max_iter=0, ma
     if first_step not in ("unit", "cauchy"):
         raise ValueError("first_step must be 'unit' or 'cauchy'")
     P.first_step = 0 if first_step == "unit" else 1
+    P.max_submin, P.restart = int(max_submin), int(bool(restart))
     t0 = time.perf_counter()
     trace, err = [], []
 
@@ -199,14 +206,15 @@ def lbfgsb(fg, x0, lo=None, hi=None, m=6, eps=1e-5, eps_rel=None, max_iter=0, ma
 
 
 def minimize(fg, f=None, x0=None, lo=None, hi=None, m=6, max_iter=100, max_ls=30, eps=1e-5,
-             time_limit=None, stop=None):
+             time_limit=None, stop=None, restart=False):
     """run_minimization's "lbfgsb" branch over box bounds: L-BFGS-B (memory m,
     at most max_ls line-search evaluations, projected-gradient tolerance eps
     absolute and relative, the reference's settings for its functional case),
     every evaluation a gradient fg(x) -> (value, gradient) as the reference's
     objective adaptor (swSpE_minimize.cpp:850-856).  f is not used (kept for
     callers that pass a cost-only function).  Stops also after time_limit
-    seconds or when stop(x, value) is true.  Returns (x, value, trace)."""
+    seconds or when stop(x, value) is true; restart as in lbfgsb.  Returns
+    (x, value, trace)."""
     t0 = time.perf_counter()
 
     def halt(xv, J):
@@ -215,7 +223,7 @@ def minimize(fg, f=None, x0=None, lo=None, hi=None, m=6, max_iter=100, max_ls=30
         return bool(stop(xv, J)) if stop is not None else False
 
     x, J, info = lbfgsb(fg, x0, lo=lo, hi=hi, m=m, eps=eps, max_iter=max_iter, max_ls=max_ls,
-                        stop=halt)
+                        stop=halt, restart=restart)
     return x, J, info["trace"]
 
 

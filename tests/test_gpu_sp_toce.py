@@ -104,21 +104,62 @@ def test_gpu_sp_toce_cost_and_gradient_equal_the_reference_log(sw):
 def test_gpu_sp_toce_lbfgsb_descent(sw):
     """The reference's functional case end to end: from phi = 1, data-only
     L-BFGS-B on [0, 1] (memory 6, <= 30 line-search evaluations, 48 tiles,
-    window from t = 10 s, truncation window frozen at the start) over the
-    GPU adjoint.  The reference's descent log goes 3.104 -> 0.751 (-75.8 %)
-    in 9 evaluations; the path is not reproducible digit for digit (the
-    case's chaotic sensitivity, a different L-BFGS-B code), the descent is:
-    below 0.9 (-71 %) within 16 iterations."""
+    window from t = 10 s, truncation window frozen after a probe gradient)
+    over the GPU adjoint.  The reference's descent log goes 3.104 -> 0.751
+    (-75.8 %) in 9 evaluations, the probe included, and stops after 14.  The
+    path cannot be followed digit for digit beyond the first trial point: the
+    last bits of that point depend on the summation order of ||xcp - x0||, and
+    the case turns a 1-ulp change of phi into ~1e-5 of J.  The descent is as
+    deep: J <= 0.76 within 30 evaluations, the probe included."""
     from sw2d_amd.porosity import PorosityProblem, run_minimization
     case, g = load_sp_toce(sw)
     print("[gpu] sp_toce L-BFGS-B descent", flush=True)
     s = sw.SpSolver(case)
     A = _assim(sw, s, g)
     P = PorosityProblem(A, np.ones(case.n_cell))
-    th, J, tr = run_minimization(P, "lbfgsb", max_iter=16)
+    n = [1]  # the probe gradient at the start
+
+    def stop(x, J):
+        n[0] += 1
+        return n[0] >= 30
+
+    th, J, tr = run_minimization(P, "lbfgsb", max_iter=100, max_ls=30, stop=stop)
     grads = [v for (_, v, k) in tr if k == "grad"]
-    print(f"[gpu] J {grads[0]:.6f} -> {J:.6f} in {len(grads)} evaluations", flush=True)
-    assert abs(grads[0] - float(g["expect_J"])) <= 1e-5 * float(g["expect_J"])
-    assert J < 0.9
+    print(f"[gpu] J {grads[0]:.6f} -> {J:.6f} in {len(grads) + 1} evaluations "
+          f"(best {min(grads):.6f})", flush=True)
+    assert f"{grads[0]:.10e}" == "3.1042911536e+00"
+    assert len(grads) + 1 <= 30
+    assert J <= 0.76
     assert np.all((th >= 0.0) & (th <= 1.0))
     s.close()
+
+
+def test_gpu_sp_toce_enzyme_cli_descent_log(sw, tmp_path):
+    """`sw2d_enzyme_gpu <sp_toce> minimize` with the case's own input.solv
+    (lbfgsb, m 6, 30 line-search evaluations, epsilon 1e-5): the first two
+    rows of descent/descent_log.csv equal the reference's
+    descent_log_reference_-75.8pct.csv in all 21 columns, the first trial
+    point has the reference's porosity statistics (phi_min, phi_mean, counts),
+    and J falls to <= 0.76 within 30 rows."""
+    import csv
+    import subprocess
+    from conftest import sp_toce_folder
+    d = str(tmp_path)
+    sp_toce_folder(sw, d)
+    p = os.path.join(d, "input", "input.solv")
+    lines = [l for l in open(p).read().splitlines() if not (l.split() and l.split()[0] == "max_it")]
+    open(p, "w").write("\n".join(lines + ["max_it\t30"]) + "\n")
+    exe = os.path.join(os.path.dirname(sw.__file__), "lib", "sw2d_enzyme_gpu")
+    print("[gpu] sw2d_enzyme_gpu sp_toce minimize", flush=True)
+    r = subprocess.run([exe, d, "minimize"], capture_output=True, text=True, timeout=300, cwd=d)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "sp_toce_input.npz"))
+    ref = [l.split(",") for l in str(g["descent_log"]).splitlines()]
+    ours = list(csv.reader(open(os.path.join(d, "descent", "descent_log.csv"))))
+    assert ours[0] == ref[0]
+    assert ours[1] == ref[1] and ours[2] == ref[2]
+    for col in (13, 14, 15, 16, 17, 18):  # phi_min phi_max phi_mean n<0.5 n<0.05 n_clamp
+        assert ours[3][col] == ref[3][col], (ref[0][col], ours[3][col], ref[3][col])
+    J = [float(x[1]) for x in ours[1:31]]
+    print(f"[gpu] CLI: {len(ours) - 1} rows, best J {min(J):.6f}", flush=True)
+    assert min(J) <= 0.76
