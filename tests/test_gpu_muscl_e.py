@@ -62,14 +62,27 @@ def test_gpu_muscl_replay_cost_matches_oracle(sw, oracle, tmp_path, name):
 @pytest.mark.parametrize("name", MUSCL)
 def test_gpu_muscl_gradient_matches_oracle_reverse_sweep(sw, oracle, tmp_path, name):
     """Over the whole run: max |gpu - oracle| / max |oracle| <= 1e-10 per
-    control kind (the two reverse sweeps sum in different orders)."""
+    control kind (the two reverse sweeps sum in different orders), and the
+    same non-finite entries where the adjoint overflows.  The
+    oracle's sweep has no explosion guard, so the guard is off here (on
+    e_scale_muscl_nofriction the adjoint of h grows past the default 1e8
+    ratio and the guarded sweep stops early; the guard has its own tests).
+    One segment spans the run, so that the forward recompute is the oracle's
+    continuous replay: restarts from dry-run checkpoints, which
+    reconstruct with the previous step's dt, differ from it by ~1e-3 of J on
+    that case, and are checked against the oracle with the same restarts
+    below."""
     from sw2d_amd import assim
     g, case, s = _open(sw, tmp_path, name)
     print(f"[gpu] muscl gradient vs oracle {name}", flush=True)
     obs_t = list(g["obs_t"])
     cells, obs = _gauges(case, len(obs_t))
-    A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs, normalize=True)
+    A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs)
+    n_steps = A.record()
+    A = assim.Assim(s, obs_t, gauge_cells=cells, gauge_h=obs, normalize=True, explosion_ratio=0.0,
+                    stride=n_steps)
     G = A.gradient()
+    assert A.truncation() == (0, 0)  # the whole run was swept
     rec = A.get_record()
     E = oracle.OracleE(case, math="device")
     O = E.gradient({"dt": rec["dt"], "bc": rec["bc"], "trig": rec["trig"]}, gauges=cells, obs=obs,
@@ -77,11 +90,23 @@ def test_gpu_muscl_gradient_matches_oracle_reverse_sweep(sw, oracle, tmp_path, n
     print(f"[gpu]   J {G['J']!r} oracle {O['J']!r}", flush=True)
     assert abs(G["J"] - O["J"]) <= 1e-12 * abs(O["J"])
     for key in ("phi", "K1", "K2", "bc"):
-        scale = np.max(np.abs(O[key])) if O[key].size else 0.0
+        # where the adjoint overflows (e_scale_muscl_nofriction: 0/0 and inf
+        # reach most cells), the cell controls are returned as 0, the
+        # reference's rule (swSpE_checkpoint.cpp:1197-1220); the boundary
+        # gradient is returned as computed
+        ok = np.isfinite(O[key])
+        if not ok.all():
+            print(f"[gpu]   {key}: {np.count_nonzero(~ok)} non-finite oracle entries", flush=True)
+            if key == "bc":
+                assert not np.isfinite(G[key][~ok]).any()
+            else:
+                assert not np.any(G[key][~ok])
+        o, gk = O[key][ok], G[key][ok]
+        scale = np.max(np.abs(o)) if o.size else 0.0
         if scale == 0.0:
-            assert not np.any(G[key])
+            assert not np.any(gk)
             continue
-        e = np.max(np.abs(G[key] - O[key])) / scale
+        e = np.max(np.abs(gk - o)) / scale
         print(f"[gpu]   {key} rel {e:.3e}", flush=True)
         assert e <= 1e-10, (key, e)
     s.close()
