@@ -333,6 +333,19 @@ int sw2dg_mlp_set_params(sw2dg_mlp *mlp, const float *params);  /* also resets A
 int sw2dg_mlp_get_params(sw2dg_mlp *mlp, float *params);
 /* y [batch x dims[L]] = MLP(x [batch x dims[0]]) (inputs rounded to BF16). */
 int sw2dg_mlp_forward(sw2dg_mlp *mlp, int32_t batch, const float *x, float *y);
+/* sw2dg_mlp_forward without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference.  x [batch x dims[0]] and y [batch x
+ * dims[L]] are DEVICE pointers on the mlp's device; the BF16 input and the
+ * hidden activations live in a fixed 64 MiB device array of the library (one
+ * per GPU, held until the process ends), so batch is bounded by that array,
+ * not by max_batch: a batch that does not fit returns SW2DG_ERR_ARG with the
+ * limit in sw2dg_mlp_last_error.  stream: a hipStream_t to enqueue on and
+ * return without waiting, or NULL for the mlp's own stream, waited for.  The
+ * output equals sw2dg_mlp_forward's bit for bit.  Every call on one GPU
+ * shares the array: calls that overlap on a GPU (other threads, other
+ * streams, other mlp objects) race. */
+int sw2dg_mlp_forward_noalloc(sw2dg_mlp *mlp, int32_t batch, const float *x, float *y, void *stream);
 /* Mean squared error against t and its gradient (flattened like the
  * parameters; may be NULL). */
 int sw2dg_mlp_grad(sw2dg_mlp *mlp, int32_t batch, const float *x, const float *t, float *loss,

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16_t *__restrict__ A, cons
         const float x = acc[v];
         if (EPIK == EPI_HIDDEN) {
             const float z = x + ep.bias[col];
-            ep.z[o] = z;
+            if (ep.z) ep.z[o] = z;  // kept for the backward pass only
             ep.h[o] = f2bf(gelu(z));
         } else if (EPIK == EPI_OUT) {
             ep.z[o] = x + ep.bias[col];
@@ -164,6 +165,43 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     if (wb) wb[i] = f2bf(wi);
 }
 
+// Scratch of sw2dg_mlp_forward_noalloc: the BF16 input and the hidden
+// activations of one batch, in a fixed device array (one copy per GPU, held
+// until the process ends) so that the call allocates nothing.  Host code uses
+// the per-device address from hipGetSymbolAddress, never the array's name.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxDevices] = {};
+
+// Address of the scratch array on `device`, looked up once per device with
+// that device current (called from sw2dg_mlp_create).
+int scratch_setup(int device, std::string &err) {
+    if (device < 0 || device >= kMaxDevices) {
+        err = "scratch: device index " + std::to_string(device) + " out of range";
+        return SW2DG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return SW2DG_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return SW2DG_ERR_HIP;
+    }
+    g_scratch_dev[device] = (unsigned char *)p;
+    return SW2DG_OK;
+}
+
+unsigned char *scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 }  // namespace
 
 struct sw2dg_mlp {
@@ -208,10 +246,24 @@ int mfail(sw2dg_mlp *m, int code, const char *msg) {
     } while (0)
 
 template <bool TA, bool TB, int EPIK>
+void gemm_on(hipStream_t s, const bf16_t *A, const bf16_t *B, int M, int N, int K, int lda, int ldb,
+             const Epi &ep) {
+    const dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPIK>), grid, dim3(256), 0, s, A, B, M, N, K, lda, ldb, ep);
+}
+
+template <bool TA, bool TB, int EPIK>
 void gemm(sw2dg_mlp *m, const bf16_t *A, const bf16_t *B, int M, int N, int K, int lda, int ldb,
           const Epi &ep) {
-    const dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPIK>), grid, dim3(256), 0, m->stream, A, B, M, N, K, lda, ldb, ep);
+    gemm_on<TA, TB, EPIK>(m->stream, A, B, M, N, K, lda, ldb, ep);
+}
+
+// bytes of scratch a forward of `batch` rows takes: the BF16 input, then the
+// BF16 activations of each hidden layer (the output goes to the caller)
+size_t noalloc_bytes(const sw2dg_mlp *m, size_t batch) {
+    size_t b = align256(batch * m->dims[0] * sizeof(bf16_t));
+    for (int l = 0; l + 1 < m->L; ++l) b += align256(batch * m->dims[l + 1] * sizeof(bf16_t));
+    return b;
 }
 
 size_t n_params(const sw2dg_mlp *m) {
@@ -290,6 +342,10 @@ int sw2dg_mlp_create(int32_t n_layers, const int32_t *dims, int32_t max_batch, i
     else m->device = device;
     if (hipSetDevice(m->device) != hipSuccess ||
         hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete m;
+        return SW2DG_ERR_HIP;
+    }
+    if (scratch_setup(m->device, m->err) != SW2DG_OK) {
         delete m;
         return SW2DG_ERR_HIP;
     }
@@ -373,6 +429,47 @@ int sw2dg_mlp_forward(sw2dg_mlp *m, int32_t batch, const float *x, float *y) {
     MHIP(m, hipMemcpyAsync(y, m->Z[m->L - 1], (size_t)batch * m->dims[m->L] * sizeof(float),
                            hipMemcpyDeviceToHost, m->stream));
     MHIP(m, hipStreamSynchronize(m->stream));
+    return SW2DG_OK;
+}
+
+int sw2dg_mlp_forward_noalloc(sw2dg_mlp *m, int32_t batch, const float *x, float *y, void *stream) {
+    if (!m || !x || !y || batch < 1) return SW2DG_ERR_ARG;
+    const size_t need = noalloc_bytes(m, (size_t)batch);
+    if (need > kScratchBytes) {
+        const size_t per_row = noalloc_bytes(m, 1);
+        m->err = "sw2dg_mlp_forward_noalloc: batch " + std::to_string(batch) + " needs " +
+                 std::to_string(need) + " bytes of scratch, above the maximum of " +
+                 std::to_string(kScratchBytes) + " (about " + std::to_string(kScratchBytes / per_row) +
+                 " rows for these layer widths)";
+        return SW2DG_ERR_ARG;
+    }
+    MHIP(m, hipSetDevice(m->device));
+    unsigned char *base = scratch_of(m->device);
+    if (!base) return mfail(m, SW2DG_ERR_HIP, "sw2dg_mlp_forward_noalloc: no scratch on this device");
+    hipStream_t s = stream ? (hipStream_t)stream : m->stream;
+    // every scratch byte read below is written first in the same call: the
+    // input conversion fills xb, each hidden GEMM fills its activations
+    // before the next layer reads them, so nothing needs clearing
+    const size_t n0 = (size_t)batch * m->dims[0];
+    bf16_t *xb = (bf16_t *)base;
+    size_t off = align256(n0 * sizeof(bf16_t));
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, x, xb, n0);
+    const bf16_t *in = xb;
+    for (int l = 0; l < m->L; ++l) {
+        const int K = m->dims[l], N = m->dims[l + 1];
+        if (l + 1 < m->L) {
+            bf16_t *h = (bf16_t *)(base + off);
+            off += align256((size_t)batch * N * sizeof(bf16_t));
+            Epi ep{m->b[l], nullptr, h, nullptr, N};
+            gemm_on<false, false, EPI_HIDDEN>(s, in, m->Wb[l], batch, N, K, K, N, ep);
+            in = h;
+        } else {
+            Epi ep{m->b[l], y, nullptr, nullptr, N};
+            gemm_on<false, false, EPI_OUT>(s, in, m->Wb[l], batch, N, K, K, N, ep);
+        }
+    }
+    MHIP(m, hipGetLastError());
+    if (!stream) MHIP(m, hipStreamSynchronize(m->stream));
     return SW2DG_OK;
 }
 

@@ -19,6 +19,7 @@ _DECL = {
     "sw2dg_mlp_set_params": ([C.c_void_p, _f], C.c_int),
     "sw2dg_mlp_get_params": ([C.c_void_p, _f], C.c_int),
     "sw2dg_mlp_forward": ([C.c_void_p, C.c_int32, _f, _f], C.c_int),
+    "sw2dg_mlp_forward_noalloc": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "sw2dg_mlp_grad": ([C.c_void_p, C.c_int32, _f, _f, _f, _f], C.c_int),
     "sw2dg_mlp_train_step": ([C.c_void_p, C.c_int32, _f, _f, C.c_float, _f], C.c_int),
     "sw2dg_mlp_last_error": ([C.c_void_p], C.c_char_p),
@@ -100,6 +101,15 @@ class MLP:
             self._check(self.lib.sw2dg_mlp_forward(self.h, len(xb), _fp(xb), _fp(y)))
             out.append(y)
         return np.concatenate(out)
+
+    def forward_noalloc(self, x_dev, y_dev, batch, stream=None):
+        """sw2dg_mlp_forward_noalloc: y = MLP(x) for `batch` rows with x and y
+        given as device addresses (ints, e.g. a tensor's data_ptr()); nothing
+        is allocated.  stream: a hipStream_t handle to enqueue on without
+        waiting, or None for the network's own stream (waited for)."""
+        self._check(self.lib.sw2dg_mlp_forward_noalloc(self.h, int(batch), C.c_void_p(int(x_dev)),
+                                                       C.c_void_p(int(y_dev)),
+                                                       None if stream is None else C.c_void_p(int(stream))))
 
     def grad(self, x, t):
         x = np.ascontiguousarray(x, dtype=np.float32)
