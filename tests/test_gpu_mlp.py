@@ -1,7 +1,8 @@
 """The MLP warm start on the GPU (BF16 MFMA kernels, FP32 accumulation)
 against the CPU oracle (oracle/mlp_oracle.py: same BF16 quantisation points,
-float64 accumulation), at the network's real shape 800 -> 512 -> 512 -> 512
--> 68.  The two differ by the accumulation precision (FP32 vs float64) and
+float64 accumulation), at the network's real shape 2400 -> 512 -> 512 -> 512
+-> 68 (20 gauges x 120 samples of 30 s in, 8 Manning zones + 60 hydrograph
+knots out).  The two differ by the accumulation precision (FP32 vs float64) and
 by the rare BF16 re-quantisation of an activation that lands on the other
 side of a rounding boundary, so outputs and gradients are compared
 norm-wise at 1e-2 (BF16 resolution 2^-8 = 4e-3 per element)."""
@@ -15,7 +16,7 @@ sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
 import mlp_oracle as M  # noqa: E402
 
 pytestmark = pytest.mark.gpu
-DIMS = [800, 512, 512, 512, 68]
+DIMS = [2400, 512, 512, 512, 68]
 
 
 def _rel(a, b):
