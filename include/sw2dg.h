@@ -282,6 +282,19 @@ int sw2dg_assim_keep_obs_states(sw2dg_ctx *ctx, int32_t on);
 int sw2dg_assim_obs_states(sw2dg_ctx *ctx, double *h, double *qx, double *qy, int32_t *reached);
 /* State at the end of the recorded run (reference order). */
 int sw2dg_assim_final_state(sw2dg_ctx *ctx, double *h, double *qx, double *qy);
+/* Resident trajectory (Godunov), mode 1 (default) or 0: the dry run of
+ * sw2dg_assim_gradient keeps, for every segment it can give a device block
+ * (72 bytes per cell and step; blocks allocated on first use while they take
+ * at most 60 % of the memory free at the first one), each step's state,
+ * balanced state and boundary fluxes, and the reverse sweep reads them in
+ * place of replaying that segment from its checkpoint (the replay
+ * recomputes the same values bit for bit).  The reference replays every
+ * segment (ckpt_segment_E, swSpE_checkpoint.cpp:584-715); segments,
+ * checkpoints, tiles and the explosion guard are unchanged.
+ * sw2dg_assim_resident: the number of segments the last gradient read from
+ * resident blocks (0 = every segment was replayed). */
+int sw2dg_assim_set_resident(sw2dg_ctx *ctx, int32_t mode);
+int sw2dg_assim_resident(sw2dg_ctx *ctx, int32_t *active);
 
 /* ---- Batched ensemble: many differentiable-forward runs over uneven meshes
  * in one persistent launch.  The reference runs one simulation per process

@@ -45,6 +45,8 @@ _DECL = {
     "sw2dg_assim_obs_states": ([C.c_void_p, N._d, N._d, N._d, N._i32], C.c_int),
     "sw2dg_assim_set_adjoint_cut": ([C.c_void_p, C.c_double], C.c_int),
     "sw2dg_assim_set_explosion_guard": ([C.c_void_p, C.c_double], C.c_int),
+    "sw2dg_assim_set_resident": ([C.c_void_p, C.c_int32], C.c_int),
+    "sw2dg_assim_resident": ([C.c_void_p, N._i32], C.c_int),
     "sw2dg_assim_set_phi_transform": ([C.c_void_p, C.c_int32, C.c_double, N._i64, C.c_int64,
                                        C.c_int32], C.c_int),
     "sw2dg_assim_truncation": ([C.c_void_p, N._i64, N._i64], C.c_int),
@@ -84,7 +86,7 @@ class Assim:
     def __init__(self, solver, obs_t, gauge_cells=None, gauge_h=None, gauge_w=None, frames=None,
                  frame_w=None, normalize=False, stride=0, n_tiles=1, t_window=-1e300,
                  max_steps=100000, adj_cut=0.0, explosion_ratio=1e8, phi_transform=None,
-                 free_cells=None, mask_wall=False):
+                 free_cells=None, mask_wall=False, resident=True):
         self.s = solver
         self.lib = solver.lib
         _declare(self.lib)
@@ -115,6 +117,9 @@ class Assim:
         # the cut is a property of the solver context: set it on every Assim
         self.s._check(self.lib.sw2dg_assim_set_adjoint_cut(solver.ctx, float(adj_cut)))
         self.s._check(self.lib.sw2dg_assim_set_explosion_guard(solver.ctx, float(explosion_ratio)))
+        # resident trajectory (Godunov gradients read the dry run's steps in
+        # place of replaying them, where device memory allows)
+        self.s._check(self.lib.sw2dg_assim_set_resident(solver.ctx, 1 if resident else 0))
         self.stride = int(stride) if stride else int(np.ceil(np.sqrt(self.max_steps)))
         # phi_transform: None or (mode, lambda), mode "clamp" | "tanh" | "normalize";
         # the phi passed to record / cost / gradient is then the control m
@@ -189,6 +194,13 @@ class Assim:
         self.n_steps = n.value
         return {"J": J.value, "phi": gphi, "K1": gK1, "K2": gK2,
                 "bc": gbc[: n.value * ns].reshape(n.value, ns), "n_steps": n.value}
+
+    def resident_segments(self):
+        """Segments the last gradient read from resident blocks (0: every
+        segment was replayed from its checkpoint)."""
+        v = np.zeros(1, dtype=np.int32)
+        self.s._check(self.lib.sw2dg_assim_resident(self.s.ctx, N.i32ptr(v)))
+        return int(v[0])
 
     def truncation(self):
         """(first kept step, first kept segment) of the last gradient: (0, 0)

@@ -46,6 +46,16 @@ struct EClock {
     double t_max, dt_max, cfl;
 };
 
+// the dry run's gauge sampling, done by the update (gcell = nullptr: none):
+// the depth at each gauge cell after a step that reaches an observation
+// time (trig[n] >= 0), the series a twin experiment observes
+struct GaugeSample {
+    const int *trig;
+    const int *gcell;
+    int n_g;
+    double *gser;
+};
+
 struct Controls {
     const double *phi, *K1, *K2;
 };
@@ -83,6 +93,17 @@ struct AssimState {
     double *d_Pseg = nullptr;  // stride x 6 nc: balanced state + internal increments per step
     double *d_sxy = nullptr;   // 2 nc: limited free-surface slopes of the step (MUSCL-EVR)
     double *d_Sseg = nullptr;  // stride x 2 nc: the slopes of every step of the replayed segment
+    // resident trajectory (Godunov): the dry run inside a gradient keeps, for
+    // the steps of each segment it has a block for, what the replay of that
+    // segment would recompute bit for bit (states, balanced states + internal
+    // increments, boundary fluxes), so the reverse sweep reads it in place of
+    // replaying.  One block per segment, allocated on first use while the
+    // blocks take at most 60 % of the device memory free at the first one.
+    int resident_mode = 1;            // 0 never, 1 when the blocks fit
+    std::vector<double *> res_blk;    // per segment: [(stride + 1) x 3nc | stride x 6nc | stride x 3nb]
+    std::vector<char> res_kept;       // per segment: the last gradient's dry run filled its block
+    size_t res_budget = 0, res_bytes = 0;
+    int64_t seg_blk = -1;             // >= 0: the seg_* views address this segment's block
     double *d_partial = nullptr;
     double *d_J = nullptr;     // [1 + n_obs]: total, then the cost of each observation
     double *d_cpart = nullptr; // frame-cost block partials
@@ -578,7 +599,8 @@ __global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls
                                                      const double *dtv, double *T, int64_t n,
                                                      double t_max, double *__restrict__ P,
                                                      double *__restrict__ Fb,
-                                                     const double *__restrict__ Fst = nullptr) {
+                                                     const double *__restrict__ Fst,
+                                                     GaugeSample gs) {
     const int nc = m.n_cell;
     if (T && !(T[n] < t_max)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) T[n + 1] = T[n];
@@ -622,6 +644,12 @@ __global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls
     Uout[i] = s.h;
     Uout[nc + i] = s.qx;
     Uout[2 * nc + i] = s.qy;
+    if (gs.gcell) {
+        const int oi = gs.trig[n];
+        if (oi >= 0)
+            for (int g = 0; g < gs.n_g; ++g)
+                if (gs.gcell[g] == i) gs.gser[(size_t)oi * gs.n_g + g] = s.h;
+    }
 }
 
 // gauge cost (compute_cost_gauges_ptr, swSpE_gauges.cpp:166-179): one thread,
@@ -704,15 +732,6 @@ __global__ void k_cost_frame_sum(const double *part, int nparts, double norm, do
     double s = 0.0;
     for (int k = 0; k < nparts; ++k) s += part[k];
     J[0] += s / norm;
-}
-
-// gauge sampling of the dry run: depth at the gauge cells after a step that
-// reaches an observation time (the series a twin experiment observes)
-__global__ void k_gauge_sample(const double *__restrict__ U, const int *gcell, int n_g,
-                               const int *trig, int64_t n, double *gser) {
-    const int oi = trig[n];
-    if (oi < 0) return;
-    for (int g = threadIdx.x; g < n_g; g += blockDim.x) gser[(size_t)oi * n_g + g] = U[gcell[g]];
 }
 
 // full state after a step that reaches an observation time (the frame the
@@ -1160,18 +1179,60 @@ int upload_ctl(sw2dg_ctx *c, AssimState *a, const double *phi, const double *K1,
 
 Controls ctl(AssimState *a) { return Controls{a->d_phi, a->d_K1, a->d_K2}; }
 
-double *seg_state(AssimState *a, sw2dg_ctx *c, int j) { return a->d_seg + (size_t)j * 3 * c->nc; }
-double *ckpt_state(AssimState *a, sw2dg_ctx *c, int64_t k) {
-    return a->d_ckpt + (size_t)k * 3 * c->nc;
-}
-
 // boundary fluxes of one step (3 per boundary edge), kept per replayed step
 // for the boundary reverse
 size_t flux_len(const sw2dg_ctx *c) { return 3 * (size_t)std::max<int64_t>(c->nb, 1); }
 
-double *seg_flux(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Fseg + (size_t)j * flux_len(c); }
-double *seg_pre(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Pseg + (size_t)j * 6 * c->nc; }
+// the three parts of a resident block (doubles)
+size_t blk_states(const AssimState *a, const sw2dg_ctx *c) { return (size_t)(a->stride + 1) * 3 * c->nc; }
+size_t blk_pre(const AssimState *a, const sw2dg_ctx *c) { return (size_t)a->stride * 6 * c->nc; }
+size_t blk_len(const AssimState *a, const sw2dg_ctx *c) {
+    return blk_states(a, c) + blk_pre(a, c) + (size_t)a->stride * flux_len(c);
+}
+
+double *seg_state(AssimState *a, sw2dg_ctx *c, int j) {
+    if (a->seg_blk >= 0) return a->res_blk[a->seg_blk] + (size_t)j * 3 * c->nc;
+    return a->d_seg + (size_t)j * 3 * c->nc;
+}
+double *ckpt_state(AssimState *a, sw2dg_ctx *c, int64_t k) {
+    return a->d_ckpt + (size_t)k * 3 * c->nc;
+}
+double *seg_flux(AssimState *a, sw2dg_ctx *c, int j) {
+    if (a->seg_blk >= 0)
+        return a->res_blk[a->seg_blk] + blk_states(a, c) + blk_pre(a, c) + (size_t)j * flux_len(c);
+    return a->d_Fseg + (size_t)j * flux_len(c);
+}
+double *seg_pre(AssimState *a, sw2dg_ctx *c, int j) {
+    if (a->seg_blk >= 0) return a->res_blk[a->seg_blk] + blk_states(a, c) + (size_t)j * 6 * c->nc;
+    return a->d_Pseg + (size_t)j * 6 * c->nc;
+}
 double *seg_slopes(AssimState *a, sw2dg_ctx *c, int j) { return a->d_Sseg + (size_t)j * 2 * c->nc; }
+
+// segment k's resident block, allocated on first use within the budget
+// (Godunov only: its dry run and replay are the same computation, while the
+// MUSCL-EVR replay reconstructs with another dt); nullptr = replay it
+double *resident_block(sw2dg_ctx *c, AssimState *a, int64_t k) {
+    if (a->resident_mode == 0 || c->o.scheme == 1) return nullptr;
+    if ((int64_t)a->res_blk.size() <= k) a->res_blk.resize(k + 1, nullptr);
+    if (a->res_blk[k]) return a->res_blk[k];
+    if (a->res_budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return nullptr;
+        a->res_budget = (size_t)(0.6 * (double)free_b);
+    }
+    const size_t b = blk_len(a, c) * sizeof(double);
+    if (a->res_bytes + b > a->res_budget) return nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, b) != hipSuccess) {
+        (void)hipGetLastError();
+        a->res_budget = a->res_bytes;  // no more blocks
+        return nullptr;
+    }
+    a->allocs.push_back(p);
+    a->res_bytes += b;
+    a->res_blk[k] = (double *)p;
+    return a->res_blk[k];
+}
 
 // one _E step U_in -> U_out.  Dry run: (MUSCL slopes,) CFL partials, the
 // schedule (dt, boundary values, observation trigger), the update.  Replay:
@@ -1205,24 +1266,24 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
         KLAUNCH(a, k_e_dt, dim3(1), dim3(kDtBlock), 0, c->stream, c->bc, a->d_ck, a->d_partial, nblocks(c->nc),
                 a->d_T, a->d_dt, a->d_bc, a->d_trig, a->d_obs_t, n);
     }
+    GaugeSample gs{};
+    if (dry && a->n_gauges > 0) gs = GaugeSample{a->d_trig, a->d_gcell, a->n_gauges, a->d_gser};
 #define E_UPDATE(STORE, MUS)                                                                       \
     KLAUNCH_NAMED(a, "k_e_update<" #STORE ", " #MUS ", false>", (k_e_update<STORE, MUS>), gc, blk, 0, c->stream, \
-                  c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n, c->prm.t_max, P, Fb)
+                  c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n, c->prm.t_max, P, Fb, \
+                  nullptr, gs)
     if (P) {
         if (muscl) E_UPDATE(true, true);
         else E_UPDATE(true, false);
     } else if (dry && muscl) {
         KLAUNCH_NAMED(a, "k_e_update<false, true, true>", (k_e_update<false, true, true>), gc, blk, 0,
                       c->stream, c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n,
-                      c->prm.t_max, P, Fb, a->d_F);
+                      c->prm.t_max, P, Fb, a->d_F, gs);
     } else {
         if (muscl) E_UPDATE(false, true);
         else E_UPDATE(false, false);
     }
 #undef E_UPDATE
-    if (dry && a->n_gauges > 0)
-        KLAUNCH(a, k_gauge_sample, dim3(1), dim3(64), 0, c->stream, Uout, a->d_gcell,
-                           a->n_gauges, a->d_trig, n, a->d_gser);
     if (dry && a->d_oframes)
         KLAUNCH(a, k_frame_sample, dim3(nblocks(3 * c->nc)), blk, 0, c->stream, Uout, c->nc, a->d_trig,
                 n, a->d_oframes);
@@ -1336,6 +1397,7 @@ int fetch_record(sw2dg_ctx *c, AssimState *a, std::vector<double> &T, std::vecto
 // buffers; store = keep what the reverse sweep reads (balanced states)
 int replay_segment(sw2dg_ctx *c, AssimState *a, int64_t k, const std::vector<int> &trig,
                    bool with_cost, const double *start, bool store = false) {
+    a->seg_blk = -1;  // the segment buffers, never a resident block
     const int64_t S = a->stride;
     const int64_t n0 = k * S, n1 = std::min(a->n_steps, n0 + S);
     HIPCHK(c, hipMemcpyAsync(seg_state(a, c, 0), start ? start : ckpt_state(a, c, k),
@@ -1547,11 +1609,15 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     return SW2DG_OK;
 }
 
-int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
-                       int64_t *n_steps) {
+// the dry run; keep = it precedes a reverse sweep at the same controls: keep
+// the resident trajectory when it fits
+static int record_run(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
+                      int64_t *n_steps, bool keep) {
     if (n_steps) *n_steps = 0;
     if (!c || !c->assim) return SW2DG_ERR_ARG;
     AssimState *a = c->assim;
+    a->seg_blk = -1;
+    a->res_kept.clear();
     HIPCHK(c, hipSetDevice(c->device));
     int st = upload_ctl(c, a, phi, K1, K2, /*new_bounds=*/true);
     if (st) return st;
@@ -1577,16 +1643,22 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
     int64_t k = 0;
     double tk = c->t0;
     while (n < a->max_steps && tk < c->prm.t_max) {
-        // one segment: ping-pong in the segment buffer, then checkpoint its end
+        // one segment: ping-pong in the segment buffer (or along its resident
+        // block, keeping what the reverse sweep reads), then checkpoint its end
+        const bool res = keep && resident_block(c, a, k) != nullptr;
+        a->seg_blk = res ? k : -1;
+        a->res_kept.push_back(res ? 1 : 0);
         HIPCHK(c, hipMemcpyAsync(seg_state(a, c, 0), ckpt_state(a, c, k), 3 * nc * sizeof(double),
                                  hipMemcpyDeviceToDevice, c->stream));
         const int64_t n1 = std::min(a->max_steps, n + S);
         for (int64_t m2 = n; m2 < n1; ++m2) {
             const int j = (int)(m2 - n);
-            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true, nullptr);
+            launch_e_step(c, a, seg_state(a, c, j), seg_state(a, c, j + 1), m2, true,
+                          res ? seg_flux(a, c, j) : nullptr, res ? seg_pre(a, c, j) : nullptr);
         }
         HIPCHK(c, hipMemcpyAsync(ckpt_state(a, c, k + 1), seg_state(a, c, (int)(n1 - n)),
                                  3 * nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        a->seg_blk = -1;
         HIPCHK(c, hipMemcpyAsync(&tk, a->d_T + n1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         n = n1;
@@ -1599,7 +1671,28 @@ int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const 
     while (cnt < n && T[cnt] < c->prm.t_max) ++cnt;
     a->n_steps = cnt;
     a->recorded = true;
+    a->seg_blk = -1;
     if (n_steps) *n_steps = cnt;  // < t_max reached when max_steps cut the window
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_record(sw2dg_ctx *c, const double *phi, const double *K1, const double *K2,
+                       int64_t *n_steps) {
+    return record_run(c, phi, K1, K2, n_steps, false);
+}
+
+int sw2dg_assim_set_resident(sw2dg_ctx *c, int32_t mode) {
+    if (!c || !c->assim || mode < 0 || mode > 1) return SW2DG_ERR_ARG;
+    c->assim->resident_mode = mode;
+    c->assim->res_kept.clear();
+    return SW2DG_OK;
+}
+
+int sw2dg_assim_resident(sw2dg_ctx *c, int32_t *active) {
+    if (!c || !c->assim || !active) return SW2DG_ERR_ARG;
+    int32_t n = 0;
+    for (char v : c->assim->res_kept) n += v ? 1 : 0;
+    *active = n;
     return SW2DG_OK;
 }
 
@@ -1672,7 +1765,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     AssimState *a = c->assim;
     HIPCHK(c, hipSetDevice(c->device));
     int64_t nst = 0;
-    int st = sw2dg_assim_record(c, phi, K1, K2, &nst);
+    int st = record_run(c, phi, K1, K2, &nst, true);
     if (st) return st;
     std::vector<double> T, dt;
     std::vector<int> trig;
@@ -1756,9 +1849,16 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             if ((st = snapshot((k + 1) * a->stride))) return st;
             wm_kept = k + 1;
         }
-        // forward recompute of segment k from its checkpoint (with its costs)
-        if ((st = replay_segment(c, a, k, trig, true, nullptr, true))) return st;
+        // forward recompute of segment k from its checkpoint (with its costs);
+        // resident: the dry run kept it, only the costs are taken
         const int64_t n0 = k * a->stride, n1 = std::min(a->n_steps, n0 + a->stride);
+        if (k < (int64_t)a->res_kept.size() && a->res_kept[k]) {
+            a->seg_blk = k;
+            for (int64_t n = n0; n < n1; ++n)
+                if (trig[n] >= 0) launch_cost(c, a, seg_state(a, c, (int)(n - n0) + 1), trig[n]);
+        } else if ((st = replay_segment(c, a, k, trig, true, nullptr, true))) {
+            return st;
+        }
         {
             const int64_t nt = n1 - 1;
             const SeedArgs sd = seed_args(c, a, trig[nt], seg_state(a, c, (int)(n1 - n0)));
@@ -1816,6 +1916,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             break;
         }
     }
+    a->seg_blk = -1;
     KLAUNCH(a, k_cost_total, dim3(1), dim3(64), 0, c->stream, a->d_J, a->n_obs);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
