@@ -110,13 +110,12 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
         # neighbours' slope terms, MUSCL)
         "k_r_gc<true>": 216 * nc + (88 if muscl else 72) * ni,
         "k_r_gc<false>": 80 * nc + (88 if muscl else 72) * ni,
-        # boundary cells: state, statics, lams, part, gphi; edges geometry, F, gbce
-        "k_r_bnd<false>": 160 * nb,
-        "k_r_bnd<true>": 160 * nb,
-        # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, length; sides out
-        "k_r_edge<false>": 80 * nc + 96 * ni,
+        # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, length; sides
+        # out; the boundary blocks: boundary cells' state, statics, lams, part,
+        # gphi, boundary edges' geometry, F, gbce
+        "k_r_edge<false>": 80 * nc + 96 * ni + 160 * nb,
         # + cell slopes and centroids, edge midpoints, slope adjoints out
-        "k_r_edge<true>": 112 * nc + 144 * ni,
+        "k_r_edge<true>": 112 * nc + 144 * ni + 160 * nb,
         # cells U(h), zb, centroid, ptr, part(h) in/out; incidences, slope
         # adjoints in, neighbour terms out, edge cells and midpoints
         "k_r_slopes": 52 * nc + 80 * ni,

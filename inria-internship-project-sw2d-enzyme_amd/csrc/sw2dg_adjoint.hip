@@ -201,13 +201,25 @@ __device__ __forceinline__ double smax_(double a, double b) { return (a < b) ? b
 // _E forward kernels
 // ---------------------------------------------------------------------------
 
-// interpolateTimeSeries_E (assim/swSpE.h:138-153)
-__device__ double interp_e(const double *tt, const double *vv, int n, double t) {
+// interpolateTimeSeries_E (assim/swSpE.h:138-153).  The bracket is the
+// first it with !(t > tt[it + 1]); on a non-decreasing table that predicate
+// is monotone and bisection finds the same it as the reference's scan
+// (sorted = 0: the scan itself)
+__device__ double interp_e(const double *tt, const double *vv, int n, double t, int sorted) {
     if (n == 0) return 0.0;
     if (n == 1 || t <= tt[0] - 1.0e-12) return vv[0];
     if (t >= tt[n - 1]) return vv[n - 1];
     int it = 0;
-    while (it < n - 1 && t > tt[it + 1]) it++;
+    if (sorted) {
+        int hi = n - 1;
+        while (it < hi) {
+            const int mid = (it + hi) >> 1;
+            if (t > tt[mid + 1]) it = mid + 1;
+            else hi = mid;
+        }
+    } else {
+        while (it < n - 1 && t > tt[it + 1]) it++;
+    }
     const double t1 = tt[it], t2 = tt[it + 1], v1 = vv[it], v2 = vv[it + 1];
     const double den = t2 - t1;
     if (den < 1e-12) return v1;
@@ -318,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_e_edges_muscl(MeshDev m, Opts o, Con
         const int i = m.bcell[j];
         const spe::Side sd = e_bnd_state<true>(m, o, U, i, j);
         const int ser = m.bser[j];
-        const double v = interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n]);
+        const double v = interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n], bc.sorted);
         double f[3];
         spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], cs.phi[i], m.zb[i], sd.z, sd.q, sd.r, sd.lp,
                      bc.type[ser], v, f);
@@ -459,6 +471,10 @@ __global__ __launch_bounds__(kDtBlock) void k_e_dt(BcDev bc, EClock *ck, const d
     v = wave_min(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) red[wid] = v;
+    // the step's boundary values depend on t alone: one thread per series
+    for (int k = (int)threadIdx.x; k < ck->n_series; k += kDtBlock)
+        bcv[n * ck->n_series + k] =
+            interp_e(bc.t, bc.v + (size_t)k * bc.n_times, bc.n_times, T[n], bc.sorted);
     __syncthreads();
     if (threadIdx.x != 0) return;
     double dtc = red[0];
@@ -476,8 +492,6 @@ __global__ __launch_bounds__(kDtBlock) void k_e_dt(BcDev bc, EClock *ck, const d
     double dt = (dtc < target) ? dtc : target;
     if ((t + dt) > t_max) dt = t_max - t;
     if (dt < 1e-6) dt = 1e-6;
-    for (int k = 0; k < ck->n_series; ++k)
-        bcv[n * ck->n_series + k] = interp_e(bc.t, bc.v + (size_t)k * bc.n_times, bc.n_times, t);
     const double tn = t + dt;
     while (oi < ck->n_obs && tn > obs_t[oi] + 1e-6) oi++;
     int tr = -1;
@@ -778,7 +792,7 @@ __device__ __forceinline__ void seed_cell(const SeedArgs &sd, int nc, int i, dou
 // the porosity derivative of its internal-edge factors.  (ah, ax, ay) in =
 // adjoint of U^{n+1}; out: lams = adjoint of the balanced state, part = the
 // cell-local part of the adjoint of U^n (boundary cells: completed by
-// k_r_bnd).  P = [balanced state | internal-edge increments] from the replay.
+// the boundary blocks of k_r_edge).  P = [balanced state | internal-edge increments] from the replay.
 __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, const Controls &cs,
                                              const double *__restrict__ P, double dt, int i,
                                              double ah, double ax, double ay,
@@ -919,19 +933,26 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
     gphi[i] = g;
 }
 
-// per boundary cell: reverse of its boundary fluxes (hypBoundaryFlux_E and
-// the Godunov or MUSCL-EVR _E boundary states) and of their factors, in reference order, added to
-// the cell part of the adjoint; boundary-value adjoint per edge
+// the boundary part of the edge reverse of a step: per boundary cell, the
+// reverse of its boundary fluxes (hypBoundaryFlux_E and the Godunov or
+// MUSCL-EVR _E boundary states) and of their factors, in reference order,
+// added to the cell part of the adjoint; boundary-value adjoint per edge
+struct BndRev {
+    BcDev bc;
+    const double *F;    // the step's boundary fluxes, 3 per boundary edge
+    const double *bcv;  // the step's boundary values per series
+    double *part, *gphi, *gbce;
+    int n_edge_blocks;  // blocks of k_r_edge before the boundary ones
+};
+
 template <bool MUSCL>
-__global__ __launch_bounds__(kBlock) void k_r_bnd(MeshDev m, Opts o, Controls cs, BcDev bc,
-                                                  const double *__restrict__ U,
-                                                  const double *__restrict__ F, double dt,
-                                                  const double *__restrict__ bcv,
-                                                  const double *__restrict__ lams,
-                                                  double *__restrict__ part, double *gphi,
-                                                  double *gbce) {
-    const int t = blockIdx.x * kBlock + (int)threadIdx.x;
+__device__ __forceinline__ void bnd_reverse(const MeshDev &m, const Opts &o, const Controls &cs,
+                                            const BndRev &br, const double *__restrict__ U, double dt,
+                                            const double *__restrict__ lams, int t) {
     if (t >= m.n_bnd_cells) return;
+    const BcDev &bc = br.bc;
+    const double *F = br.F, *bcv = br.bcv;
+    double *part = br.part, *gphi = br.gphi, *gbce = br.gbce;
     const int i = m.bnd_cells[t];
     const int nc = m.n_cell;
     const double ah = lams[i], ax = lams[nc + i], ay = lams[2 * nc + i];
@@ -981,15 +1002,22 @@ __device__ __forceinline__ double fresh(double x) {
 // reconstruction of each side from the step's slopes sxy with the step's dt,
 // whose slope adjoints go to sd[4e ..] = {L: dsx, dsy, R: dsx, dsy}
 // register budgets: 4 waves per SIMD (128 VGPRs) fit the Godunov reverse
-// without spills, the MUSCL-EVR one needs 144 (3 waves)
+// without spills, the MUSCL-EVR one needs 144 (3 waves).  Blocks past
+// br.n_edge_blocks run the boundary reverse of the step (a few blocks: the
+// spills its larger register need takes under this budget stay there)
 template <bool MUSCL>
 __global__ __launch_bounds__(kBlock, MUSCL ? 3 : 4) void k_r_edge(MeshDev m, Opts o, Controls cs,
                                                    const double *__restrict__ U, double dt,
                                                    const double *__restrict__ lams,
                                                    double *__restrict__ side_out,
                                                    const double *__restrict__ sxy,
-                                                   double *__restrict__ sd) {
-    const int e = xcd_block() * kBlock + (int)threadIdx.x;
+                                                   double *__restrict__ sd, BndRev br) {
+    if ((int)blockIdx.x >= br.n_edge_blocks) {
+        bnd_reverse<MUSCL>(m, o, cs, br, U, dt, lams,
+                           ((int)blockIdx.x - br.n_edge_blocks) * kBlock + (int)threadIdx.x);
+        return;
+    }
+    const int e = xcd_remap((int)blockIdx.x, br.n_edge_blocks) * kBlock + (int)threadIdx.x;
     if (e >= m.n_int) return;
     const int nc = m.n_cell;
     const int L = m.eL[e], R = m.eR[e];
@@ -1802,27 +1830,21 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
                     ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn, a->d_mirror);
     };
-    auto bnd = [&](int64_t n, int j) {
-        if (c->m.n_bnd_cells <= 0) return;
-        if (muscl)
-            KLAUNCH(a, k_r_bnd<true>, dim3(nblocks(c->m.n_bnd_cells)), blk, 0, c->stream, c->m, c->o,
-                    cs, c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
-                    a->d_lams, a->d_part, a->d_gphi, a->d_gbce);
-        else
-            KLAUNCH(a, k_r_bnd<false>, dim3(nblocks(c->m.n_bnd_cells)), blk, 0, c->stream, c->m, c->o,
-                    cs, c->bc, seg_state(a, c, j), seg_flux(a, c, j), dt[n], a->d_bc + (size_t)n * ns,
-                    a->d_lams, a->d_part, a->d_gphi, a->d_gbce);
-    };
     // edge reverse of step n (segment slot j); MUSCL-EVR: then the slopes
+    // edge reverse of step n (with its boundary part in extra blocks)
     auto edges = [&](int64_t n, int j) {
+        const int neb = nblocks(c->ni);
+        const int nbb = c->m.n_bnd_cells > 0 ? nblocks(c->m.n_bnd_cells) : 0;
+        const BndRev br{c->bc, seg_flux(a, c, j), a->d_bc + (size_t)n * ns, a->d_part, a->d_gphi,
+                        a->d_gbce, neb};
         if (muscl) {
-            KLAUNCH(a, k_r_edge<true>, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs,
-                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side, seg_slopes(a, c, j), a->d_sd);
+            KLAUNCH(a, k_r_edge<true>, dim3(neb + nbb), blk, 0, c->stream, c->m, c->o, cs,
+                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side, seg_slopes(a, c, j), a->d_sd, br);
             KLAUNCH(a, k_r_slopes, dim3(ncb), blk, 0, c->stream, c->m, seg_state(a, c, j), a->d_sd,
                     a->d_part, a->d_zn);
         } else {
-            KLAUNCH(a, k_r_edge<false>, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->o, cs,
-                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side, nullptr, nullptr);
+            KLAUNCH(a, k_r_edge<false>, dim3(neb + nbb), blk, 0, c->stream, c->m, c->o, cs,
+                    seg_state(a, c, j), dt[n], a->d_lams, a->d_side, nullptr, nullptr, br);
         }
     };
     // explosion guard (swSpE_checkpoint.cpp:1035-1117): ||adjoint of h|| at
@@ -1867,7 +1889,6 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             gc(true, (!started || p.tile_top[k]) ? -1 : 2, -1, sd, seg_pre(a, c, (int)(nt - n0)),
                dt[nt]);
             started = true;
-            bnd(nt, (int)(nt - n0));
         }
         for (int64_t n = n1 - 1; n >= n0; --n) {
             const int j = (int)(n - n0);
@@ -1875,7 +1896,6 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             if (n > n0) {
                 const SeedArgs sd = seed_args(c, a, trig[n - 1], seg_state(a, c, j));
                 gc(true, 1, n, sd, seg_pre(a, c, j - 1), dt[n - 1]);
-                bnd(n - 1, j - 1);
             }
         }
         // close the segment: adjoint of its start state (no seed: an

@@ -1130,6 +1130,7 @@ int sw2dg_set_bc_series(sw2dg_ctx *c, int32_t n_series, int32_t n_times, const i
         HIPCHK(c, hipMemcpy((void *)c->bc.v, v, (size_t)n_series * n_times * sizeof(double),
                             hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy((void *)c->bc.type, type, n_series * sizeof(int), hipMemcpyHostToDevice));
+        c->bc.sorted = std::is_sorted(t, t + n_times) ? 1 : 0;
         c->series_t_host.assign(t, t + n_times);
         c->series_v_host.assign(v, v + (size_t)n_series * n_times);
         c->series_type_host.assign(type, type + n_series);
@@ -1157,6 +1158,7 @@ int sw2dg_set_bc_series(sw2dg_ctx *c, int32_t n_series, int32_t n_times, const i
     c->bc.rank = c->dalloc<int>(ns);
     c->bc.atend = c->dalloc<int>(ns);
     c->bc.n_times = nt;
+    c->bc.sorted = std::is_sorted(tv.begin(), tv.end()) ? 1 : 0;
     c->hck.n_series = n_series;
     if (c->gexec) {  // arguments changed: rebuild the graph
         hipGraphExecDestroy(c->gexec);

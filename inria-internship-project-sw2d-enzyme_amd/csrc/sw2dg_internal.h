@@ -44,6 +44,7 @@ struct BcDev {
     double *vinf, *vsup, *tinf, *tsup, *val;
     int *rank, *atend;
     int n_times;
+    int sorted;          // t is non-decreasing: the _E interpolation may bisect
 };
 
 struct MeshDev {
