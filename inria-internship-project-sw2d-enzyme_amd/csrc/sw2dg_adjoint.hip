@@ -942,7 +942,7 @@ struct BndRev {
     const double *F;    // the step's boundary fluxes, 3 per boundary edge
     const double *bcv;  // the step's boundary values per series
     double *part, *gphi, *gbce;
-    int n_edge_blocks;  // blocks of k_r_edge before the boundary ones
+    int n_bnd_blocks;   // leading blocks of k_r_edge that run it (a multiple of 8)
 };
 
 template <bool MUSCL>
@@ -1002,9 +1002,11 @@ __device__ __forceinline__ double fresh(double x) {
 // reconstruction of each side from the step's slopes sxy with the step's dt,
 // whose slope adjoints go to sd[4e ..] = {L: dsx, dsy, R: dsx, dsy}
 // register budgets: 4 waves per SIMD (128 VGPRs) fit the Godunov reverse
-// without spills, the MUSCL-EVR one needs 144 (3 waves).  Blocks past
-// br.n_edge_blocks run the boundary reverse of the step (a few blocks: the
-// spills its larger register need takes under this budget stay there)
+// without spills, the MUSCL-EVR one needs 144 (3 waves).  The first
+// br.n_bnd_blocks blocks run the boundary reverse of the step, dispatched
+// first so that its latency overlaps the edges (a few blocks: the spills its
+// larger register need takes under this budget stay there); a multiple of 8,
+// so that the edge blocks keep their XCD-aware order
 template <bool MUSCL>
 __global__ __launch_bounds__(kBlock, MUSCL ? 3 : 4) void k_r_edge(MeshDev m, Opts o, Controls cs,
                                                    const double *__restrict__ U, double dt,
@@ -1012,12 +1014,12 @@ __global__ __launch_bounds__(kBlock, MUSCL ? 3 : 4) void k_r_edge(MeshDev m, Opt
                                                    double *__restrict__ side_out,
                                                    const double *__restrict__ sxy,
                                                    double *__restrict__ sd, BndRev br) {
-    if ((int)blockIdx.x >= br.n_edge_blocks) {
-        bnd_reverse<MUSCL>(m, o, cs, br, U, dt, lams,
-                           ((int)blockIdx.x - br.n_edge_blocks) * kBlock + (int)threadIdx.x);
+    if ((int)blockIdx.x < br.n_bnd_blocks) {
+        bnd_reverse<MUSCL>(m, o, cs, br, U, dt, lams, (int)blockIdx.x * kBlock + (int)threadIdx.x);
         return;
     }
-    const int e = xcd_remap((int)blockIdx.x, br.n_edge_blocks) * kBlock + (int)threadIdx.x;
+    const int e = xcd_remap((int)blockIdx.x - br.n_bnd_blocks, (int)gridDim.x - br.n_bnd_blocks) * kBlock +
+                  (int)threadIdx.x;
     if (e >= m.n_int) return;
     const int nc = m.n_cell;
     const int L = m.eL[e], R = m.eR[e];
@@ -1834,9 +1836,9 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
     // edge reverse of step n (with its boundary part in extra blocks)
     auto edges = [&](int64_t n, int j) {
         const int neb = nblocks(c->ni);
-        const int nbb = c->m.n_bnd_cells > 0 ? nblocks(c->m.n_bnd_cells) : 0;
+        const int nbb = c->m.n_bnd_cells > 0 ? (nblocks(c->m.n_bnd_cells) + 7) / 8 * 8 : 0;
         const BndRev br{c->bc, seg_flux(a, c, j), a->d_bc + (size_t)n * ns, a->d_part, a->d_gphi,
-                        a->d_gbce, neb};
+                        a->d_gbce, nbb};
         if (muscl) {
             KLAUNCH(a, k_r_edge<true>, dim3(neb + nbb), blk, 0, c->stream, c->m, c->o, cs,
                     seg_state(a, c, j), dt[n], a->d_lams, a->d_side, seg_slopes(a, c, j), a->d_sd, br);
