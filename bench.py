@@ -251,7 +251,12 @@ def forward_bench(sw, args):
     step_ms = sum(prof.values())
     s.close()
     log(f"native MUSCL forward: {n} steps in {el:.3f} s")
+    cpu = None
+    if not args.no_cpu:
+        log("native forward: CPU baselines (OpenMP and serial oracle)")
+        cpu = cpu_forward_baseline(case, args)
     return {"workload": "SP forward at scale, MUSCL-EVR + Manning 0.03, native semantics",
+            "cpu_baseline": cpu,
             "value": case.n_cell * n / el, "unit": "cell-steps/s", "steps": n,
             "ms_per_step": el * 1e3 / n, "kernels_ms": prof,
             "roofline_step": {"algorithmic_bytes": Bf, "device_ms": step_ms,
@@ -259,32 +264,86 @@ def forward_bench(sw, args):
                               "frac": Bf / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
-def cpu_baseline(res, args):
-    """The oracle's _E dry run + frozen replay + reverse sweep on the first
-    cpu_steps time steps of the same workload, one host thread."""
+def host_info():
+    """The host the CPU baselines ran on: logical CPUs available to this
+    process and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return {"nproc": n, "cpu_model": model}
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
-    case = res["case"]
-    cal = res["cal"]
-    E = O.OracleE(case)
-    m = args.cpu_steps
-    t_m = float(res["rec"]["t"][m])
-    K0 = res["K0"]
-    obs = res["obs"][:1]
-    tc = time.perf_counter()
-    rec = E.record([t_m], K1=K0, K2=K0, max_steps=m)
-    g = E.gradient(rec, K1=K0, K2=K0, gauges=cal.gauge_cells, obs=obs, n_obs=1)
-    tc = time.perf_counter() - tc
-    n = len(rec["dt"])
+    return O
+
+
+def cpu_baseline(res, args):
+    """The oracle's _E dry run + reverse sweep (all states kept, no
+    recomputation) on a prefix of the same gradient: the OpenMP build on
+    every host thread it is given (OMP_NUM_THREADS), and the serial build
+    (1 thread, the reference is single-threaded) on a shorter prefix."""
+    O = _oracle()
+    case, cal, K0 = res["case"], res["cal"], res["K0"]
+
+    def leg(math, m):
+        E = O.OracleE(case, math=math)
+        t_m = float(res["rec"]["t"][m])
+        tc = time.perf_counter()
+        rec = E.record([t_m], K1=K0, K2=K0, max_steps=m)
+        g = E.gradient(rec, K1=K0, K2=K0, gauges=cal.gauge_cells, obs=res["obs"][:1], n_obs=1)
+        tc = time.perf_counter() - tc
+        n = len(rec["dt"])
+        return case.n_cell * n / tc, n, tc, g["J"]
+
+    threads = O.lib("omp").orc_threads()
+    v, n, tc, J = leg("omp", args.cpu_steps_omp)
+    v1, n1, tc1, J1 = leg("glibc", args.cpu_steps)
     return {
-        "value": case.n_cell * n / tc,
+        "value": v,
         "unit": "fwd+adj cell-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"first {n} time steps of the same {case.n_cell}-cell gradient (C oracle: _E dry run "
-                  f"+ replay + reverse sweep, all states kept, no recomputation), 1 thread, "
-                  f"{tc:.1f} s; J = {g['J']:.6e}",
+        "sample": f"first {n} time steps of the same {case.n_cell}-cell gradient (C oracle, OpenMP "
+                  f"build: _E dry run + reverse sweep, all states kept, no recomputation), "
+                  f"{threads} threads, {tc:.1f} s; J = {J:.6e}",
+        "host": host_info(),
+        "single_thread": {"value": v1, "cores": 1, "sample": f"first {n1} time steps, serial build, "
+                                                             f"{tc1:.1f} s; J = {J1:.6e}"},
     }
+
+
+def cpu_forward_baseline(case, args):
+    """The oracle's native forward (the GPU line's workload) on a bounded
+    number of steps: OpenMP build on every host thread, serial build on 1."""
+    O = _oracle()
+
+    def leg(math, m):
+        o = O.OracleSolver(case, math=math)
+        tc = time.perf_counter()
+        o.step(m)
+        return case.n_cell * m / (time.perf_counter() - tc), time.perf_counter() - tc
+
+    threads = O.lib("omp").orc_threads()
+    v, tc = leg("omp", args.cpu_fwd_steps_omp)
+    v1, tc1 = leg("glibc", args.cpu_fwd_steps)
+    return {"value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {args.cpu_fwd_steps_omp} steps of the same {case.n_cell}-cell native "
+                      f"MUSCL-EVR forward (C oracle, OpenMP build), {threads} threads, {tc:.1f} s",
+            "host": host_info(),
+            "single_thread": {"value": v1, "cores": 1,
+                              "sample": f"first {args.cpu_fwd_steps} steps, serial build, {tc1:.1f} s"}}
 
 
 def time_to_misfit_bench(sw, args):
@@ -466,7 +525,14 @@ def main():
     ap.add_argument("--window", type=float, default=90.0, help="assimilation window (s)")
     ap.add_argument("--max-steps", type=int, default=2000)
     ap.add_argument("--fwd-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="time steps of the serial CPU fwd+adj baseline")
+    ap.add_argument("--cpu-steps-omp", type=int, default=400,
+                    help="time steps of the OpenMP CPU fwd+adj baseline")
+    ap.add_argument("--cpu-fwd-steps", type=int, default=60,
+                    help="steps of the serial CPU native-forward baseline")
+    ap.add_argument("--cpu-fwd-steps-omp", type=int, default=1000,
+                    help="steps of the OpenMP CPU native-forward baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forward", action="store_true")
     ap.add_argument("--scheme", choices=("muscl_evr", "godunov"), default="godunov",

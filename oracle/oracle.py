@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 # the same restatement built with the kernels' exp / log / cbrt (orc_math.h)
 LIB_DM = os.path.join(HERE, "_build", "liboracle_dm.so")
-MATHS = ("glibc", "device")
+# the glibc restatement's OpenMP build (orc_omp.h): the all-core CPU baseline
+LIB_OMP = os.path.join(HERE, "_build", "liboracle_omp.so")
+MATHS = ("glibc", "device", "omp")
 REF_SOLVER = os.path.join(HERE, "_ref", "sw2dSolver")
 
 _d = C.POINTER(C.c_double)
@@ -51,12 +53,13 @@ _libs = {}
 
 
 def lib(math="glibc"):
-    """The oracle library: math "glibc" (the reference's exp / log / cbrt) or
-    "device" (the kernels' own, every call audited against glibc)."""
+    """The oracle library: math "glibc" (the reference's exp / log / cbrt),
+    "device" (the kernels' own, every call audited against glibc) or "omp"
+    (glibc's, the OpenMP build: OMP_NUM_THREADS threads)."""
     if math not in MATHS:
         raise ValueError(f"math must be one of {MATHS}")
     if math not in _libs:
-        path = LIB if math == "glibc" else LIB_DM
+        path = {"glibc": LIB, "device": LIB_DM, "omp": LIB_OMP}[math]
         if not os.path.exists(path):
             subprocess.check_call(["make", "-C", HERE], stdout=subprocess.DEVNULL)
         L = C.CDLL(path)
@@ -78,6 +81,7 @@ def lib(math="glibc"):
         L.orc_bnd_flux.argtypes = [C.c_double] * 8 + [C.c_int, C.c_double, _d, C.POINTER(C.c_int)]
         L.orc_interp_series.argtypes = [_d, _d, C.c_int64, C.c_double]
         L.orc_interp_series.restype = C.c_double
+        L.orc_threads.restype = C.c_int
         L.orc_friction_factor.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_friction_factor.restype = C.c_double
         L.orc_math_audit_reset.argtypes = []

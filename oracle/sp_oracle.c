@@ -9,6 +9,7 @@
  */
 #include "sp_oracle.h"
 #include "orc_math.h"
+#include "orc_omp.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -53,6 +54,7 @@ struct orc_solver {
     double *rain_now;   /* rain rate per station at t + dt/2 */
     double *pfac, *pcoef, *pflux;  /* momentum diffusion per internal edge */
     double t, dt;
+    orc_inc inc; /* per-cell incidences (the OpenMP build's gathers) */
 };
 
 /* swSp::lambdaMinus / lambdaPlus -- swSp.cpp:972-995 */
@@ -333,6 +335,7 @@ static void edge_factors(orc_solver *s) {
  * swSpHypMusclEvr.cpp:144-162) */
 static void lr_bnd(orc_solver *s) {
     const orc_case *c = &s->c;
+    ORC_PAR
     for (int64_t e = 0; e < c->n_bnd; e++) {
         double nx = c->bnd_nx[e], ny = c->bnd_ny[e];
         int64_t id = c->bnd_cell[e];
@@ -354,6 +357,7 @@ static void lr_bnd(orc_solver *s) {
 static void lr_godunov(orc_solver *s) {
     const orc_case *c = &s->c;
     lr_bnd(s);
+    ORC_PAR
     for (int64_t e = 0; e < c->n_int; e++) {
         double nx = c->int_nx[e], ny = c->int_ny[e];
         for (int side = 0; side < 2; side++) {
@@ -384,6 +388,7 @@ static void lr_godunov(orc_solver *s) {
 /* swSp::computeMusclSlopes -- swSpHypMusclEvr.cpp:11-115 */
 static void muscl_slopes(orc_solver *s) {
     const orc_case *c = &s->c;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_cell; i++) {
         s->dzdx[i] = 0.0;
         s->dzdy[i] = 0.0;
@@ -404,6 +409,35 @@ static void muscl_slopes(orc_solver *s) {
             s->dzdy[i] /= wsum;
         }
     }
+#ifdef ORC_OMP
+    /* the limiter of cell k only changes its own slope, in its edges' order */
+    ORC_PAR
+    for (int64_t k = 0; k < c->n_cell; k++) {
+        for (int64_t p = s->inc.iptr[k]; p < s->inc.iptr[k + 1]; p++) {
+            const int64_t e = s->inc.iinc[p] >> 1;
+            int64_t L = c->int_L[e], R = c->int_R[e];
+            double fmx = smax(s->zc[L], s->zc[R]);
+            double fmn = smin(s->zc[L], s->zc[R]);
+            double fr = s->zc[k] + (c->int_xm[e] - c->cell_x[k]) * s->dzdx[k] +
+                        (c->int_ym[e] - c->cell_y[k]) * s->dzdy[k];
+            if (fr < fmn) {
+                double dfr = fr - s->zc[k];
+                double dft = fmn - s->zc[k];
+                double sf = dft / dfr;
+                s->dzdx[k] *= sf;
+                s->dzdy[k] *= sf;
+            }
+            if (fr > fmx) {
+                double dfr = fr - s->zc[k];
+                double dft = fmx - s->zc[k];
+                double sf = dft / dfr;
+                s->dzdx[k] *= sf;
+                s->dzdy[k] *= sf;
+            }
+        }
+    }
+    return;
+#endif
     for (int64_t e = 0; e < c->n_int; e++) {
         int64_t L = c->int_L[e], R = c->int_R[e];
         double fmx = smax(s->zc[L], s->zc[R]);
@@ -434,10 +468,12 @@ static void muscl_slopes(orc_solver *s) {
  * previous step's dt (the solver member, 0 before the first step, swSp.h:155) */
 static void lr_muscl(orc_solver *s) {
     const orc_case *c = &s->c;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_cell; i++) s->zc[i] = s->h[i] + c->cell_zb[i];
     muscl_slopes(s);
     lr_bnd(s);
     double dt = s->dt;
+    ORC_PAR
     for (int64_t e = 0; e < c->n_int; e++) {
         double nx = c->int_nx[e], ny = c->int_ny[e];
         for (int side = 0; side < 2; side++) {
@@ -497,6 +533,41 @@ static void lr_muscl(orc_solver *s) {
 static double time_step(orc_solver *s) {
     const orc_case *c = &s->c;
     double dt = c->dt_max;
+#ifdef ORC_OMP
+    /* per cell: its boundary edges, then its internal edges, each ascending */
+#pragma omp parallel
+    {
+        double dtl = dt;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < c->n_cell; i++) {
+            double sw = 0.;
+            for (int64_t p = s->inc.bptr[i]; p < s->inc.bptr[i + 1]; p++) {
+                const int64_t e = s->inc.binc[p];
+                if (s->h[i] > c->h_min) sw += s->lmaxb[e] * c->bnd_len[e];
+            }
+            for (int64_t p = s->inc.iptr[i]; p < s->inc.iptr[i + 1]; p++) {
+                const int64_t e = s->inc.iinc[p] >> 1;
+                const int side = (int)(s->inc.iinc[p] & 1);
+                double lambda = 0.;
+                if (s->h[c->int_L[e]] > c->h_min) lambda = s->lmaxi[e];
+                if (side && s->h[c->int_R[e]] > c->h_min) lambda = s->lmaxi[e];
+                sw += lambda * c->int_len[e];
+            }
+            s->swept[i] = sw;
+            double x = c->cfl_max * (c->cell_area[i] / smax(sw, SMALL_REAL));
+            if (x < dtl) dtl = x;
+        }
+#pragma omp critical
+        if (dtl < dt) dt = dtl;
+    }
+    if (c->dt_map > SMALL_REAL) {
+        int now = (int)(s->t / c->dt_map);
+        int later = (int)((s->t + dt) / c->dt_map);
+        if (later > now) dt = later * c->dt_map - s->t;
+    }
+    if ((s->t + dt) > c->t_max) dt = c->t_max - s->t;
+    return dt;
+#endif
     for (int64_t i = 0; i < c->n_cell; i++) s->swept[i] = 0.;
     for (int64_t e = 0; e < c->n_bnd; e++) {
         int64_t id = c->bnd_cell[e];
@@ -527,6 +598,51 @@ static double time_step(orc_solver *s) {
 static int fluxes(orc_solver *s) {
     const orc_case *c = &s->c;
     bc_compute(s, s->t);
+#ifdef ORC_OMP
+    int err = ORC_OK;
+    ORC_PAR
+    for (int64_t e = 0; e < c->n_bnd; e++) {
+        int64_t id = c->bnd_cell[e];
+        int k = c->bnd_series[e];
+        double out[4];
+        int st = bnd_flux(c->h_min, c->beta[id], c->phi[id], c->cell_zb[id], s->zrb[e], s->qrb[e],
+                          s->rrb[e], s->lpb[e], c->series_type[k], s->bc_val[k], out);
+        if (st) {
+#pragma omp critical
+            err = err ? err : st;
+        }
+        s->F1b[e] = out[0];
+        s->F2b[e] = out[1];
+        s->F3b[e] = out[2];
+        s->S2Rb[e] = out[3];
+    }
+    if (err) return err;
+    ORC_PAR
+    for (int64_t e = 0; e < c->n_int; e++) {
+        int64_t L = c->int_L[e], R = c->int_R[e];
+        double zbL = c->cell_zb[L], zbR = c->cell_zb[R];
+        double zL = s->zri[2 * e], zR = s->zri[2 * e + 1];
+        double hL = zL - zbL, hR = zR - zbR;
+        if ((hL > c->h_min) || (hR > c->h_min)) {
+            double out[5];
+            int st = hllc(c->topo == ORC_TOPO_BLENDING, c->h_min, c->phi_min, c->beta[L], c->beta[R],
+                          c->phi[L], c->phi[R], zbL, zbR, zL, s->qri[2 * e], s->rri[2 * e], zR,
+                          s->qri[2 * e + 1], s->rri[2 * e + 1], s->lmi[2 * e], s->lmi[2 * e + 1],
+                          s->lpi[2 * e], s->lpi[2 * e + 1], out);
+            if (!st && isnan(out[0])) st = ORC_ERR_NAN;
+            if (st) {
+#pragma omp critical
+                err = err ? err : st;
+            }
+            s->F1i[e] = out[0];
+            s->F2i[e] = out[1];
+            s->F3i[e] = out[2];
+            s->S2Li[e] = out[3];
+            s->S2Ri[e] = out[4];
+        }
+    }
+    return err;
+#endif
     for (int64_t e = 0; e < c->n_bnd; e++) {
         int64_t id = c->bnd_cell[e];
         int k = c->bnd_series[e];
@@ -566,6 +682,33 @@ static int fluxes(orc_solver *s) {
 static void balance(orc_solver *s, double dt) {
     const orc_case *c = &s->c;
     double *h = s->h, *qx = s->qx, *qy = s->qy;
+#ifdef ORC_OMP
+    ORC_PAR
+    for (int64_t i = 0; i < c->n_cell; i++) {
+        for (int64_t p = s->inc.bptr[i]; p < s->inc.bptr[i + 1]; p++) {
+            const int64_t e = s->inc.binc[p];
+            h[i] += s->bf1[e] * s->F1b[e] * dt;
+            qx[i] += (s->bf2F2[e] * s->F2b[e] + s->bf2S2[e] * s->S2Rb[e] + s->bf2F3[e] * s->F3b[e]) * dt;
+            qy[i] += (s->bf3F2[e] * s->F2b[e] + s->bf3S2[e] * s->S2Rb[e] + s->bf3F3[e] * s->F3b[e]) * dt;
+        }
+        for (int64_t p = s->inc.iptr[i]; p < s->inc.iptr[i + 1]; p++) {
+            const int64_t e = s->inc.iinc[p] >> 1;
+            if (s->inc.iinc[p] & 1) {
+                h[i] += s->fR1[e] * s->F1i[e] * dt;
+                qx[i] += (s->f2F2R[e] * s->F2i[e] + s->f2S2R[e] * s->S2Ri[e] + s->f2F3R[e] * s->F3i[e]) * dt;
+                qy[i] += (s->f3F2R[e] * s->F2i[e] + s->f3S2R[e] * s->S2Ri[e] + s->f3F3R[e] * s->F3i[e]) * dt;
+            } else {
+                h[i] += s->fL1[e] * s->F1i[e] * dt;
+                qx[i] += (s->f2F2L[e] * s->F2i[e] + s->f2S2L[e] * s->S2Li[e] + s->f2F3L[e] * s->F3i[e]) * dt;
+                qy[i] += (s->f3F2L[e] * s->F2i[e] + s->f3S2L[e] * s->S2Li[e] + s->f3F3L[e] * s->F3i[e]) * dt;
+            }
+        }
+        if (fabs(h[i]) < SMALL_REAL) h[i] = 0.;
+        if (fabs(qx[i]) < SMALL_REAL) qx[i] = 0.;
+        if (fabs(qy[i]) < SMALL_REAL) qy[i] = 0.;
+    }
+    return;
+#endif
     for (int64_t e = 0; e < c->n_bnd; e++) {
         int64_t id = c->bnd_cell[e];
         h[id] += s->bf1[e] * s->F1b[e] * dt;
@@ -591,6 +734,7 @@ static void balance(orc_solver *s, double dt) {
 /* swSp::sourceFriction{Manning,Strickler,Chezy} -- swSpSourceFriction.cpp:39-171 */
 static void source_friction(orc_solver *s, double dt) {
     const orc_case *c = &s->c;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_cell; i++) {
         double h = s->h[i];
         if (h > smax(c->h_min, SMALL_REAL)) {
@@ -774,6 +918,7 @@ static void source_wind(orc_solver *s, double dt) {
 /* swSp::correctVelocity -- swSp.cpp:447-479 */
 static void correct_velocity(orc_solver *s) {
     const orc_case *c = &s->c;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_cell; i++) {
         double h = s->h[i];
         if (h > c->h_min) {
@@ -804,14 +949,22 @@ int orc_step(orc_solver *s) {
     int st = fluxes(s);
     if (st) return st;
     balance(s, dt);
+    int neg = 0;
+#ifdef ORC_OMP
+#pragma omp parallel for schedule(static) reduction(| : neg)
+#endif
     for (int64_t n = 0; n < c->n_cell; ++n) {
         if (s->h[n] < 0.) {
-            if (c->neg_depth_detection) return ORC_ERR_NEG_DEPTH;
+            if (c->neg_depth_detection) {
+                neg = 1;
+                continue;
+            }
             s->h[n] = 0.;
             s->qx[n] = 0.;
             s->qy[n] = 0.;
         }
     }
+    if (neg) return ORC_ERR_NEG_DEPTH;
     if (c->momentum_diffusion) source_parab(s, dt);
     if (c->friction != ORC_FRIC_NONE) source_friction(s, dt);
     if (c->head_loss) source_head_loss(s, dt);
@@ -996,6 +1149,9 @@ orc_solver *orc_create(const orc_case *c, const double *h0, const double *qx0, c
         s->nb_idx[s->nb_ptr[R] + fill[R]++] = L;
     }
     free(fill);
+#ifdef ORC_OMP
+    orc_inc_build(&s->inc, nc, ni, c->n_bnd, c->int_L, c->int_R, c->bnd_cell);
+#endif
 
     /* friction angle: degrees -> radians, then cos/sin -- swSp.cpp:1062-1070 */
     if (c->friction != ORC_FRIC_NONE && c->fric_alpha_deg) {
@@ -1037,7 +1193,19 @@ void orc_destroy(orc_solver *s) {
         (void *)s->c.rain_t, (void *)s->c.rain_v, (void *)s->c.runoff_coeff,
         (void *)s->c.infil_rate, s->rain_now, s->pfac, s->pcoef, s->pflux};
     for (size_t i = 0; i < sizeof(owned) / sizeof(owned[0]); i++) free(owned[i]);
+#ifdef ORC_OMP
+    orc_inc_free(&s->inc);
+#endif
     free(s);
+}
+
+/* threads of the OpenMP build (1 in the serial one) */
+int orc_threads(void) {
+#ifdef ORC_OMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 void orc_get_state(const orc_solver *s, double *h, double *qx, double *qy) {

@@ -91,6 +91,8 @@ void orc_destroy(orc_solver *s);
 int orc_step(orc_solver *s);
 /* Run while t < t_end (and t < t_max), at most max_steps steps.  *n_done gets the count. */
 int orc_run(orc_solver *s, double t_end, int64_t max_steps, int64_t *n_done);
+/* threads of the OpenMP build (liboracle_omp.so); 1 in the serial ones */
+int orc_threads(void);
 void orc_get_state(const orc_solver *s, double *h, double *qx, double *qy);
 void orc_set_state(orc_solver *s, const double *h, const double *qx, const double *qy);
 double orc_time(const orc_solver *s);

@@ -21,6 +21,7 @@
  * and full-frame misfit (swSpE_ad.cpp:273-308), optional normalisation by the
  * number of observations.
  */
+#include "orc_omp.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -47,6 +48,7 @@ typedef struct {
     double *zc, *dzdx, *dzdy;
     int64_t *nb_ptr, *nb_idx;
     double dt_recon;
+    orc_inc inc; /* per-cell incidences (the OpenMP build's gathers) */
 } esolver;
 
 static double lam_m(double b, double c, double u) { return b * u - sqrt((b - 1.0) * b * u * u + c * c); }
@@ -71,11 +73,13 @@ double orc_e_interp(const double *tt, const double *vv, int n, double t) {
 /* computeEdgeFactors_E -- swSpE_hyp.cpp:277-333 */
 static void factors(esolver *s) {
     const orc_case *c = s->c;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_bnd; i++) {
         int64_t k = c->bnd_cell[i];
         double ph = (s->phi[k] > c->phi_min) ? s->phi[k] : c->phi_min;
         s->fac_b[i] = (s->phi[k] > c->phi_min) ? (c->bnd_len[i] / c->cell_area[k] / ph) : 0.0;
     }
+    ORC_PAR
     for (int64_t i = 0; i < c->n_int; i++) {
         int64_t L = c->int_L[i], R = c->int_R[i];
         double pL = (s->phi[L] > c->phi_min) ? s->phi[L] : c->phi_min;
@@ -89,6 +93,7 @@ static void factors(esolver *s) {
 static void lr(esolver *s) {
     const orc_case *c = s->c;
     const double hmin = c->h_min;
+    ORC_PAR
     for (int64_t e = 0; e < c->n_bnd; e++) {
         double nx = c->bnd_nx[e], ny = c->bnd_ny[e];
         int64_t id = c->bnd_cell[e];
@@ -102,6 +107,7 @@ static void lr(esolver *s) {
         s->lpb[e] = lp;
         s->lmaxb[e] = (fabs(lp) < fabs(lm)) ? fabs(lm) : fabs(lp);
     }
+    ORC_PAR
     for (int64_t e = 0; e < c->n_int; e++) {
         double nx = c->int_nx[e], ny = c->int_ny[e];
         for (int sd = 0; sd < 2; sd++) {
@@ -136,6 +142,7 @@ static double desing(double q, double h, double he) {
 /* computeMusclSlopes on the free surface -- swSpHypMusclEvr.cpp:11-115 */
 static void e_slopes(esolver *s) {
     const orc_case *c = s->c;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_cell; i++) {
         s->dzdx[i] = 0.0;
         s->dzdy[i] = 0.0;
@@ -155,6 +162,31 @@ static void e_slopes(esolver *s) {
             s->dzdy[i] /= wsum;
         }
     }
+#ifdef ORC_OMP
+    /* the limiter of cell k only changes its own slope, in its edges' order */
+    ORC_PAR
+    for (int64_t k = 0; k < c->n_cell; k++) {
+        for (int64_t p = s->inc.iptr[k]; p < s->inc.iptr[k + 1]; p++) {
+            const int64_t e = s->inc.iinc[p] >> 1;
+            int64_t L = c->int_L[e], R = c->int_R[e];
+            double fmx = (s->zc[L] < s->zc[R]) ? s->zc[R] : s->zc[L];
+            double fmn = (s->zc[R] < s->zc[L]) ? s->zc[R] : s->zc[L];
+            double fr = s->zc[k] + (c->int_xm[e] - c->cell_x[k]) * s->dzdx[k] +
+                        (c->int_ym[e] - c->cell_y[k]) * s->dzdy[k];
+            if (fr < fmn) {
+                double sf = (fmn - s->zc[k]) / (fr - s->zc[k]);
+                s->dzdx[k] *= sf;
+                s->dzdy[k] *= sf;
+            }
+            if (fr > fmx) {
+                double sf = (fmx - s->zc[k]) / (fr - s->zc[k]);
+                s->dzdx[k] *= sf;
+                s->dzdy[k] *= sf;
+            }
+        }
+    }
+    return;
+#endif
     for (int64_t e = 0; e < c->n_int; e++) {
         int64_t L = c->int_L[e], R = c->int_R[e];
         double fmx = (s->zc[L] < s->zc[R]) ? s->zc[R] : s->zc[L];
@@ -181,8 +213,10 @@ static void e_slopes(esolver *s) {
 static void lr_muscl(esolver *s) {
     const orc_case *c = s->c;
     const double dt = s->dt_recon;
+    ORC_PAR
     for (int64_t i = 0; i < c->n_cell; i++) s->zc[i] = s->h[i] + c->cell_zb[i];
     e_slopes(s);
+    ORC_PAR
     for (int64_t e = 0; e < c->n_bnd; e++) {
         double nx = c->bnd_nx[e], ny = c->bnd_ny[e];
         int64_t id = c->bnd_cell[e];
@@ -196,6 +230,7 @@ static void lr_muscl(esolver *s) {
         s->lpb[e] = lp;
         s->lmaxb[e] = (fabs(lm) < fabs(lp)) ? fabs(lp) : fabs(lm);
     }
+    ORC_PAR
     for (int64_t e = 0; e < c->n_int; e++) {
         double nx = c->int_nx[e], ny = c->int_ny[e];
         for (int sd = 0; sd < 2; sd++) {
@@ -254,6 +289,34 @@ static void lr_any(esolver *s) {
 static double cfl_dt(esolver *s) {
     const orc_case *c = s->c;
     double dt = c->dt_max;
+#ifdef ORC_OMP
+    /* per cell: its boundary edges, then its internal edges, each ascending */
+#pragma omp parallel
+    {
+        double dtl = dt;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < c->n_cell; i++) {
+            double sw = 0.0;
+            if (s->h[i] > c->h_min) {
+                for (int64_t p = s->inc.bptr[i]; p < s->inc.bptr[i + 1]; p++) {
+                    const int64_t e = s->inc.binc[p];
+                    sw += s->lmaxb[e] * c->bnd_len[e];
+                }
+                for (int64_t p = s->inc.iptr[i]; p < s->inc.iptr[i + 1]; p++) {
+                    const int64_t e = s->inc.iinc[p] >> 1;
+                    sw += s->lmaxi[e] * c->int_len[e];
+                }
+            }
+            s->swept[i] = sw;
+            double den = (sw < SMALL) ? SMALL : sw;
+            double x = c->cfl_max * (c->cell_area[i] / den);
+            if (x < dtl) dtl = x;
+        }
+#pragma omp critical
+        if (dtl < dt) dt = dtl;
+    }
+    return dt;
+#endif
     for (int64_t i = 0; i < c->n_cell; i++) s->swept[i] = 0.0;
     for (int64_t e = 0; e < c->n_bnd; e++) {
         int64_t id = c->bnd_cell[e];
@@ -380,6 +443,7 @@ void orc_e_hllc(double h_min, double phi_min, double bL, double bR, double phiL,
 
 static void fluxes(esolver *s, const double *bc_step) {
     const orc_case *c = s->c;
+    ORC_PAR
     for (int64_t e = 0; e < c->n_bnd; e++) {
         int64_t id = c->bnd_cell[e];
         int k = c->bnd_series[e];
@@ -390,6 +454,7 @@ static void fluxes(esolver *s, const double *bc_step) {
         s->F2b[e] = f[1];
         s->F3b[e] = f[2];
     }
+    ORC_PAR
     for (int64_t e = 0; e < c->n_int; e++) {
         int64_t L = c->int_L[e], R = c->int_R[e];
         double f[5];
@@ -407,9 +472,51 @@ static void fluxes(esolver *s, const double *bc_step) {
 
 /* hypComputeBalance_E -- swSpE_hyp.cpp:336-394 (signed factors of
  * computeEdgeFactors_E written out) */
+#ifdef ORC_OMP
+/* the balance of the cells of U (h | qx | qy, the cell's own additions in
+ * the serial order), without the clean-up */
+static void balance_gather(const esolver *s, double dt, double *h, double *qx, double *qy) {
+    const orc_case *c = s->c;
+    ORC_PAR
+    for (int64_t i = 0; i < c->n_cell; i++) {
+        for (int64_t p = s->inc.bptr[i]; p < s->inc.bptr[i + 1]; p++) {
+            const int64_t e = s->inc.binc[p];
+            double co = s->fac_b[e], nx = c->bnd_nx[e], ny = c->bnd_ny[e];
+            h[i] += (+co) * s->F1b[e] * dt;
+            qx[i] += ((+co * nx) * s->F2b[e] + (+co * nx) * 0.0 + (-co * ny) * s->F3b[e]) * dt;
+            qy[i] += ((+co * ny) * s->F2b[e] + (+co * ny) * 0.0 + (+co * nx) * s->F3b[e]) * dt;
+        }
+        for (int64_t p = s->inc.iptr[i]; p < s->inc.iptr[i + 1]; p++) {
+            const int64_t e = s->inc.iinc[p] >> 1;
+            double cl = s->fac_L[e], cr = s->fac_R[e], nx = c->int_nx[e], ny = c->int_ny[e];
+            double dh = s->F1i[e] * dt;
+            if (s->inc.iinc[p] & 1) {
+                h[i] += (+cr) * dh;
+                qx[i] += ((+cr * nx) * s->F2i[e] + (+cr * nx) * s->sRi[e] + (-cr * ny) * s->F3i[e]) * dt;
+                qy[i] += ((+cr * ny) * s->F2i[e] + (+cr * ny) * s->sRi[e] + (+cr * nx) * s->F3i[e]) * dt;
+            } else {
+                h[i] += (-cl) * dh;
+                qx[i] += ((-cl * nx) * s->F2i[e] + (+cl * nx) * s->sLi[e] + (+cl * ny) * s->F3i[e]) * dt;
+                qy[i] += ((-cl * ny) * s->F2i[e] + (+cl * ny) * s->sLi[e] + (-cl * nx) * s->F3i[e]) * dt;
+            }
+        }
+    }
+}
+#endif
+
 static void balance(esolver *s, double dt) {
     const orc_case *c = s->c;
     double *h = s->h, *qx = s->qx, *qy = s->qy;
+#ifdef ORC_OMP
+    balance_gather(s, dt, h, qx, qy);
+    ORC_PAR
+    for (int64_t n = 0; n < c->n_cell; ++n) {
+        if (fabs(h[n]) < SMALL) h[n] = 0.0;
+        if (fabs(qx[n]) < SMALL) qx[n] = 0.0;
+        if (fabs(qy[n]) < SMALL) qy[n] = 0.0;
+    }
+    return;
+#endif
     for (int64_t e = 0; e < c->n_bnd; e++) {
         int64_t id = c->bnd_cell[e];
         double co = s->fac_b[e], nx = c->bnd_nx[e], ny = c->bnd_ny[e];
@@ -440,6 +547,7 @@ static void sources(esolver *s, double dt) {
     const double hthr = (c->h_min > SMALL) ? c->h_min : SMALL;
     if (c->friction != ORC_FRIC_NONE) {
         /* sourceFrictionStrickler_E -- swSpE_friction.cpp:19-76 */
+        ORC_PAR
         for (int64_t i = 0; i < c->n_cell; i++) {
             double h = s->h[i];
             if (h > hthr) {
@@ -465,6 +573,7 @@ static void sources(esolver *s, double dt) {
     }
     if (c->head_loss) {
         /* sourceSingularHeadLoss_E -- swSpE_friction.cpp:79-126 */
+        ORC_PAR
         for (int64_t i = 0; i < c->n_cell; i++) {
             double ca = s->hcos[i], sa = s->hsin[i];
             double qx = s->qx[i] * ca + s->qy[i] * sa;
@@ -497,6 +606,7 @@ static void e_step(esolver *s, double dt, const double *bc_step) {
     lr_any(s);
     fluxes(s, bc_step);
     balance(s, dt);
+    ORC_PAR
     for (int64_t n = 0; n < c->n_cell; ++n) {
         if (s->h[n] < 0.) {
             s->h[n] = 0.;
@@ -543,6 +653,9 @@ static void e_init(esolver *s, const orc_case *c, const double *phi, const doubl
         }
         free(fill);
     }
+#ifdef ORC_OMP
+    orc_inc_build(&s->inc, nc, c->n_int, c->n_bnd, c->int_L, c->int_R, c->bnd_cell);
+#endif
     s->dt_recon = 0.0; /* the solver's dt member before any step (swSp.h:155) */
     memcpy(s->h, h0, sizeof(double) * (size_t)nc);
     memcpy(s->qx, qx0, sizeof(double) * (size_t)nc);
@@ -566,6 +679,9 @@ static void e_free(esolver *s) {
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
     free(s->nb_ptr);
     free(s->nb_idx);
+#ifdef ORC_OMP
+    orc_inc_free(&s->inc);
+#endif
 }
 
 /* Dry run (build_checkpoint_records, swSpE_checkpoint.cpp:210-401): steps from
@@ -1427,6 +1543,10 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
     const int muscl = c->scheme == ORC_MUSCL_EVR;
     double *asx = (double *)calloc((size_t)nc, sizeof(double)); /* adjoint of the limited slopes */
     double *asy = (double *)calloc((size_t)nc, sizeof(double));
+#ifdef ORC_OMP
+    double *wb = (double *)calloc(6 * (size_t)(c->n_bnd + 1), sizeof(double));   /* per boundary edge */
+    double *wi = (double *)calloc(14 * (size_t)(c->n_int + 1), sizeof(double));  /* per internal edge */
+#endif
     for (int64_t i = 0; i < nc; i++) g_phi[i] = g_K1[i] = g_K2[i] = 0.0;
     for (int64_t k = 0; k < n_steps * (ns > 0 ? ns : 1); k++) g_bc[k] = 0.0;
     for (int64_t n = n_steps - 1; n >= 0; n--) {
@@ -1464,6 +1584,10 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
         s.dt_recon = (g_plant == 2) ? (n > 0 ? dt_seq[n - 1] : 0.0) : dt;
         lr_any(&s);
         fluxes(&s, bcn);
+#ifdef ORC_OMP
+        memcpy(bh, Un, sizeof(double) * 3 * nc);
+        balance_gather(&s, dt, bh, bh + nc, bh + 2 * nc);
+#else
         memcpy(bh, Un, sizeof(double) * 3 * nc);
         {
             double *h = bh, *qx = bh + nc, *qy = bh + 2 * nc;
@@ -1486,7 +1610,9 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
                 qy[R] += ((+cr * ny) * s.F2i[e] + (+cr * ny) * s.sRi[e] + (+cr * nx) * s.F3i[e]) * dt;
             }
         }
+#endif
         /* per cell: sources, clamp and clean-up in reverse */
+        ORC_PAR
         for (int64_t i = 0; i < nc; i++) {
             double h = bh[i], qx = bh[nc + i], qy = bh[2 * nc + i];
             int kh = !(fabs(h) < SMALL), kx = !(fabs(qx) < SMALL), ky = !(fabs(qy) < SMALL);
@@ -1530,6 +1656,106 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
             lams[nc + i] = ax;
             lams[2 * nc + i] = ay;
         }
+#ifdef ORC_OMP
+        /* every edge's reverse reads lams (fixed here) only: the terms of each
+         * edge side go to per-edge slots, then each cell adds its own in
+         * ascending edge order, boundary edges first (porosity and
+         * boundary-value sums in the serial order; the adjoint state sums
+         * each edge's terms before adding them) */
+        memcpy(lam, lams, sizeof(double) * 3 * nc);
+        ORC_PAR
+        for (int64_t e = 0; e < c->n_bnd; e++) {
+            int64_t id = c->bnd_cell[e];
+            double co = s.fac_b[e], nx = c->bnd_nx[e], ny = c->bnd_ny[e];
+            double lh = lams[id], lx = lams[nc + id], ly = lams[2 * nc + id];
+            double df[3] = {lh * co * dt, (lx * co * nx + ly * co * ny) * dt,
+                            (-lx * co * ny + ly * co * nx) * dt};
+            double dco = lh * s.F1b[e] * dt +
+                         (lx * (nx * s.F2b[e] - ny * s.F3b[e]) + ly * (ny * s.F2b[e] + nx * s.F3b[e])) * dt;
+            double dfp;
+            fac_dphi(c, phi, id, c->bnd_len[e], &dfp);
+            double *tb = wb + 6 * e;
+            tb[0] = dco * dfp;
+            int k = c->bnd_series[e];
+            double d[6];
+            bnd_adj(c->h_min, c->phi_min, c->beta[id], phi[id], c->cell_zb[id], s.zrb[e], s.qrb[e],
+                    s.rrb[e], s.lpb[e], c->series_type[k], bcn[k], df, d);
+            tb[1] = d[4];
+            tb[2] = d[5];
+            tb[3] = tb[4] = tb[5] = 0.0;
+            if (muscl)
+                lr_bnd_muscl_adj(Un[id], Un[nc + id], Un[2 * nc + id], c->beta[id], nx, ny, d[0],
+                                 d[1], d[2], d[3], &tb[3], &tb[4], &tb[5]);
+            else
+                lr_adj(Un[id], Un[nc + id], Un[2 * nc + id], c->beta[id], nx, ny, c->h_min, d[0],
+                       d[1], d[2], 0.0, d[3], &tb[3], &tb[4], &tb[5]);
+        }
+        for (int64_t e = 0; e < c->n_bnd; e++) g_bc[n * ns + c->bnd_series[e]] += wb[6 * e + 2];
+        ORC_PAR
+        for (int64_t e = 0; e < c->n_int; e++) {
+            int64_t L = c->int_L[e], R = c->int_R[e];
+            double cl = s.fac_L[e], cr = s.fac_R[e], nx = c->int_nx[e], ny = c->int_ny[e];
+            double lhL = lams[L], lxL = lams[nc + L], lyL = lams[2 * nc + L];
+            double lhR = lams[R], lxR = lams[nc + R], lyR = lams[2 * nc + R];
+            double F1 = s.F1i[e], F2 = s.F2i[e], F3 = s.F3i[e], sL = s.sLi[e], sR = s.sRi[e];
+            double df[5];
+            df[0] = (-lhL * cl + lhR * cr) * dt;
+            df[1] = (-lxL * cl * nx + lxR * cr * nx - lyL * cl * ny + lyR * cr * ny) * dt;
+            df[2] = (lxL * cl * ny - lxR * cr * ny - lyL * cl * nx + lyR * cr * nx) * dt;
+            df[3] = (lxL * cl * nx + lyL * cl * ny) * dt;
+            df[4] = (lxR * cr * nx + lyR * cr * ny) * dt;
+            double dcl = (-lhL * F1 + lxL * (-nx * F2 + nx * sL + ny * F3) + lyL * (-ny * F2 + ny * sL - nx * F3)) * dt;
+            double dcr = (lhR * F1 + lxR * (nx * F2 + nx * sR - ny * F3) + lyR * (ny * F2 + ny * sR + nx * F3)) * dt;
+            double dpl, dpr;
+            fac_dphi(c, phi, L, c->int_len[e], &dpl);
+            fac_dphi(c, phi, R, c->int_len[e], &dpr);
+            double *tL = wi + 14 * e, *tR = tL + 7;
+            for (int q = 0; q < 14; q++) tL[q] = 0.0;
+            tL[0] = dcl * dpl;
+            tR[0] = dcr * dpr;
+            double d[12];
+            hllc_adj(c->h_min, c->phi_min, c->beta[L], c->beta[R], phi[L], phi[R], c->cell_zb[L],
+                     c->cell_zb[R], s.zri[2 * e], s.qri[2 * e], s.rri[2 * e], s.zri[2 * e + 1],
+                     s.qri[2 * e + 1], s.rri[2 * e + 1], s.lmi[2 * e], s.lmi[2 * e + 1], s.lpi[2 * e],
+                     s.lpi[2 * e + 1], df, d);
+            tL[1] = d[10];
+            tR[1] = d[11];
+            if (muscl) {
+                recon_adj(0, Un[L], Un[nc + L], Un[2 * nc + L], c->cell_zb[L], c->beta[L], s.dzdx[L],
+                          s.dzdy[L], nx, ny, c->int_xm[e] - c->cell_x[L], c->int_ym[e] - c->cell_y[L],
+                          s.dt_recon, d[0], d[1], d[2], d[6], d[8], &tL[2], &tL[3], &tL[4], &tL[5], &tL[6]);
+                recon_adj(1, Un[R], Un[nc + R], Un[2 * nc + R], c->cell_zb[R], c->beta[R], s.dzdx[R],
+                          s.dzdy[R], nx, ny, c->int_xm[e] - c->cell_x[R], c->int_ym[e] - c->cell_y[R],
+                          s.dt_recon, d[3], d[4], d[5], d[7], d[9], &tR[2], &tR[3], &tR[4], &tR[5], &tR[6]);
+            } else {
+                lr_adj(Un[L], Un[nc + L], Un[2 * nc + L], c->beta[L], nx, ny, c->h_min, d[0], d[1],
+                       d[2], d[6], d[8], &tL[2], &tL[3], &tL[4]);
+                lr_adj(Un[R], Un[nc + R], Un[2 * nc + R], c->beta[R], nx, ny, c->h_min, d[3], d[4],
+                       d[5], d[7], d[9], &tR[2], &tR[3], &tR[4]);
+            }
+        }
+        ORC_PAR
+        for (int64_t i = 0; i < nc; i++) {
+            for (int64_t p = s.inc.bptr[i]; p < s.inc.bptr[i + 1]; p++) {
+                const double *tb = wb + 6 * s.inc.binc[p];
+                g_phi[i] += tb[0];
+                g_phi[i] += tb[1];
+                lam[i] += tb[3];
+                lam[nc + i] += tb[4];
+                lam[2 * nc + i] += tb[5];
+            }
+            for (int64_t p = s.inc.iptr[i]; p < s.inc.iptr[i + 1]; p++) {
+                const double *t = wi + 14 * (s.inc.iinc[p] >> 1) + 7 * (s.inc.iinc[p] & 1);
+                g_phi[i] += t[0];
+                g_phi[i] += t[1];
+                lam[i] += t[2];
+                lam[nc + i] += t[3];
+                lam[2 * nc + i] += t[4];
+                asx[i] += t[5];
+                asy[i] += t[6];
+            }
+        }
+#else
         /* balance: identity part, then every edge in reverse */
         memcpy(lam, lams, sizeof(double) * 3 * nc);
         for (int64_t e = 0; e < c->n_bnd; e++) {
@@ -1598,6 +1824,7 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
                        d[5], d[7], d[9], &lam[R], &lam[nc + R], &lam[2 * nc + R]);
             }
         }
+#endif
         if (muscl) { /* slopes: free surface zc = h + zb of the cell and its neighbours */
             if (g_plant != 1) e_slopes_adj(&s, asx, asy, lam);
             memset(asx, 0, sizeof(double) * nc);
@@ -1631,6 +1858,10 @@ static double e_gradient(const orc_case *c, const double *phi, const double *K1,
         }
     }
     free(wm);
+#ifdef ORC_OMP
+    free(wb);
+    free(wi);
+#endif
     free(asx);
     free(asy);
     free(Jstep);
