@@ -76,14 +76,19 @@ __global__ __launch_bounds__(kBlock) void k_slopes(MeshDev m, Clock *ck, int p) 
     const double zi = m.h[i] + m.zb[i];
     const double xi = m.cx[i], yi = m.cy[i];
     const int b = m.int_ptr[i], e_end = m.int_ptr[i + 1];
-    double sx = 0.0, sy = 0.0;
-    if (e_end - b > 1) {
-        double wsum = 0.0;
-        for (int k = b; k < e_end; ++k) {
-            const int inc = m.int_inc[k];
-            const int e = inc >> 1;
-            const int j = (inc & 1) ? m.eL[e] : m.eR[e];
-            const double zj = m.h[j] + m.zb[j];
+    const int nk = e_end - b;
+    // the first KC neighbours' free surface and incidence stay in registers
+    // for the limiter pass (triangles and quads: all of them)
+    constexpr int KC = 4;
+    double zn[KC];
+    int in[KC];
+    double sx = 0.0, sy = 0.0, wsum = 0.0;
+    auto grad_term = [&](int inc, double &zj_out) {
+        const int e = inc >> 1;
+        const int j = (inc & 1) ? m.eL[e] : m.eR[e];
+        const double zj = m.h[j] + m.zb[j];
+        zj_out = zj;
+        if (nk > 1) {
             const double dx = m.cx[j] - xi;
             const double dy = m.cy[j] - yi;
             const double dist = sqrt(dx * dx + dy * dy);
@@ -92,15 +97,28 @@ __global__ __launch_bounds__(kBlock) void k_slopes(MeshDev m, Clock *ck, int p) 
             sy += (zj - zi) * dy / dist / dist * w;
             wsum += w;
         }
+    };
+#pragma unroll
+    for (int t = 0; t < KC; ++t) {
+        if (t < nk) {
+            in[t] = m.int_inc[b + t];
+            grad_term(in[t], zn[t]);
+        }
+    }
+    for (int k = b + KC; k < e_end; ++k) {
+        double zj;
+        grad_term(m.int_inc[k], zj);
+    }
+    if (nk > 1) {
         sx /= wsum;
         sy /= wsum;
     }
-    for (int k = b; k < e_end; ++k) {
-        const int inc = m.int_inc[k];
+    // limiter replayed over the incidences in order; (zL, zR) of the edge is
+    // (zi, zj) or (zj, zi) by the side of this cell
+    auto limit = [&](int inc, double zj) {
         const int e = inc >> 1;
-        const int L = m.eL[e], R = m.eR[e];
-        const double zL = m.h[L] + m.zb[L];
-        const double zR = m.h[R] + m.zb[R];
+        const double zL = (inc & 1) ? zj : zi;
+        const double zR = (inc & 1) ? zi : zj;
         const double fmx = smax(zL, zR);
         const double fmn = smin(zL, zR);
         const double fr = zi + (m.exm[e] - xi) * sx + (m.eym[e] - yi) * sy;
@@ -114,6 +132,15 @@ __global__ __launch_bounds__(kBlock) void k_slopes(MeshDev m, Clock *ck, int p) 
             sx *= sf;
             sy *= sf;
         }
+    };
+#pragma unroll
+    for (int t = 0; t < KC; ++t)
+        if (t < nk) limit(in[t], zn[t]);
+    for (int k = b + KC; k < e_end; ++k) {
+        const int inc = m.int_inc[k];
+        const int e = inc >> 1;
+        const int j = (inc & 1) ? m.eL[e] : m.eR[e];
+        limit(inc, m.h[j] + m.zb[j]);
     }
     m.dzdx[i] = sx;
     m.dzdy[i] = sy;
@@ -210,12 +237,13 @@ __global__ __launch_bounds__(kBlock) void k_edges(MeshDev m, Opts o, Clock *ck, 
     if (lmax < v) lmax = v;
     v = fabs(sR.lp);
     if (lmax < v) lmax = v;
-    double *rec = m.rec + 8 * (size_t)e;
-    // native CFL quirk: the lambda picked up on the left carries to the right
+    // native CFL quirk: the lambda picked up on the left carries to the right.
+    // Swept terms and contributions go to separate dense arrays with 16-byte
+    // stores: the CFL gather then reads 16 bytes per edge, not the record.
     const bool wL = hL > o.h_min;
     const bool wR = hR > o.h_min;
-    rec[3] = (wL ? lmax : 0.) * len;
-    rec[7] = ((wL || wR) ? lmax : 0.) * len;
+    reinterpret_cast<double2 *>(m.swp)[e] =
+        make_double2((wL ? lmax : 0.) * len, ((wL || wR) ? lmax : 0.) * len);
     if ((sL.z - zbL > o.h_min) || (sR.z - zbR > o.h_min)) {
         const double phiL = m.phi[L], phiR = m.phi[R];
         double f[5];
@@ -231,12 +259,16 @@ __global__ __launch_bounds__(kBlock) void k_edges(MeshDev m, Opts o, Clock *ck, 
         }
         const double cl = (phiL > o.phi_min) ? len / m.area[L] / phiL : 0.0;
         const double cr = (phiR > o.phi_min) ? len / m.area[R] / phiR : 0.0;
-        rec[0] = (-cl) * f[0];
-        rec[1] = ((-cl) * nx) * f[1] + (cl * nx) * f[3] + (cl * ny) * f[2];
-        rec[2] = ((-cl) * ny) * f[1] + (cl * ny) * f[3] + ((-cl) * nx) * f[2];
-        rec[4] = cr * f[0];
-        rec[5] = (cr * nx) * f[1] + (cr * nx) * f[4] + ((-cr) * ny) * f[2];
-        rec[6] = (cr * ny) * f[1] + (cr * ny) * f[4] + (cr * nx) * f[2];
+        const double a0 = (-cl) * f[0];
+        const double a1 = ((-cl) * nx) * f[1] + (cl * nx) * f[3] + (cl * ny) * f[2];
+        const double a2 = ((-cl) * ny) * f[1] + (cl * ny) * f[3] + ((-cl) * nx) * f[2];
+        const double b0 = cr * f[0];
+        const double b1 = (cr * nx) * f[1] + (cr * nx) * f[4] + ((-cr) * ny) * f[2];
+        const double b2 = (cr * ny) * f[1] + (cr * ny) * f[4] + (cr * nx) * f[2];
+        double2 *rec = reinterpret_cast<double2 *>(m.flx + 6 * (size_t)e);
+        rec[0] = make_double2(a0, a1);
+        rec[1] = make_double2(a2, b0);
+        rec[2] = make_double2(b1, b2);
     }
 }
 
@@ -315,10 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, Clock *ck, in
             }
         }
         const int ib = m.int_ptr[i], ie = m.int_ptr[i + 1];
-        for (int k = ib; k < ie; ++k) {
-            const int inc = m.int_inc[k];
-            swept += m.rec[8 * (size_t)(inc >> 1) + 4 * (inc & 1) + 3];
-        }
+        for (int k = ib; k < ie; ++k) swept += m.swp[m.int_inc[k]];  // (edge << 1) | side
         const double x = ck->cfl * (m.area[i] / smax(swept, kSmall));
         if (x < cand) cand = x;
     }
@@ -528,9 +557,13 @@ __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck,
 // ---------------------------------------------------------------------------
 // part 0: the whole update; with momentum diffusion the parabolic step sits
 // between the clamp and friction (swSp.cpp:535-545): part 1 stops after the
-// clamp, part 2 starts from the diffused discharges.
+// clamp, part 2 starts from the diffused discharges.  EXTRA = false compiles
+// the step without the runoff, infiltration, wind and diffusion options
+// (part 0 only), which frees registers for more waves per SIMD.
+template <bool EXTRA>
 __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, Clock *ck, int p,
-                                                   int part) {
+                                                   int part_arg) {
+    const int part = EXTRA ? part_arg : 0;
     if (!step_active(ck, p)) {
         carry_clock(ck, p);
         return;
@@ -567,8 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
     }
     const int ib = m.int_ptr[i], ie = m.int_ptr[i + 1];
     for (int k = ib; k < ie; ++k) {
-        const int inc = m.int_inc[k];
-        const double *r = m.rec + 8 * (size_t)(inc >> 1) + 4 * (inc & 1);
+        const double *r = m.flx + 3 * (size_t)m.int_inc[k];  // 6 * edge + 3 * side
         h += r[0] * dt;
         qx += r[1] * dt;
         qy += r[2] * dt;
@@ -630,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
         qx = ax * ca - ay * sa;
         qy = ax * sa + ay * ca;
     }
-    if (o.runoff) {
+    if (EXTRA && o.runoff) {
         // swSp::sourceRunoffEverywhere / sourceRunoffWateronly
         // (swSpSourceRunoffInfiltration.cpp:43-263): rain rate of the cell's
         // station at t + dt/2; rain adds (a share of) the depth, evaporation
@@ -663,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
             qy *= ratio;
         }
     }
-    if (o.infil) {
+    if (EXTRA && o.infil) {
         // swSp::sourceInfiltration, Horton (swSpSourceRunoffInfiltration.cpp:12-40)
         const double h0 = h;
         h = smax(0.0, h0 + -m.infil[i] * dt);
@@ -671,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
         qx *= ratio;
         qy *= ratio;
     }
-    if (o.wind) {
+    if (EXTRA && o.wind) {
         // swSp::sourceWind + sourceWindSmithBanke (swSpSourceWind.cpp:11-85):
         // station velocity interpolated at t + dt/2, then the Smith & Banke
         // stress (its drag coefficient grows with the water speed |q|/h)
@@ -743,15 +775,20 @@ namespace {
 void launch_update(sw2dg_ctx *c, int p) {
     const dim3 blk(kBlock);
     if (!c->o.mdiff) {
-        hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 0);
+        if (c->o.runoff || c->o.infil || c->o.wind)
+            hipLaunchKernelGGL(k_update<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc,
+                               c->ck, p, 0);
+        else
+            hipLaunchKernelGGL(k_update<false>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc,
+                               c->ck, p, 0);
         return;
     }
-    hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 1);
+    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 1);
     hipLaunchKernelGGL(k_parab_dt, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->ck, p);
     hipLaunchKernelGGL(k_parab_n, dim3(1), dim3(kDtBlock), 0, c->stream, c->m, c->o, c->ck, p,
                        nblocks(c->ni));
     hipLaunchKernelGGL(k_parab_sub, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->ck, p);
-    hipLaunchKernelGGL(k_update, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 2);
+    hipLaunchKernelGGL(k_update<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 2);
 }
 
 void launch_step(sw2dg_ctx *c, int p) {
@@ -1030,7 +1067,8 @@ int sw2dg_ctx_create(const sw2dg_mesh *mesh, const sw2dg_params *prm, int32_t de
     m.eny = c->upload(eny);
     m.exm = c->upload(exm);
     m.eym = c->upload(eym);
-    m.rec = c->dalloc<double>(8 * (size_t)ni);
+    m.flx = c->dalloc<double>(6 * (size_t)ni);
+    m.swp = c->dalloc<double>(2 * (size_t)ni);
     m.blen = c->upload(blen);
     m.bnx = c->upload(bnx);
     m.bny = c->upload(bny);
@@ -1342,7 +1380,8 @@ int sw2dg_set_state(sw2dg_ctx *c, const double *h, const double *qx, const doubl
     }
     c->t0 = t;
     // fluxes of dry-dry edges start at zero (reference vectors are value-initialised)
-    HIPCHK(c, hipMemset(c->m.rec, 0, 8 * (size_t)std::max<int64_t>(c->ni, 1) * sizeof(double)));
+    HIPCHK(c, hipMemset(c->m.flx, 0, 6 * (size_t)std::max<int64_t>(c->ni, 1) * sizeof(double)));
+    HIPCHK(c, hipMemset(c->m.swp, 0, 2 * (size_t)std::max<int64_t>(c->ni, 1) * sizeof(double)));
     // BC cursors rewound (swBoundaryConditionTimeSeries::initialise)
     const int ns = c->n_series;
     if (ns > 0) {

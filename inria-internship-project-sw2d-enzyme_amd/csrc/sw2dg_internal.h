@@ -61,7 +61,8 @@ struct MeshDev {
     // internal edges (renumbered)
     const int *eL, *eR;
     const double *elen, *enx, *eny, *exm, *eym;
-    double *rec;                   // [8*n_int]: side 0 {dh, dqx, dqy, swept}, side 1 same
+    double *flx;                   // [6*n_int]: side 0 {dh, dqx, dqy}, side 1 the same
+    double *swp;                   // [2*n_int]: CFL swept term of side 0, side 1
     // boundary edges (renumbered)
     const double *blen, *bnx, *bny;
     const int *bser;
