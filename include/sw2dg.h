@@ -128,10 +128,13 @@ int sw2dg_set_infiltration(sw2dg_ctx *ctx, int32_t model, const double *rate);
  * and qy diffuse with the edge coefficient of the harmonic mean of
  * 1e-6 h phi of its cells (the reference's fixed coefficient, swSp.h:244)
  * and the factor length / |centre distance along the normal|
- * (swSp.cpp:1236-1246).  The reference divides q by h of both cells, so a
- * dry cell makes NaN there as in the reference.  One explicit substep per
- * step is built: a step whose bound asks for more stops with SW2DG_ERR_ARG
- * (it needs a cell area below 2e-6 dt h). */
+ * (swSp.cpp:1236-1246), in n = int(dt / min(t_max - t_0, dt, area phi /
+ * (2 D))) explicit substeps of dt / n.  A step with n > 1 (cells of about a
+ * micron or less) ends the device time loop's launch; the library runs its
+ * substeps and resumes, so the results do not depend on it.  n above 2^20
+ * stops the run with SW2DG_ERR_ARG.  The reference divides q by h of both
+ * cells, so a dry cell makes q NaN next to it and the next step stops with
+ * SW2DG_ERR_NAN, where the reference stops (swSpHyp.cpp:391). */
 int sw2dg_set_momentum_diffusion(sw2dg_ctx *ctx, int32_t model);
 /* Replaces swSp::setInitialConditions (swSp.cpp:1119-1224): sets the state
  * and the clock (t, previous dt = 0, step count 0, BC cursors rewound). */

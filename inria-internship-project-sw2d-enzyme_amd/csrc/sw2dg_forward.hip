@@ -45,6 +45,7 @@ using namespace sw2dg_detail;
 namespace {
 
 constexpr int kStepsPerGraph = 32;  // even: the clock double buffer returns to slot 0
+constexpr int kMaxParabSubsteps = 1 << 20;  // explicit diffusion substeps of one step, at most
 
 __device__ __forceinline__ bool step_active(const Clock *ck, int p) {
     return ck->ACT[p] != 0 && ck->err == 0;
@@ -387,8 +388,10 @@ __device__ int record_bracket(const double *tv, int n, double tw) {
 // momentum diffusion, velocity_independent (swSpParab.cpp:10-96): edge
 // coefficient = harmonic mean of nu h phi of its cells, explicit substeps of
 // dt / n with n = int(dt / min(t_max - t_0, dt, min over edge sides of
-// area phi / (2 D))), on qx and qy.  One substep (n = 1) is built here; a
-// step that needs more stops with an error.
+// area phi / (2 D))), on qx and qy.  The time-loop graph holds one substep;
+// a step that needs n > 1 leaves them pending (k_parab_n): the rest of the
+// graph's steps idle, and the host runs the n substeps and the end of the
+// step before resuming (finish_parab).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double parab_coef(const MeshDev &m, int L, int R) {
     const double dL = sp::kMomDiff * m.h[L] * m.phi[L];
@@ -448,32 +451,43 @@ __global__ __launch_bounds__(kDtBlock) void k_parab_n(MeshDev m, Opts o, Clock *
     for (int w = 1; w < kDtBlock / 64; ++w) dtp = (red[w] < dtp) ? red[w] : dtp;
     const double dt = ck->DT[1 - p];
     dtp = (dt < dtp) ? dt : dtp;
-    const int n = (int)(dt / dtp);
-    if (n != 1) {
+    const double ratio = dt / dtp;
+    if (!(ratio < (double)kMaxParabSubsteps + 1.0)) {
         raise_err(ck, sp::ST_PARAB_SUBSTEPS);
         return;
     }
+    const int n = (int)ratio;
     ck->parab_dt = dt / n;
+    ck->parab_n = n;
+    if (n > 1) {  // pending: the following steps of this launch idle
+        ck->parab_pending = 1;
+        ck->act_saved = ck->ACT[1 - p];
+        ck->ACT[1 - p] = 0;
+    }
 }
 
-// one explicit substep of qx and qy; each internal edge flux is formed by
-// both of its cells from the same values, in the reference's edge order
-__global__ __launch_bounds__(kBlock) void k_parab_sub(MeshDev m, Clock *ck, int p) {
-    if (!step_active(ck, p)) return;
+// one explicit substep of qx and qy, (sx, sy) -> (dx, dy); each internal
+// edge flux is formed by both of its cells from the same values, in the
+// reference's edge order
+__global__ __launch_bounds__(kBlock) void k_parab_sub(MeshDev m, Clock *ck, int p,
+                                                      const double *__restrict__ sx,
+                                                      const double *__restrict__ sy,
+                                                      double *__restrict__ dx, double *__restrict__ dy) {
+    if (!step_active(ck, p) || ck->parab_pending) return;
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= m.n_cell) return;
     const double dtp = ck->parab_dt;
     const double den = smax(kSmall, m.phi[i]);
     const double area = m.area[i];
-    double vx = m.qx[i], vy = m.qy[i];
+    double vx = sx[i], vy = sy[i];
     const int ib = m.int_ptr[i], ie = m.int_ptr[i + 1];
     for (int k = ib; k < ie; ++k) {
         const int e = m.int_inc[k] >> 1;
         const int L = m.eL[e], R = m.eR[e];
         const double w = parab_coef(m, L, R) * m.pfac[e];
         const double hL = m.h[L], hR = m.h[R];
-        const double fx = w * (m.qx[L] / hL - m.qx[R] / hR);
-        const double fy = w * (m.qy[L] / hL - m.qy[R] / hR);
+        const double fx = w * (sx[L] / hL - sx[R] / hR);
+        const double fy = w * (sy[L] / hL - sy[R] / hR);
         if (i == L) {
             vx = vx - dtp * fx / area / den;
             vy = vy - dtp * fy / area / den;
@@ -482,8 +496,8 @@ __global__ __launch_bounds__(kBlock) void k_parab_sub(MeshDev m, Clock *ck, int 
             vy = vy + dtp * fy / area / den;
         }
     }
-    m.pqx[i] = vx;
-    m.pqy[i] = vy;
+    dx[i] = vx;
+    dy[i] = vy;
 }
 
 // Value of one station's records at time tw given its bracket k
@@ -568,6 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
         carry_clock(ck, p);
         return;
     }
+    if (part == 2 && ck->parab_pending) return;  // finished by the host
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= m.n_cell) return;
     const double dt = ck->DT[1 - p];
@@ -787,7 +802,8 @@ void launch_update(sw2dg_ctx *c, int p) {
     hipLaunchKernelGGL(k_parab_dt, dim3(nblocks(c->ni)), blk, 0, c->stream, c->m, c->ck, p);
     hipLaunchKernelGGL(k_parab_n, dim3(1), dim3(kDtBlock), 0, c->stream, c->m, c->o, c->ck, p,
                        nblocks(c->ni));
-    hipLaunchKernelGGL(k_parab_sub, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->ck, p);
+    hipLaunchKernelGGL(k_parab_sub, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->ck, p, c->m.qx, c->m.qy,
+                       c->m.pqx, c->m.pqy);
     hipLaunchKernelGGL(k_update<true>, dim3(nblocks(c->nc)), blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 2);
 }
 
@@ -825,6 +841,38 @@ int push_clock(sw2dg_ctx *c) {
     return SW2DG_OK;
 }
 
+// The momentum-diffusion substeps a step left pending (k_parab_n: n > 1),
+// then the rest of that step (k_update part 2); slot q of the clock holds
+// the step's outcome (the live slot).  The substeps alternate between
+// (qx, qy) and (pqx, pqy); part 2 reads the latter.  Afterwards the next
+// step's activity is restored.
+int finish_parab(sw2dg_ctx *c, int q) {
+    Clock &k = c->hck;
+    const int p = 1 - q;  // the kernels run as the step of parity p: dt = DT[q]
+    const int n = k.parab_n;
+    k.parab_pending = 0;
+    k.ACT[p] = 1;
+    int st = push_clock(c);
+    if (st) return st;
+    const dim3 blk(kBlock), gc(nblocks(c->nc));
+    double *bx[2] = {c->m.qx, c->m.pqx}, *by[2] = {c->m.qy, c->m.pqy};
+    for (int s = 0; s < n; ++s)
+        hipLaunchKernelGGL(k_parab_sub, gc, blk, 0, c->stream, c->m, c->ck, p, bx[s & 1], by[s & 1],
+                           bx[(s + 1) & 1], by[(s + 1) & 1]);
+    if ((n & 1) == 0) {  // the result is in (qx, qy)
+        HIPCHK(c, hipMemcpyAsync(c->m.pqx, c->m.qx, c->nc * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->m.pqy, c->m.qy, c->nc * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    }
+    hipLaunchKernelGGL(k_update<true>, gc, blk, 0, c->stream, c->m, c->o, c->bc, c->ck, p, 2);
+    HIPCHK(c, hipGetLastError());
+    if ((st = pull_clock(c))) return st;
+    k.ACT[p] = 0;
+    k.ACT[q] = k.act_saved;
+    return push_clock(c);
+}
+
 int map_err(int st) {
     switch (st) {
         case sp::ST_NAN: return SW2DG_ERR_NAN;
@@ -843,8 +891,7 @@ const char *err_text(int st) {
         case sp::ST_FLOW_CONFIG: return "Impossible flow configuration in HLLC";
         case sp::ST_BC: return "invalid boundary condition value or type";
         case sp::ST_PARAB_SUBSTEPS:
-            return "momentum diffusion needs more than one explicit substep in this step "
-                   "(only one is built)";
+            return "momentum diffusion needs more than 2^20 explicit substeps in one step";
         default: return "device error";
     }
 }
@@ -1419,6 +1466,7 @@ int sw2dg_set_state(sw2dg_ctx *c, const double *h, const double *qx, const doubl
     c->hck.paused = 0;
     c->hck.store_flags = 0;
     c->hck.dt_last = 0.0;
+    c->hck.parab_pending = 0;
     c->state_set = true;
     return push_clock(c);
 }
@@ -1464,6 +1512,10 @@ int sw2dg_run(sw2dg_ctx *c, double t_stop, int64_t max_steps, int32_t pause_on_s
     while (c->hck.ACT[0]) {
         HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
         if ((st = pull_clock(c))) return st;
+        if (c->hck.err) break;
+        // a step with more than one diffusion substep idled the rest of the
+        // launch: the live clock is in slot 0 (the idle steps carried it)
+        if (c->hck.parab_pending && (st = finish_parab(c, 0))) return st;
         if (c->hck.err) break;
     }
     if (n_done) *n_done = c->hck.STEP[0] - step0;
@@ -1546,6 +1598,10 @@ int sw2dg_profile_kernels(sw2dg_ctx *c, int64_t n_steps, int32_t k_max, double *
         launch_update(c, p);
         hipEventRecord(ev[2 * k + 1], c->stream);
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->o.mdiff) {  // substeps left pending by the step (outside the timed kernels)
+            if ((st = pull_clock(c))) return st;
+            if (c->hck.parab_pending && (st = finish_parab(c, 1 - p))) return st;
+        }
         for (int j = 0; j < nk; ++j) {
             float ms = 0.f;
             hipEventElapsedTime(&ms, ev[2 * j], ev[2 * j + 1]);

@@ -35,6 +35,9 @@ struct Clock {
     int wind_k;          // record bracket at t_half: -1 first record, -2 last, k >= 1 (k-1, k)
     int rain_k;          // the same on the rain records
     double parab_dt;     // explicit momentum-diffusion substep of the step
+    int parab_n;         // its number of substeps
+    int parab_pending;   // n > 1: substeps left to the host (the launch's later steps idle)
+    int act_saved;       // ACT of the next step, restored once the substeps are done
 };
 
 struct BcDev {

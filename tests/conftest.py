@@ -78,6 +78,15 @@ def time_of(result_file):
     return d * 86400 + h * 3600 + m * 60 + sec + ms / 1000.0
 
 
+def map_time(g, k):
+    """Time of map k of a golden fixture: from its file name, or from the
+    fixture's own record where the name cannot carry it (sub-millisecond
+    runs: every map of those is named for t = 0)."""
+    if "times" in g:
+        return float(g["times"][k])
+    return time_of(str(g["files"][k]))
+
+
 def rel_err(a, b):
     """max |a-b| / max |b| (field-normalised), 0 when both are 0."""
     scale = max(float(np.max(np.abs(b))), 1e-300)

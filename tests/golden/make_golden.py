@@ -107,6 +107,22 @@ def golden_cases():
         "mdiff_dam_small_muscl_manning_wind": ("dambreak_case", dict(
             nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0, scheme="muscl_evr", friction="manning",
             t_max=3.0, dt_map=1.0, mdiff=True, wind=True)),
+        # momentum diffusion with several explicit substeps per step: the
+        # bound area phi / (2 D) falls below dt / 2 only on sub-micron cells
+        # (about 3 and 2 substeps at the start of these two); the map names
+        # carry milliseconds, so the one map written is the final state
+        "mdiff_micro_substeps3_godunov": ("dambreak_case", dict(
+            nx=20, ny=10, Lx=1.0e-6, Ly=5.0e-7, x_dam=5.0e-7, scheme="godunov", friction="none",
+            t_max=1.0e-7, mdiff=True)),
+        "mdiff_micro_substeps2_godunov": ("dambreak_case", dict(
+            nx=20, ny=10, Lx=1.3e-6, Ly=6.5e-7, x_dam=6.5e-7, scheme="godunov", friction="none",
+            t_max=1.3e-7, mdiff=True)),
+        # momentum diffusion divides q by h of both cells of every edge
+        # (swSpParab.cpp:74-75): a dry cell gives 0 / 0, and the reference
+        # stops at the next flux computation (swSpHyp.cpp:391)
+        "mdiff_dry_cell_error": ("dambreak_case", dict(
+            nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0, h_right=0.0, scheme="godunov",
+            friction="none", t_max=3.0, dt_map=1.0, mdiff=True)),
         # the reference stops on an impossible HLLC regime (swSpHyp.cpp:729-739)
         "mixed_muscl_closed_cells_error": ("mixed_case", dict(scheme="muscl_evr", friction="strickler",
                                                              t_max=20.0, closed_fraction=0.04)),
@@ -230,6 +246,8 @@ def main(only=None):
             keep = list(range(len(outs)))
             if name.startswith("dambreak_20k"):
                 keep = [len(outs) - 1]  # final map only for the full-size case
+            if "_micro_" in name:  # one map, named for t = 0, holding the state at t_max
+                meta["times"] = np.array([kw["t_max"]])
             np.savez_compressed(os.path.join(OUT, name + ".npz"), files=np.array([times[k] for k in keep]),
                                 h=np.array([H[k] for k in keep]), qx=np.array([QX[k] for k in keep]),
                                 qy=np.array([QY[k] for k in keep]), **meta)

@@ -120,7 +120,7 @@ def test_run_minimization_dispatch(sw):
         run_minimization(Quad(), "nelder-mead")
 
 
-def test_qsva_window_ends():
+def test_qsva_window_ends(sw):
     from sw2d_amd.porosity import qsva_window_ends
     assert qsva_window_ends(0.0, 10.0, 3.0) == [3.0, 6.0, 9.0, 10.0]
     assert qsva_window_ends(5.0, 11.0, 2.0) == [7.0, 9.0, 11.0]
@@ -129,7 +129,7 @@ def test_qsva_window_ends():
         qsva_window_ends(0.0, 1.0, 0.0)
 
 
-def test_qsva_follows_the_moving_minimum_with_warm_starts():
+def test_qsva_follows_the_moving_minimum_with_warm_starts(sw):
     """A cost whose minimum moves with the window end (phi* = t_end / 10 on
     every cell) and whose far-away basin traps a cold start on the full
     window: the continuation reaches the full window's absolute minimum from

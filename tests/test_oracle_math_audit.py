@@ -18,8 +18,8 @@ import os
 import numpy as np
 import pytest
 
-from conftest import (e_golden_names, golden_names, load_golden, load_sp_toce, regen_case,
-                      rel_err, time_of)
+from conftest import (e_golden_names, golden_names, load_golden, load_sp_toce, map_time, regen_case,
+                      rel_err)
 
 
 def test_kernel_math_equals_glibc_exp_cbrt_and_rounds_log_correctly(oracle):
@@ -46,7 +46,7 @@ def test_native_builds_part_only_after_a_differing_transcendental(sw, oracle, tm
     A = oracle.OracleSolver(case, math="glibc")
     oracle.math_audit_reset()
     B = oracle.OracleSolver(case, math="device")
-    t_end = time_of(str(g["files"][-1]))
+    t_end = map_time(g, len(g["files"]) - 1)
     first_diff = None
     n = 0
     while A.t < t_end:

@@ -5,12 +5,15 @@ maps on the mdiff_* golden cases (test_oracle_golden.py runs them).  Its
 coefficient is the reference's fixed 1e-6 (swSp.h:244; the input key is never
 read), so the effect is small: here, switching it off moves the final maps
 by 5e-8 (depth) to 1e-6 (discharge) relative, four to six orders above the
-1e-12 parity tolerance.
+1e-12 parity tolerance.  The mdiff_micro_* fixtures (sub-micron cells) take
+several explicit substeps per step, odd and even counts among them; on a
+case with a dry cell the reference's q / h is 0 / 0 and it stops at the next
+flux computation, which the oracle reproduces.
 """
 import numpy as np
 import pytest
 
-from conftest import regen_case, rel_err, time_of
+from conftest import map_time, regen_case, rel_err, time_of
 
 
 @pytest.mark.parametrize("name", ["mdiff_dam_small_godunov", "mdiff_dam_small_muscl_manning_wind"])
@@ -41,3 +44,48 @@ def test_any_value_but_none_turns_it_on(sw, tmp_path):
     with open(solv, "w") as f:
         f.write(text.replace("velocity_independent", "anything"))
     assert sw.Case(folder)["source_models"][3] == 1
+
+
+def _substeps(case, h, dt, span):
+    """n of swSpParab.cpp:57-67 for the depths h of a step of length dt."""
+    a = case.a
+    L, R, phi, A = a["int_L"], a["int_R"], a["phi"], a["cell_area"]
+    dL, dR = 1e-6 * h[L] * phi[L], 1e-6 * h[R] * phi[R]
+    D = np.where(np.maximum(dL, dR) > 1e-16, 2.0 * dL * dR / (dL + dR), 0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        b = np.fmin(0.5 * A[L] * phi[L] / D, 0.5 * A[R] * phi[R] / D)
+    return int(dt / min(span, dt, float(np.nanmin(b))))
+
+
+@pytest.mark.parametrize("name", ["mdiff_micro_substeps3_godunov", "mdiff_micro_substeps2_godunov"])
+def test_micro_fixtures_take_several_substeps(sw, oracle, tmp_path, name):
+    """The oracle matches these fixtures bit for bit (test_oracle_golden.py);
+    here: their steps really take several substeps, both odd and even
+    counts (the diffusion step leaves h unchanged, so the depths after a
+    step are the ones its diffusion saw)."""
+    g = regen_case(sw, name, str(tmp_path))
+    case = sw.Case(str(tmp_path))
+    t_end = map_time(g, 0)
+    o = oracle.OracleSolver(case)
+    counts = []
+    while o.t < t_end:
+        o.step(1)
+        counts.append(_substeps(case, o.state()[0], o.dt, t_end))
+    assert counts[0] >= 2
+    multi = {n for n in counts if n > 1}
+    assert any(n % 2 == 0 for n in multi) and any(n % 2 == 1 for n in multi), counts
+
+
+def test_dry_cell_stops_like_the_reference(sw, oracle, tmp_path):
+    """Dry cells (h = 0 right of the dam) make q NaN after one step on every
+    cell with a dry neighbour (0 / 0 in the edge flux; a zero coefficient
+    does not mask it), and the next step's flux check stops the run, where
+    the reference stops (swSpHyp.cpp:391, recorded in the fixture)."""
+    g = regen_case(sw, "mdiff_dry_cell_error", str(tmp_path))
+    assert "hypComputeFluxes 391" in str(g["error"])
+    o = oracle.OracleSolver(sw.Case(str(tmp_path)))
+    o.step(1)
+    h, qx, qy = o.state()
+    assert np.all(np.isfinite(h)) and np.isnan(qx).sum() >= 100 and np.isnan(qy).sum() >= 100
+    with pytest.raises(RuntimeError, match="oracle error 2"):  # ORC_ERR_NAN
+        o.step(1)

@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import e_golden_names, golden_names, load_golden, regen_case, time_of
+from conftest import e_golden_names, golden_names, load_golden, map_time, regen_case
 
 NATIVE = [n for n in golden_names() if not n.endswith("_error")]
 
@@ -31,7 +31,7 @@ def test_omp_native_matches_reference_maps(sw, oracle, tmp_path, name):
     g = regen_case(sw, name, str(tmp_path))
     o = oracle.OracleSolver(sw.Case(str(tmp_path)), math="omp")
     for k, f in enumerate(g["files"]):
-        t = time_of(str(f))
+        t = map_time(g, k)
         if t > 0:
             o.run(t_end=t)
         h, qx, qy = o.state()
