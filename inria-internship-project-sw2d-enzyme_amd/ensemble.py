@@ -200,11 +200,17 @@ class EnsembleProblem:
     problem's prior run in one launch and return the gauge series."""
 
     def __init__(self, n_runs=256, seed=7, window=1200.0, level_quantile=0.9, sizes=None,
-                 device=-1, replicate=1):
+                 device=-1, replicate=1, workers=None):
+        from concurrent.futures import ThreadPoolExecutor
+
         from .calibration import Calibration
         px = log_uniform_sizes(n_runs, seed) if sizes is None else np.asarray(sizes)
-        meshes = [Calibration(px=int(p), window=window, level_quantile=level_quantile)
-                  for p in px]
+        # the meshes are independent and deterministic: built on `workers`
+        # host threads (the Delaunay and zoning work releases the GIL)
+        workers = workers or min(8, os.cpu_count() or 1)
+        build = lambda p: Calibration(px=int(p), window=window, level_quantile=level_quantile)
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            meshes = list(ex.map(build, px))
         # `replicate` runs per mesh (distinct controls, one device copy each)
         self.cals = [c for c in meshes for _ in range(int(replicate))]
         self.ens = Ensemble([c.case for c in self.cals], device=device)
