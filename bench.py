@@ -255,13 +255,24 @@ def forward_bench(sw, args):
     if not args.no_cpu:
         log("native forward: CPU baselines (OpenMP and serial oracle)")
         cpu = cpu_forward_baseline(case, args)
+    traffic = None  # HBM bytes of one step from the copy-calibrated counters
+    tpath = os.path.join(REPO, "profiles", "traffic_fwd_adj_1m.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f).get("forward_muscl_evr", {})
+        if all(k in tk for k in ("k_slopes", "k_edges<1>", "k_cfl", "k_update<false>")):
+            traffic = sum(tk[k] for k in ("k_slopes", "k_edges<1>", "k_cfl", "k_dt", "k_update<false>") if k in tk)
     return {"workload": "SP forward at scale, MUSCL-EVR + Manning 0.03, native semantics",
             "cpu_baseline": cpu,
             "value": case.n_cell * n / el, "unit": "cell-steps/s", "steps": n,
             "ms_per_step": el * 1e3 / n, "kernels_ms": prof,
             "roofline_step": {"algorithmic_bytes": Bf, "device_ms": step_ms,
                               "achieved": Bf / (step_ms * 1e-3) / 1e9,
-                              "frac": Bf / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+                              "frac": Bf / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "traffic": traffic,
+                              "note": "algorithmic bytes of one step over the summed per-kernel "
+                                      "device time of one step; traffic: the kernels' HBM bytes "
+                                      "per step (profiles/counters_forward_1m.md)"}}
 
 
 def host_info():
@@ -548,6 +559,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched-ensemble leg instead")
     ap.add_argument("--ens-runs", type=int, default=256)
+    ap.add_argument("--forward-only", action="store_true",
+                    help="only the native MUSCL-EVR forward line (for kernel traces and counters)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain copy kernel of known size (counter scaling)")
     ap.add_argument("--time-to-misfit", action="store_true",
@@ -582,6 +595,9 @@ def main():
         return
     if args.ensemble:
         print(json.dumps(ensemble_bench(sw, args)))
+        return
+    if args.forward_only:
+        print(json.dumps(forward_bench(sw, args)))
         return
     ratio = lambda sc: 0.0 if sc == "muscl_evr" else 1e8
     res = fwd_adj_bench(sw, args, args.scheme, ratio(args.scheme))
