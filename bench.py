@@ -95,13 +95,15 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
         # cells; the bytes count its inputs once.
         "k_e_update<false, false, false>": 120 * nc + 40 * ni + 32 * nb,
         "k_e_update<true, false, false>": 168 * nc + 40 * ni + 56 * nb,
-        "k_e_update<false, true, false>": 152 * nc + 56 * ni + 32 * nb,
-        "k_e_update<true, true, false>": 200 * nc + 56 * ni + 56 * nb,
-        # MUSCL-EVR dry run: edges once (states, fluxes and swept terms out),
-        # the CFL from the swept terms, the update from the stored fluxes
-        "k_e_edges_muscl": 80 * nc + 80 * ni + 48 * nb,
+        # MUSCL-EVR: edges once (states, fluxes and, dry run, swept terms
+        # out), the CFL from the swept terms, the update from the stored
+        # fluxes (F1 and each side's momentum sum, 40 bytes per edge) and the
+        # edge length; the replay's update also writes the balanced state +
+        # increments
+        "k_e_edges_muscl": 88 * nc + 80 * ni + 48 * nb,
         "k_e_cfl_sw": 56 * nc + 24 * ni + 28 * nb,
-        "k_e_update<false, true, true>": 104 * nc + 72 * ni + 52 * nb,
+        "k_e_update<false, true, true>": 104 * nc + 56 * ni + 52 * nb,
+        "k_e_update<true, true, true>": 152 * nc + 56 * ni + 52 * nb,
         # cells U(h), zb, centroid, CSR ptr, slopes out; incidences, edge cells
         # and midpoints
         "k_e_slopes": 52 * nc + 32 * ni,

@@ -309,20 +309,27 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
-// MUSCL-EVR dry run: the reconstruction is costly enough that the dry run
-// computes each edge once and keeps its fluxes and swept terms for the CFL
-// and the update (the replay and every Godunov pass compute fluxes in the
-// update itself).  Threads [0, n_int): internal edges -> F[5e..5e+4] and the
-// swept terms sw[2e + side]; threads [n_int, n_int + n_bnd): boundary edges
-// -> F[5 n_int + 3j ..], with the boundary value interpolated at T[n], as
-// build_checkpoint_records does.  dtp: the previous step's dt (nullptr: 0).
+// MUSCL-EVR: the reconstruction is costly enough that the forward computes
+// each edge once and keeps its fluxes (the Godunov passes compute fluxes in
+// the update itself).  Threads [0, n_int): internal edges -> F[5e..5e+4] =
+// {F1, then per side the momentum contribution without dt: (mx, my) of side
+// 0, (mx, my) of side 1} (side_contrib's sums, the factor of each side's
+// cell applied here), and, in the dry run, the swept terms sw[2e + side];
+// threads [n_int, n_int + n_bnd): boundary edges -> F[5 n_int + 3j ..] (and
+// Fb, the reverse sweep's copy, when given).  Dry run (T given): the
+// boundary value interpolated at T[n] and the reconstruction with the
+// previous step's dt (*dtp, 0 before the first), as
+// build_checkpoint_records does; replay (T null): the recorded boundary
+// values bcv and the step's own dt, dt_fix.
 __global__ __launch_bounds__(kBlock) void k_e_edges_muscl(MeshDev m, Opts o, Controls cs, BcDev bc,
                                                           const double *__restrict__ U,
                                                           double *__restrict__ F, double *__restrict__ sw,
                                                           const double *T, int64_t n, double t_max,
                                                           const double *__restrict__ sxy,
-                                                          const double *dtp) {
-    if (!(T[n] < t_max)) return;
+                                                          const double *dtp, double dt_fix,
+                                                          const double *__restrict__ bcv,
+                                                          double *__restrict__ Fb_out) {
+    if (T && !(T[n] < t_max)) return;
     const int e = xcd_block() * kBlock + (int)threadIdx.x;
     if (e >= m.n_int) {
         const int j = e - m.n_int;
@@ -330,7 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_e_edges_muscl(MeshDev m, Opts o, Con
         const int i = m.bcell[j];
         const spe::Side sd = e_bnd_state<true>(m, o, U, i, j);
         const int ser = m.bser[j];
-        const double v = interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n], bc.sorted);
+        const double v = T ? interp_e(bc.t, bc.v + (size_t)ser * bc.n_times, bc.n_times, T[n], bc.sorted)
+                           : bcv[ser];
         double f[3];
         spe::bnd_fwd(o.h_min, o.phi_min, m.beta[i], cs.phi[i], m.zb[i], sd.z, sd.q, sd.r, sd.lp,
                      bc.type[ser], v, f);
@@ -338,24 +346,36 @@ __global__ __launch_bounds__(kBlock) void k_e_edges_muscl(MeshDev m, Opts o, Con
         Fb[0] = f[0];
         Fb[1] = f[1];
         Fb[2] = f[2];
+        if (Fb_out) {
+            Fb_out[3 * (size_t)j] = f[0];
+            Fb_out[3 * (size_t)j + 1] = f[1];
+            Fb_out[3 * (size_t)j + 2] = f[2];
+        }
         return;
     }
     const int L = m.eL[e], R = m.eR[e];
     spe::Side sL, sR;
-    e_int_states<true>(m, o, U, sxy, dtp ? *dtp : 0.0, e, sL, sR);
+    e_int_states<true>(m, o, U, sxy, T ? (dtp ? *dtp : 0.0) : dt_fix, e, sL, sR);
     double f[5];
     spe::hllc_fwd(o.h_min, o.phi_min, m.beta[L], m.beta[R], cs.phi[L], cs.phi[R], m.zb[L], m.zb[R],
                   sL.z, sL.q, sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+    const double len = m.elen[e], nx = m.enx[e], ny = m.eny[e];
+    const double coL = spe::edge_coeff(len, m.area[L], cs.phi[L], o.phi_min);
+    const double coR = spe::edge_coeff(len, m.area[R], cs.phi[R], o.phi_min);
     double *Fe = F + 5 * (size_t)e;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Fe[k] = f[k];
-    double m1 = fabs(sL.lm), v = fabs(sL.lp);
-    if (m1 < v) m1 = v;
-    if (m1 < (v = fabs(sR.lm))) m1 = v;
-    if (m1 < (v = fabs(sR.lp))) m1 = v;
-    const double len = m.elen[e];
-    sw[2 * e] = (U[L] > o.h_min) ? m1 * len : 0.0;
-    sw[2 * e + 1] = (U[R] > o.h_min) ? m1 * len : 0.0;
+    Fe[0] = f[0];
+    Fe[1] = (-coL * nx) * f[1] + (coL * nx) * f[3] + (coL * ny) * f[2];
+    Fe[2] = (-coL * ny) * f[1] + (coL * ny) * f[3] + (-coL * nx) * f[2];
+    Fe[3] = (coR * nx) * f[1] + (coR * nx) * f[4] + (-coR * ny) * f[2];
+    Fe[4] = (coR * ny) * f[1] + (coR * ny) * f[4] + (coR * nx) * f[2];
+    if (sw) {
+        double m1 = fabs(sL.lm), v = fabs(sL.lp);
+        if (m1 < v) m1 = v;
+        if (m1 < (v = fabs(sR.lm))) m1 = v;
+        if (m1 < (v = fabs(sR.lp))) m1 = v;
+        sw[2 * e] = (U[L] > o.h_min) ? m1 * len : 0.0;
+        sw[2 * e + 1] = (U[R] > o.h_min) ? m1 * len : 0.0;
+    }
 }
 
 // the MUSCL-EVR dry run's CFL from the swept terms of k_e_edges_muscl (the
@@ -565,11 +585,13 @@ __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, con
     return s;
 }
 
-// the same balance from stored fluxes F (k_e_edges_muscl's layout: 5 per
-// internal edge, then 3 per boundary edge)
+// the same balance from the fluxes k_e_edges_muscl stored (5 per internal
+// edge: F1 and the two sides' momentum sums; then 3 per boundary edge):
+// side_contrib's operations, its sums taken from the edge pass
 __device__ __forceinline__ CellSt e_balance_stored(const MeshDev &m, const Opts &o, const Controls &cs,
                                                    const double *__restrict__ U,
-                                                   const double *__restrict__ F, int i, double dt) {
+                                                   const double *__restrict__ F, int i, double dt,
+                                                   CellSt *dint = nullptr) {
     const int nc = m.n_cell;
     CellSt s{U[i], U[nc + i], U[2 * nc + i]};
     const double phi = cs.phi[i], area = m.area[i];
@@ -587,23 +609,32 @@ __device__ __forceinline__ CellSt e_balance_stored(const MeshDev &m, const Opts 
         const int inc = m.int_inc[k];
         const int e = inc >> 1, side = inc & 1;
         const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
-        double a, b, c;
-        spe::side_contrib(side, co, m.enx[e], m.eny[e], F + 5 * (size_t)e, dt, a, b, c);
+        const double *f = F + 5 * (size_t)e;
+        const double a = (side ? co : -co) * (f[0] * dt);
+        const double b = f[1 + 2 * side] * dt;
+        const double c = f[2 + 2 * side] * dt;
         s.h += a;
         s.qx += b;
         s.qy += c;
+        if (dint) {
+            dint->h += a;
+            dint->qx += b;
+            dint->qy += c;
+        }
     }
     return s;
 }
 
 // per cell: edge fluxes and balance, clean-up, clamp, friction _E, head loss
-// _E.  STORED (the MUSCL-EVR dry run): the fluxes come from k_e_edges_muscl
-// in Fst instead.  The dry run (T given) takes the boundary values the schedule kernel
-// recorded for the step and reconstructs (MUSCL) with the previous step's
-// dt (*dtp, 0 before the first); the replay takes dtr, the step's own.
-// STORE (the replay before a reverse sweep): also keep the balanced state
-// before the clean-up and the sum of its internal-edge increments,
-// P = [pre | dint], and the boundary fluxes Fb.
+// _E.  STORED (MUSCL-EVR, dry run and replay): the fluxes come from
+// k_e_edges_muscl in Fst instead.  The dry run (T given) takes the boundary
+// values the schedule kernel recorded for the step and reconstructs (MUSCL)
+// with the previous step's dt (*dtp, 0 before the first); the replay takes
+// dtr, the step's own.  STORE (a replay before a reverse sweep, or a
+// Godunov dry run that keeps the resident trajectory): also keep the
+// balanced state before the clean-up and the sum of its internal-edge
+// increments, P = [pre | dint], and (computing the fluxes here) the
+// boundary fluxes Fb.
 // (the Godunov dry run that keeps the resident trajectory fits 96 VGPRs
 // without spills: 5 waves per SIMD instead of 4, 68 -> 65.5 us at 1M cells)
 template <bool STORE, bool MUSCL, bool STORED = false>
@@ -633,7 +664,7 @@ __global__ __launch_bounds__(kBlock, (STORE && !MUSCL) ? 5 : 1) void k_e_update(
     const double dt = dtv[n];
     const double dt_recon = MUSCL ? (T ? (dtp ? *dtp : 0.0) : dtr) : 0.0;
     CellSt dint{0.0, 0.0, 0.0};
-    CellSt s = STORED ? e_balance_stored(m, o, cs, Uin, Fst, i, dt)
+    CellSt s = STORED ? e_balance_stored(m, o, cs, Uin, Fst, i, dt, STORE ? &dint : nullptr)
                       : e_balance<MUSCL>(m, o, cs, bc, Uin, sxy, dt_recon, bcv, i, dt, Fb,
                                          STORE ? &dint : nullptr);
     if (STORE) {
@@ -1288,7 +1319,7 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
     if (dry) {
         if (muscl) {  // edges once, their fluxes and swept terms kept
             KLAUNCH(a, k_e_edges_muscl, dim3(nblocks(c->ni + c->nb)), blk, 0, c->stream, c->m, c->o, cs,
-                    c->bc, Uin, a->d_F, a->d_sw, a->d_T, n, c->prm.t_max, sxy, dtp);
+                    c->bc, Uin, a->d_F, a->d_sw, a->d_T, n, c->prm.t_max, sxy, dtp, 0.0, nullptr, nullptr);
             KLAUNCH(a, k_e_cfl_sw, gc, blk, 0, c->stream, c->m, c->o, Uin, a->d_sw, a->d_ck, a->d_partial,
                     a->d_T, n);
         } else {
@@ -1304,16 +1335,26 @@ void launch_e_step(sw2dg_ctx *c, AssimState *a, const double *Uin, double *Uout,
     KLAUNCH_NAMED(a, "k_e_update<" #STORE ", " #MUS ", false>", (k_e_update<STORE, MUS>), gc, blk, 0, c->stream, \
                   c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n, c->prm.t_max, P, Fb, \
                   nullptr, gs)
-    if (P) {
-        if (muscl) E_UPDATE(true, true);
-        else E_UPDATE(true, false);
-    } else if (dry && muscl) {
-        KLAUNCH_NAMED(a, "k_e_update<false, true, true>", (k_e_update<false, true, true>), gc, blk, 0,
-                      c->stream, c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n,
-                      c->prm.t_max, P, Fb, a->d_F, gs);
+    if (muscl) {
+        // MUSCL-EVR: the update from the fluxes of one edge pass (the dry
+        // run's above; the replay's here: reconstruction with the step's own
+        // dt and the recorded boundary values, boundary fluxes also into the
+        // segment's slot Fb for the reverse sweep)
+        if (!dry)
+            KLAUNCH(a, k_e_edges_muscl, dim3(nblocks(c->ni + c->nb)), blk, 0, c->stream, c->m, c->o, cs, c->bc,
+                    Uin, a->d_F, nullptr, nullptr, n, c->prm.t_max, sxy, nullptr, dtr, bcv, Fb);
+        if (P)
+            KLAUNCH_NAMED(a, "k_e_update<true, true, true>", (k_e_update<true, true, true>), gc, blk, 0,
+                          c->stream, c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n,
+                          c->prm.t_max, P, Fb, a->d_F, gs);
+        else
+            KLAUNCH_NAMED(a, "k_e_update<false, true, true>", (k_e_update<false, true, true>), gc, blk, 0,
+                          c->stream, c->m, c->o, cs, c->bc, Uin, Uout, sxy, dtp, dtr, bcv, a->d_dt, T, n,
+                          c->prm.t_max, P, Fb, a->d_F, gs);
+    } else if (P) {
+        E_UPDATE(true, false);
     } else {
-        if (muscl) E_UPDATE(false, true);
-        else E_UPDATE(false, false);
+        E_UPDATE(false, false);
     }
 #undef E_UPDATE
     if (dry && a->d_oframes)
