@@ -176,7 +176,7 @@ class Calibration:
 
     def fourdvar(self, solver, obs, n0, Q0, J_tol=0.0, max_iter=60, max_steps=200000, m=6,
                  n_bounds=(0.01, 0.1), Q_bounds=(0.0, 5000.0), max_ls=30, time_limit=None,
-                 n_tiles=1, ls_cost="auto", adj_cut=0.0):
+                 n_tiles=1, ls_cost="auto", adj_cut=0.0, log=None):
         """Projected L-BFGS (memory m) on the 68 controls from (n0, Q0)
         against the gauge series `obs`, organised like the reference's
         minimiser (assim/swSpE_minimize.cpp:398-511): a gradient evaluation
@@ -192,7 +192,8 @@ class Calibration:
         (n / 0.03, Q / max Q_default) and kept in their bounds by projection.
         Stops when J <= J_tol, after max_iter iterations or time_limit
         seconds.  Returns (n, Q, trace) with trace = [(wall s, J, kind)] for
-        every evaluation, kind 'grad' or 'ls'."""
+        every evaluation, kind 'grad' or 'ls' (each also passed to `log`, if
+        given, as it happens)."""
         from .assim import Assim, interp_weights
         A = Assim(solver, self.obs_t, gauge_cells=self.gauge_cells, gauge_h=obs,
                   max_steps=max_steps, n_tiles=n_tiles, adj_cut=adj_cut)
@@ -203,6 +204,11 @@ class Calibration:
         trace = []
         t0 = time.perf_counter()
         state = {}
+
+        def note(J, kind):
+            trace.append((time.perf_counter() - t0, J, kind))
+            if log:
+                log(f"4D-Var {kind}: {trace[-1][0]:.1f} s, J = {J:.4e}")
 
         def controls(z):
             return z[:nz] * sn, z[nz:] * sq
@@ -217,7 +223,7 @@ class Calibration:
             gn, gQ = self.grad_controls(G, n, rec["t"][:-1])
             state["bc"] = rec["bc"].copy()
             state["W"] = interp_weights(self.t_knots, rec["t"][:-1])
-            trace.append((time.perf_counter() - t0, G["J"], "grad"))
+            note(G["J"], "grad")
             return G["J"], np.concatenate([gn * sn, gQ * sq])
 
         if ls_cost == "auto":
@@ -232,7 +238,7 @@ class Calibration:
                 A.record(K1=K, K2=K)
                 self._check_window(A.get_record(), max_steps)
                 J = float(np.sum((A.gauge_series() - obs_arr) ** 2))
-                trace.append((time.perf_counter() - t0, J, "ls"))
+                note(J, "ls")
                 return J
             # frozen schedule of the last gradient evaluation
             bc = state["bc"].copy()
@@ -240,7 +246,7 @@ class Calibration:
             A.set_bc_seq(bc)
             K = self.K(n)
             J = A.cost(K1=K, K2=K)
-            trace.append((time.perf_counter() - t0, J, "ls"))
+            note(J, "ls")
             return J
 
         z = np.clip(np.concatenate([np.asarray(n0) / sn, np.asarray(Q0) / sq]), lo, hi)

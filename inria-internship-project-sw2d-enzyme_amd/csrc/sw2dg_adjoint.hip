@@ -110,8 +110,7 @@ struct AssimState {
     // adjoint
     double *d_lam = nullptr, *d_lams = nullptr, *d_part = nullptr, *d_side = nullptr;
     double *d_sd = nullptr;    // MUSCL-EVR: 4 n_int slope adjoints per edge side
-    double *d_zn = nullptr;    // MUSCL-EVR: 2 n_int neighbour free-surface adjoints (CSR slots)
-    int *d_mirror = nullptr;   // MUSCL-EVR: CSR slot of the other cell's incidence on the same edge
+    double *d_zn = nullptr;    // MUSCL-EVR: 2 n_int neighbour free-surface adjoints (per edge side)
     double *d_gphi = nullptr, *d_gK1 = nullptr, *d_gK2 = nullptr, *d_gbce = nullptr,
            *d_gbc = nullptr;
     // change of variable of the porosity control (phi_correct / dphi_dm,
@@ -891,7 +890,7 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
 //   seed    + dJ_obs/dU^n when U^n is observed
 //   CELL    reverse of the cell part of step n-1 (else: write lam = adj U^n)
 // gather: -1 nothing to gather (the last step, or a tile top: the future
-// adjoint is dropped), 1 gather part + edge sides (+ zn[mirror], MUSCL-EVR:
+// adjoint is dropped), 1 gather part + edge sides (+ zn, MUSCL-EVR:
 // the neighbours' slope adjoints through this cell's free surface), 2 take lam
 // (the gather already done by the CELL = false launch that closed the next
 // segment).
@@ -907,8 +906,7 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
                                                  double *gK1, double *gK2, int n_cell_blocks,
                                                  const int *ser_ptr, const int *ser_edges,
                                                  const double *gbce, double *gbc_n,
-                                                 const double *__restrict__ zn,
-                                                 const int *__restrict__ mirror) {
+                                                 const double *__restrict__ zn) {
     if ((int)blockIdx.x >= n_cell_blocks) {
         __shared__ double red[kBlock];
         const int k = (int)blockIdx.x - n_cell_blocks;
@@ -941,7 +939,7 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
             b += sv[1];
             c += sv[2];
             gp += sv[3];
-            if (zn) a += zn[mirror[k]];  // the neighbour's slope through this cell's surface
+            if (zn) a += zn[inc];  // the neighbour's slope through this cell's surface
         }
         if (gather > 0) {  // not a tile top
             ah = a;
@@ -1113,14 +1111,16 @@ __global__ __launch_bounds__(kBlock, MUSCL ? 3 : 4) void k_r_edge(MeshDev m, Opt
 }
 
 // the slope adjoint's term for the neighbour across a cell's incidence k
-// goes to zn[b + k], the incidence's own CSR slot (contiguous per cell); the
-// neighbour reads it through the mirror index of its own incidence
+// (edge e) goes to zn[2 e + the neighbour's side], the slot of the
+// neighbour's own incidence on e: each slot has one writer, and the gather
+// reads it at the index of its incidence, next to the edge's side record
 struct ZnOut {
     double *zn;
-    int b;
-    __device__ void set(int k, double v) const { zn[b + k] = v; }
-    __device__ void add(int k, double v) const { zn[b + k] += v; }
-    __device__ void put(int k, double v) const { zn[b + k] = v; }
+    const int *inc;  // the cell's incidences: entry (e << 1) | side
+    __device__ size_t slot(int k) const { return (size_t)(inc[k] ^ 1); }
+    __device__ void set(int k, double v) const { zn[slot(k)] = v; }
+    __device__ void add(int k, double v) const { zn[slot(k)] += v; }
+    __device__ void put(int k, double v) const { zn[slot(k)] = v; }
 };
 
 // cells of at most this many internal edges (every triangle or quad) keep
@@ -1145,7 +1145,7 @@ __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__
         gy += sd[2 * (size_t)inc + 1];
     }
     const NbDev nb{m, U, b};
-    const ZnOut out{zn, b};
+    const ZnOut out{zn, m.int_inc + b};
     const double zi = U[i] + m.zb[i], xi = m.cx[i], yi = m.cy[i];
     double dzi = 0.0;
     if (!spe::slope_vjp_reg<kSlopeRegDeg>(nb, deg, zi, xi, yi, gx, gy, dzi, out))
@@ -1640,14 +1640,6 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_Sseg = a->alloc<double>(muscl ? (size_t)a->stride * 2 * (size_t)nc : 1);
     a->d_sd = a->alloc<double>(muscl ? 4 * (size_t)std::max<int64_t>(ni, 1) : 1);
     a->d_zn = a->alloc<double>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
-    a->d_mirror = a->alloc<int>(muscl ? 2 * (size_t)std::max<int64_t>(ni, 1) : 1);
-    if (muscl && ni > 0) {  // mirror[p] = slot of (e, 1 - side) for the slot p of (e, side)
-        std::vector<int> inc(2 * ni), slot(2 * ni), mir(2 * ni);
-        HIPCHK(c, hipMemcpy(inc.data(), c->m.int_inc, inc.size() * sizeof(int), hipMemcpyDeviceToHost));
-        for (int64_t p = 0; p < 2 * ni; ++p) slot[inc[p]] = (int)p;
-        for (int64_t p = 0; p < 2 * ni; ++p) mir[p] = slot[inc[p] ^ 1];
-        HIPCHK(c, hipMemcpy(a->d_mirror, mir.data(), mir.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
     a->d_partial = a->alloc<double>(nblocks(nc));
     a->d_cpart = a->alloc<double>(nblocks(nc));
     a->d_J = a->alloc<double>(1 + (size_t)std::max(1, obs->n_obs));
@@ -1869,11 +1861,11 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
         if (cell)
             KLAUNCH(a, k_r_gc<true>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
-                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn, a->d_mirror);
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn);
         else
             KLAUNCH(a, k_r_gc<false>, dim3(ncb + extra), blk, 0, c->stream, c->m, c->o, cs, a->d_part,
                     a->d_side, gather, sd, P, dtv, a->d_lams, a->d_lam, a->d_gphi, a->d_gK1, a->d_gK2,
-                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn, a->d_mirror);
+                    ncb, a->d_ser_ptr, a->d_ser_edges, a->d_gbce, gbcn, zn);
     };
     // edge reverse of step n (segment slot j); MUSCL-EVR: then the slopes
     // edge reverse of step n (with its boundary part in extra blocks)

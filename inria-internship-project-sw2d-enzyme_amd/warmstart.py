@@ -171,7 +171,7 @@ def time_to_misfit(cal, solver, obs, starts, J_tol, max_iter=200, time_limit=600
     for name, (n0, Q0, setup_s) in starts.items():
         n, Q, trace = cal.fourdvar(solver, obs, n0, Q0, J_tol=J_tol, max_iter=max_iter,
                                    max_steps=max_steps, time_limit=time_limit, n_tiles=n_tiles,
-                                   max_ls=12, adj_cut=adj_cut)
+                                   max_ls=12, adj_cut=adj_cut, log=log)
         hit = next((t for t, J, _ in trace if J <= J_tol), None)
         res[name] = dict(
             seconds=None if hit is None else hit + setup_s, setup_s=setup_s,
