@@ -182,19 +182,24 @@ SPD int hllc(bool blending, double h_min, double phi_min, double bL, double bR, 
     return ST_OK;
 }
 
-// velocity damping factor over one step: swSourceFriction.cpp:7-79
-SPD double friction_factor(int model, double coeff, double h, double u_norm, double dt) {
+// h^(-4/3) as the reference's friction laws form it, exp(-4/3 log h)
+SPD double friction_hpow(double h) { return sw_exp(-4.0 / 3.0 * sw_log(h)); }
+
+// velocity damping factor over one step: swSourceFriction.cpp:7-79.  hp:
+// friction_hpow(h) for Manning and Strickler (formed once per cell by the
+// caller: both principal directions use the same value)
+SPD double friction_factor(int model, double coeff, double h, double u_norm, double dt, double hp) {
     double factor = 1.0;
     if (model == FRIC_MANNING) {
         if (h > kSmall) {
-            double arg = -kG * u_norm * coeff * coeff * sw_exp(-4.0 / 3.0 * sw_log(h));
+            double arg = -kG * u_norm * coeff * coeff * hp;
             factor = sw_exp(arg * dt);
         } else {
             factor = 0.0;
         }
     } else if (model == FRIC_STRICKLER) {
         if ((h > kSmall) && (coeff > kSmall)) {
-            double arg = -kG * u_norm / coeff / coeff * sw_exp(-4.0 / 3.0 * sw_log(h));
+            double arg = -kG * u_norm / coeff / coeff * hp;
             factor = sw_exp(arg * dt);
         } else {
             factor = 0.0;

@@ -647,8 +647,10 @@ __global__ __launch_bounds__(kBlock) void k_update(MeshDev m, Opts o, BcDev bc, 
             double ul = u * ca + v * sa;
             double vl = -u * sa + v * ca;
             const double un = sqrt(u * u + v * v);
-            const double f1 = sp::friction_factor(o.friction, m.fc1[i], h, un, dt);
-            const double f2 = sp::friction_factor(o.friction, m.fc2[i], h, un, dt);
+            const double hp = (o.friction != sp::FRIC_CHEZY && h > kSmall) ? sp::friction_hpow(h) : 0.0;
+            const double c1 = m.fc1[i], c2 = m.fc2[i];
+            const double f1 = sp::friction_factor(o.friction, c1, h, un, dt, hp);
+            const double f2 = (c2 == c1) ? f1 : sp::friction_factor(o.friction, c2, h, un, dt, hp);
             ul *= f1;
             vl *= f2;
             qx = (ul * ca - vl * sa) * h;
