@@ -206,16 +206,23 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double
             double f[5];
             spe::hllc_fwd(o.h_min, o.phi_min, beta[L], beta[Rr], phi[L], phi[Rr], zb[L], zb[Rr], sL.z, sL.q,
                           sL.r, sR.z, sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+            // F1 and each side's momentum sum with its cell's factor (the
+            // sums of side_contrib; the update multiplies by dt)
+            const double len = elen[e];
+            const double coL = spe::edge_coeff(len, area[L], phi[L], o.phi_min);
+            const double coR = spe::edge_coeff(len, area[Rr], phi[Rr], o.phi_min);
             double *Fe = F + 5 * (size_t)e;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) Fe[k] = f[k];
+            Fe[0] = f[0];
+            Fe[1] = (-coL * nx) * f[1] + (coL * nx) * f[3] + (coL * ny) * f[2];
+            Fe[2] = (-coL * ny) * f[1] + (coL * ny) * f[3] + (-coL * nx) * f[2];
+            Fe[3] = (coR * nx) * f[1] + (coR * nx) * f[4] + (-coR * ny) * f[2];
+            Fe[4] = (coR * ny) * f[1] + (coR * ny) * f[4] + (coR * nx) * f[2];
             double m1 = fabs(sL.lm), v = fabs(sL.lp);
             if (m1 < v) m1 = v;
             if (m1 < (v = fabs(sR.lm))) m1 = v;
             if (m1 < (v = fabs(sR.lp))) m1 = v;
-            const double len = elen[e];
-            sw[2 * e] = (hL > o.h_min) ? m1 * len : 0.0;
-            sw[2 * e + 1] = (hR > o.h_min) ? m1 * len : 0.0;
+            reinterpret_cast<double2 *>(sw)[e] =
+                make_double2((hL > o.h_min) ? m1 * len : 0.0, (hR > o.h_min) ? m1 * len : 0.0);
         }
         if (!run_sync(cl, r, c, target, &s_flag)) break;
         // ---- CFL (k_e_cfl): per-cell candidate, cluster minimum
@@ -231,10 +238,7 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double
                     swept += lm * blen[j];
                 }
             }
-            for (int k = iptr[i]; k < iptr[i + 1]; ++k) {
-                const int inc = iinc[k];
-                swept += sw[2 * (size_t)(inc >> 1) + (inc & 1)];
-            }
+            for (int k = iptr[i]; k < iptr[i + 1]; ++k) swept += sw[iinc[k]];  // (edge << 1) | side
             const double den = (swept < kSmall) ? kSmall : swept;
             const double x = cfl * (area[i] / den);
             if (x < cand) cand = x;
@@ -294,11 +298,10 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double
                 const int inc = iinc[k];
                 const int e = inc >> 1, side = inc & 1;
                 const double co = spe::edge_coeff(elen[e], ar, ph, o.phi_min);
-                double a, b, c;
-                spe::side_contrib(side, co, enx[e], eny[e], F + 5 * (size_t)e, dt, a, b, c);
-                sh += a;
-                sx += b;
-                sy += c;
+                const double *f = F + 5 * (size_t)e;
+                sh += (side ? co : -co) * (f[0] * dt);
+                sx += f[1 + 2 * side] * dt;
+                sy += f[2 + 2 * side] * dt;
             }
             if (fabs(sh) < kSmall) sh = 0.0;
             if (fabs(sx) < kSmall) sx = 0.0;
