@@ -369,6 +369,15 @@ int sw2dg_mlp_grad(sw2dg_mlp *mlp, int32_t batch, const float *x, const float *t
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on a batch; loss may be NULL. */
 int sw2dg_mlp_train_step(sw2dg_mlp *mlp, int32_t batch, const float *x, const float *t, float lr,
                          float *loss);
+/* A training set kept on the device: n rows of x [n x dims[0]] (stored in
+ * BF16) and t [n x dims[L]]; replaces the previous one. */
+int sw2dg_mlp_set_data(sw2dg_mlp *mlp, int64_t n, const float *x, const float *t);
+/* n_steps Adam steps on the resident training set, step k on its rows
+ * rows[k * batch .. (k + 1) * batch), with one wait at the end: the same
+ * results as n_steps calls of sw2dg_mlp_train_step on those rows, bit for
+ * bit.  step_loss [n_steps] (may be NULL): each step's loss. */
+int sw2dg_mlp_train_epoch(sw2dg_mlp *mlp, int32_t batch, int64_t n_steps, const int32_t *rows, float lr,
+                          float *step_loss);
 const char *sw2dg_mlp_last_error(sw2dg_mlp *mlp);
 void sw2dg_mlp_destroy(sw2dg_mlp *mlp);
 

@@ -52,40 +52,117 @@ struct Epi {
     int ldz;
 };
 
-constexpr int TM = 64, TN = 64, TK = 32, PADK = 8;
+constexpr int TM = 64, TN = 64, TK = 64, LDK = TK + 8;  // LDS rows of 72 BF16 (144 bytes, 16-byte aligned)
+// Split-K: tiles x splits never exceed this, so the FP32 partials of one GEMM
+// hold at most kMaxSplitTiles x 64 x 64 floats (8 MiB).
+constexpr int kMaxSplitTiles = 512;
 
+// 8 consecutive BF16 from global memory, zero past the first `cnt`; one
+// 16-byte load when all 8 are in range and the address is 16-byte aligned
+__device__ __forceinline__ uint4 ld8(const bf16_t *p, int cnt, bool vec) {
+    if (vec && cnt == 8) return *reinterpret_cast<const uint4 *>(p);
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (i < cnt) w[i >> 1] |= (uint32_t)p[i] << (16 * (i & 1));
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ bf16_t half_of(const uint4 &r, int i) {
+    const uint32_t w = i < 2 ? r.x : i < 4 ? r.y : i < 6 ? r.z : r.w;
+    return (bf16_t)(w >> (16 * (i & 1)));
+}
+
+template <int EPIK>
+__device__ __forceinline__ void epi_store(const Epi &ep, int row, int col, float x) {
+    const size_t o = (size_t)row * ep.ldz + col;
+    if (EPIK == EPI_HIDDEN) {
+        const float z = x + ep.bias[col];
+        if (ep.z) ep.z[o] = z;  // kept for the backward pass only
+        ep.h[o] = f2bf(gelu(z));
+    } else if (EPIK == EPI_OUT) {
+        ep.z[o] = x + ep.bias[col];
+    } else if (EPIK == EPI_BACK_ACT) {
+        const float dz = x * gelu_d(ep.zsave[o]);
+        ep.z[o] = dz;
+        ep.h[o] = f2bf(dz);
+    } else {
+        ep.z[o] = x;
+    }
+}
+
+// One 64 x 64 output tile per workgroup (4 waves, 32 x 32 each), over the
+// k-tiles [blockIdx.z * kt_per, +kt_per) of 64.  Each thread stages two
+// 8-element vectors of A and two of B per k-tile (16-byte loads along the
+// contiguous index; a vector along m or n is written to LDS transposed), and
+// loads the next k-tile into registers before the MFMAs of the current one.
+// part == nullptr: the epilogue writes the result; else the raw FP32 sums of
+// this split go to part[blockIdx.z][M][N] and k_split_sum finishes.
 template <bool TA, bool TB, int EPIK>
 __global__ __launch_bounds__(256) void k_gemm(const bf16_t *__restrict__ A, const bf16_t *__restrict__ B,
-                                             int M, int N, int K, int lda, int ldb, Epi ep) {
-    __shared__ __attribute__((aligned(16))) bf16_t sA[TM][TK + PADK];  // [m][k]
-    __shared__ __attribute__((aligned(16))) bf16_t sB[TN][TK + PADK];  // [n][k]
+                                             int M, int N, int K, int lda, int ldb, int kt_per,
+                                             float *__restrict__ part, Epi ep) {
+    __shared__ __attribute__((aligned(16))) bf16_t sA[TM][LDK];  // [m][k]
+    __shared__ __attribute__((aligned(16))) bf16_t sB[TN][LDK];  // [n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int mb = blockIdx.y * TM, nb = blockIdx.x * TN;
+    const int nkt = (K + TK - 1) / TK;
+    const int kt0 = blockIdx.z * kt_per, kt1 = min(nkt, kt0 + kt_per);
+    const bool va = ((lda & 7) == 0) && (((uintptr_t)A & 15) == 0);
+    const bool vb = ((ldb & 7) == 0) && (((uintptr_t)B & 15) == 0);
+    uint4 ra[2], rb[2];
+    auto fetch = [&](int kt) {
+        const int kb = kt * TK;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int v = tid + 256 * j, r = v >> 3, q = (v & 7) * 8;
+            if (!TA) {  // row m = r of A, k from q
+                const int gm = mb + r, gk = kb + q;
+                const int cnt = gm < M ? max(0, min(8, K - gk)) : 0;
+                ra[j] = ld8(A + (size_t)gm * lda + gk, cnt, va);
+            } else {    // row k = r of A^T storage, m from q
+                const int gk = kb + r, gm = mb + q;
+                const int cnt = gk < K ? max(0, min(8, M - gm)) : 0;
+                ra[j] = ld8(A + (size_t)gk * lda + gm, cnt, va);
+            }
+            if (TB) {   // row n = r of B^T storage, k from q
+                const int gn = nb + r, gk = kb + q;
+                const int cnt = gn < N ? max(0, min(8, K - gk)) : 0;
+                rb[j] = ld8(B + (size_t)gn * ldb + gk, cnt, vb);
+            } else {    // row k = r of B, n from q
+                const int gk = kb + r, gn = nb + q;
+                const int cnt = gk < K ? max(0, min(8, N - gn)) : 0;
+                rb[j] = ld8(B + (size_t)gk * ldb + gn, cnt, vb);
+            }
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int v = tid + 256 * j, r = v >> 3, q = (v & 7) * 8;
+            if (!TA) {
+                *reinterpret_cast<uint4 *>(&sA[r][q]) = ra[j];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) sA[q + i][r] = half_of(ra[j], i);
+            }
+            if (TB) {
+                *reinterpret_cast<uint4 *>(&sB[r][q]) = rb[j];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) sB[q + i][r] = half_of(rb[j], i);
+            }
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-    for (int kb = 0; kb < K; kb += TK) {
-        // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
-        for (int q = tid; q < TM * TK; q += 256) {
-            int m, k;
-            if (TA) { m = q % TM; k = q / TM; }  // walk m fastest: contiguous in A^T storage
-            else { k = q % TK; m = q / TK; }
-            const int gm = mb + m, gk = kb + k;
-            bf16_t v = 0;
-            if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            sA[m][k] = v;
-        }
-        for (int q = tid; q < TN * TK; q += 256) {
-            int n, k;
-            if (TB) { k = q % TK; n = q / TK; }
-            else { n = q % TN; k = q / TN; }
-            const int gn = nb + n, gk = kb + k;
-            bf16_t v = 0;
-            if (gn < N && gk < K) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-            sB[n][k] = v;
-        }
+    if (kt0 < kt1) fetch(kt0);
+    for (int kt = kt0; kt < kt1; ++kt) {
+        stash();
         __syncthreads();
+        if (kt + 1 < kt1) fetch(kt + 1);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 16) {
             const int r = lane & 31, k0 = kk + 8 * (lane >> 5);
@@ -95,28 +172,37 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16_t *__restrict__ A, cons
         }
         __syncthreads();
     }
-    // epilogue: lane holds column (lane % 32), rows 8 (v / 4) + 4 (lane / 32) + v % 4
+    // lane holds column (lane % 32), rows 8 (v / 4) + 4 (lane / 32) + v % 4
     const int col = nb + 32 * wn + (lane & 31);
+    float *pz = part ? part + (size_t)blockIdx.z * M * N : nullptr;
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         const int row = mb + 32 * wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         if (row >= M || col >= N) continue;
-        const size_t o = (size_t)row * ep.ldz + col;
-        const float x = acc[v];
-        if (EPIK == EPI_HIDDEN) {
-            const float z = x + ep.bias[col];
-            if (ep.z) ep.z[o] = z;  // kept for the backward pass only
-            ep.h[o] = f2bf(gelu(z));
-        } else if (EPIK == EPI_OUT) {
-            ep.z[o] = x + ep.bias[col];
-        } else if (EPIK == EPI_BACK_ACT) {
-            const float dz = x * gelu_d(ep.zsave[o]);
-            ep.z[o] = dz;
-            ep.h[o] = f2bf(dz);
-        } else {
-            ep.z[o] = x;
-        }
+        if (pz) pz[(size_t)row * N + col] = acc[v];
+        else epi_store<EPIK>(ep, row, col, acc[v]);
     }
+}
+
+// the splits of a split-K GEMM summed in order (split 0 first), then the epilogue
+template <int EPIK>
+__global__ __launch_bounds__(256) void k_split_sum(const float *__restrict__ part, int S, int M, int N, Epi ep) {
+    const size_t MN = (size_t)M * N, i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= MN) return;
+    float x = part[i];
+    for (int s = 1; s < S; ++s) x += part[(size_t)s * MN + i];
+    epi_store<EPIK>(ep, (int)(i / N), (int)(i % N), x);
+}
+
+// training batch: rows idx[0..B) of the resident data set (BF16 inputs, FP32 targets)
+__global__ void k_gather_rows(const bf16_t *__restrict__ dx, const float *__restrict__ dt,
+                              const int32_t *__restrict__ idx, int B, int d0, int dl,
+                              bf16_t *__restrict__ xb, float *__restrict__ tb) {
+    const int r = blockIdx.x;
+    if (r >= B) return;
+    const size_t src = (size_t)idx[r];
+    for (int c = threadIdx.x; c < d0; c += blockDim.x) xb[(size_t)r * d0 + c] = dx[src * d0 + c];
+    for (int c = threadIdx.x; c < dl; c += blockDim.x) tb[(size_t)r * dl + c] = dt[src * dl + c];
 }
 
 __global__ void k_to_bf16(const float *__restrict__ x, bf16_t *__restrict__ y, size_t n) {
@@ -124,30 +210,51 @@ __global__ void k_to_bf16(const float *__restrict__ x, bf16_t *__restrict__ y, s
     if (i < n) y[i] = f2bf(x[i]);
 }
 
-// column sums (bias gradient), fixed order over the rows
-__global__ void k_colsum(const float *__restrict__ x, int M, int N, float *__restrict__ out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// column sums (bias gradient): a workgroup per 64 columns, its 4 waves each
+// summing every 4th row in row order (8 loads in flight), the 4 partial sums
+// added in wave order -- a fixed order, so the result is deterministic
+__global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ x, int M, int N,
+                                               float *__restrict__ out) {
+    __shared__ float part[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), w = threadIdx.x >> 6;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += x[(size_t)m * N + n];
-    out[n] = s;
+    if (c < N) {
+        int m = w;
+        for (; m + 28 < M; m += 32) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = x[(size_t)(m + 4 * j) * N + c];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += v[j];
+        }
+        for (; m < M; m += 4) s += x[(size_t)m * N + c];
+    }
+    part[w][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (w == 0 && c < N) out[c] = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                                  part[3][threadIdx.x];
 }
 
-// mean squared error over the batch: dY = 2 (Y - T) / (M N); per-row partial losses
-__global__ void k_mse(const float *__restrict__ y, const float *__restrict__ t, int M, int N,
-                      float *__restrict__ dy, bf16_t *__restrict__ dyb, float *__restrict__ row_loss) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
+// mean squared error over the batch: dY = 2 (Y - T) / (M N); per-row
+// partial losses.  One wave per row: lane j takes columns j, j + 64, ...;
+// the lanes' sums meet in a fixed butterfly order.
+__global__ __launch_bounds__(256) void k_mse(const float *__restrict__ y, const float *__restrict__ t, int M,
+                                            int N, float *__restrict__ dy, bf16_t *__restrict__ dyb,
+                                            float *__restrict__ row_loss) {
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (m >= M) return;  // a whole wave leaves together
     const float sc = 2.0f / ((float)M * (float)N);
     float l = 0.0f;
-    for (int n = 0; n < N; ++n) {
+    for (int n = lane; n < N; n += 64) {
         const size_t o = (size_t)m * N + n;
         const float d = y[o] - t[o];
         l += d * d;
         dy[o] = sc * d;
         dyb[o] = f2bf(sc * d);
     }
-    row_loss[m] = l;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+    if (lane == 0) row_loss[m] = l;
 }
 
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
@@ -210,15 +317,29 @@ struct sw2dg_mlp {
     std::string err;
     int L = 0, max_batch = 0;
     std::vector<int> dims;
-    // per layer: W [in x out] fp32 master + bf16 copy, b [out]; Adam moments
-    std::vector<float *> W, b, gW, gb, mW, vW, mb, vb;
+    // FP32 parameters P (per layer W [in x out] then b [out]), gradients G,
+    // Adam moments M1 / M2, BF16 copy PB, all with the same padded layout of
+    // n_pad elements; per-layer pointers into them
+    float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;
+    bf16_t *PB = nullptr;
+    size_t n_pad = 0;
+    std::vector<float *> W, b, gW, gb;
     std::vector<bf16_t *> Wb;
     // activations: X bf16 [B x d0]; per layer: Z fp32, H bf16 (hidden), dZ fp32 / bf16
     bf16_t *xb = nullptr;
     std::vector<float *> Z, dZ;
     std::vector<bf16_t *> H, dZb;
     float *X32 = nullptr, *T32 = nullptr, *row_loss = nullptr;
+    float *part = nullptr;  // FP32 partials of the split-K training GEMMs (kMaxSplitTiles x 64 x 64)
     int64_t step = 0;
+    // resident training set (sw2dg_mlp_set_data) and the row indices / row
+    // losses of one sw2dg_mlp_train_epoch call; freed on replacement
+    bf16_t *data_x = nullptr;
+    float *data_t = nullptr;
+    int64_t data_n = 0;
+    int32_t *ep_rows = nullptr;
+    float *ep_loss = nullptr;
+    size_t ep_cap = 0;
     std::vector<void *> allocs;
     template <class T_>
     T_ *dalloc(size_t n) {
@@ -230,6 +351,8 @@ struct sw2dg_mlp {
     }
     ~sw2dg_mlp() {
         for (void *p : allocs) (void)hipFree(p);
+        for (void *p : {(void *)data_x, (void *)data_t, (void *)ep_rows, (void *)ep_loss})
+            if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -245,17 +368,38 @@ int mfail(sw2dg_mlp *m, int code, const char *msg) {
         if (e_ != hipSuccess) return mfail(m, SW2DG_ERR_HIP, hipGetErrorString(e_));   \
     } while (0)
 
+// Splits of the K loop for a GEMM of `tiles` output tiles and `nkt` k-tiles:
+// none when the tiles fill the chip (256 CUs); else as many as keep at least
+// 2 k-tiles per split and tiles x splits <= kMaxSplitTiles, with no empty split.
+int k_splits(int tiles, int nkt, int &kt_per) {
+    int S = 1;
+    if (tiles < 256 && nkt >= 4) S = std::max(1, std::min(kMaxSplitTiles / tiles, nkt / 2));
+    kt_per = (nkt + S - 1) / S;
+    return (nkt + kt_per - 1) / kt_per;
+}
+
+// C = op(A) op(B) with the epilogue; part != nullptr allows split-K through
+// it (training), nullptr keeps one split (inference: every output row is then
+// the same whatever the batch it is computed in)
 template <bool TA, bool TB, int EPIK>
 void gemm_on(hipStream_t s, const bf16_t *A, const bf16_t *B, int M, int N, int K, int lda, int ldb,
-             const Epi &ep) {
-    const dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPIK>), grid, dim3(256), 0, s, A, B, M, N, K, lda, ldb, ep);
+             const Epi &ep, float *part) {
+    const int tiles_n = (N + TN - 1) / TN, tiles_m = (M + TM - 1) / TM, nkt = (K + TK - 1) / TK;
+    int kt_per = nkt;
+    const int S = part ? k_splits(tiles_n * tiles_m, nkt, kt_per) : 1;
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPIK>), dim3(tiles_n, tiles_m, S), dim3(256), 0, s, A, B, M, N, K, lda,
+                       ldb, kt_per, S > 1 ? part : nullptr, ep);
+    if (S > 1) {
+        const size_t MN = (size_t)M * N;
+        hipLaunchKernelGGL((k_split_sum<EPIK>), dim3((unsigned)((MN + 255) / 256)), dim3(256), 0, s, part, S, M, N,
+                           ep);
+    }
 }
 
 template <bool TA, bool TB, int EPIK>
 void gemm(sw2dg_mlp *m, const bf16_t *A, const bf16_t *B, int M, int N, int K, int lda, int ldb,
-          const Epi &ep) {
-    gemm_on<TA, TB, EPIK>(m->stream, A, B, M, N, K, lda, ldb, ep);
+          const Epi &ep, bool split) {
+    gemm_on<TA, TB, EPIK>(m->stream, A, B, M, N, K, lda, ldb, ep, split ? m->part : nullptr);
 }
 
 // bytes of scratch a forward of `batch` rows takes: the BF16 input, then the
@@ -273,13 +417,14 @@ size_t n_params(const sw2dg_mlp *m) {
 }
 
 // forward of a batch whose inputs are in m->xb; output fp32 in m->Z[L-1]
-void forward(sw2dg_mlp *m, int B) {
+// (split: the training path, split-K allowed)
+void forward(sw2dg_mlp *m, int B, bool split) {
     const bf16_t *in = m->xb;
     for (int l = 0; l < m->L; ++l) {
         const int K = m->dims[l], N = m->dims[l + 1];
         Epi ep{m->b[l], m->Z[l], l + 1 < m->L ? m->H[l] : nullptr, nullptr, N};
-        if (l + 1 < m->L) gemm<false, false, EPI_HIDDEN>(m, in, m->Wb[l], B, N, K, K, N, ep);
-        else gemm<false, false, EPI_OUT>(m, in, m->Wb[l], B, N, K, K, N, ep);
+        if (l + 1 < m->L) gemm<false, false, EPI_HIDDEN>(m, in, m->Wb[l], B, N, K, K, N, ep, split);
+        else gemm<false, false, EPI_OUT>(m, in, m->Wb[l], B, N, K, K, N, ep, split);
         in = m->H[l];
     }
 }
@@ -291,11 +436,11 @@ void backward(sw2dg_mlp *m, int B) {
         const bf16_t *in = l ? m->H[l - 1] : m->xb;
         // dW = in^T dZ  [K x N]
         Epi ew{nullptr, m->gW[l], nullptr, nullptr, N};
-        gemm<true, false, EPI_PLAIN>(m, in, m->dZb[l], K, N, B, K, N, ew);
-        hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, m->stream, m->dZ[l], B, N, m->gb[l]);
+        gemm<true, false, EPI_PLAIN>(m, in, m->dZb[l], K, N, B, K, N, ew, true);
+        hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(256), 0, m->stream, m->dZ[l], B, N, m->gb[l]);
         if (l > 0) {  // dZ_{l-1} = (dZ W^T) * GELU'(Z_{l-1})
             Epi ed{nullptr, m->dZ[l - 1], m->dZb[l - 1], m->Z[l - 1], K};
-            gemm<false, true, EPI_BACK_ACT>(m, m->dZb[l], m->Wb[l], B, K, N, N, N, ed);
+            gemm<false, true, EPI_BACK_ACT>(m, m->dZb[l], m->Wb[l], B, K, N, N, N, ed, true);
         }
     }
 }
@@ -309,20 +454,40 @@ int load_batch(sw2dg_mlp *m, int B, const float *x, const float *t) {
     return SW2DG_OK;
 }
 
-int loss_and_backward(sw2dg_mlp *m, int B, float *loss) {
+// the loss of a batch from its per-row squared errors: summed in row order in double
+float batch_loss(const float *rl, int B, int dl) {
+    double s = 0.0;
+    for (int i = 0; i < B; ++i) s += rl[i];
+    return (float)(s / ((double)B * dl));
+}
+
+// MSE of the forward's output against m->T32 (per-row sums to row_loss), then the backward
+void mse_and_backward(sw2dg_mlp *m, int B, float *row_loss) {
     const int dl = m->dims[m->L];
-    hipLaunchKernelGGL(k_mse, dim3((B + 255) / 256), dim3(256), 0, m->stream, m->Z[m->L - 1], m->T32, B, dl,
-                       m->dZ[m->L - 1], m->dZb[m->L - 1], m->row_loss);
+    hipLaunchKernelGGL(k_mse, dim3((B + 3) / 4), dim3(256), 0, m->stream, m->Z[m->L - 1], m->T32, B, dl,
+                       m->dZ[m->L - 1], m->dZb[m->L - 1], row_loss);
     backward(m, B);
+}
+
+int loss_and_backward(sw2dg_mlp *m, int B, float *loss) {
+    mse_and_backward(m, B, m->row_loss);
     if (loss) {
         std::vector<float> rl(B);
         MHIP(m, hipMemcpyAsync(rl.data(), m->row_loss, B * sizeof(float), hipMemcpyDeviceToHost, m->stream));
         MHIP(m, hipStreamSynchronize(m->stream));
-        double s = 0.0;
-        for (int i = 0; i < B; ++i) s += rl[i];
-        *loss = (float)(s / ((double)B * dl));
+        *loss = batch_loss(rl.data(), B, m->dims[m->L]);
     }
     return SW2DG_OK;
+}
+
+// one Adam step over every layer's weights and biases (the BF16 weight copies follow)
+void adam_step(sw2dg_mlp *m, float lr) {
+    m->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
+    // padding elements have zero gradient and moments: they stay zero
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->n_pad + 255) / 256)), dim3(256), 0, m->stream, m->P, m->G, m->M1,
+                       m->M2, m->PB, m->n_pad, lr, b1, b2, eps, c1, c2);
 }
 }  // namespace
 
@@ -352,17 +517,33 @@ int sw2dg_mlp_create(int32_t n_layers, const int32_t *dims, int32_t max_batch, i
     m->L = n_layers;
     m->max_batch = max_batch;
     m->dims.assign(dims, dims + n_layers + 1);
+    // parameters, gradients and Adam moments each in one array (every
+    // layer's W then b, segments padded to 64 elements so that each starts
+    // 256-byte aligned), so that one Adam launch covers the whole network
+    std::vector<size_t> offW, offb;
+    size_t P = 0;
     for (int l = 0; l < n_layers; ++l) {
-        const size_t nw = (size_t)dims[l] * dims[l + 1], nb = dims[l + 1];
-        m->W.push_back(m->dalloc<float>(nw));
-        m->Wb.push_back(m->dalloc<bf16_t>(nw));
-        m->b.push_back(m->dalloc<float>(nb));
-        m->gW.push_back(m->dalloc<float>(nw));
-        m->gb.push_back(m->dalloc<float>(nb));
-        m->mW.push_back(m->dalloc<float>(nw));
-        m->vW.push_back(m->dalloc<float>(nw));
-        m->mb.push_back(m->dalloc<float>(nb));
-        m->vb.push_back(m->dalloc<float>(nb));
+        offW.push_back(P);
+        P += ((size_t)dims[l] * dims[l + 1] + 63) / 64 * 64;
+        offb.push_back(P);
+        P += ((size_t)dims[l + 1] + 63) / 64 * 64;
+    }
+    m->n_pad = P;
+    m->P = m->dalloc<float>(P);
+    m->G = m->dalloc<float>(P);
+    m->M1 = m->dalloc<float>(P);
+    m->M2 = m->dalloc<float>(P);
+    m->PB = m->dalloc<bf16_t>(P);
+    if (!m->P || !m->G || !m->M1 || !m->M2 || !m->PB) {
+        delete m;
+        return SW2DG_ERR_HIP;
+    }
+    for (int l = 0; l < n_layers; ++l) {
+        m->W.push_back(m->P + offW[l]);
+        m->Wb.push_back(m->PB + offW[l]);
+        m->b.push_back(m->P + offb[l]);
+        m->gW.push_back(m->G + offW[l]);
+        m->gb.push_back(m->G + offb[l]);
         m->Z.push_back(m->dalloc<float>((size_t)max_batch * dims[l + 1]));
         m->dZ.push_back(m->dalloc<float>((size_t)max_batch * dims[l + 1]));
         m->H.push_back(m->dalloc<bf16_t>((size_t)max_batch * dims[l + 1]));
@@ -372,6 +553,7 @@ int sw2dg_mlp_create(int32_t n_layers, const int32_t *dims, int32_t max_batch, i
     m->X32 = m->dalloc<float>((size_t)max_batch * dims[0]);
     m->T32 = m->dalloc<float>((size_t)max_batch * dims[n_layers]);
     m->row_loss = m->dalloc<float>(max_batch);
+    m->part = m->dalloc<float>((size_t)kMaxSplitTiles * TM * TN);
     for (void *p : m->allocs)
         if (!p) {
             delete m;
@@ -394,11 +576,9 @@ int sw2dg_mlp_set_params(sw2dg_mlp *m, const float *params) {
         MHIP(m, hipMemcpy(m->b[l], params + o, nb * sizeof(float), hipMemcpyHostToDevice));
         o += nb;
         hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, m->stream, m->W[l], m->Wb[l], nw);
-        MHIP(m, hipMemsetAsync(m->mW[l], 0, nw * sizeof(float), m->stream));
-        MHIP(m, hipMemsetAsync(m->vW[l], 0, nw * sizeof(float), m->stream));
-        MHIP(m, hipMemsetAsync(m->mb[l], 0, nb * sizeof(float), m->stream));
-        MHIP(m, hipMemsetAsync(m->vb[l], 0, nb * sizeof(float), m->stream));
     }
+    MHIP(m, hipMemsetAsync(m->M1, 0, m->n_pad * sizeof(float), m->stream));
+    MHIP(m, hipMemsetAsync(m->M2, 0, m->n_pad * sizeof(float), m->stream));
     m->step = 0;
     MHIP(m, hipStreamSynchronize(m->stream));
     return SW2DG_OK;
@@ -424,7 +604,7 @@ int sw2dg_mlp_forward(sw2dg_mlp *m, int32_t batch, const float *x, float *y) {
     MHIP(m, hipSetDevice(m->device));
     int st = load_batch(m, batch, x, nullptr);
     if (st) return st;
-    forward(m, batch);
+    forward(m, batch, false);
     MHIP(m, hipGetLastError());
     MHIP(m, hipMemcpyAsync(y, m->Z[m->L - 1], (size_t)batch * m->dims[m->L] * sizeof(float),
                            hipMemcpyDeviceToHost, m->stream));
@@ -461,11 +641,11 @@ int sw2dg_mlp_forward_noalloc(sw2dg_mlp *m, int32_t batch, const float *x, float
             bf16_t *h = (bf16_t *)(base + off);
             off += align256((size_t)batch * N * sizeof(bf16_t));
             Epi ep{m->b[l], nullptr, h, nullptr, N};
-            gemm_on<false, false, EPI_HIDDEN>(s, in, m->Wb[l], batch, N, K, K, N, ep);
+            gemm_on<false, false, EPI_HIDDEN>(s, in, m->Wb[l], batch, N, K, K, N, ep, nullptr);
             in = h;
         } else {
             Epi ep{m->b[l], y, nullptr, nullptr, N};
-            gemm_on<false, false, EPI_OUT>(s, in, m->Wb[l], batch, N, K, K, N, ep);
+            gemm_on<false, false, EPI_OUT>(s, in, m->Wb[l], batch, N, K, K, N, ep, nullptr);
         }
     }
     MHIP(m, hipGetLastError());
@@ -479,7 +659,7 @@ int sw2dg_mlp_grad(sw2dg_mlp *m, int32_t batch, const float *x, const float *t, 
     MHIP(m, hipSetDevice(m->device));
     int st = load_batch(m, batch, x, t);
     if (st) return st;
-    forward(m, batch);
+    forward(m, batch, true);
     if ((st = loss_and_backward(m, batch, loss))) return st;
     MHIP(m, hipGetLastError());
     MHIP(m, hipStreamSynchronize(m->stream));
@@ -502,20 +682,79 @@ int sw2dg_mlp_train_step(sw2dg_mlp *m, int32_t batch, const float *x, const floa
     MHIP(m, hipSetDevice(m->device));
     int st = load_batch(m, batch, x, t);
     if (st) return st;
-    forward(m, batch);
+    forward(m, batch, true);
     if ((st = loss_and_backward(m, batch, loss))) return st;
-    m->step += 1;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
-    for (int l = 0; l < m->L; ++l) {
-        const size_t nw = (size_t)m->dims[l] * m->dims[l + 1], nb = m->dims[l + 1];
-        hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, m->stream, m->W[l], m->gW[l],
-                           m->mW[l], m->vW[l], m->Wb[l], nw, lr, b1, b2, eps, c1, c2);
-        hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, m->stream, m->b[l], m->gb[l],
-                           m->mb[l], m->vb[l], (bf16_t *)nullptr, nb, lr, b1, b2, eps, c1, c2);
-    }
+    adam_step(m, lr);
     MHIP(m, hipGetLastError());
     if (!loss) MHIP(m, hipStreamSynchronize(m->stream));
+    return SW2DG_OK;
+}
+
+int sw2dg_mlp_set_data(sw2dg_mlp *m, int64_t n, const float *x, const float *t) {
+    if (!m || !x || !t || n < 1) return SW2DG_ERR_ARG;
+    MHIP(m, hipSetDevice(m->device));
+    MHIP(m, hipStreamSynchronize(m->stream));
+    const int d0 = m->dims[0], dl = m->dims[m->L];
+    if (m->data_x) (void)hipFree(m->data_x);
+    if (m->data_t) (void)hipFree(m->data_t);
+    m->data_x = nullptr;
+    m->data_t = nullptr;
+    m->data_n = 0;
+    MHIP(m, hipMalloc((void **)&m->data_x, (size_t)n * d0 * sizeof(bf16_t)));
+    MHIP(m, hipMalloc((void **)&m->data_t, (size_t)n * dl * sizeof(float)));
+    // inputs through the FP32 staging buffer, max_batch rows at a time, with
+    // the same conversion as a batch of sw2dg_mlp_train_step
+    for (int64_t r = 0; r < n; r += m->max_batch) {
+        const size_t rows = (size_t)std::min<int64_t>(m->max_batch, n - r), cnt = rows * d0;
+        MHIP(m, hipMemcpyAsync(m->X32, x + (size_t)r * d0, cnt * sizeof(float), hipMemcpyHostToDevice, m->stream));
+        hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m->stream, m->X32,
+                           m->data_x + (size_t)r * d0, cnt);
+    }
+    MHIP(m, hipMemcpyAsync(m->data_t, t, (size_t)n * dl * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    MHIP(m, hipGetLastError());
+    MHIP(m, hipStreamSynchronize(m->stream));
+    m->data_n = n;
+    return SW2DG_OK;
+}
+
+int sw2dg_mlp_train_epoch(sw2dg_mlp *m, int32_t batch, int64_t n_steps, const int32_t *rows, float lr,
+                          float *step_loss) {
+    if (!m || !rows || batch < 1 || batch > m->max_batch || n_steps < 1 || !(lr > 0.f)) return SW2DG_ERR_ARG;
+    if (!m->data_x) return mfail(m, SW2DG_ERR_ARG, "sw2dg_mlp_train_epoch: no training set (sw2dg_mlp_set_data)");
+    const size_t nr = (size_t)n_steps * batch;
+    for (size_t i = 0; i < nr; ++i)
+        if (rows[i] < 0 || rows[i] >= m->data_n)
+            return mfail(m, SW2DG_ERR_ARG, "sw2dg_mlp_train_epoch: row index outside the training set");
+    MHIP(m, hipSetDevice(m->device));
+    MHIP(m, hipStreamSynchronize(m->stream));
+    if (nr > m->ep_cap) {
+        if (m->ep_rows) (void)hipFree(m->ep_rows);
+        if (m->ep_loss) (void)hipFree(m->ep_loss);
+        m->ep_rows = nullptr;
+        m->ep_loss = nullptr;
+        m->ep_cap = 0;
+        MHIP(m, hipMalloc((void **)&m->ep_rows, nr * sizeof(int32_t)));
+        MHIP(m, hipMalloc((void **)&m->ep_loss, nr * sizeof(float)));
+        m->ep_cap = nr;
+    }
+    MHIP(m, hipMemcpyAsync(m->ep_rows, rows, nr * sizeof(int32_t), hipMemcpyHostToDevice, m->stream));
+    const int d0 = m->dims[0], dl = m->dims[m->L];
+    for (int64_t k = 0; k < n_steps; ++k) {
+        hipLaunchKernelGGL(k_gather_rows, dim3(batch), dim3(256), 0, m->stream, m->data_x, m->data_t,
+                           m->ep_rows + (size_t)k * batch, batch, d0, dl, m->xb, m->T32);
+        forward(m, batch, true);
+        mse_and_backward(m, batch, m->ep_loss + (size_t)k * batch);
+        adam_step(m, lr);
+    }
+    MHIP(m, hipGetLastError());
+    if (step_loss) {
+        std::vector<float> rl(nr);
+        MHIP(m, hipMemcpyAsync(rl.data(), m->ep_loss, nr * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        MHIP(m, hipStreamSynchronize(m->stream));
+        for (int64_t k = 0; k < n_steps; ++k) step_loss[k] = batch_loss(rl.data() + (size_t)k * batch, batch, dl);
+    } else {
+        MHIP(m, hipStreamSynchronize(m->stream));
+    }
     return SW2DG_OK;
 }
 

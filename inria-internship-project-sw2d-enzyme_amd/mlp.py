@@ -4,7 +4,8 @@ BF16 MFMA kernels (FP32 accumulation, FP32 master weights).
 
 ``MLP(dims)`` holds the network; ``forward`` evaluates it, ``grad`` returns
 the mean-squared-error loss and its gradient, ``train`` runs Adam over
-shuffled minibatches.
+shuffled minibatches with the training set resident on the device (one
+wait per epoch).
 """
 import ctypes as C
 
@@ -22,6 +23,8 @@ _DECL = {
     "sw2dg_mlp_forward_noalloc": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "sw2dg_mlp_grad": ([C.c_void_p, C.c_int32, _f, _f, _f, _f], C.c_int),
     "sw2dg_mlp_train_step": ([C.c_void_p, C.c_int32, _f, _f, C.c_float, _f], C.c_int),
+    "sw2dg_mlp_set_data": ([C.c_void_p, C.c_int64, _f, _f], C.c_int),
+    "sw2dg_mlp_train_epoch": ([C.c_void_p, C.c_int32, C.c_int64, N._i32, C.c_float, _f], C.c_int),
     "sw2dg_mlp_last_error": ([C.c_void_p], C.c_char_p),
     "sw2dg_mlp_destroy": ([C.c_void_p], None),
 }
@@ -119,8 +122,51 @@ class MLP:
         self._check(self.lib.sw2dg_mlp_grad(self.h, len(x), _fp(x), _fp(t), C.byref(loss), _fp(g)))
         return float(loss.value), g
 
+    def set_data(self, x, t):
+        """Keeps the training set (x [n x dims[0]], t [n x dims[-1]]) on the device."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        if x.ndim != 2 or t.ndim != 2 or len(x) != len(t) or x.shape[1] != self.dims[0] \
+                or t.shape[1] != self.dims[-1]:
+            raise ValueError(f"training set shapes {x.shape}, {t.shape} do not fit dims {self.dims}")
+        self._check(self.lib.sw2dg_mlp_set_data(self.h, len(x), _fp(x), _fp(t)))
+        self.n_data = len(x)
+
     def train(self, x, t, epochs=50, batch=256, lr=1e-3, seed=0, lr_decay=1.0):
-        """Adam over shuffled minibatches; returns the mean loss of every epoch."""
+        """Adam over shuffled minibatches (each epoch a fresh permutation,
+        the last partial batch left out); returns the mean loss of every
+        epoch.  The set goes to the device once (set_data), then
+        train_resident."""
+        self.set_data(x, t)
+        return self.train_resident(epochs=epochs, batch=batch, lr=lr, seed=seed, lr_decay=lr_decay)
+
+    def train_resident(self, epochs=50, batch=256, lr=1e-3, seed=0, lr_decay=1.0):
+        """``train`` on the set last given to set_data: one
+        sw2dg_mlp_train_epoch call (one wait) per epoch."""
+        n = getattr(self, "n_data", 0)
+        if n == 0:
+            raise ValueError("no training set on the device: call set_data first")
+        rng = np.random.default_rng(seed)
+        n_steps = n // batch
+        hist = []
+        for ep in range(epochs):
+            idx = rng.permutation(n)
+            tot = 0.0
+            if n_steps:
+                rows = np.ascontiguousarray(idx[: n_steps * batch], dtype=np.int32)
+                losses = np.empty(n_steps, dtype=np.float32)
+                self._check(self.lib.sw2dg_mlp_train_epoch(self.h, int(batch), n_steps,
+                                                           rows.ctypes.data_as(N._i32), float(lr),
+                                                           _fp(losses)))
+                for v in losses:
+                    tot += float(v)
+            hist.append(tot / max(n_steps, 1))
+            lr *= lr_decay
+        return hist
+
+    def train_steps(self, x, t, epochs=1, batch=256, lr=1e-3, seed=0, lr_decay=1.0):
+        """``train`` one sw2dg_mlp_train_step call per minibatch (the batch
+        copied from the host each step); the same results."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
         rng = np.random.default_rng(seed)

@@ -127,9 +127,10 @@ class WarmStart:
         xs = ((X - self.x_mu) / self.x_sd).astype(np.float32)
         ys = ((Y - self.y_mu) / self.y_sd).astype(np.float32)
         self.net = MLP([X.shape[1], *self.hidden, Y.shape[1]], max_batch=max(batch, 1024), seed=seed)
+        self.net.set_data(xs[tr], ys[tr])
         hist = []
         for ep in range(epochs):
-            tl = self.net.train(xs[tr], ys[tr], epochs=1, batch=batch, lr=lr, seed=seed + ep)[0]
+            tl = self.net.train_resident(epochs=1, batch=batch, lr=lr, seed=seed + ep)[0]
             vl = float(np.mean((self.net.forward(xs[va]) - ys[va]) ** 2)) if n_val else float("nan")
             hist.append((tl, vl))
             lr *= lr_decay

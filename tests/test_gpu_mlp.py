@@ -71,3 +71,29 @@ def test_gpu_mlp_training_fits_a_smooth_map(sw):
     hist = net.train(x, t, epochs=20, batch=256, lr=1e-3)
     l1 = net.grad(x[:256], t[:256])[0]
     assert hist[-1] < hist[0] and l1 < l0 / 20, (l0, l1, hist[:3], hist[-3:])
+
+
+def test_gpu_mlp_resident_epochs_equal_host_fed_steps(sw):
+    """sw2dg_mlp_train_epoch (training set resident on the device, one wait
+    per epoch) against sw2dg_mlp_train_step per minibatch copied from the
+    host: same permutations, parameters and losses bit for bit; a set that
+    is not a multiple of the batch (its last partial batch left out)."""
+    from sw2d_amd import _native as N
+    from sw2d_amd import mlp
+    print("[gpu] mlp resident epochs vs host-fed steps", flush=True)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((1100, DIMS[0])).astype(np.float32)
+    t = rng.standard_normal((1100, DIMS[-1])).astype(np.float32)
+    a = mlp.MLP(DIMS, max_batch=256, seed=5)
+    b = mlp.MLP(DIMS, max_batch=256, seed=5)
+    ha = a.train(x, t, epochs=2, batch=256, lr=1e-3, seed=9, lr_decay=0.9)
+    hb = b.train_steps(x, t, epochs=2, batch=256, lr=1e-3, seed=9, lr_decay=0.9)
+    assert ha == hb, (ha, hb)
+    np.testing.assert_array_equal(a.params(), b.params())
+    # one more epoch on the resident set, as warm-start fitting does
+    assert a.train_resident(epochs=1, batch=128, lr=5e-4, seed=3) == \
+        b.train_steps(x, t, epochs=1, batch=128, lr=5e-4, seed=3)
+    np.testing.assert_array_equal(a.params(), b.params())
+    rows = np.full(256, 1100, dtype=np.int32)  # outside the set: refused before any launch
+    with pytest.raises(N.SW2DError, match="outside"):
+        a._check(a.lib.sw2dg_mlp_train_epoch(a.h, 256, 1, rows.ctypes.data_as(N._i32), 1e-3, None))
