@@ -4,9 +4,10 @@
 // estimate of the calibration controls; the 4D-Var refines from there.
 //
 // GEMM: C[M x N] = op(A)[M x K] op(B)[K x N], A/B in BF16 (row-major, op = T
-// reads the transpose), 64 x 64 output tile per workgroup of 4 waves, each
-// wave a 32 x 32 block of v_mfma_f32_32x32x16_bf16, K staged through LDS 32 at
-// a time.  Epilogues fuse the bias, GELU (forward), GELU' (backward) and the
+// reads the transpose), a 64 x 64 or 128 x 128 output tile per workgroup of 4
+// waves on v_mfma_f32_32x32x16_bf16, K staged through LDS 64 at a time with
+// 16-byte loads and a register prefetch; small grids split K (fixed-order
+// sum of the splits).  Epilogues fuse the bias, GELU (forward), GELU' (backward) and the
 // BF16 re-quantisation of activations.  Training: mean squared error, Adam
 // with FP32 master weights and BF16 working copies.
 #include <hip/hip_runtime.h>
@@ -52,7 +53,8 @@ struct Epi {
     int ldz;
 };
 
-constexpr int TM = 64, TN = 64, TK = 64, LDK = TK + 8;  // LDS rows of 72 BF16 (144 bytes, 16-byte aligned)
+constexpr int TK = 64, LDK = TK + 8;  // LDS rows of 72 BF16 (144 bytes, 16-byte aligned)
+constexpr int TM = 64, TN = 64;       // the small output tile (split-K allowed)
 // Split-K: tiles x splits never exceed this, so the FP32 partials of one GEMM
 // hold at most kMaxSplitTiles x 64 x 64 floats (8 MiB).
 constexpr int kMaxSplitTiles = 512;
@@ -91,47 +93,53 @@ __device__ __forceinline__ void epi_store(const Epi &ep, int row, int col, float
     }
 }
 
-// One 64 x 64 output tile per workgroup (4 waves, 32 x 32 each), over the
-// k-tiles [blockIdx.z * kt_per, +kt_per) of 64.  Each thread stages two
-// 8-element vectors of A and two of B per k-tile (16-byte loads along the
+// One BT x BT output tile per workgroup (BT = 64 or 128; 4 waves, each a
+// BT/2 x BT/2 block of 32 x 32 MFMA tiles), over the k-tiles
+// [blockIdx.z * kt_per, +kt_per) of 64.  Each thread stages BT/32
+// 8-element vectors of A and of B per k-tile (16-byte loads along the
 // contiguous index; a vector along m or n is written to LDS transposed), and
 // loads the next k-tile into registers before the MFMAs of the current one.
+// Every output element accumulates its k in the same order whatever BT is.
 // part == nullptr: the epilogue writes the result; else the raw FP32 sums of
 // this split go to part[blockIdx.z][M][N] and k_split_sum finishes.
-template <bool TA, bool TB, int EPIK>
+template <bool TA, bool TB, int EPIK, int BT>
 __global__ __launch_bounds__(256) void k_gemm(const bf16_t *__restrict__ A, const bf16_t *__restrict__ B,
                                              int M, int N, int K, int lda, int ldb, int kt_per,
                                              float *__restrict__ part, Epi ep) {
-    __shared__ __attribute__((aligned(16))) bf16_t sA[TM][LDK];  // [m][k]
-    __shared__ __attribute__((aligned(16))) bf16_t sB[TN][LDK];  // [n][k]
+    constexpr int NV = BT / 32, WT = BT / 2, NB = WT / 32;
+    __shared__ __attribute__((aligned(16))) bf16_t sA[BT][LDK];  // [m][k]
+    __shared__ __attribute__((aligned(16))) bf16_t sB[BT][LDK];  // [n][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int mb = blockIdx.y * TM, nb = blockIdx.x * TN;
+    const int mb = blockIdx.y * BT, nb = blockIdx.x * BT;
     const int nkt = (K + TK - 1) / TK;
     const int kt0 = blockIdx.z * kt_per, kt1 = min(nkt, kt0 + kt_per);
     const bool va = ((lda & 7) == 0) && (((uintptr_t)A & 15) == 0);
     const bool vb = ((ldb & 7) == 0) && (((uintptr_t)B & 15) == 0);
-    uint4 ra[2], rb[2];
+    // vector v of a tile: along k (row r = v / 8 of BT, k = 8 (v % 8)), or
+    // along m / n (k-row v / (BT / 8) of TK, m or n = 8 (v % (BT / 8)))
+    uint4 ra[NV], rb[NV];
     auto fetch = [&](int kt) {
         const int kb = kt * TK;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int v = tid + 256 * j, r = v >> 3, q = (v & 7) * 8;
-            if (!TA) {  // row m = r of A, k from q
-                const int gm = mb + r, gk = kb + q;
+        for (int j = 0; j < NV; ++j) {
+            const int v = tid + 256 * j;
+            const int rd = v >> 3, qd = (v & 7) * 8, rt = v / (BT / 8), qt = (v % (BT / 8)) * 8;
+            if (!TA) {  // row m = rd of A, k from qd
+                const int gm = mb + rd, gk = kb + qd;
                 const int cnt = gm < M ? max(0, min(8, K - gk)) : 0;
                 ra[j] = ld8(A + (size_t)gm * lda + gk, cnt, va);
-            } else {    // row k = r of A^T storage, m from q
-                const int gk = kb + r, gm = mb + q;
+            } else {    // row k = rt of A^T storage, m from qt
+                const int gk = kb + rt, gm = mb + qt;
                 const int cnt = gk < K ? max(0, min(8, M - gm)) : 0;
                 ra[j] = ld8(A + (size_t)gk * lda + gm, cnt, va);
             }
-            if (TB) {   // row n = r of B^T storage, k from q
-                const int gn = nb + r, gk = kb + q;
+            if (TB) {   // row n = rd of B^T storage, k from qd
+                const int gn = nb + rd, gk = kb + qd;
                 const int cnt = gn < N ? max(0, min(8, K - gk)) : 0;
                 rb[j] = ld8(B + (size_t)gn * ldb + gk, cnt, vb);
-            } else {    // row k = r of B, n from q
-                const int gk = kb + r, gn = nb + q;
+            } else {    // row k = rt of B, n from qt
+                const int gk = kb + rt, gn = nb + qt;
                 const int cnt = gk < K ? max(0, min(8, N - gn)) : 0;
                 rb[j] = ld8(B + (size_t)gk * ldb + gn, cnt, vb);
             }
@@ -139,25 +147,30 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16_t *__restrict__ A, cons
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int v = tid + 256 * j, r = v >> 3, q = (v & 7) * 8;
+        for (int j = 0; j < NV; ++j) {
+            const int v = tid + 256 * j;
+            const int rd = v >> 3, qd = (v & 7) * 8, rt = v / (BT / 8), qt = (v % (BT / 8)) * 8;
             if (!TA) {
-                *reinterpret_cast<uint4 *>(&sA[r][q]) = ra[j];
+                *reinterpret_cast<uint4 *>(&sA[rd][qd]) = ra[j];
             } else {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) sA[q + i][r] = half_of(ra[j], i);
+                for (int i = 0; i < 8; ++i) sA[qt + i][rt] = half_of(ra[j], i);
             }
             if (TB) {
-                *reinterpret_cast<uint4 *>(&sB[r][q]) = rb[j];
+                *reinterpret_cast<uint4 *>(&sB[rd][qd]) = rb[j];
             } else {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) sB[q + i][r] = half_of(rb[j], i);
+                for (int i = 0; i < 8; ++i) sB[qt + i][rt] = half_of(rb[j], i);
             }
         }
     };
-    f32x16 acc;
+    f32x16 acc[NB][NB];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
     if (kt0 < kt1) fetch(kt0);
     for (int kt = kt0; kt < kt1; ++kt) {
         stash();
@@ -166,22 +179,34 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16_t *__restrict__ A, cons
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 16) {
             const int r = lane & 31, k0 = kk + 8 * (lane >> 5);
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sA[32 * wm + r][k0]);
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[32 * wn + r][k0]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            bf16x8 a[NB], b[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&sA[WT * wm + 32 * i + r][k0]);
+#pragma unroll
+            for (int j = 0; j < NB; ++j) b[j] = *reinterpret_cast<const bf16x8 *>(&sB[WT * wn + 32 * j + r][k0]);
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
-    // lane holds column (lane % 32), rows 8 (v / 4) + 4 (lane / 32) + v % 4
-    const int col = nb + 32 * wn + (lane & 31);
+    // lane holds column (lane % 32), rows 8 (v / 4) + 4 (lane / 32) + v % 4 of each 32 x 32 tile
     float *pz = part ? part + (size_t)blockIdx.z * M * N : nullptr;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int row = mb + 32 * wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (row >= M || col >= N) continue;
-        if (pz) pz[(size_t)row * N + col] = acc[v];
-        else epi_store<EPIK>(ep, row, col, acc[v]);
-    }
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int col = nb + WT * wn + 32 * j + (lane & 31);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = mb + WT * wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (row >= M || col >= N) continue;
+                if (pz) pz[(size_t)row * N + col] = acc[i][j][v];
+                else epi_store<EPIK>(ep, row, col, acc[i][j][v]);
+            }
+        }
 }
 
 // the splits of a split-K GEMM summed in order (split 0 first), then the epilogue
@@ -257,9 +282,19 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ y, const 
     if (lane == 0) row_loss[m] = l;
 }
 
+// the weight segments of the parameter array: layer l's W [K x N] at off[l]
+struct WSegs {
+    int L;
+    size_t off[16];
+    int K[16], N[16];
+};
+
+// Adam over the whole parameter array; the BF16 copies follow: wb in the
+// same layout, wbt with every weight segment transposed ([N x K], the
+// forward GEMM's k-contiguous operand)
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m1,
-                       float *__restrict__ m2, bf16_t *__restrict__ wb, size_t n, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
+                       float *__restrict__ m2, bf16_t *__restrict__ wb, bf16_t *__restrict__ wbt, WSegs sg,
+                       size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -269,7 +304,22 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     m2[i] = b;
     const float wi = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
     w[i] = wi;
-    if (wb) wb[i] = f2bf(wi);
+    const bf16_t q = f2bf(wi);
+    wb[i] = q;
+    for (int l = 0; l < sg.L; ++l) {
+        const size_t e = i - sg.off[l];
+        if (i >= sg.off[l] && e < (size_t)sg.K[l] * sg.N[l]) {
+            wbt[sg.off[l] + (e % sg.N[l]) * sg.K[l] + e / sg.N[l]] = q;
+            break;
+        }
+    }
+}
+
+// W [K x N] FP32 -> its BF16 transpose [N x K]
+__global__ void k_to_bf16_t(const float *__restrict__ w, int K, int N, bf16_t *__restrict__ wt) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * N) return;
+    wt[(i % N) * K + i / N] = f2bf(w[i]);
 }
 
 // Scratch of sw2dg_mlp_forward_noalloc: the BF16 input and the hidden
@@ -321,10 +371,11 @@ struct sw2dg_mlp {
     // Adam moments M1 / M2, BF16 copy PB, all with the same padded layout of
     // n_pad elements; per-layer pointers into them
     float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;
-    bf16_t *PB = nullptr;
+    bf16_t *PB = nullptr, *PBT = nullptr;  // PBT: the weight segments transposed ([out x in])
     size_t n_pad = 0;
+    WSegs segs{};
     std::vector<float *> W, b, gW, gb;
-    std::vector<bf16_t *> Wb;
+    std::vector<bf16_t *> Wb, WbT;
     // activations: X bf16 [B x d0]; per layer: Z fp32, H bf16 (hidden), dZ fp32 / bf16
     bf16_t *xb = nullptr;
     std::vector<float *> Z, dZ;
@@ -385,9 +436,14 @@ template <bool TA, bool TB, int EPIK>
 void gemm_on(hipStream_t s, const bf16_t *A, const bf16_t *B, int M, int N, int K, int lda, int ldb,
              const Epi &ep, float *part) {
     const int tiles_n = (N + TN - 1) / TN, tiles_m = (M + TM - 1) / TM, nkt = (K + TK - 1) / TK;
+    if (tiles_n * tiles_m >= 1024) {  // enough 128 x 128 tiles to fill the chip twice over
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPIK, 128>), dim3((N + 127) / 128, (M + 127) / 128, 1), dim3(256), 0, s,
+                           A, B, M, N, K, lda, ldb, nkt, (float *)nullptr, ep);
+        return;
+    }
     int kt_per = nkt;
     const int S = part ? k_splits(tiles_n * tiles_m, nkt, kt_per) : 1;
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPIK>), dim3(tiles_n, tiles_m, S), dim3(256), 0, s, A, B, M, N, K, lda,
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPIK, 64>), dim3(tiles_n, tiles_m, S), dim3(256), 0, s, A, B, M, N, K, lda,
                        ldb, kt_per, S > 1 ? part : nullptr, ep);
     if (S > 1) {
         const size_t MN = (size_t)M * N;
@@ -423,8 +479,8 @@ void forward(sw2dg_mlp *m, int B, bool split) {
     for (int l = 0; l < m->L; ++l) {
         const int K = m->dims[l], N = m->dims[l + 1];
         Epi ep{m->b[l], m->Z[l], l + 1 < m->L ? m->H[l] : nullptr, nullptr, N};
-        if (l + 1 < m->L) gemm<false, false, EPI_HIDDEN>(m, in, m->Wb[l], B, N, K, K, N, ep, split);
-        else gemm<false, false, EPI_OUT>(m, in, m->Wb[l], B, N, K, K, N, ep, split);
+        if (l + 1 < m->L) gemm<false, true, EPI_HIDDEN>(m, in, m->WbT[l], B, N, K, K, K, ep, split);
+        else gemm<false, true, EPI_OUT>(m, in, m->WbT[l], B, N, K, K, K, ep, split);
         in = m->H[l];
     }
 }
@@ -487,7 +543,7 @@ void adam_step(sw2dg_mlp *m, float lr) {
     const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
     // padding elements have zero gradient and moments: they stay zero
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->n_pad + 255) / 256)), dim3(256), 0, m->stream, m->P, m->G, m->M1,
-                       m->M2, m->PB, m->n_pad, lr, b1, b2, eps, c1, c2);
+                       m->M2, m->PB, m->PBT, m->segs, m->n_pad, lr, b1, b2, eps, c1, c2);
 }
 }  // namespace
 
@@ -515,6 +571,7 @@ int sw2dg_mlp_create(int32_t n_layers, const int32_t *dims, int32_t max_batch, i
         return SW2DG_ERR_HIP;
     }
     m->L = n_layers;
+    m->segs.L = n_layers;
     m->max_batch = max_batch;
     m->dims.assign(dims, dims + n_layers + 1);
     // parameters, gradients and Adam moments each in one array (every
@@ -534,13 +591,18 @@ int sw2dg_mlp_create(int32_t n_layers, const int32_t *dims, int32_t max_batch, i
     m->M1 = m->dalloc<float>(P);
     m->M2 = m->dalloc<float>(P);
     m->PB = m->dalloc<bf16_t>(P);
-    if (!m->P || !m->G || !m->M1 || !m->M2 || !m->PB) {
+    m->PBT = m->dalloc<bf16_t>(P);
+    if (!m->P || !m->G || !m->M1 || !m->M2 || !m->PB || !m->PBT) {
         delete m;
         return SW2DG_ERR_HIP;
     }
     for (int l = 0; l < n_layers; ++l) {
         m->W.push_back(m->P + offW[l]);
         m->Wb.push_back(m->PB + offW[l]);
+        m->WbT.push_back(m->PBT + offW[l]);
+        m->segs.off[l] = offW[l];
+        m->segs.K[l] = dims[l];
+        m->segs.N[l] = dims[l + 1];
         m->b.push_back(m->P + offb[l]);
         m->gW.push_back(m->G + offW[l]);
         m->gb.push_back(m->G + offb[l]);
@@ -576,6 +638,8 @@ int sw2dg_mlp_set_params(sw2dg_mlp *m, const float *params) {
         MHIP(m, hipMemcpy(m->b[l], params + o, nb * sizeof(float), hipMemcpyHostToDevice));
         o += nb;
         hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, m->stream, m->W[l], m->Wb[l], nw);
+        hipLaunchKernelGGL(k_to_bf16_t, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, m->stream, m->W[l],
+                           m->dims[l], m->dims[l + 1], m->WbT[l]);
     }
     MHIP(m, hipMemsetAsync(m->M1, 0, m->n_pad * sizeof(float), m->stream));
     MHIP(m, hipMemsetAsync(m->M2, 0, m->n_pad * sizeof(float), m->stream));
@@ -641,11 +705,11 @@ int sw2dg_mlp_forward_noalloc(sw2dg_mlp *m, int32_t batch, const float *x, float
             bf16_t *h = (bf16_t *)(base + off);
             off += align256((size_t)batch * N * sizeof(bf16_t));
             Epi ep{m->b[l], nullptr, h, nullptr, N};
-            gemm_on<false, false, EPI_HIDDEN>(s, in, m->Wb[l], batch, N, K, K, N, ep, nullptr);
+            gemm_on<false, true, EPI_HIDDEN>(s, in, m->WbT[l], batch, N, K, K, K, ep, nullptr);
             in = h;
         } else {
             Epi ep{m->b[l], y, nullptr, nullptr, N};
-            gemm_on<false, false, EPI_OUT>(s, in, m->Wb[l], batch, N, K, K, N, ep, nullptr);
+            gemm_on<false, true, EPI_OUT>(s, in, m->WbT[l], batch, N, K, K, K, ep, nullptr);
         }
     }
     MHIP(m, hipGetLastError());

@@ -97,3 +97,19 @@ def test_gpu_mlp_resident_epochs_equal_host_fed_steps(sw):
     rows = np.full(256, 1100, dtype=np.int32)  # outside the set: refused before any launch
     with pytest.raises(N.SW2DError, match="outside"):
         a._check(a.lib.sw2dg_mlp_train_epoch(a.h, 256, 1, rows.ctypes.data_as(N._i32), 1e-3, None))
+
+
+def test_gpu_mlp_large_batch_tiles_match_small_batches(sw):
+    """A batch large enough for the 128 x 128 output tiles (8229 rows: the
+    hidden layers' 64 x 64 grid reaches 1032 tiles; the last block of rows
+    ragged) gives each row exactly what a small batch gives it: every
+    output accumulates its k in the same order whatever the tile."""
+    from sw2d_amd import mlp
+    print("[gpu] mlp large-batch tiles vs small batches", flush=True)
+    net = mlp.MLP(DIMS, max_batch=8229, seed=6)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((8229, DIMS[0])).astype(np.float32)
+    y = net.forward(x)
+    for lo, hi in ((0, 100), (4000, 4130), (8192, 8229)):
+        np.testing.assert_array_equal(y[lo:hi], net.forward(x[lo:hi]))
+    assert _rel(y[:64], M.forward(net.params(), DIMS, x[:64])[0]) <= 1e-2
