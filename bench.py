@@ -515,6 +515,48 @@ def ensemble_bench(sw, args):
             "max_cluster": cmax, "autotune_ms_per_step": {str(k): v for k, v in tuner.last.items()}}
 
 
+def mlp_bench(sw, args):
+    """The warm start's network 2400-512-512-512-68 on synthetic data:
+    Adam steps of 256 rows on a 15,360-row training set resident on the
+    device (an epoch of 60 steps per call, one wait each), and the forward
+    of 16,384 rows (host arrays in and out)."""
+    from sw2d_amd import mlp
+    dims = [2400, 512, 512, 512, 68]
+    rng = np.random.default_rng(3)
+    n, batch = 15360, 256
+    x = rng.standard_normal((n, dims[0])).astype(np.float32)
+    t = rng.standard_normal((n, dims[-1])).astype(np.float32)
+    net = mlp.MLP(dims, max_batch=16384, seed=1)
+    t0 = time.perf_counter()
+    net.set_data(x, t)
+    t_data = time.perf_counter() - t0
+    net.train_resident(epochs=1, batch=batch, lr=1e-4, seed=0)
+    reps = 5
+    t0 = time.perf_counter()
+    net.train_resident(epochs=reps, batch=batch, lr=1e-4, seed=1)
+    t_ep = (time.perf_counter() - t0) / reps
+    steps = n // batch
+    flop_row = 2 * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+    # forward 1x, backward 2x (weight and input gradients; the first layer's input gradient is not formed)
+    flop_step = batch * (3 * flop_row - 2 * dims[0] * dims[1])
+    xf = rng.standard_normal((16384, dims[0])).astype(np.float32)
+    net.forward(xf)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        net.forward(xf)
+    t_fwd = (time.perf_counter() - t0) / 3
+    net.close()
+    ms_step = t_ep * 1e3 / steps
+    log(f"mlp: {ms_step * 1e3:.1f} us per training step of {batch} rows "
+        f"({flop_step / (ms_step * 1e-3) / 1e12:.1f} TFLOP/s), 16384-row forward {t_fwd * 1e3:.2f} ms")
+    return {"leg": "MLP of the learned warm start (BF16 MFMA, FP32 accumulation)",
+            "network": dims, "batch": batch, "training_rows": n, "steps_per_epoch": steps,
+            "ms_per_training_step": ms_step, "training_tflops": flop_step / (ms_step * 1e-3) / 1e12,
+            "set_data_s": t_data, "forward_16384_rows_ms": t_fwd * 1e3,
+            "note": "host-timed, one wait per epoch; the forward includes the host-to-device copy "
+                    "of its 157 MB of inputs; per-kernel GEMM times come from a kernel trace"}
+
+
 def copy_calibration(args):
     """A plain device copy of known size (torch's elementwise copy kernel,
     2 x 2^28 doubles read and written once) for scaling the FETCH_SIZE /
@@ -561,6 +603,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched-ensemble leg instead")
     ap.add_argument("--ens-runs", type=int, default=256)
+    ap.add_argument("--mlp", action="store_true",
+                    help="only the MLP leg (training steps and a large forward)")
     ap.add_argument("--forward-only", action="store_true",
                     help="only the native MUSCL-EVR forward line (for kernel traces and counters)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -600,6 +644,9 @@ def main():
         return
     if args.forward_only:
         print(json.dumps(forward_bench(sw, args)))
+        return
+    if args.mlp:
+        print(json.dumps(mlp_bench(sw, args)))
         return
     ratio = lambda sc: 0.0 if sc == "muscl_evr" else 1e8
     res = fwd_adj_bench(sw, args, args.scheme, ratio(args.scheme))
