@@ -115,7 +115,7 @@ class Assim:
         self.s._check(self.lib.sw2dg_assim_setup(solver.ctx, C.byref(o), int(stride), int(n_tiles),
                                                  float(t_window), self.max_steps))
         # the cut is a property of the solver context: set it on every Assim
-        self.s._check(self.lib.sw2dg_assim_set_adjoint_cut(solver.ctx, float(adj_cut)))
+        self.set_adjoint_cut(adj_cut)
         self.s._check(self.lib.sw2dg_assim_set_explosion_guard(solver.ctx, float(explosion_ratio)))
         # resident trajectory (Godunov gradients read the dry run's steps in
         # place of replaying them, where device memory allows)
@@ -132,6 +132,12 @@ class Assim:
             1 if mask_wall else 0))
         self.n_steps = 0
         self._tmp = []
+
+    def set_adjoint_cut(self, cut):
+        """Thin-film cut of the adjoint (m; 0 = the exact adjoint) for the
+        following gradients (sw2dg_assim_set_adjoint_cut)."""
+        self.adj_cut = float(cut)
+        self.s._check(self.lib.sw2dg_assim_set_adjoint_cut(self.s.ctx, self.adj_cut))
 
     def _hold(self, a):
         a = np.ascontiguousarray(a)

@@ -188,7 +188,12 @@ class Calibration:
         the recorded trajectory; the probes then run a dry run at the trial
         controls and take the true misfit of its gauge series (ls_cost
         "record", the default for tiles).  adj_cut > 0 drops the adjoint
-        of thin films (sw2dg_assim_set_adjoint_cut).  Controls are scaled to O(1)
+        of thin films (sw2dg_assim_set_adjoint_cut); where the line search
+        then finds no decrease (the cut gradient need not be a descent
+        direction), the gradient at that point is evaluated again with the
+        exact adjoint and the search repeated along it before giving up;
+        the cut applies again after the step.  Controls
+        are scaled to O(1)
         (n / 0.03, Q / max Q_default) and kept in their bounds by projection.
         Stops when J <= J_tol, after max_iter iterations or time_limit
         seconds.  Returns (n, Q, trace) with trace = [(wall s, J, kind)] for
@@ -249,6 +254,18 @@ class Calibration:
             note(J, "ls")
             return J
 
+        def search(z, J, g, d, step):
+            """Backtracking (Armijo) along d from `step` (None: a first step
+            of 0.1 in the largest control); the accepted point, or None."""
+            if step is None:
+                step = min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
+            for _ in range(max_ls):
+                zn = np.clip(z + step * d, lo, hi)
+                if cost(zn) <= J + 1e-4 * np.dot(g, zn - z):
+                    return zn
+                step *= 0.5
+            return None
+
         z = np.clip(np.concatenate([np.asarray(n0) / sn, np.asarray(Q0) / sq]), lo, hi)
         J, g = grad(z)
         S, Y = [], []
@@ -271,21 +288,21 @@ class Calibration:
             if np.dot(d, g) >= 0.0:  # not a descent direction: restart from steepest descent
                 S, Y = [], []
                 d = -np.where(free, g, 0.0)
-            accepted = False
-            for attempt in range(2):  # the quasi-Newton direction, then steepest descent
-                step = 1.0 if S else min(1.0, 0.1 / max(np.max(np.abs(d)), 1e-300))
-                for _ in range(max_ls):
-                    zn = np.clip(z + step * d, lo, hi)
-                    Jn = cost(zn)
-                    if Jn <= J + 1e-4 * np.dot(g, zn - z):
-                        accepted = True
-                        break
-                    step *= 0.5
-                if accepted or not S:
-                    break
+            # the quasi-Newton direction, then steepest descent (with a cut
+            # adjoint: along the exact gradient, evaluated here)
+            zn = search(z, J, g, d, 1.0 if S else None)
+            if zn is None and S and A.adj_cut == 0.0:
                 S, Y = [], []
-                d = -np.where(free, g, 0.0)
-            if not accepted:
+                zn = search(z, J, g, -np.where(free, g, 0.0), None)
+            if zn is None and A.adj_cut > 0.0:
+                cut = A.adj_cut
+                A.set_adjoint_cut(0.0)
+                J, g = grad(z)
+                A.set_adjoint_cut(cut)
+                S, Y = [], []
+                free = ~(((z <= lo) & (g > 0)) | ((z >= hi) & (g < 0)))
+                zn = search(z, J, g, -np.where(free, g, 0.0), None)
+            if zn is None:
                 break
             Jn, gn = grad(zn)
             sk, yk = zn - z, gn - g
