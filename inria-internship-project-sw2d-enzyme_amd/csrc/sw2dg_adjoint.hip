@@ -446,7 +446,34 @@ __global__ __launch_bounds__(kBlock) void k_e_cfl(MeshDev m, Opts o, const doubl
             }
             // the dry run reconstructs with the previous step's dt (0 before the first)
             const double dt_recon = (MUSCL && dtp) ? *dtp : 0.0;
-            for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
+            if (!MUSCL) {  // own data once, neighbours through int_nbr (as e_balance)
+                const double qx0 = U[nc + i], qy0 = U[2 * nc + i], zb0 = m.zb[i], be0 = m.beta[i];
+                const int k1 = m.int_ptr[i + 1];
+                for (int kb = m.int_ptr[i]; kb < k1; kb += 4) {
+                    int inc[4], nb[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        inc[j] = (kb + j < k1) ? m.int_inc[kb + j] : 0;
+                        nb[j] = (kb + j < k1) ? m.int_nbr[kb + j] : 0;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (kb + j >= k1) break;
+                        const int e = inc[j] >> 1, side = inc[j] & 1, c2 = nb[j];
+                        const double nx = m.enx[e], ny = m.eny[e];
+                        const spe::Side so = spe::lr_fwd(h, qx0, qy0, zb0, be0, nx, ny, o.h_min);
+                        const spe::Side sn =
+                            spe::lr_fwd(U[c2], U[nc + c2], U[2 * nc + c2], m.zb[c2], m.beta[c2], nx, ny, o.h_min);
+                        const spe::Side &sL = side ? sn : so, &sR = side ? so : sn;
+                        double m1 = fabs(sL.lm), v = fabs(sL.lp);
+                        if (m1 < v) m1 = v;
+                        if (m1 < (v = fabs(sR.lm))) m1 = v;
+                        if (m1 < (v = fabs(sR.lp))) m1 = v;
+                        swept += m1 * m.elen[e];
+                    }
+                }
+            }
+            for (int k = m.int_ptr[i]; MUSCL && k < m.int_ptr[i + 1]; ++k) {
                 const int e = m.int_inc[k] >> 1;
                 spe::Side sL, sR;
                 e_int_states<MUSCL>(m, o, U, sxy, dt_recon, e, sL, sR);
@@ -560,6 +587,47 @@ __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, con
         s.qx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
         s.qy += ((+co * ny) * f[1] + (+co * ny) * 0.0 + (+co * nx) * f[2]) * dt;
     }
+    if (!MUSCL) {
+        // the cell's own data once; per incidence only the edge and the
+        // neighbour (int_nbr), the indices of up to 4 incidences loaded
+        // together; the same operands and operations as the MUSCL branch
+        const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i], zb0 = m.zb[i], be0 = m.beta[i];
+        const int k1 = m.int_ptr[i + 1];
+        for (int kb = m.int_ptr[i]; kb < k1; kb += 4) {
+            int inc[4], nb[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                inc[j] = (kb + j < k1) ? m.int_inc[kb + j] : 0;
+                nb[j] = (kb + j < k1) ? m.int_nbr[kb + j] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (kb + j >= k1) break;
+                const int e = inc[j] >> 1, side = inc[j] & 1, c2 = nb[j];
+                const double nx = m.enx[e], ny = m.eny[e];
+                const double zb2 = m.zb[c2], be2 = m.beta[c2], ph2 = cs.phi[c2];
+                const spe::Side so = spe::lr_fwd(h0, qx0, qy0, zb0, be0, nx, ny, o.h_min);
+                const spe::Side sn = spe::lr_fwd(U[c2], U[nc + c2], U[2 * nc + c2], zb2, be2, nx, ny, o.h_min);
+                const spe::Side &sL = side ? sn : so, &sR = side ? so : sn;
+                double f[5];
+                spe::hllc_fwd(o.h_min, o.phi_min, side ? be2 : be0, side ? be0 : be2, side ? ph2 : phi,
+                              side ? phi : ph2, side ? zb2 : zb0, side ? zb0 : zb2, sL.z, sL.q, sL.r, sR.z,
+                              sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
+                const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
+                double a, b, c;
+                spe::side_contrib(side, co, nx, ny, f, dt, a, b, c);
+                s.h += a;
+                s.qx += b;
+                s.qy += c;
+                if (dint) {
+                    dint->h += a;
+                    dint->qx += b;
+                    dint->qy += c;
+                }
+            }
+        }
+        return s;
+    }
     for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
         const int inc = m.int_inc[k];
         const int e = inc >> 1, side = inc & 1;
@@ -634,10 +702,10 @@ __device__ __forceinline__ CellSt e_balance_stored(const MeshDev &m, const Opts 
 // balanced state before the clean-up and the sum of its internal-edge
 // increments, P = [pre | dint], and (computing the fluxes here) the
 // boundary fluxes Fb.
-// (the Godunov dry run that keeps the resident trajectory fits 96 VGPRs
-// without spills: 5 waves per SIMD instead of 4, 68 -> 65.5 us at 1M cells)
+// (the Godunov dry run that keeps the resident trajectory takes 125 VGPRs,
+// 4 waves per SIMD: held at 96 for 5 waves, its incidence loop spills)
 template <bool STORE, bool MUSCL, bool STORED = false>
-__global__ __launch_bounds__(kBlock, (STORE && !MUSCL) ? 5 : 1) void k_e_update(MeshDev m, Opts o, Controls cs, BcDev bc,
+__global__ __launch_bounds__(kBlock) void k_e_update(MeshDev m, Opts o, Controls cs, BcDev bc,
                                                      const double *__restrict__ Uin,
                                                      double *__restrict__ Uout,
                                                      const double *__restrict__ sxy, const double *dtp,

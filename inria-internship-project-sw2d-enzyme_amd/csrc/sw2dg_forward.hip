@@ -1107,6 +1107,11 @@ int sw2dg_ctx_create(const sw2dg_mesh *mesh, const sw2dg_params *prm, int32_t de
     m.dzdy = c->dalloc<double>(nc);
     m.int_ptr = c->upload(iptr);
     m.int_inc = c->upload(iinc);
+    {
+        std::vector<int> inbr(iinc.size());
+        for (size_t k = 0; k < iinc.size(); ++k) inbr[k] = (iinc[k] & 1) ? eL[iinc[k] >> 1] : eR[iinc[k] >> 1];
+        m.int_nbr = c->upload(inbr);
+    }
     m.bnd_ptr = c->upload(bptr);
     m.bnd_inc = c->upload(binc);
     m.eL = c->upload(eL);

@@ -60,6 +60,7 @@ struct MeshDev {
     double *dzdx, *dzdy;                                 // MUSCL slopes
     // incidence lists in reference order
     const int *int_ptr, *int_inc;  // entries (edge << 1) | side
+    const int *int_nbr;            // the other cell of each incidence
     const int *bnd_ptr, *bnd_inc;  // entries: renumbered boundary edge
     // internal edges (renumbered)
     const int *eL, *eR;
