@@ -232,10 +232,7 @@ struct NbDev {
     const double *U;
     int b;
     __device__ int inc(int k) const { return m.int_inc[b + k]; }
-    __device__ int nbr(int k) const {
-        const int c = inc(k);
-        return (c & 1) ? m.eL[c >> 1] : m.eR[c >> 1];
-    }
+    __device__ int nbr(int k) const { return m.int_nbr[b + k]; }
     __device__ double zj(int k) const {
         const int j = nbr(k);
         return U[j] + m.zb[j];

@@ -84,9 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_slopes(MeshDev m, Clock *ck, int p) 
     double zn[KC];
     int in[KC];
     double sx = 0.0, sy = 0.0, wsum = 0.0;
-    auto grad_term = [&](int inc, double &zj_out) {
-        const int e = inc >> 1;
-        const int j = (inc & 1) ? m.eL[e] : m.eR[e];
+    auto grad_term = [&](int j, double &zj_out) {  // j: the neighbour (int_nbr)
         const double zj = m.h[j] + m.zb[j];
         zj_out = zj;
         if (nk > 1) {
@@ -103,12 +101,12 @@ __global__ __launch_bounds__(kBlock) void k_slopes(MeshDev m, Clock *ck, int p) 
     for (int t = 0; t < KC; ++t) {
         if (t < nk) {
             in[t] = m.int_inc[b + t];
-            grad_term(in[t], zn[t]);
+            grad_term(m.int_nbr[b + t], zn[t]);
         }
     }
     for (int k = b + KC; k < e_end; ++k) {
         double zj;
-        grad_term(m.int_inc[k], zj);
+        grad_term(m.int_nbr[k], zj);
     }
     if (nk > 1) {
         sx /= wsum;
@@ -138,10 +136,8 @@ __global__ __launch_bounds__(kBlock) void k_slopes(MeshDev m, Clock *ck, int p) 
     for (int t = 0; t < KC; ++t)
         if (t < nk) limit(in[t], zn[t]);
     for (int k = b + KC; k < e_end; ++k) {
-        const int inc = m.int_inc[k];
-        const int e = inc >> 1;
-        const int j = (inc & 1) ? m.eL[e] : m.eR[e];
-        limit(inc, m.h[j] + m.zb[j]);
+        const int j = m.int_nbr[k];
+        limit(m.int_inc[k], m.h[j] + m.zb[j]);
     }
     m.dzdx[i] = sx;
     m.dzdy[i] = sy;
