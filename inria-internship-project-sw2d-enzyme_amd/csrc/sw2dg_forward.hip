@@ -344,7 +344,17 @@ __global__ __launch_bounds__(kBlock) void k_cfl(MeshDev m, Opts o, Clock *ck, in
             }
         }
         const int ib = m.int_ptr[i], ie = m.int_ptr[i + 1];
-        for (int k = ib; k < ie; ++k) swept += m.swp[m.int_inc[k]];  // (edge << 1) | side
+        for (int kb = ib; kb < ie; kb += 4) {  // four incidences' loads in flight, summed in order
+            int in[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) in[j] = (kb + j < ie) ? m.int_inc[kb + j] : 0;
+            double w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = (kb + j < ie) ? m.swp[in[j]] : 0.0;  // (edge << 1) | side
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (kb + j < ie) swept += w[j];
+        }
         const double x = ck->cfl * (m.area[i] / smax(swept, kSmall));
         if (x < cand) cand = x;
     }
