@@ -395,9 +395,17 @@ __global__ __launch_bounds__(kBlock) void k_e_cfl_sw(MeshDev m, Opts o, const do
                 swept += lm * m.blen[j];
             }
         }
-        for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
-            const int inc = m.int_inc[k];
-            swept += sw[2 * (size_t)(inc >> 1) + (inc & 1)];
+        const int ie = m.int_ptr[i + 1];
+        for (int kb = m.int_ptr[i]; kb < ie; kb += 4) {  // four incidences' loads in flight, summed in order
+            int in[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) in[j] = (kb + j < ie) ? m.int_inc[kb + j] : 0;
+            double w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = (kb + j < ie) ? sw[in[j]] : 0.0;  // 2 e + side
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (kb + j < ie) swept += w[j];
         }
         const double den = smax_(swept, kSmall);
         const double x = ck->cfl * (m.area[i] / den);
