@@ -1005,14 +1005,35 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
     } else if (gather >= 0) {
         const double a0 = part[i], b0 = part[nc + i], c0 = part[2 * nc + i];
         double a = a0, b = b0, c = c0;
-        for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
-            const int inc = m.int_inc[k];
-            const double *sv = side + 8 * (size_t)(inc >> 1) + 4 * (inc & 1);
-            a += sv[0];
-            b += sv[1];
-            c += sv[2];
-            gp += sv[3];
-            if (zn) a += zn[inc];  // the neighbour's slope through this cell's surface
+        // four incidences at a time: their indices, then their side records
+        // (and slope terms), then the sums in incidence order
+        const int ie = m.int_ptr[i + 1];
+        for (int kb = m.int_ptr[i]; kb < ie; kb += 4) {
+            int in[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) in[j] = (kb + j < ie) ? m.int_inc[kb + j] : 0;
+            double s0[4], s1[4], s2[4], s3[4], z[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (kb + j < ie) {
+                    const double *sv = side + 8 * (size_t)(in[j] >> 1) + 4 * (in[j] & 1);
+                    s0[j] = sv[0];
+                    s1[j] = sv[1];
+                    s2[j] = sv[2];
+                    s3[j] = sv[3];
+                    z[j] = zn ? zn[in[j]] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (kb + j < ie) {
+                    a += s0[j];
+                    b += s1[j];
+                    c += s2[j];
+                    gp += s3[j];
+                    if (zn) a += z[j];  // the neighbour's slope through this cell's surface
+                }
+            }
         }
         if (gather > 0) {  // not a tile top
             ah = a;
