@@ -677,21 +677,41 @@ __device__ __forceinline__ CellSt e_balance_stored(const MeshDev &m, const Opts 
         s.qx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
         s.qy += ((+co * ny) * f[1] + (+co * ny) * 0.0 + (+co * nx) * f[2]) * dt;
     }
-    for (int k = m.int_ptr[i]; k < m.int_ptr[i + 1]; ++k) {
-        const int inc = m.int_inc[k];
-        const int e = inc >> 1, side = inc & 1;
-        const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
-        const double *f = F + 5 * (size_t)e;
-        const double a = (side ? co : -co) * (f[0] * dt);
-        const double b = f[1 + 2 * side] * dt;
-        const double c = f[2 + 2 * side] * dt;
-        s.h += a;
-        s.qx += b;
-        s.qy += c;
-        if (dint) {
-            dint->h += a;
-            dint->qx += b;
-            dint->qy += c;
+    // four incidences at a time: their indices, then their edge lengths and
+    // stored values, then the sums in incidence order
+    const int ie = m.int_ptr[i + 1];
+    for (int kb = m.int_ptr[i]; kb < ie; kb += 4) {
+        int in[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) in[j] = (kb + j < ie) ? m.int_inc[kb + j] : 0;
+        double len[4], f0[4], f1[4], f2[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (kb + j < ie) {
+                const int e = in[j] >> 1, side = in[j] & 1;
+                const double *f = F + 5 * (size_t)e;
+                len[j] = m.elen[e];
+                f0[j] = f[0];
+                f1[j] = f[1 + 2 * side];
+                f2[j] = f[2 + 2 * side];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (kb + j < ie) {
+                const double co = spe::edge_coeff(len[j], area, phi, o.phi_min);
+                const double a = ((in[j] & 1) ? co : -co) * (f0[j] * dt);
+                const double b = f1[j] * dt;
+                const double c = f2[j] * dt;
+                s.h += a;
+                s.qx += b;
+                s.qy += c;
+                if (dint) {
+                    dint->h += a;
+                    dint->qx += b;
+                    dint->qy += c;
+                }
+            }
         }
     }
     return s;
