@@ -244,6 +244,18 @@ struct NbDev {
     __device__ int side(int k) const { return inc(k) & 1; }
 };
 
+// the same, from registers (cells of at most 4 internal edges)
+struct NbReg {
+    double z[4], x[4], y[4], xm_[4], ym_[4];
+    int s[4];
+    __device__ double zj(int k) const { return z[k]; }
+    __device__ double xj(int k) const { return x[k]; }
+    __device__ double yj(int k) const { return y[k]; }
+    __device__ double xm(int k) const { return xm_[k]; }
+    __device__ double ym(int k) const { return ym_[k]; }
+    __device__ int side(int k) const { return s[k]; }
+};
+
 // MUSCL-EVR: per cell, free-surface gradient over the cell's neighbours and
 // the reference's in-place limiter replayed over the cell's internal edges
 // in ascending order (computeMusclSlopes, swSpHypMusclEvr.cpp:11-115; a
@@ -256,9 +268,36 @@ __global__ __launch_bounds__(kBlock) void k_e_slopes(MeshDev m, const double *__
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     const int nc = m.n_cell;
     if (i >= nc) return;
-    const NbDev nb{m, U, m.int_ptr[i]};
+    const int b = m.int_ptr[i], deg = m.int_ptr[i + 1] - b;
+    const double zi = U[i] + m.zb[i], xi = m.cx[i], yi = m.cy[i];
     double sx, sy;
-    spe::slope_fwd(nb, m.int_ptr[i + 1] - nb.b, U[i] + m.zb[i], m.cx[i], m.cy[i], sx, sy);
+    if (deg >= 2 && deg <= 4) {  // triangles and quads: every incidence's data loaded once, together
+        NbReg r;
+        int in[4], nb[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            in[j] = (j < deg) ? m.int_inc[b + j] : 0;
+            nb[j] = (j < deg) ? m.int_nbr[b + j] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j < deg) {
+                const int c2 = nb[j], e = in[j] >> 1;
+                r.z[j] = U[c2] + m.zb[c2];
+                r.x[j] = m.cx[c2];
+                r.y[j] = m.cy[c2];
+                r.xm_[j] = m.exm[e];
+                r.ym_[j] = m.eym[e];
+                r.s[j] = in[j] & 1;
+            }
+        }
+        if (deg == 2) spe::slope_fwd(r, 2, zi, xi, yi, sx, sy);
+        else if (deg == 3) spe::slope_fwd(r, 3, zi, xi, yi, sx, sy);
+        else spe::slope_fwd(r, 4, zi, xi, yi, sx, sy);
+    } else {
+        const NbDev nb{m, U, b};
+        spe::slope_fwd(nb, deg, zi, xi, yi, sx, sy);
+    }
     sxy[i] = sx;
     sxy[nc + i] = sy;
 }
