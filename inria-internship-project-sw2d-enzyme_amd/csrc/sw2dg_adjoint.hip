@@ -1291,18 +1291,52 @@ __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= m.n_cell) return;
     const int b = m.int_ptr[i], deg = m.int_ptr[i + 1] - b;
-    double gx = 0.0, gy = 0.0;
-    for (int k = 0; k < deg; ++k) {
-        const int inc = m.int_inc[b + k];
-        gx += sd[2 * (size_t)inc];
-        gy += sd[2 * (size_t)inc + 1];
-    }
-    const NbDev nb{m, U, b};
     const ZnOut out{zn, m.int_inc + b};
     const double zi = U[i] + m.zb[i], xi = m.cx[i], yi = m.cy[i];
-    double dzi = 0.0;
-    if (!spe::slope_vjp_reg<kSlopeRegDeg>(nb, deg, zi, xi, yi, gx, gy, dzi, out))
+    double gx = 0.0, gy = 0.0, dzi = 0.0;
+    if (deg <= kSlopeRegDeg) {
+        // every incidence's slope adjoints, neighbour and edge data loaded
+        // once, together; the slope adjoints summed in incidence order
+        static_assert(kSlopeRegDeg == 4, "NbReg holds 4 incidences");
+        NbReg r;
+        int in[4], nb[4];
+        double g0[4], g1[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            in[j] = (j < deg) ? m.int_inc[b + j] : 0;
+            nb[j] = (j < deg) ? m.int_nbr[b + j] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j < deg) {
+                const int c2 = nb[j], e = in[j] >> 1;
+                g0[j] = sd[2 * (size_t)in[j]];
+                g1[j] = sd[2 * (size_t)in[j] + 1];
+                r.z[j] = U[c2] + m.zb[c2];
+                r.x[j] = m.cx[c2];
+                r.y[j] = m.cy[c2];
+                r.xm_[j] = m.exm[e];
+                r.ym_[j] = m.eym[e];
+                r.s[j] = in[j] & 1;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j < deg) {
+                gx += g0[j];
+                gy += g1[j];
+            }
+        }
+        spe::slope_vjp_reg<kSlopeRegDeg>(r, deg, zi, xi, yi, gx, gy, dzi, out);
+    } else {
+        for (int k = 0; k < deg; ++k) {
+            const int inc = m.int_inc[b + k];
+            gx += sd[2 * (size_t)inc];
+            gy += sd[2 * (size_t)inc + 1];
+        }
+        const NbDev nb{m, U, b};
         spe::slope_vjp(nb, deg, zi, xi, yi, gx, gy, dzi, out);
+    }
     part[i] += dzi;
 }
 
