@@ -238,7 +238,18 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double
                     swept += lm * blen[j];
                 }
             }
-            for (int k = iptr[i]; k < iptr[i + 1]; ++k) swept += sw[iinc[k]];  // (edge << 1) | side
+            const int ie = iptr[i + 1];
+            for (int kb = iptr[i]; kb < ie; kb += 4) {  // four incidences' loads in flight, summed in order
+                int in[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) in[j] = (kb + j < ie) ? iinc[kb + j] : 0;
+                double w[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) w[j] = (kb + j < ie) ? sw[in[j]] : 0.0;  // (edge << 1) | side
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (kb + j < ie) swept += w[j];
+            }
             const double den = (swept < kSmall) ? kSmall : swept;
             const double x = cfl * (area[i] / den);
             if (x < cand) cand = x;
