@@ -33,6 +33,12 @@ int sw2dh_geom_rect(int32_t nx, int32_t ny, double Lx, double Ly, sw2dh_geom **o
 /* seeded jittered (px+1)x(py+1) lattice, Delaunay by Lawson flips */
 int sw2dh_geom_delaunay(int32_t px, int32_t py, double Lx, double Ly, double jitter,
                         uint64_t seed, sw2dh_geom **out, int64_t *n_flips);
+/* any polygon mesh from its nodes and cell->node lists: cell c has nodes
+ * cell_nodes[cell_ptr[c] .. cell_ptr[c+1]) in order around it (either way);
+ * edges, normals, areas and centres as the reference's geometry classes
+ * derive them; every boundary edge gets code 1 (sw2dh_geom_set_bnd_codes) */
+int sw2dh_geom_cells(int64_t n_node, const double *x, const double *y, const double *z, int64_t n_cell,
+                     const int64_t *cell_ptr, const int64_t *cell_nodes, sw2dh_geom **out);
 int sw2dh_geom_read(const char *path, sw2dh_geom **out);
 int sw2dh_geom_write(const sw2dh_geom *g, const char *path);
 int sw2dh_geom_set_node_z(sw2dh_geom *g, const double *z);

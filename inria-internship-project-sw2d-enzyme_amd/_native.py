@@ -113,6 +113,7 @@ _HOST_DECL = {
     "sw2dh_geom_rect": ([C.c_int32, C.c_int32, C.c_double, C.c_double, C.POINTER(C.c_void_p)], C.c_int),
     "sw2dh_geom_delaunay": ([C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                              C.POINTER(C.c_void_p), _i64], C.c_int),
+    "sw2dh_geom_cells": ([C.c_int64, _d, _d, _d, C.c_int64, _i64, _i64, C.POINTER(C.c_void_p)], C.c_int),
     "sw2dh_geom_read": ([C.c_char_p, C.POINTER(C.c_void_p)], C.c_int),
     "sw2dh_geom_write": ([C.c_void_p, C.c_char_p], C.c_int),
     "sw2dh_geom_set_node_z": ([C.c_void_p, _d], C.c_int),

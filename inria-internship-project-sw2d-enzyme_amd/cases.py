@@ -48,6 +48,44 @@ class Geometry:
         return g
 
     @classmethod
+    def cells(cls, x, y, z, cells):
+        """Any polygon mesh: node coordinates and elevations, and per cell
+        its nodes in order around it (sw2dh_geom_cells)."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        ptr = np.concatenate([[0], np.cumsum([len(c) for c in cells])]).astype(np.int64)
+        nodes = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int64) for c in cells]))
+        out = C.c_void_p()
+        N.host_check(N.host_lib().sw2dh_geom_cells(len(x), N.dptr(x), N.dptr(y), N.dptr(z), len(cells),
+                                                   N.i64ptr(ptr), N.i64ptr(nodes), C.byref(out)))
+        return cls(out)
+
+    @classmethod
+    def bricks(cls, nx, ny, Lx, Ly):
+        """Staggered brick wall of nx x ny bricks (rows offset by half a
+        brick, half bricks closing alternate rows): every node of a row's
+        boundary is a vertex of the bricks on both sides, so an interior
+        brick is a 6-node polygon with 6 internal edges (2 above, 2 below,
+        left, right) and the half bricks have 4 or 5 edges."""
+        w, h = Lx / nx, Ly / ny
+        xs = [np.arange(0, 2 * nx + 1) * (0.5 * w)] * (ny + 1)  # every line: nodes every half brick
+        idx, x, y = [], [], []
+        for j in range(ny + 1):
+            idx.append(len(x) + np.arange(len(xs[j])))
+            x.extend(xs[j])
+            y.extend([j * h] * len(xs[j]))
+        cells = []
+        for j in range(ny):
+            lo, hi = idx[j], idx[j + 1]
+            # brick edges at half-brick indices: even rows 0, 2, 4, ...; odd rows 0, 1, 3, 5, ..., 2 nx
+            cuts = list(range(0, 2 * nx + 1, 2)) if j % 2 == 0 else [0] + list(range(1, 2 * nx, 2)) + [2 * nx]
+            for a, b in zip(cuts[:-1], cuts[1:]):
+                ring = [lo[k] for k in range(a, b + 1)] + [hi[k] for k in range(b, a - 1, -1)]
+                cells.append(ring)
+        x = np.asarray(x)
+        y = np.asarray(y)
+        return cls.cells(x, y, np.zeros_like(x), cells)
+
+    @classmethod
     def read(cls, path):
         out = C.c_void_p()
         N.host_check(N.host_lib().sw2dh_geom_read(str(path).encode(), C.byref(out)))
@@ -285,7 +323,7 @@ def write_distrib_map(path, names, cols):
 def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
                topo="discontinuous", friction="strickler", head_loss=True, t_max=20.0,
                dt_map=1e6, dt_ts=1e6, cfl=0.9, store_flag="default", closed_fraction=0.04,
-               wind=False, rain=None, rain_where="water_only", infil=False):
+               wind=False, rain=None, rain_where="water_only", infil=False, mesh="delaunay"):
     """Small case that exercises every branch of the SP step: Delaunay mesh,
     bumpy bed with wet/dry fronts, porosity field including closed cells
     (phi = 0), all four boundary types (west q hydrograph, east fr, north h,
@@ -296,10 +334,13 @@ def mixed_case(folder, px=12, py=10, L=(60.0, 50.0), seed=7, scheme="muscl_evr",
     after t = 0 and end before t_max, so both clamps and the interpolation
     run.  rain (a model_runoff value) adds rain and evaporation from two
     stations, rain_where is model_porosity_rainfall, infil adds Horton
-    infiltration."""
+    infiltration.  mesh "bricks": the staggered brick wall of px x py bricks
+    (Geometry.bricks: interior cells are 6-sided polygons) instead of the
+    Delaunay triangles."""
     rng = np.random.default_rng(seed)
     Lx, Ly = L
-    g = Geometry.delaunay(px, py, Lx, Ly, jitter=0.25, seed=seed)
+    g = (Geometry.bricks(px, py, Lx, Ly) if mesh == "bricks"
+         else Geometry.delaunay(px, py, Lx, Ly, jitter=0.25, seed=seed))
     nx_, ny_ = g.array("node_x"), g.array("node_y")
     z = 0.02 * (Lx - nx_)
     for _ in range(4):

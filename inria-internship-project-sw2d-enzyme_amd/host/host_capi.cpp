@@ -99,6 +99,29 @@ int sw2dh_geom_delaunay(int32_t px, int32_t py, double Lx, double Ly, double jit
     return 0;
 }
 
+int sw2dh_geom_cells(int64_t n_node, const double *x, const double *y, const double *z, int64_t n_cell,
+                     const int64_t *cell_ptr, const int64_t *cell_nodes, sw2dh_geom **out) {
+    if (!out || n_node < 3 || n_cell < 1 || !x || !y || !z || !cell_ptr || !cell_nodes)
+        return fail("bad arguments");
+    if (cell_ptr[0] != 0) return fail("cell_ptr[0] must be 0");
+    std::vector<std::vector<uint64_t>> cells((size_t)n_cell);
+    for (int64_t c = 0; c < n_cell; ++c) {
+        const int64_t b = cell_ptr[c], e = cell_ptr[c + 1];
+        if (e - b < 3) return fail("cell " + std::to_string(c) + " has fewer than 3 nodes");
+        for (int64_t k = b; k < e; ++k) {
+            if (cell_nodes[k] < 0 || cell_nodes[k] >= n_node)
+                return fail("cell " + std::to_string(c) + " names a node outside [0, n_node)");
+            cells[(size_t)c].push_back((uint64_t)cell_nodes[k]);
+        }
+    }
+    auto *h = new sw2dh_geom();
+    sw2d::build_geometry(std::vector<double>(x, x + n_node), std::vector<double>(y, y + n_node),
+                         std::vector<double>(z, z + n_node), cells, nullptr, nullptr, h->g);
+    h->refresh();
+    *out = h;
+    return 0;
+}
+
 int sw2dh_geom_read(const char *path, sw2dh_geom **out) {
     if (!out || !path) return fail("bad arguments");
     auto *h = new sw2dh_geom();

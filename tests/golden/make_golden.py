@@ -49,6 +49,13 @@ def load_pkg():
 # name -> (generator, kwargs).  Shared with the tests through golden_cases().
 def golden_cases():
     return {
+        # polygon cells: interior bricks have 6 internal edges
+        "bricks_muscl_evr_strickler": ("mixed_case", dict(px=14, py=12, mesh="bricks",
+                                                           scheme="muscl_evr", friction="strickler",
+                                                           closed_fraction=0.0, t_max=20.0,
+                                                           dt_map=5.0)),
+        "bricks_godunov_manning": ("mixed_case", dict(px=14, py=12, mesh="bricks", scheme="godunov",
+                                                      friction="manning", t_max=20.0, dt_map=5.0)),
         "dam_small_godunov": ("dambreak_case", dict(nx=20, ny=10, Lx=40.0, Ly=20.0, x_dam=20.0,
                                                      scheme="godunov", friction="none", t_max=3.0,
                                                      dt_map=1.0)),
@@ -143,6 +150,13 @@ def e_golden_cases():
         "e_mixed_godunov_manning_nohl": ("mixed_case", dict(scheme="godunov", friction="manning",
                                                             t_max=20.0, head_loss=False),
                                          [5.0, 10.0, 15.0, 20.0]),
+        "e_bricks_godunov_strickler": ("mixed_case", dict(px=14, py=12, mesh="bricks",
+                                                          scheme="godunov", friction="strickler",
+                                                          t_max=20.0), [5.0, 10.0, 15.0, 20.0]),
+        "e_bricks_muscl_strickler": ("mixed_case", dict(px=14, py=12, mesh="bricks",
+                                                        scheme="muscl_evr", friction="strickler",
+                                                        closed_fraction=0.0, t_max=20.0),
+                                     [5.0, 10.0, 15.0, 20.0]),
         "e_scale_muscl_strickler": ("scale_case", dict(px=30, py=30, L=300.0, scheme="muscl_evr",
                                                        t_max=120.0, friction="strickler",
                                                        manning=30.0), [30.0, 60.0, 90.0, 120.0]),

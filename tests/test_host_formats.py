@@ -26,11 +26,12 @@ def test_geometry_round_trip(sw, tmp_path):
         np.testing.assert_array_equal(g.array(k), g2.array(k), err_msg=k)
 
 
-@pytest.mark.parametrize("kind", ["rect", "delaunay"])
+@pytest.mark.parametrize("kind", ["rect", "delaunay", "bricks"])
 def test_mesh_geometry_is_consistent(sw, kind):
-    g = (sw.Geometry.rect(10, 7, 40.0, 21.0) if kind == "rect"
-         else sw.Geometry.delaunay(14, 11, 140.0, 110.0, jitter=0.3, seed=11))
-    Lx, Ly = (40.0, 21.0) if kind == "rect" else (140.0, 110.0)
+    g = {"rect": lambda: sw.Geometry.rect(10, 7, 40.0, 21.0),
+         "delaunay": lambda: sw.Geometry.delaunay(14, 11, 140.0, 110.0, jitter=0.3, seed=11),
+         "bricks": lambda: sw.Geometry.bricks(9, 6, 45.0, 30.0)}[kind]()
+    Lx, Ly = {"rect": (40.0, 21.0), "delaunay": (140.0, 110.0), "bricks": (45.0, 30.0)}[kind]
     nc, nb, ni, nn = g.sizes()
     A = g.array("cell_area")
     assert np.isclose(A.sum(), Lx * Ly, rtol=1e-13)
@@ -127,3 +128,29 @@ def test_uniform_and_distributed_maps(sw, tmp_path):
     c = sw.Case(str(tmp_path))
     np.testing.assert_array_equal(c["phi"], vals)
     np.testing.assert_array_equal(c["beta"], np.ones(nc))  # 'Unif 1' map
+
+
+def test_brick_mesh_cells_are_hexagons(sw):
+    """The polygon mesh of sw2dh_geom_cells: interior bricks have 6 nodes and
+    6 internal edges (2 above, 2 below, left, right), the rows' end bricks
+    fewer; every edge is shared by at most two cells."""
+    g = sw.Geometry.bricks(9, 6, 45.0, 30.0)
+    nc, nb, ni, nn = g.sizes()
+    assert nc == 3 * 9 + 3 * (8 + 2)  # even rows 9 bricks, odd rows 8 + 2 half bricks
+    L, R = g.array("int_L"), g.array("int_R")
+    deg = np.bincount(np.concatenate([L, R]), minlength=nc)
+    cx, cy = g.array("cell_x"), g.array("cell_y")
+    inner = (cx > 5.0) & (cx < 40.0) & (cy > 5.0) & (cy < 25.0)
+    assert np.all(deg[inner] == 6) and deg.max() == 6
+    A = g.array("cell_area")
+    np.testing.assert_allclose(A[inner], 5.0 * 5.0, rtol=1e-13)
+
+
+def test_polygon_mesh_rejects_bad_cells(sw):
+    from sw2d_amd import _native as N
+    x = np.array([0.0, 1.0, 1.0, 0.0])
+    y = np.array([0.0, 0.0, 1.0, 1.0])
+    with pytest.raises(N.SW2DError, match="fewer than 3 nodes"):
+        sw.Geometry.cells(x, y, np.zeros(4), [[0, 1]])
+    with pytest.raises(N.SW2DError, match="outside"):
+        sw.Geometry.cells(x, y, np.zeros(4), [[0, 1, 7]])
