@@ -954,14 +954,15 @@ __device__ __forceinline__ void seed_cell(const SeedArgs &sd, int nc, int i, dou
 
 // Reverse of the cell-local part of one step (sources, clamp, clean-up) and
 // the porosity derivative of its internal-edge factors.  (ah, ax, ay) in =
-// adjoint of U^{n+1}; out: lams = adjoint of the balanced state, part = the
-// cell-local part of the adjoint of U^n (boundary cells: completed by
-// the boundary blocks of k_r_edge).  P = [balanced state | internal-edge increments] from the replay.
+// adjoint of U^{n+1}; out: lams = adjoint of the balanced state, which is
+// also the cell-local part of the adjoint of U^n (boundary cells: completed
+// into part by the boundary blocks of k_r_edge).  P = [balanced state |
+// internal-edge increments] from the replay.
 __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, const Controls &cs,
                                              const double *__restrict__ P, double dt, int i,
                                              double ah, double ax, double ay,
-                                             double *__restrict__ lams, double *__restrict__ part,
-                                             double &gp, double *gK1, double *gK2) {
+                                             double *__restrict__ lams, double &gp, double *gK1,
+                                             double *gK2) {
     const int nc = m.n_cell;
     const CellSt dint{P[3 * nc + i], P[4 * nc + i], P[5 * nc + i]};
     CellSt s{P[i], P[nc + i], P[2 * nc + i]};
@@ -1008,9 +1009,6 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
     // balance: identity part; internal-edge factors co_e = w_e / area / phi
     // enter every contribution linearly, so
     // sum_e (dJ/dco_e) (dco_e/dphi) = -(lam . dint) / phi
-    part[i] = ah;
-    part[nc + i] = ax;
-    part[2 * nc + i] = ay;
     const double phi = cs.phi[i];
     if (phi > o.phi_min) gp += -((ah * dint.h + ax * dint.qx) + ay * dint.qy) / phi;
 }
@@ -1062,7 +1060,12 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
         ax = lam[nc + i];
         ay = lam[2 * nc + i];
     } else if (gather >= 0) {
-        const double a0 = part[i], b0 = part[nc + i], c0 = part[2 * nc + i];
+        // the cell part: lams as the cell reverse left it, or, for a cell
+        // with boundary edges, part (lams plus its boundary terms)
+        const bool bcell = m.bnd_ptr[i] < m.bnd_ptr[i + 1];
+        const double a0 = bcell ? part[i] : lams[i];
+        const double b0 = bcell ? part[nc + i] : lams[nc + i];
+        const double c0 = bcell ? part[2 * nc + i] : lams[2 * nc + i];
         double a = a0, b = b0, c = c0;
         // four incidences at a time: their indices, then their side records
         // (and slope terms), then the sums in incidence order
@@ -1107,7 +1110,7 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
     double g = gphi[i] + gp;
     if (CELL) {
         double gc_ = 0.0;
-        cell_reverse(m, o, cs, P, dt, i, ah, ax, ay, lams, part, gc_, gK1, gK2);
+        cell_reverse(m, o, cs, P, dt, i, ah, ax, ay, lams, gc_, gK1, gK2);
         g += gc_;
     } else {
         lam[i] = ah;
@@ -1140,7 +1143,7 @@ __device__ __forceinline__ void bnd_reverse(const MeshDev &m, const Opts &o, con
     const int i = m.bnd_cells[t];
     const int nc = m.n_cell;
     const double ah = lams[i], ax = lams[nc + i], ay = lams[2 * nc + i];
-    double ph = part[i], px = part[nc + i], py = part[2 * nc + i], gp = 0.0;
+    double ph = ah, px = ax, py = ay, gp = 0.0;  // the cell part (lams), completed here
     const double phi = cs.phi[i], area = m.area[i];
     const double h0 = U[i], qx0 = U[nc + i], qy0 = U[2 * nc + i];
     const double *Fb = F;  // the step's boundary fluxes, 3 per boundary edge
@@ -1287,6 +1290,7 @@ constexpr int kSlopeRegDeg = 4;
 __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__restrict__ U,
                                                      const double *__restrict__ sd,
                                                      double *__restrict__ part,
+                                                     double *__restrict__ lams,
                                                      double *__restrict__ zn) {
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
     if (i >= m.n_cell) return;
@@ -1337,7 +1341,12 @@ __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__
         const NbDev nb{m, U, b};
         spe::slope_vjp(nb, deg, zi, xi, yi, gx, gy, dzi, out);
     }
-    part[i] += dzi;
+    // into the cell part of the adjoint of U^n: part for a cell with
+    // boundary edges, lams otherwise (the edge reverse has read lams)
+    if (m.bnd_ptr[i] < m.bnd_ptr[i + 1])
+        part[i] += dzi;
+    else
+        lams[i] += dzi;
 }
 
 __global__ void k_zero(double *p, size_t n) {
@@ -2065,7 +2074,7 @@ int sw2dg_assim_gradient(sw2dg_ctx *c, const double *phi, const double *K1, cons
             KLAUNCH(a, k_r_edge<true>, dim3(neb + nbb), blk, 0, c->stream, c->m, c->o, cs,
                     seg_state(a, c, j), dt[n], a->d_lams, a->d_side, seg_slopes(a, c, j), a->d_sd, br);
             KLAUNCH(a, k_r_slopes, dim3(ncb), blk, 0, c->stream, c->m, seg_state(a, c, j), a->d_sd,
-                    a->d_part, a->d_zn);
+                    a->d_part, a->d_lams, a->d_zn);
         } else {
             KLAUNCH(a, k_r_edge<false>, dim3(neb + nbb), blk, 0, c->stream, c->m, c->o, cs,
                     seg_state(a, c, j), dt[n], a->d_lams, a->d_side, nullptr, nullptr, br);
