@@ -107,11 +107,12 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
         # cells U(h), zb, centroid, CSR ptr, slopes out; incidences, edge cells
         # and midpoints
         "k_e_slopes": 52 * nc + 32 * ni,
-        # cells part in/out, ptrs, P in, phi, K1, K2, cos, sin, lams out,
+        # cells: the cell part in (lams, or part for a boundary cell), ptrs
+        # (internal and boundary), P in, phi, K1, K2, cos, sin, lams out,
         # gphi/gK1/gK2 read-modify-write; incidences + edge sides (+ the
         # neighbours' slope terms, MUSCL)
-        "k_r_gc<true>": 216 * nc + (80 if muscl else 72) * ni,
-        "k_r_gc<false>": 80 * nc + (80 if muscl else 72) * ni,
+        "k_r_gc<true>": 196 * nc + (80 if muscl else 72) * ni,
+        "k_r_gc<false>": 84 * nc + (80 if muscl else 72) * ni,
         # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, length; sides
         # out; the boundary blocks: boundary cells' state, statics, lams, part,
         # gphi, boundary edges' geometry, F, gbce
