@@ -520,9 +520,12 @@ __global__ __launch_bounds__(kDtBlock) void k_dt(MeshDev m, BcDev bc, Clock *ck,
     __shared__ double red[kDtBlock / 64];
     const double dt_max = ck->dt_max;
     double v = dt_max;
-    for (int k = threadIdx.x; k < n_part; k += kDtBlock) {
-        const double x = m.partial[k];
-        v = (x < v) ? x : v;
+    for (int k0 = threadIdx.x; k0 < n_part; k0 += 4 * kDtBlock) {  // four loads in flight per thread
+        double x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = (k0 + j * kDtBlock < n_part) ? m.partial[k0 + j * kDtBlock] : dt_max;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v = (x[j] < v) ? x[j] : v;
     }
     v = wave_min(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
