@@ -1017,13 +1017,14 @@ __device__ __forceinline__ void cell_reverse(const MeshDev &m, const Opts &o, co
 }
 
 // One fused reverse kernel per step boundary n:
-//   gather  the adjoint of U^n = part + the edge-side contributions of step
+//   gather  the adjoint of U^n = the cell part (lams, or part for a cell
+//           with boundary edges) + the edge-side contributions of step
 //           n in the cell's reference incidence order (reset: the tile top,
 //           the adjoint restarts from 0; the porosity terms still count)
 //   seed    + dJ_obs/dU^n when U^n is observed
 //   CELL    reverse of the cell part of step n-1 (else: write lam = adj U^n)
 // gather: -1 nothing to gather (the last step, or a tile top: the future
-// adjoint is dropped), 1 gather part + edge sides (+ zn, MUSCL-EVR:
+// adjoint is dropped), 1 gather the cell part + edge sides (+ zn, MUSCL-EVR:
 // the neighbours' slope adjoints through this cell's free surface), 2 take lam
 // (the gather already done by the CELL = false launch that closed the next
 // segment).
@@ -1126,7 +1127,8 @@ __global__ __launch_bounds__(kBlock) void k_r_gc(MeshDev m, Opts o, Controls cs,
 // the boundary part of the edge reverse of a step: per boundary cell, the
 // reverse of its boundary fluxes (hypBoundaryFlux_E and the Godunov or
 // MUSCL-EVR _E boundary states) and of their factors, in reference order,
-// added to the cell part of the adjoint; boundary-value adjoint per edge
+// added to the cell part of the adjoint (lams) into part; boundary-value
+// adjoint per edge
 struct BndRev {
     BcDev bc;
     const double *F;    // the step's boundary fluxes, 3 per boundary edge
