@@ -305,14 +305,32 @@ __global__ __launch_bounds__(B) void k_ens(EnsDev d, Clusters cl, Opts o, double
                 sx += ((+co * nx) * f[1] + (+co * nx) * 0.0 + (-co * ny) * f[2]) * dt;
                 sy += ((+co * ny) * f[1] + (+co * ny) * 0.0 + (+co * nx) * f[2]) * dt;
             }
-            for (int k = iptr[i]; k < iptr[i + 1]; ++k) {
-                const int inc = iinc[k];
-                const int e = inc >> 1, side = inc & 1;
-                const double co = spe::edge_coeff(elen[e], ar, ph, o.phi_min);
-                const double *f = F + 5 * (size_t)e;
-                sh += (side ? co : -co) * (f[0] * dt);
-                sx += f[1 + 2 * side] * dt;
-                sy += f[2 + 2 * side] * dt;
+            const int ie = iptr[i + 1];
+            for (int kb = iptr[i]; kb < ie; kb += 2) {  // two incidences' loads together, sums in order
+                int in[2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) in[j] = (kb + j < ie) ? iinc[kb + j] : 0;
+                double len[2], f0[2], f1[2], f2[2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (kb + j < ie) {
+                        const int e = in[j] >> 1, side = in[j] & 1;
+                        const double *f = F + 5 * (size_t)e;
+                        len[j] = elen[e];
+                        f0[j] = f[0];
+                        f1[j] = f[1 + 2 * side];
+                        f2[j] = f[2 + 2 * side];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (kb + j < ie) {
+                        const double co = spe::edge_coeff(len[j], ar, ph, o.phi_min);
+                        sh += ((in[j] & 1) ? co : -co) * (f0[j] * dt);
+                        sx += f1[j] * dt;
+                        sy += f2[j] * dt;
+                    }
+                }
             }
             if (fabs(sh) < kSmall) sh = 0.0;
             if (fabs(sx) < kSmall) sx = 0.0;
