@@ -58,6 +58,12 @@ struct GaugeSample {
 
 struct Controls {
     const double *phi, *K1, *K2;
+    // each internal edge side's factor edge_coeff(length, area, phi) of its
+    // cell, at 2 e + side (so also at a cell's incidence entry); computed
+    // once per control upload (k_coe) and read by the Godunov update and the
+    // edge reverse (the MUSCL-EVR edge pass and stored update, with VALU to
+    // spare, form it in place: the two-sided array gathers worse than elen)
+    const double *coe;
 };
 
 struct AssimState {
@@ -132,6 +138,7 @@ struct AssimState {
     double dh_peak = 0.0;       // largest ||lam_h|| at a segment start in the last gradient
     // controls (device order)
     double *d_phi = nullptr, *d_K1 = nullptr, *d_K2 = nullptr;
+    double *d_coe = nullptr;  // [2 n_int] edge-side factors of the current phi
     // series -> boundary edges (renumbered) in reference order
     int *d_ser_ptr = nullptr, *d_ser_edges = nullptr;
     // optional per-launch timing (sw2dg_assim_profile): an event pair per launch
@@ -660,7 +667,7 @@ __device__ __forceinline__ CellSt e_balance(const MeshDev &m, const Opts &o, con
                 spe::hllc_fwd(o.h_min, o.phi_min, side ? be2 : be0, side ? be0 : be2, side ? ph2 : phi,
                               side ? phi : ph2, side ? zb2 : zb0, side ? zb0 : zb2, sL.z, sL.q, sL.r, sR.z,
                               sR.q, sR.r, sL.lm, sR.lm, sL.lp, sR.lp, f);
-                const double co = spe::edge_coeff(m.elen[e], area, phi, o.phi_min);
+                const double co = cs.coe[inc[j]];  // edge_coeff(elen[e], area, phi)
                 double a, b, c;
                 spe::side_contrib(side, co, nx, ny, f, dt, a, b, c);
                 s.h += a;
@@ -1215,10 +1222,9 @@ __global__ __launch_bounds__(kBlock, MUSCL ? 3 : 4) void k_r_edge(MeshDev m, Opt
     if (e >= m.n_int) return;
     const int nc = m.n_cell;
     const int L = m.eL[e], R = m.eR[e];
-    const double nx = m.enx[e], ny = m.eny[e], len = m.elen[e];
+    const double nx = m.enx[e], ny = m.eny[e];
     const double phiL = cs.phi[L], phiR = cs.phi[R];
-    const double coL = spe::edge_coeff(len, m.area[L], phiL, o.phi_min);
-    const double coR = spe::edge_coeff(len, m.area[R], phiR, o.phi_min);
+    const double coL = cs.coe[2 * (size_t)e], coR = cs.coe[2 * (size_t)e + 1];
     const double Fz[5] = {0, 0, 0, 0, 0};
     double dF[5] = {0, 0, 0, 0, 0};
     spe::side_contrib_vjp(0, coL, nx, ny, Fz, dt, lams[L], lams[nc + L], lams[2 * nc + L], dF);
@@ -1354,6 +1360,17 @@ __global__ __launch_bounds__(kBlock) void k_r_slopes(MeshDev m, const double *__
         lams[i] += dzi;
 }
 
+// the edge-side factors of the current phi (Controls::coe)
+__global__ __launch_bounds__(kBlock) void k_coe(MeshDev m, Opts o, const double *__restrict__ phi,
+                                                double *__restrict__ coe) {
+    const int e = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    if (e >= m.n_int) return;
+    const int L = m.eL[e], R = m.eR[e];
+    const double len = m.elen[e];
+    coe[2 * (size_t)e] = spe::edge_coeff(len, m.area[L], phi[L], o.phi_min);
+    coe[2 * (size_t)e + 1] = spe::edge_coeff(len, m.area[R], phi[R], o.phi_min);
+}
+
 __global__ void k_zero(double *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -1436,12 +1453,14 @@ int upload_ctl(sw2dg_ctx *c, AssimState *a, const double *phi, const double *K1,
     };
     int st;
     if ((st = put(phi, a->d_phi, c->m.phi))) return st;
+    if (c->ni > 0)
+        KLAUNCH(a, k_coe, dim3(nblocks(c->ni)), dim3(kBlock), 0, c->stream, c->m, c->o, a->d_phi, a->d_coe);
     if ((st = put(K1, a->d_K1, c->m.fc1))) return st;
     if ((st = put(K2, a->d_K2, c->m.fc2))) return st;
     return SW2DG_OK;
 }
 
-Controls ctl(AssimState *a) { return Controls{a->d_phi, a->d_K1, a->d_K2}; }
+Controls ctl(AssimState *a) { return Controls{a->d_phi, a->d_K1, a->d_K2, a->d_coe}; }
 
 // boundary fluxes of one step (3 per boundary edge), kept per replayed step
 // for the boundary reverse
@@ -1856,6 +1875,7 @@ int sw2dg_assim_setup(sw2dg_ctx *c, const sw2dg_obs *obs, int32_t stride, int32_
     a->d_wm = a->alloc<double>(3 * nc);
     a->d_nrm = a->alloc<double>(nblocks(nc) + 1);
     a->d_phi = a->alloc<double>(nc);
+    a->d_coe = a->alloc<double>(2 * (size_t)std::max<int64_t>(c->ni, 1));
     a->d_K1 = a->alloc<double>(nc);
     a->d_K2 = a->alloc<double>(nc);
     // boundary edges of each series in reference order (renumbered indices)
