@@ -88,13 +88,14 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
         "k_e_cfl<true>": 88 * nc + 56 * ni + 28 * nb,
         "k_e_dt": 8 * ((nc + 255) // 256),
         # fused flux + update: cells U in/out, zb, beta, phi, area, K1, K2, cos,
-        # sin, ptrs (+ slopes, centroids); incidences, edge cells and geometry
-        # (+ midpoints); boundary incidences, geometry, series; the replay
+        # sin, ptrs (+ slopes, centroids); incidences, their neighbours and
+        # edge-side factors (Godunov), edge geometry (+ midpoints); boundary
+        # incidences, geometry, series; the replay
         # (<true, ., .>) also writes the balanced state + increments and the
         # boundary fluxes.  Each internal edge's flux is computed by both of its
         # cells; the bytes count its inputs once.
-        "k_e_update<false, false, false>": 120 * nc + 40 * ni + 32 * nb,
-        "k_e_update<true, false, false>": 168 * nc + 40 * ni + 56 * nb,
+        "k_e_update<false, false, false>": 120 * nc + 48 * ni + 32 * nb,
+        "k_e_update<true, false, false>": 168 * nc + 48 * ni + 56 * nb,
         # MUSCL-EVR: edges once (states, fluxes and, dry run, swept terms
         # out), the CFL from the swept terms, the update from the stored
         # fluxes (F1 and each side's momentum sum, 40 bytes per edge) and the
@@ -113,12 +114,12 @@ def kernel_bytes(name, nc, ni, nb, n_gauges, scheme="godunov"):
         # neighbours' slope terms, MUSCL)
         "k_r_gc<true>": 196 * nc + (80 if muscl else 72) * ni,
         "k_r_gc<false>": 84 * nc + (80 if muscl else 72) * ni,
-        # cells phi, area, zb, beta, U, lams; edges L, R, nx, ny, length; sides
-        # out; the boundary blocks: boundary cells' state, statics, lams, part,
-        # gphi, boundary edges' geometry, F, gbce
-        "k_r_edge<false>": 80 * nc + 96 * ni + 160 * nb,
+        # cells phi, zb, beta, U, lams; edges L, R, nx, ny, the two sides'
+        # factors; sides out; the boundary blocks: boundary cells' state,
+        # statics, lams, part, gphi, boundary edges' geometry, F, gbce
+        "k_r_edge<false>": 72 * nc + 104 * ni + 160 * nb,
         # + cell slopes and centroids, edge midpoints, slope adjoints out
-        "k_r_edge<true>": 112 * nc + 144 * ni + 160 * nb,
+        "k_r_edge<true>": 104 * nc + 152 * ni + 160 * nb,
         # cells U(h), zb, centroid, ptr, part(h) in/out; incidences, slope
         # adjoints in, neighbour terms out, edge cells and midpoints
         "k_r_slopes": 52 * nc + 80 * ni,
